@@ -1,0 +1,13 @@
+// Internal entry points shared between translation units of libflub.so
+// (not part of the public C-ABI in include/flub.h).
+#pragma once
+#include <stdint.h>
+
+// divergence_tag + mask_with_tag + scale_by_3d(s) in one pass over packed tags.
+int flub_divergence_fused_u8(const float *u, const float *v, const float *w, float *div,
+                             const float *ur, const float *vr, const float *wr, const uint8_t *tag,
+                             int nx, int ny, int nz, float s, void *stream);
+// subtract_gradient_tag reading the solver's x and applying the reference's
+// pressure scale chain (x*h)*(1/h) on the fly.
+int flub_subtract_gradient_x_u8(float *uf, const float *u_rigid, const float *x, const uint8_t *tag,
+                                int nx, int ny, int nz, int d, float h, void *stream);

@@ -1,0 +1,203 @@
+// Cell classification, MAC divergence, masking, scaling and gradient
+// subtraction: the grid kernels the reference launches around its pressure
+// solve (geometry.py:335-397, divergence.py:44-180, utils/util.py:162-165,
+// subtract_gradient.py:38-68).  One thread per cell or face, k fastest across
+// consecutive lanes so every global access of a wavefront is one contiguous
+// 256-byte run.
+#include "common.h"
+#include "grid_ops.h"
+
+__global__ void k_tag_dirichlet(const float *__restrict__ phi, int32_t *__restrict__ tag, size_t n)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) tag[t] = phi[t] > 0.0f ? 0 : 1;
+}
+
+__global__ void k_tag_neumann(const float *__restrict__ phi, int32_t *__restrict__ tag, size_t n)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n && phi[t] < 0.0f) tag[t] = 2;
+}
+
+FLUB_API int flub_tag_dirichlet(const float *phi, int32_t *tag, size_t n, void *stream)
+{
+    if (!n) return FLUB_OK;
+    FLUB_REQUIRE(phi && tag, FLUB_EINVAL, "flub_tag_dirichlet: null pointer");
+    k_tag_dirichlet<<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(phi, tag, n);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+FLUB_API int flub_tag_neumann(const float *phi_rigid, int32_t *tag, size_t n, void *stream)
+{
+    if (!n) return FLUB_OK;
+    FLUB_REQUIRE(phi_rigid && tag, FLUB_EINVAL, "flub_tag_neumann: null pointer");
+    k_tag_neumann<<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(phi_rigid, tag, n);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+// divergence.py:44-99.  FUSE additionally applies mask_with_tag (:174-180) and
+// scale_by_3d(div, s) in registers: (tag==FLUID ? F : 0) * s, which is the
+// value the three separate launches store.
+template <typename TagT, bool FUSE>
+__global__ void k_divergence(const float *__restrict__ u, const float *__restrict__ v,
+                             const float *__restrict__ w, float *__restrict__ div,
+                             const float *__restrict__ ur, const float *__restrict__ vr,
+                             const float *__restrict__ wr, const TagT *__restrict__ tag, int nx,
+                             int ny, int nz, float s)
+{
+    const uint32_t n = (uint32_t)nx * ny * nz;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int k = t % nz;
+    const int j = (t / nz) % ny;
+    const int i = t / ((uint32_t)nz * ny);
+    const uint32_t plane = (uint32_t)ny * nz;
+    const uint32_t iu0 = t, iu1 = t + plane;
+    const uint32_t iv0 = ((uint32_t)i * (ny + 1) + j) * nz + k, iv1 = iv0 + nz;
+    const uint32_t iw0 = ((uint32_t)i * ny + j) * (nz + 1) + k, iw1 = iw0 + 1;
+    float vl = u[iu0], vrr = u[iu1], vb = v[iv0], vt = v[iv1], vf = w[iw0], vn = w[iw1];
+    if (i == 0) vl = 0.0f;
+    else if (tag[t - plane] == 2) vl = ur[iu0];
+    if (i == nx - 1) vrr = 0.0f;
+    else if (tag[t + plane] == 2) vrr = ur[iu1];
+    if (j == 0) vb = 0.0f;
+    else if (tag[t - nz] == 2) vb = vr[iv0];
+    if (j == ny - 1) vt = 0.0f;
+    else if (tag[t + nz] == 2) vt = vr[iv1];
+    if (k == 0) vf = 0.0f;
+    else if (tag[t - 1] == 2) vf = wr[iw0];
+    if (k == nz - 1) vn = 0.0f;
+    else if (tag[t + 1] == 2) vn = wr[iw1];
+    float F = vrr - vl + vt - vb + vn - vf;
+    if (FUSE) F = (tag[t] == 1 ? F : 0.0f) * s;
+    div[t] = F;
+}
+
+static bool grid_ok(int nx, int ny, int nz)
+{
+    if (nx < 1 || ny < 1 || nz < 1) return false;
+    const double cells = (double)(nx + 1) * (ny + 1) * (nz + 1);
+    return cells < 2147483647.0;
+}
+
+template <typename TagT, bool FUSE>
+static int divergence_impl(const float *u, const float *v, const float *w, float *div, const float *ur,
+                           const float *vr, const float *wr, const TagT *tag, int nx, int ny, int nz,
+                           float s, void *stream)
+{
+    FLUB_REQUIRE(u && v && w && div && ur && vr && wr && tag, FLUB_EINVAL, "divergence: null pointer");
+    FLUB_REQUIRE(grid_ok(nx, ny, nz), FLUB_ESHAPE, "divergence: bad grid %dx%dx%d", nx, ny, nz);
+    const size_t n = (size_t)nx * ny * nz;
+    k_divergence<TagT, FUSE><<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(
+        u, v, w, div, ur, vr, wr, tag, nx, ny, nz, s);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+FLUB_API int flub_divergence_tag(const float *u, const float *v, const float *w, float *div,
+                                 const float *u_rigid, const float *v_rigid, const float *w_rigid,
+                                 const int32_t *tag, int nx, int ny, int nz, void *stream)
+{
+    return divergence_impl<int32_t, false>(u, v, w, div, u_rigid, v_rigid, w_rigid, tag, nx, ny, nz,
+                                           1.0f, stream);
+}
+
+int flub_divergence_fused_u8(const float *u, const float *v, const float *w, float *div,
+                             const float *ur, const float *vr, const float *wr, const uint8_t *tag,
+                             int nx, int ny, int nz, float s, void *stream)
+{
+    return divergence_impl<uint8_t, true>(u, v, w, div, ur, vr, wr, tag, nx, ny, nz, s, stream);
+}
+
+__global__ void k_mask_with_tag(float *__restrict__ div, const int32_t *__restrict__ tag, size_t n)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n && tag[t] != 1) div[t] = 0.0f;
+}
+
+FLUB_API int flub_mask_with_tag(float *div, const int32_t *tag, size_t n, void *stream)
+{
+    if (!n) return FLUB_OK;
+    FLUB_REQUIRE(div && tag, FLUB_EINVAL, "flub_mask_with_tag: null pointer");
+    k_mask_with_tag<<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(div, tag, n);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+__global__ void k_scale(float *__restrict__ q, float s, size_t n)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) q[t] = q[t] * s;
+}
+
+FLUB_API int flub_scale(float *q, float s, size_t n, void *stream)
+{
+    if (!n) return FLUB_OK;
+    FLUB_REQUIRE(q, FLUB_EINVAL, "flub_scale: null pointer");
+    k_scale<<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(q, s, n);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+// subtract_gradient.py:38-68 over the face grid of axis d.  PS != 0 reads the
+// pressure as (x * h) * (1/h) -- the value the reference's scale chain
+// (water_3d.py:67-68, example_fluid.py:134) hands to the kernel -- so the
+// fused projection can read x directly.
+template <typename TagT, bool PS>
+__global__ void k_subtract_gradient(float *__restrict__ uf, const float *__restrict__ urig,
+                                    const float *__restrict__ p, const TagT *__restrict__ tag, int nx,
+                                    int ny, int nz, int d, float h, float inv_h)
+{
+    const int fx = nx + (d == 1), fy = ny + (d == 2), fz = nz + (d == 3);
+    const uint32_t nf = (uint32_t)fx * fy * fz;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nf) return;
+    const int k = t % fz;
+    const int j = (t / fz) % fy;
+    const int i = t / ((uint32_t)fz * fy);
+    const int ii = d == 1 ? i : (d == 2 ? j : k);
+    const int n_d = d == 1 ? nx : (d == 2 ? ny : nz);
+    float out;
+    if (0 < ii && ii < n_d) {
+        const uint32_t I = cidx(i, j, k, ny, nz);
+        const uint32_t off = d == 1 ? (uint32_t)ny * nz : (d == 2 ? (uint32_t)nz : 1u);
+        float pl = p[I - off], pr = p[I];
+        if (PS) { pl = (pl * h) * inv_h; pr = (pr * h) * inv_h; }
+        const int tl = tag[I - off], tr = tag[I];
+        if (tl == 2 || tr == 2) out = urig[t];
+        else if (tl == 1 || tr == 1) out = uf[t] - (pr - pl);
+        else out = 0.0f;
+    } else {
+        out = 0.0f;
+    }
+    uf[t] = out;
+}
+
+template <typename TagT, bool PS>
+static int subgrad_impl(float *uf, const float *urig, const float *p, const TagT *tag, int nx, int ny,
+                        int nz, int d, float h, void *stream)
+{
+    FLUB_REQUIRE(uf && urig && p && tag, FLUB_EINVAL, "subtract_gradient_tag: null pointer");
+    FLUB_REQUIRE(d >= 1 && d <= 3, FLUB_EINVAL, "subtract_gradient_tag: d must be 1, 2 or 3 (got %d)", d);
+    FLUB_REQUIRE(grid_ok(nx, ny, nz), FLUB_ESHAPE, "subtract_gradient_tag: bad grid");
+    const size_t nf = (size_t)(nx + (d == 1)) * (ny + (d == 2)) * (nz + (d == 3));
+    const float inv_h = (float)(1.0 / (double)h);
+    k_subtract_gradient<TagT, PS><<<blocks_for(nf, 256), 256, 0, as_stream(stream)>>>(
+        uf, urig, p, tag, nx, ny, nz, d, h, inv_h);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+FLUB_API int flub_subtract_gradient_tag(float *uf, const float *u_rigid, const float *p,
+                                        const int32_t *tag, int nx, int ny, int nz, int d, void *stream)
+{
+    return subgrad_impl<int32_t, false>(uf, u_rigid, p, tag, nx, ny, nz, d, 1.0f, stream);
+}
+
+int flub_subtract_gradient_x_u8(float *uf, const float *u_rigid, const float *x, const uint8_t *tag,
+                                int nx, int ny, int nz, int d, float h, void *stream)
+{
+    return subgrad_impl<uint8_t, true>(uf, u_rigid, x, tag, nx, ny, nz, d, h, stream);
+}

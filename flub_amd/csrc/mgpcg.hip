@@ -1,0 +1,982 @@
+// MGPCG pressure solver (multigrid/mgpcg.py:234-490) for one MI355X.
+//
+// Handle = level pyramid (r, z in FP32, one-byte tags), x, p, Ap, FP64 device
+// scalars and a partial-sum buffer for the deterministic FP64 reductions.
+// Per-cell arithmetic follows the reference kernels operation by operation,
+// so every kernel here is bitwise equal to the FP32 oracle on equal inputs;
+// the dot products accumulate in FP64 (fixed two-stage tree) instead of
+// Warp's FP32 array_inner.
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "common.h"
+#include "grid_ops.h"
+#include "mgpcg_internal.h"
+
+// ---------------------------------------------------------------- tags
+__global__ void k_pack_tags(const int32_t *__restrict__ in, uint8_t *__restrict__ out, uint32_t n)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] = (uint8_t)in[t];
+}
+
+// mgpcg.py:162-174 restrict_tag, literally: t = 2; a 0 child -> 0; else a 1
+// child -> 1 (equal to min over the children for tags in {0,1,2}).
+__global__ void k_restrict_tag(const uint8_t *__restrict__ f, uint8_t *__restrict__ c, int cx, int cy,
+                               int cz)
+{
+    const uint32_t n = (uint32_t)cx * cy * cz;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int K = t % cz, J = (t / cz) % cy, I = t / ((uint32_t)cz * cy);
+    const int ny = cy * 2, nz = cz * 2;
+    int tt = 2;
+#pragma unroll
+    for (int ii = 0; ii < 2; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+            for (int kk = 0; kk < 2; ++kk) {
+                const int tg = f[cidx(2 * I + ii, 2 * J + jj, 2 * K + kk, ny, nz)];
+                if (tg == 0) tt = 0;
+                else if (tt > 0 && tg == 1) tt = 1;
+            }
+    c[t] = (uint8_t)tt;
+}
+
+// mgpcg.py:51-58 _compute_divd (attribute view; the solver derives divd from
+// the tags inside each stencil and never stores it).
+__global__ void k_divd(const uint8_t *__restrict__ tag, float *__restrict__ divd, int nx, int ny, int nz)
+{
+    const uint32_t n = (uint32_t)nx * ny * nz;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int k = t % nz, j = (t / nz) % ny, i = t / ((uint32_t)nz * ny);
+    int d = 0;
+    if (tag[t] == 1) {
+        d += (i + 1 < nx && tag[cidx(i + 1, j, k, ny, nz)] < 2);
+        d += (i > 0 && tag[cidx(i - 1, j, k, ny, nz)] < 2);
+        d += (j + 1 < ny && tag[cidx(i, j + 1, k, ny, nz)] < 2);
+        d += (j > 0 && tag[cidx(i, j - 1, k, ny, nz)] < 2);
+        d += (k + 1 < nz && tag[cidx(i, j, k + 1, ny, nz)] < 2);
+        d += (k > 0 && tag[cidx(i, j, k - 1, ny, nz)] < 2);
+    }
+    divd[t] = (float)d;
+}
+
+// ---------------------------------------------------------------- elementwise
+__global__ void k_fill_f32(float *__restrict__ p, float v, uint32_t n)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+__global__ void k_scale_copy(float *__restrict__ dst, const float *__restrict__ src, float k, uint32_t n)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = src[t] * k;
+}
+
+__global__ void k_mask_interior(const uint8_t *__restrict__ tag, float *__restrict__ r, uint32_t n)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n && tag[t] != 1) r[t] = 0.0f;
+}
+
+// p = ((x*h)*(1/h))*h : the stored pressure of example_fluid.py:134,145.
+__global__ void k_store_pressure(float *__restrict__ p, const float *__restrict__ x, float h, float inv_h,
+                                 uint32_t n)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = ((x[t] * h) * inv_h) * h;
+}
+
+// ---------------------------------------------------------------- stencils
+struct GX {
+    const float *x;
+    int ny, nz;
+    __device__ float operator()(int i, int j, int k) const { return x[cidx(i, j, k, ny, nz)]; }
+};
+struct GT {
+    const uint8_t *t;
+    int ny, nz;
+    __device__ int operator()(int i, int j, int k) const { return t[cidx(i, j, k, ny, nz)]; }
+};
+
+// mgpcg.py:72-86 _smooth_mcadams over (nx, ny, nz/2): phase p updates the
+// cells with (i+j+k)&1 == p^1.  divd from the tags (equal to _compute_divd).
+__global__ void k_smooth_mc(float *__restrict__ z, const float *__restrict__ r,
+                            const uint8_t *__restrict__ tag, int nx, int ny, int nz, int phase)
+{
+    const int nzh = nz >> 1;
+    const uint32_t n = (uint32_t)nx * ny * nzh;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int kk = t % nzh, j = (t / nzh) % ny, i = t / ((uint32_t)nzh * ny);
+    const int k = (((i + j) & 1) == phase) ? kk * 2 + 1 : kk * 2;
+    const uint32_t I = cidx(i, j, k, ny, nz);
+    const int tI = tag[I];
+    const float zI = z[I];
+    int divd;
+    const float nd = ndt_stencil(GX{z, ny, nz}, GT{tag, ny, nz}, i, j, k, nx, ny, nz, zI, tI, &divd);
+    z[I] = divd != 0 ? zI + (r[I] + nd) / (float)divd : 0.0f;
+}
+
+// mgpcg.py:41-49 _smooth (tag-free): updates (i+j+k)&1 == phase, with the
+// reference's res = shape[0] on every axis.
+__global__ void k_smooth_def(float *__restrict__ z, const float *__restrict__ r, int nx, int ny, int nz,
+                             int phase)
+{
+    const int nzh = nz >> 1;
+    const uint32_t n = (uint32_t)nx * ny * nzh;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int kk = t % nzh, j = (t / nzh) % ny, i = t / ((uint32_t)nzh * ny);
+    const int k = kk * 2 + ((i + j + phase) & 1);
+    const int res = nx;
+    const float dx = (i == 0 || i == res - 1) ? 1.0f : 2.0f;
+    const float dy = (j == 0 || j == res - 1) ? 1.0f : 2.0f;
+    const float dz = (k == 0 || k == res - 1) ? 1.0f : 2.0f;
+    const uint32_t I = cidx(i, j, k, ny, nz);
+    const float zI = z[I];
+    const float ns = nsum_stencil(GX{z, ny, nz}, i, j, k, nx, ny, nz, zI);
+    z[I] = zI + (r[I] + ns) / (dx + dy + dz);
+}
+
+// mgpcg.py:94-101 _restrict_mcadams as a gather: a FLUID parent sums
+// r - (-ndt(z)) of its 8 children in the serial launch order (kk fastest)
+// starting from 0; any other parent gets 0.  (nx,ny,nz) are fine dims.
+template <bool TAGGED>
+__global__ void k_restrict(const float *__restrict__ r, const float *__restrict__ z,
+                           const uint8_t *__restrict__ tag, float *__restrict__ rc,
+                           const uint8_t *__restrict__ tagc, int nx, int ny, int nz)
+{
+    const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;
+    const uint32_t n = (uint32_t)cx * cy * cz;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    if (TAGGED && tagc[t] != 1) { rc[t] = 0.0f; return; }
+    const int K = t % cz, J = (t / cz) % cy, I = t / ((uint32_t)cz * cy);
+    float s = 0.0f;
+#pragma unroll
+    for (int ii = 0; ii < 2; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+            for (int kk = 0; kk < 2; ++kk) {
+                const int i = 2 * I + ii, j = 2 * J + jj, k = 2 * K + kk;
+                const uint32_t c = cidx(i, j, k, ny, nz);
+                const float zc = z[c];
+                float a;
+                if (TAGGED) a = ndt_stencil(GX{z, ny, nz}, GT{tag, ny, nz}, i, j, k, nx, ny, nz, zc, tag[c], nullptr);
+                else a = nsum_stencil(GX{z, ny, nz}, i, j, k, nx, ny, nz, zc);
+                s += r[c] - (-a);
+            }
+    rc[t] = s;
+}
+
+// mgpcg.py:113-116 _prolongate.
+__global__ void k_prolong(float *__restrict__ z, const float *__restrict__ zc, int nx, int ny, int nz)
+{
+    const uint32_t n = (uint32_t)nx * ny * nz;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int k = t % nz, j = (t / nz) % ny, i = t / ((uint32_t)nz * ny);
+    z[t] = z[t] + zc[cidx(i >> 1, j >> 1, k >> 1, ny >> 1, nz >> 1)];
+}
+
+// Bottom level (mgpcg.py:319-336) in one workgroup: nb x (phase 0, 1) then
+// nb x (phase 1, 0), a workgroup barrier between half-sweeps.
+template <bool TAGGED>
+__global__ void __launch_bounds__(1024) k_bottom(float *__restrict__ z, const float *__restrict__ r,
+                                                 const uint8_t *__restrict__ tag, int nx, int ny, int nz,
+                                                 int nb)
+{
+    const int nzh = nz >> 1;
+    const uint32_t n = (uint32_t)nx * ny * nzh;
+    for (int s = 0; s < 4 * nb; ++s) {
+        const int pair = s >> 1, second = s & 1;
+        const int phase = pair < nb ? second : 1 - second;
+        for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) {
+            const int kk = t % nzh, j = (t / nzh) % ny, i = t / ((uint32_t)nzh * ny);
+            if (TAGGED) {
+                const int k = (((i + j) & 1) == phase) ? kk * 2 + 1 : kk * 2;
+                const uint32_t I = cidx(i, j, k, ny, nz);
+                const int tI = tag[I];
+                const float zI = z[I];
+                int divd;
+                const float nd = ndt_stencil(GX{z, ny, nz}, GT{tag, ny, nz}, i, j, k, nx, ny, nz, zI, tI, &divd);
+                z[I] = divd != 0 ? zI + (r[I] + nd) / (float)divd : 0.0f;
+            } else {
+                const int k = kk * 2 + ((i + j + phase) & 1);
+                const int res = nx;
+                const float dx = (i == 0 || i == res - 1) ? 1.0f : 2.0f;
+                const float dy = (j == 0 || j == res - 1) ? 1.0f : 2.0f;
+                const float dz = (k == 0 || k == res - 1) ? 1.0f : 2.0f;
+                const uint32_t I = cidx(i, j, k, ny, nz);
+                const float zI = z[I];
+                const float ns = nsum_stencil(GX{z, ny, nz}, i, j, k, nx, ny, nz, zI);
+                z[I] = zI + (r[I] + ns) / (dx + dy + dz);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- reductions
+// Every reduction: RBS-thread blocks, grid-stride over the cells with a grid
+// of fixed size, one FP64 partial per block, then k_finish sums the partials
+// in a fixed order.  Results are identical run to run.
+
+// Ap = ndt(p) (compute_Ap_mcadams) or nsum(p) (compute_Ap); partial p.Ap.
+template <bool TAGGED>
+__global__ void __launch_bounds__(RBS) k_Ap_dot(float *__restrict__ Ap, const float *__restrict__ p,
+                                                const uint8_t *__restrict__ tag, int nx, int ny, int nz,
+                                                double *__restrict__ part)
+{
+    __shared__ double lds[RBS / 64];
+    const uint32_t n = (uint32_t)nx * ny * nz;
+    double acc = 0.0;
+    for (uint32_t t = blockIdx.x * RBS + threadIdx.x; t < n; t += gridDim.x * RBS) {
+        const int k = t % nz, j = (t / nz) % ny, i = t / ((uint32_t)nz * ny);
+        const float pI = p[t];
+        float a;
+        if (TAGGED) a = ndt_stencil(GX{p, ny, nz}, GT{tag, ny, nz}, i, j, k, nx, ny, nz, pI, tag[t], nullptr);
+        else a = nsum_stencil(GX{p, ny, nz}, i, j, k, nx, ny, nz, pI);
+        Ap[t] = a;
+        acc += (double)pI * (double)a;
+    }
+    const double s = block_sum<RBS>(acc, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(RBS) k_dot(const float *__restrict__ a, const float *__restrict__ b,
+                                             uint32_t n, double *__restrict__ part)
+{
+    __shared__ double lds[RBS / 64];
+    double acc = 0.0;
+    for (uint32_t t = blockIdx.x * RBS + threadIdx.x; t < n; t += gridDim.x * RBS)
+        acc += (double)a[t] * (double)b[t];
+    const double s = block_sum<RBS>(acc, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// Sum over FLUID cells of q and their count (mgpcg.py:191-196 _mean).
+__global__ void __launch_bounds__(RBS) k_masked_sum(const float *__restrict__ q, const uint8_t *__restrict__ tag,
+                                                    uint32_t n, double *__restrict__ part)
+{
+    __shared__ double lds[RBS / 64];
+    double acc = 0.0, cnt = 0.0;
+    for (uint32_t t = blockIdx.x * RBS + threadIdx.x; t < n; t += gridDim.x * RBS)
+        if (tag[t] == 1) { acc += (double)q[t]; cnt += 1.0; }
+    const double s = block_sum<RBS>(acc, lds);
+    const double c = block_sum<RBS>(cnt, lds);
+    if (threadIdx.x == 0) { part[blockIdx.x] = s; part[gridDim.x + blockIdx.x] = c; }
+}
+
+// mgpcg.py:198-205 _sub: q -= mean on FLUID, 0 elsewhere; mean = sum/count in FP32.
+__global__ void k_sub_mean(float *__restrict__ q, const uint8_t *__restrict__ tag, const double *__restrict__ sc,
+                           uint32_t n)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float mean = (float)sc[SC_MEAN] / (float)sc[SC_COUNT];
+    q[t] = tag[t] == 1 ? q[t] - mean : 0.0f;
+}
+
+// r -= alpha*Ap and x += alpha*p (the two are independent of the
+// preconditioner between them in mgpcg.py:421-457), partial r.r.
+__global__ void __launch_bounds__(RBS) k_update_rx_dot(float *__restrict__ r, float *__restrict__ x,
+                                                       const float *__restrict__ Ap, const float *__restrict__ p,
+                                                       const double *__restrict__ sc, uint32_t n,
+                                                       double *__restrict__ part)
+{
+    __shared__ double lds[RBS / 64];
+    const float alpha = (float)sc[SC_ALPHA];
+    double acc = 0.0;
+    for (uint32_t t = blockIdx.x * RBS + threadIdx.x; t < n; t += gridDim.x * RBS) {
+        const float rn = fmaf(-alpha, Ap[t], r[t]);
+        r[t] = rn;
+        x[t] = fmaf(alpha, p[t], x[t]);
+        acc += (double)rn * (double)rn;
+    }
+    const double s = block_sum<RBS>(acc, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// mgpcg.py:224-227 _update_r, :207-210 _update_x, :229-232 _update_p.
+__global__ void k_update_r(float *__restrict__ r, const float *__restrict__ Ap, const double *__restrict__ sc,
+                           uint32_t n)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) r[t] = fmaf(-(float)sc[SC_ALPHA], Ap[t], r[t]);
+}
+__global__ void k_update_x(float *__restrict__ x, const float *__restrict__ p, const double *__restrict__ sc,
+                           uint32_t n)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) x[t] = fmaf((float)sc[SC_ALPHA], p[t], x[t]);
+}
+__global__ void k_update_p(float *__restrict__ p, const float *__restrict__ z, const double *__restrict__ sc,
+                           uint32_t n)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = fmaf((float)sc[SC_BETA], p[t], z[t]);
+}
+
+// Sum the partials (fixed order) and apply the scalar update `op`.
+__global__ void __launch_bounds__(RBS) k_finish(const double *__restrict__ part, int nblk, double *__restrict__ sc,
+                                                int op, double eps)
+{
+    __shared__ double lds[RBS / 64];
+    double acc = 0.0, acc2 = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += RBS) {
+        acc += part[b];
+        if (op == FIN_MEAN) acc2 += part[nblk + b];
+    }
+    const double s = block_sum<RBS>(acc, lds);
+    const double s2 = op == FIN_MEAN ? block_sum<RBS>(acc2, lds) : 0.0;
+    if (threadIdx.x != 0) return;
+    switch (op) {
+    case FIN_SUM: sc[SC_SUM] = s; break;
+    case FIN_ALPHA: sc[SC_SUM] = s; sc[SC_ALPHA] = sc[SC_OLDZTR] / (s + eps); break;
+    case FIN_BETA: sc[SC_SUM] = s; sc[SC_BETA] = s / (sc[SC_OLDZTR] + eps); sc[SC_OLDZTR] = s; break;
+    case FIN_OLDZTR: sc[SC_OLDZTR] = s; break;
+    case FIN_RTR: sc[SC_RTR] = s; break;
+    case FIN_MEAN: sc[SC_MEAN] = s; sc[SC_COUNT] = s2; break;
+    case FIN_OUT: sc[0] = s; break;
+    }
+}
+
+// mgpcg.py:212-222 _update_alpha / _update_beta on the handle scalars.
+__global__ void k_alpha(double *sc, double eps) { sc[SC_ALPHA] = sc[SC_OLDZTR] / (sc[SC_SUM] + eps); }
+__global__ void k_beta(double *sc, double eps)
+{
+    const double nz = sc[SC_SUM];
+    sc[SC_BETA] = nz / (sc[SC_OLDZTR] + eps);
+    sc[SC_OLDZTR] = nz;
+}
+
+// ================================================================ host side
+static inline unsigned nb256(uint32_t n) { return blocks_for(n, 256); }
+
+static int red_blocks(uint32_t n)
+{
+    int b = (int)blocks_for(n, RBS);
+    return b < MAXBLK ? (b < 1 ? 1 : b) : MAXBLK;
+}
+
+static int launch_finish(flub_mgpcg *h, int nblk, int op, double eps, double *dst = nullptr)
+{
+    k_finish<<<1, RBS, 0, h->s>>>(h->part, nblk, dst ? dst : h->sc, op, eps);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+int mg_dot(flub_mgpcg *h, const float *a, const float *b, int op, double eps)
+{
+    const uint32_t n = h->lv[0].n;
+    const int nb = red_blocks(n);
+    k_dot<<<nb, RBS, 0, h->s>>>(a, b, n, h->part);
+    FLUB_CHECK_LAUNCH();
+    return launch_finish(h, nb, op, eps);
+}
+
+int mg_zero_centered(flub_mgpcg *h)
+{
+    Level &L0 = h->lv[0];
+    const int nb = red_blocks(L0.n);
+    k_masked_sum<<<nb, RBS, 0, h->s>>>(L0.r, L0.tag, L0.n, h->part);
+    FLUB_CHECK_LAUNCH();
+    int rc = launch_finish(h, nb, FIN_MEAN, 0.0);
+    if (rc) return rc;
+    k_sub_mean<<<nb256(L0.n), 256, 0, h->s>>>(L0.r, L0.tag, h->sc, L0.n);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+static int half_sweep(flub_mgpcg *h, int l, int tagged, int phase)
+{
+    Level &L = h->lv[l];
+    const uint32_t nh = L.n / 2;
+    if (tagged) k_smooth_mc<<<nb256(nh), 256, 0, h->s>>>(L.z, L.r, L.tag, L.nx, L.ny, L.nz, phase);
+    else k_smooth_def<<<nb256(nh), 256, 0, h->s>>>(L.z, L.r, L.nx, L.ny, L.nz, phase);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+#define TRY(x)                    \
+    do {                          \
+        int rc_ = (x);            \
+        if (rc_) return rc_;      \
+    } while (0)
+
+// mgpcg.py:296-349, launched kernel by kernel (captured into a graph by the caller).
+static int vcycle_fp32(flub_mgpcg *h, int smoother)
+{
+    const int tagged = smoother != FLUB_SMOOTHER_DEFAULT;
+    Level &L0 = h->lv[0];
+    k_fill_f32<<<nb256(L0.n), 256, 0, h->s>>>(L0.z, 0.0f, L0.n);
+    FLUB_CHECK_LAUNCH();
+    for (int l = 0; l + 1 < h->L; ++l) {
+        for (int it = 0; it < (h->pre_post << l); ++it) {
+            TRY(half_sweep(h, l, tagged, 0));
+            TRY(half_sweep(h, l, tagged, 1));
+        }
+        Level &C = h->lv[l + 1];
+        k_fill_f32<<<nb256(C.n), 256, 0, h->s>>>(C.z, 0.0f, C.n);
+        FLUB_CHECK_LAUNCH();
+        Level &F = h->lv[l];
+        if (tagged) k_restrict<true><<<nb256(C.n), 256, 0, h->s>>>(F.r, F.z, F.tag, C.r, C.tag, F.nx, F.ny, F.nz);
+        else k_restrict<false><<<nb256(C.n), 256, 0, h->s>>>(F.r, F.z, F.tag, C.r, C.tag, F.nx, F.ny, F.nz);
+        FLUB_CHECK_LAUNCH();
+    }
+    const int lb = h->L - 1;
+    Level &B = h->lv[lb];
+    const int nb = h->bottom / 2;
+    if (B.n / 2 <= BOTTOM_ONE_WG_MAX) {
+        if (nb > 0) {
+            if (tagged) k_bottom<true><<<1, 1024, 0, h->s>>>(B.z, B.r, B.tag, B.nx, B.ny, B.nz, nb);
+            else k_bottom<false><<<1, 1024, 0, h->s>>>(B.z, B.r, B.tag, B.nx, B.ny, B.nz, nb);
+            FLUB_CHECK_LAUNCH();
+        }
+    } else {
+        for (int it = 0; it < nb; ++it) { TRY(half_sweep(h, lb, tagged, 0)); TRY(half_sweep(h, lb, tagged, 1)); }
+        for (int it = 0; it < nb; ++it) { TRY(half_sweep(h, lb, tagged, 1)); TRY(half_sweep(h, lb, tagged, 0)); }
+    }
+    for (int l = h->L - 2; l >= 0; --l) {
+        Level &F = h->lv[l];
+        k_prolong<<<nb256(F.n), 256, 0, h->s>>>(F.z, h->lv[l + 1].z, F.nx, F.ny, F.nz);
+        FLUB_CHECK_LAUNCH();
+        for (int it = 0; it < (h->pre_post << l); ++it) {
+            TRY(half_sweep(h, l, tagged, 1));
+            TRY(half_sweep(h, l, tagged, 0));
+        }
+    }
+    return FLUB_OK;
+}
+
+int mg_precondition(flub_mgpcg *h, int smoother, bool use_graph)
+{
+    if (h->prec != FLUB_PREC_FP32) return mg_vcycle_mixed(h, smoother);
+    if (!use_graph || h->no_graph) return vcycle_fp32(h, smoother);
+    if (!h->graph_ok[smoother]) {
+        // Capture on the handle's private stream (the work stream may be the
+        // null stream, which cannot be captured); replay on the work stream.
+        hipGraph_t g;
+        hipStream_t work = h->s;
+        FLUB_CHECK_HIP(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
+        h->s = h->cap;
+        int rc = vcycle_fp32(h, smoother);
+        h->s = work;
+        hipError_t e = hipStreamEndCapture(h->cap, &g);
+        if (rc) { if (e == hipSuccess) hipGraphDestroy(g); return rc; }
+        FLUB_CHECK_HIP(e);
+        FLUB_CHECK_HIP(hipGraphInstantiate(&h->graph[smoother], g, nullptr, nullptr, 0));
+        FLUB_CHECK_HIP(hipGraphDestroy(g));
+        h->graph_ok[smoother] = true;
+    }
+    FLUB_CHECK_HIP(hipGraphLaunch(h->graph[smoother], h->s));
+    return FLUB_OK;
+}
+
+static void drop_graphs(flub_mgpcg *h)
+{
+    for (int s = 0; s < 3; ++s)
+        if (h->graph_ok[s]) { hipGraphExecDestroy(h->graph[s]); h->graph_ok[s] = false; }
+}
+
+FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int levels, int precision,
+                               void *stream)
+{
+    FLUB_REQUIRE(out, FLUB_EINVAL, "flub_mgpcg_create: null output");
+    *out = nullptr;
+    FLUB_REQUIRE(levels >= 1 && levels <= MAXL, FLUB_EINVAL, "n_mg_levels must be in [1, %d]", MAXL);
+    FLUB_REQUIRE(precision >= 0 && precision <= 2, FLUB_EINVAL, "precision must be 0, 1 or 2");
+    FLUB_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1 && (double)(nx + 1) * (ny + 1) * (nz + 1) < 2147483647.0,
+                 FLUB_ESHAPE, "grid %dx%dx%d out of range", nx, ny, nz);
+    for (int l = 0; l < levels; ++l) {
+        const int a = nx >> l, b = ny >> l, c = nz >> l;
+        FLUB_REQUIRE(a >= 1 && b >= 1 && c >= 2 && (c % 2) == 0, FLUB_ESHAPE,
+                     "level %d shape %dx%dx%d: the red-black smoother needs an even z extent", l, a, b, c);
+        if (l + 1 < levels)
+            FLUB_REQUIRE(a % 2 == 0 && b % 2 == 0, FLUB_ESHAPE,
+                         "level %d shape %dx%dx%d must be even to restrict", l, a, b, c);
+    }
+    flub_mgpcg *h = (flub_mgpcg *)calloc(1, sizeof(flub_mgpcg));
+    FLUB_REQUIRE(h, FLUB_EINVAL, "out of host memory");
+    h->L = levels;
+    h->prec = precision;
+    h->pre_post = 6;
+    h->bottom = 6;
+    h->no_graph = getenv("FLUB_NO_GRAPH") != nullptr;
+    int rc = FLUB_OK;
+#define ALLOC(ptr, bytes)                                              \
+    do {                                                               \
+        hipError_t e_ = hipMalloc((void **)&(ptr), (bytes));           \
+        if (e_ != hipSuccess) {                                        \
+            flub_set_error("hipMalloc(%zu): %s", (size_t)(bytes), hipGetErrorString(e_)); \
+            rc = (int)e_;                                              \
+            goto fail;                                                 \
+        }                                                              \
+    } while (0)
+    {
+        // All work is ordered on the caller's stream (NULL = the null stream);
+        // a private stream only records the V-cycle graphs.
+        h->s = as_stream(stream);
+        {
+            hipError_t e = hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking);
+            if (e != hipSuccess) { flub_set_error("hipStreamCreate: %s", hipGetErrorString(e)); rc = (int)e; goto fail; }
+        }
+        for (int l = 0; l < levels; ++l) {
+            Level &L = h->lv[l];
+            L.nx = nx >> l; L.ny = ny >> l; L.nz = nz >> l;
+            L.n = (uint32_t)L.nx * L.ny * L.nz;
+            ALLOC(L.tag, L.n);
+            ALLOC(L.r, (size_t)L.n * 4);
+            ALLOC(L.z, (size_t)L.n * 4);
+            if (precision != FLUB_PREC_FP32) {
+                ALLOC(L.rh, (size_t)L.n * 2);
+                ALLOC(L.zh, (size_t)L.n * 2);
+            }
+        }
+        const size_t n0 = h->lv[0].n;
+        ALLOC(h->x, n0 * 4);
+        ALLOC(h->p, n0 * 4);
+        ALLOC(h->Ap, n0 * 4);
+        ALLOC(h->sc, SC_NUM * sizeof(double));
+        ALLOC(h->part, 2 * MAXBLK * sizeof(double));
+        hipError_t e = hipHostMalloc((void **)&h->hsc, SC_NUM * sizeof(double), hipHostMallocDefault);
+        if (e != hipSuccess) { flub_set_error("hipHostMalloc: %s", hipGetErrorString(e)); rc = (int)e; goto fail; }
+        e = hipMemsetAsync(h->sc, 0, SC_NUM * sizeof(double), h->s);
+        if (e == hipSuccess) e = hipMemsetAsync(h->x, 0, n0 * 4, h->s);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->s);
+        if (e != hipSuccess) { flub_set_error("init: %s", hipGetErrorString(e)); rc = (int)e; goto fail; }
+        rc = mg_mixed_setup(h);
+        if (rc) goto fail;
+    }
+#undef ALLOC
+    *out = h;
+    return FLUB_OK;
+fail:
+    flub_mgpcg_destroy(h);
+    return rc;
+}
+
+FLUB_API int flub_mgpcg_destroy(flub_mgpcg *h)
+{
+    if (!h) return FLUB_OK;
+    hipStreamSynchronize(h->s);
+    drop_graphs(h);
+    mg_mixed_teardown(h);
+    for (int l = 0; l < h->L; ++l) {
+        Level &L = h->lv[l];
+        hipFree(L.tag); hipFree(L.r); hipFree(L.z); hipFree(L.rh); hipFree(L.zh);
+    }
+    hipFree(h->x); hipFree(h->p); hipFree(h->Ap); hipFree(h->sc); hipFree(h->part);
+    if (h->hsc) hipHostFree(h->hsc);
+    if (h->cap) hipStreamDestroy(h->cap);
+    free(h);
+    return FLUB_OK;
+}
+
+FLUB_API int flub_mgpcg_set_smoothing(flub_mgpcg *h, int pre_post, int bottom)
+{
+    FLUB_REQUIRE(h, FLUB_EINVAL, "null handle");
+    FLUB_REQUIRE(pre_post >= 0 && pre_post <= 64 && bottom >= 0 && bottom <= 256, FLUB_EINVAL,
+                 "smoothing counts out of range");
+    if (pre_post != h->pre_post || bottom != h->bottom) {
+        FLUB_CHECK_HIP(hipStreamSynchronize(h->s));
+        drop_graphs(h);
+    }
+    h->pre_post = pre_post;
+    h->bottom = bottom;
+    return FLUB_OK;
+}
+
+FLUB_API int flub_mgpcg_set_tags(flub_mgpcg *h, const int32_t *tag)
+{
+    FLUB_REQUIRE(h, FLUB_EINVAL, "null handle");
+    Level &L0 = h->lv[0];
+    if (tag) {
+        k_pack_tags<<<nb256(L0.n), 256, 0, h->s>>>(tag, L0.tag, L0.n);
+        FLUB_CHECK_LAUNCH();
+    } else {
+        FLUB_REQUIRE(h->tags_set, FLUB_ESTATE, "set_tags(None) before any tags were given");
+    }
+    for (int l = 0; l + 1 < h->L; ++l) {
+        Level &C = h->lv[l + 1];
+        k_restrict_tag<<<nb256(C.n), 256, 0, h->s>>>(h->lv[l].tag, C.tag, C.nx, C.ny, C.nz);
+        FLUB_CHECK_LAUNCH();
+    }
+    h->tags_set = true;
+    return FLUB_OK;
+}
+
+FLUB_API int flub_mgpcg_init(flub_mgpcg *h, const float *r, float k)
+{
+    FLUB_REQUIRE(h && r, FLUB_EINVAL, "flub_mgpcg_init: null argument");
+    Level &L0 = h->lv[0];
+    const uint32_t n = L0.n;
+    k_scale_copy<<<nb256(n), 256, 0, h->s>>>(L0.r, r, k, n);
+    FLUB_CHECK_LAUNCH();
+    k_fill_f32<<<nb256(n), 256, 0, h->s>>>(L0.z, 0.0f, n);
+    k_fill_f32<<<nb256(n), 256, 0, h->s>>>(h->Ap, 0.0f, n);
+    k_fill_f32<<<nb256(n), 256, 0, h->s>>>(h->p, 0.0f, n);
+    k_fill_f32<<<nb256(n), 256, 0, h->s>>>(h->x, 0.0f, n);
+    FLUB_CHECK_LAUNCH();
+    FLUB_CHECK_HIP(hipMemsetAsync(h->sc, 0, SC_NUM * sizeof(double), h->s));
+    return FLUB_OK;
+}
+
+FLUB_API int flub_mgpcg_mask_interior(flub_mgpcg *h)
+{
+    FLUB_REQUIRE(h, FLUB_EINVAL, "null handle");
+    FLUB_REQUIRE(h->tags_set, FLUB_ESTATE, "mask_interior before set_tags");
+    Level &L0 = h->lv[0];
+    k_mask_interior<<<nb256(L0.n), 256, 0, h->s>>>(L0.tag, L0.r, L0.n);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+FLUB_API int flub_mgpcg_zero_centered(flub_mgpcg *h)
+{
+    FLUB_REQUIRE(h, FLUB_EINVAL, "null handle");
+    FLUB_REQUIRE(h->tags_set, FLUB_ESTATE, "zero_centered before set_tags");
+    return mg_zero_centered(h);
+}
+
+static bool smoother_ok(int s) { return s >= 0 && s <= 2; }
+
+FLUB_API int flub_mgpcg_apply_preconditioner(flub_mgpcg *h, int smoother)
+{
+    FLUB_REQUIRE(h && smoother_ok(smoother), FLUB_EINVAL, "bad handle or smoother");
+    FLUB_REQUIRE(h->tags_set, FLUB_ESTATE, "apply_preconditioner before set_tags");
+    return mg_precondition(h, smoother, false);
+}
+
+FLUB_API int flub_mgpcg_graph_precondition(flub_mgpcg *h, int smoother)
+{
+    FLUB_REQUIRE(h && smoother_ok(smoother), FLUB_EINVAL, "bad handle or smoother");
+    FLUB_REQUIRE(h->tags_set, FLUB_ESTATE, "graph_precondition before set_tags");
+    return mg_precondition(h, smoother, true);
+}
+
+static int compute_Ap_dot(flub_mgpcg *h, int smoother, int op, double eps)
+{
+    Level &L0 = h->lv[0];
+    const int nb = red_blocks(L0.n);
+    if (smoother == FLUB_SMOOTHER_DEFAULT)
+        k_Ap_dot<false><<<nb, RBS, 0, h->s>>>(h->Ap, h->p, L0.tag, L0.nx, L0.ny, L0.nz, h->part);
+    else
+        k_Ap_dot<true><<<nb, RBS, 0, h->s>>>(h->Ap, h->p, L0.tag, L0.nx, L0.ny, L0.nz, h->part);
+    FLUB_CHECK_LAUNCH();
+    return launch_finish(h, nb, op, eps);
+}
+
+FLUB_API int flub_mgpcg_compute_Ap(flub_mgpcg *h, int smoother)
+{
+    FLUB_REQUIRE(h && smoother_ok(smoother), FLUB_EINVAL, "bad handle or smoother");
+    FLUB_REQUIRE(h->tags_set || smoother == FLUB_SMOOTHER_DEFAULT, FLUB_ESTATE, "compute_Ap before set_tags");
+    return compute_Ap_dot(h, smoother, FIN_SUM, 0.0);
+}
+
+FLUB_API int flub_mgpcg_update_p(flub_mgpcg *h)
+{
+    FLUB_REQUIRE(h, FLUB_EINVAL, "null handle");
+    k_update_p<<<nb256(h->lv[0].n), 256, 0, h->s>>>(h->p, h->lv[0].z, h->sc, h->lv[0].n);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+FLUB_API int flub_mgpcg_update_r(flub_mgpcg *h)
+{
+    FLUB_REQUIRE(h, FLUB_EINVAL, "null handle");
+    k_update_r<<<nb256(h->lv[0].n), 256, 0, h->s>>>(h->lv[0].r, h->Ap, h->sc, h->lv[0].n);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+FLUB_API int flub_mgpcg_update_x(flub_mgpcg *h)
+{
+    FLUB_REQUIRE(h, FLUB_EINVAL, "null handle");
+    k_update_x<<<nb256(h->lv[0].n), 256, 0, h->s>>>(h->x, h->p, h->sc, h->lv[0].n);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+FLUB_API int flub_mgpcg_update_alpha(flub_mgpcg *h, double eps)
+{
+    FLUB_REQUIRE(h, FLUB_EINVAL, "null handle");
+    k_alpha<<<1, 1, 0, h->s>>>(h->sc, eps);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+FLUB_API int flub_mgpcg_update_beta(flub_mgpcg *h, double eps)
+{
+    FLUB_REQUIRE(h, FLUB_EINVAL, "null handle");
+    k_beta<<<1, 1, 0, h->s>>>(h->sc, eps);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+static float *buf_ptr(flub_mgpcg *h, int b)
+{
+    switch (b) {
+    case 0: return h->x;
+    case 1: return h->p;
+    case 2: return h->Ap;
+    case 3: return h->lv[0].r;
+    case 4: return h->lv[0].z;
+    default: return nullptr;
+    }
+}
+
+FLUB_API int flub_mgpcg_inner(flub_mgpcg *h, int a, int b, int scalar_out)
+{
+    FLUB_REQUIRE(h, FLUB_EINVAL, "null handle");
+    float *pa = buf_ptr(h, a), *pb = buf_ptr(h, b);
+    FLUB_REQUIRE(pa && pb && scalar_out >= 0 && scalar_out <= 3, FLUB_EINVAL, "bad buffer/scalar id");
+    static const int ops[4] = {-1, -1, FIN_SUM, FIN_OLDZTR};
+    const uint32_t n = h->lv[0].n;
+    const int nb = red_blocks(n);
+    k_dot<<<nb, RBS, 0, h->s>>>(pa, pb, n, h->part);
+    FLUB_CHECK_LAUNCH();
+    if (ops[scalar_out] >= 0) return launch_finish(h, nb, ops[scalar_out], 0.0);
+    return launch_finish(h, nb, FIN_OUT, 0.0, h->sc + (scalar_out == 0 ? SC_ALPHA : SC_BETA));
+}
+
+static int read_scalar(flub_mgpcg *h, int idx, double *out)
+{
+    FLUB_CHECK_HIP(hipMemcpyAsync(h->hsc + idx, h->sc + idx, sizeof(double), hipMemcpyDeviceToHost, h->s));
+    FLUB_CHECK_HIP(hipStreamSynchronize(h->s));
+    *out = h->hsc[idx];
+    return FLUB_OK;
+}
+
+// mgpcg.py:378-462.  Same control flow; x's update is fused into r's (both
+// use this iteration's alpha and p, which the preconditioner does not touch).
+static int solve_fp32(flub_mgpcg *h, int max_iters, double eps, double abs_tol, double rel_tol, int smoother,
+                      int verbose, int *iters, double *rTr_out)
+{
+    Level &L0 = h->lv[0];
+    const uint32_t n = L0.n;
+    const int nb = red_blocks(n);
+    const bool centred = smoother != FLUB_SMOOTHER_PHASES;
+    if (centred) TRY(mg_zero_centered(h));
+    TRY(mg_dot(h, L0.r, L0.r, FIN_RTR, 0.0));
+    double rTr0;
+    TRY(read_scalar(h, SC_RTR, &rTr0));
+    const double tol = fmax(abs_tol, rTr0 * rel_tol);
+    TRY(mg_precondition(h, smoother, true));
+    k_update_p<<<nb256(n), 256, 0, h->s>>>(h->p, L0.z, h->sc, n);
+    FLUB_CHECK_LAUNCH();
+    TRY(mg_dot(h, L0.z, L0.r, FIN_OLDZTR, 0.0));
+    const int cap = max_iters < 0 ? SOLVE_SAFETY_CAP : max_iters;
+    int it = 0;
+    double rTr = rTr0;
+    int status = FLUB_OK;
+    while (it < cap) {
+        TRY(compute_Ap_dot(h, smoother, FIN_ALPHA, eps));
+        if (centred) {
+            k_update_r<<<nb256(n), 256, 0, h->s>>>(L0.r, h->Ap, h->sc, n);
+            k_update_x<<<nb256(n), 256, 0, h->s>>>(h->x, h->p, h->sc, n);
+            FLUB_CHECK_LAUNCH();
+            TRY(mg_zero_centered(h));
+            TRY(mg_dot(h, L0.r, L0.r, FIN_RTR, 0.0));
+        } else {
+            k_update_rx_dot<<<nb, RBS, 0, h->s>>>(L0.r, h->x, h->Ap, h->p, h->sc, n, h->part);
+            FLUB_CHECK_LAUNCH();
+            TRY(launch_finish(h, nb, FIN_RTR, 0.0));
+        }
+        TRY(read_scalar(h, SC_RTR, &rTr));
+        if (verbose) printf("after iter %d, |residual|_2=%.9g\n", it + 1, sqrt(rTr));
+        if (!isfinite(rTr)) { status = FLUB_ENONFINITE; flub_set_error("non-finite residual at iteration %d", it + 1); break; }
+        if (rTr < tol) break;
+        TRY(mg_precondition(h, smoother, true));
+        TRY(mg_dot(h, L0.z, L0.r, FIN_BETA, eps));
+        k_update_p<<<nb256(n), 256, 0, h->s>>>(h->p, L0.z, h->sc, n);
+        FLUB_CHECK_LAUNCH();
+        it += 1;
+    }
+    if (status == FLUB_OK && it >= cap && max_iters < 0) {
+        status = FLUB_ENOTCONV;
+        flub_set_error("no convergence within %d iterations", cap);
+    }
+    if (verbose) printf("finished with  %d  iterations\n", it);
+    if (iters) *iters = it;
+    if (rTr_out) *rTr_out = rTr;
+    return status;
+}
+
+FLUB_API int flub_mgpcg_solve(flub_mgpcg *h, int max_iters, double eps, double abs_tol, double rel_tol,
+                              int smoother, int verbose, int *iters, double *rTr)
+{
+    FLUB_REQUIRE(h && smoother_ok(smoother), FLUB_EINVAL, "bad handle or smoother");
+    FLUB_REQUIRE(h->tags_set, FLUB_ESTATE, "solve before set_tags");
+    if (h->prec != FLUB_PREC_FP32)
+        return mg_solve_mixed(h, max_iters, eps, abs_tol, rel_tol, smoother, verbose, iters, rTr);
+    return solve_fp32(h, max_iters, eps, abs_tol, rel_tol, smoother, verbose, iters, rTr);
+}
+
+FLUB_API int flub_mgpcg_buffer(flub_mgpcg *h, int which, int level, void **ptr)
+{
+    FLUB_REQUIRE(h && ptr, FLUB_EINVAL, "null argument");
+    FLUB_REQUIRE(level >= 0 && level < h->L, FLUB_EINVAL, "level %d out of range", level);
+    switch (which) {
+    case 0: *ptr = h->x; break;
+    case 1: *ptr = h->p; break;
+    case 2: *ptr = h->Ap; break;
+    case 3: *ptr = h->lv[level].r; break;
+    case 4: *ptr = h->lv[level].z; break;
+    case 5: *ptr = h->lv[level].tag; break;
+    case 6: *ptr = h->sc; break;
+    default: FLUB_REQUIRE(false, FLUB_EINVAL, "unknown buffer id %d", which);
+    }
+    return FLUB_OK;
+}
+
+FLUB_API int flub_mgpcg_compute_divd(flub_mgpcg *h, int level, float *divd)
+{
+    FLUB_REQUIRE(h && divd, FLUB_EINVAL, "null argument");
+    FLUB_REQUIRE(level >= 0 && level < h->L, FLUB_EINVAL, "level out of range");
+    Level &L = h->lv[level];
+    k_divd<<<nb256(L.n), 256, 0, h->s>>>(L.tag, divd, L.nx, L.ny, L.nz);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+FLUB_API int flub_mgpcg_scalars(flub_mgpcg *h, double out[5])
+{
+    FLUB_REQUIRE(h && out, FLUB_EINVAL, "null argument");
+    FLUB_CHECK_HIP(hipMemcpyAsync(h->hsc, h->sc, SC_NUM * sizeof(double), hipMemcpyDeviceToHost, h->s));
+    FLUB_CHECK_HIP(hipStreamSynchronize(h->s));
+    out[0] = h->hsc[SC_ALPHA];
+    out[1] = h->hsc[SC_BETA];
+    out[2] = h->hsc[SC_SUM];
+    out[3] = h->hsc[SC_OLDZTR];
+    out[4] = h->hsc[SC_RTR];
+    return FLUB_OK;
+}
+
+FLUB_API int flub_mgpcg_level_shape(flub_mgpcg *h, int level, int shape[3])
+{
+    FLUB_REQUIRE(h && shape, FLUB_EINVAL, "null argument");
+    FLUB_REQUIRE(level >= 0 && level < h->L, FLUB_EINVAL, "level out of range");
+    shape[0] = h->lv[level].nx;
+    shape[1] = h->lv[level].ny;
+    shape[2] = h->lv[level].nz;
+    return FLUB_OK;
+}
+
+FLUB_API int flub_inner(const float *a, const float *b, size_t n, double *out_dev, void *stream)
+{
+    FLUB_REQUIRE(a && b && out_dev, FLUB_EINVAL, "flub_inner: null pointer");
+    FLUB_REQUIRE(n < 2147483648ull, FLUB_ESHAPE, "flub_inner: n too large");
+    static thread_local double *part = nullptr;
+    if (!part) FLUB_CHECK_HIP(hipMalloc((void **)&part, MAXBLK * sizeof(double)));
+    hipStream_t s = as_stream(stream);
+    const int nb = red_blocks((uint32_t)n);
+    if (n) {
+        k_dot<<<nb, RBS, 0, s>>>(a, b, (uint32_t)n, part);
+        FLUB_CHECK_LAUNCH();
+    }
+    k_finish<<<1, RBS, 0, s>>>(part, n ? nb : 0, out_dev, FIN_OUT, 0.0);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+// example_fluid.py:120-147 + water_3d.py:58-68.
+FLUB_API int flub_project(flub_mgpcg *h, float *u, float *v, float *w, const float *ur, const float *vr,
+                          const float *wr, const int32_t *tag, float h_cell, int max_iters, double abs_tol,
+                          double rel_tol, int smoother, float *p, float *div, int *iters, double *rTr)
+{
+    FLUB_REQUIRE(h && u && v && w && ur && vr && wr && tag && p, FLUB_EINVAL, "flub_project: null argument");
+    FLUB_REQUIRE(smoother_ok(smoother), FLUB_EINVAL, "bad smoother");
+    FLUB_REQUIRE(h_cell > 0.0f, FLUB_EINVAL, "cell size must be positive");
+    Level &L0 = h->lv[0];
+    const int nx = L0.nx, ny = L0.ny, nz = L0.nz;
+    TRY(flub_mgpcg_set_tags(h, tag));
+    float *d = div ? div : h->Ap;   // Ap is rewritten by init
+    const float s_div = (float)(-1.0 / (double)h_cell);
+    TRY(flub_divergence_fused_u8(u, v, w, d, ur, vr, wr, L0.tag, nx, ny, nz, s_div, h->s));
+    TRY(flub_mgpcg_init(h, d, -h_cell));
+    TRY(flub_mgpcg_mask_interior(h));
+    int st = flub_mgpcg_solve(h, max_iters, 1e-12, abs_tol, rel_tol, smoother, 0, iters, rTr);
+    if (st != FLUB_OK && st != FLUB_ENOTCONV) return st;
+    TRY(flub_subtract_gradient_x_u8(u, ur, h->x, L0.tag, nx, ny, nz, 1, h_cell, h->s));
+    TRY(flub_subtract_gradient_x_u8(v, vr, h->x, L0.tag, nx, ny, nz, 2, h_cell, h->s));
+    TRY(flub_subtract_gradient_x_u8(w, wr, h->x, L0.tag, nx, ny, nz, 3, h_cell, h->s));
+    const float inv_h = (float)(1.0 / (double)h_cell);
+    k_store_pressure<<<nb256(L0.n), 256, 0, h->s>>>(p, h->x, h_cell, inv_h, L0.n);
+    FLUB_CHECK_LAUNCH();
+    return st;
+}
+
+// ---------------------------------------------------------------- measurement
+FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_ms, double *bytes)
+{
+    FLUB_REQUIRE(h && avg_ms && reps > 0, FLUB_EINVAL, "flub_mgpcg_profile: bad argument");
+    FLUB_REQUIRE(h->tags_set, FLUB_ESTATE, "profile before set_tags");
+    Level &L0 = h->lv[0];
+    const double N = (double)L0.n, v = h->prec == FLUB_PREC_FP32 ? 4.0 : 2.0;
+    double b = 0.0;
+    hipEvent_t e0, e1;
+    FLUB_CHECK_HIP(hipEventCreate(&e0));
+    FLUB_CHECK_HIP(hipEventCreate(&e1));
+    const int nb = red_blocks(L0.n);
+    auto launch_once = [&]() -> int {
+        switch (kernel) {
+        case 0: return half_sweep(h, 0, 1, 0);
+        case 1: return compute_Ap_dot(h, FLUB_SMOOTHER_PHASES, FIN_SUM, 0.0);
+        case 2:
+            k_update_rx_dot<<<nb, RBS, 0, h->s>>>(L0.r, h->x, h->Ap, h->p, h->sc, L0.n, h->part);
+            FLUB_CHECK_LAUNCH();
+            return launch_finish(h, nb, FIN_RTR, 0.0);
+        case 3: return mg_precondition(h, FLUB_SMOOTHER_PHASES, true);
+        case 4: {
+            Level &C = h->lv[1];
+            k_restrict<true><<<nb256(C.n), 256, 0, h->s>>>(L0.r, L0.z, L0.tag, C.r, C.tag, L0.nx, L0.ny, L0.nz);
+            FLUB_CHECK_LAUNCH();
+            return FLUB_OK;
+        }
+        case 5:
+            k_prolong<<<nb256(L0.n), 256, 0, h->s>>>(L0.z, h->lv[1].z, L0.nx, L0.ny, L0.nz);
+            FLUB_CHECK_LAUNCH();
+            return FLUB_OK;
+        case 6:
+            k_update_p<<<nb256(L0.n), 256, 0, h->s>>>(h->p, L0.z, h->sc, L0.n);
+            FLUB_CHECK_LAUNCH();
+            return FLUB_OK;
+        default: flub_set_error("unknown kernel id %d", kernel); return FLUB_EINVAL;
+        }
+    };
+    switch (kernel) {
+    case 0: b = N * (2 * v + 1); break;                      // z (7-pt), r (own colour), tags
+    case 1: b = N * (2 * 4 + 1); break;                      // p, tags -> Ap
+    case 2: b = N * 6 * 4; break;                            // r, x, Ap, p -> r, x
+    case 3: b = N * 0; break;                                // filled by the caller's model
+    case 4: b = N * (2 * v + 1) + N / 8 * (v + 1); break;
+    case 5: b = N * 2 * v + N / 8 * v; break;
+    case 6: b = N * 3 * 4; break;
+    default: break;
+    }
+    int rc = launch_once();   // warm-up (captures the graph for id 3)
+    if (rc == FLUB_OK) {
+        FLUB_CHECK_HIP(hipStreamSynchronize(h->s));
+        FLUB_CHECK_HIP(hipEventRecord(e0, h->s));
+        for (int i = 0; i < reps && rc == FLUB_OK; ++i) rc = launch_once();
+        FLUB_CHECK_HIP(hipEventRecord(e1, h->s));
+        FLUB_CHECK_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        FLUB_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *avg_ms = ms / reps;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (bytes) *bytes = b;
+    return rc;
+}

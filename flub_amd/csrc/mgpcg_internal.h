@@ -1,0 +1,54 @@
+// Internal layout of the MGPCG handle, shared by the FP32 solver (mgpcg.hip)
+// and the mixed-precision flexible-CG solver (mixed.hip).
+#pragma once
+#include "common.h"
+
+#define MAXL 12
+#define RBS 256                 // reduction block size
+#define MAXBLK 2048             // max partials per reduction
+#define BOTTOM_ONE_WG_MAX 65536 // bottom levels up to this many cells per colour run in one workgroup
+#define SOLVE_SAFETY_CAP 20000  // iteration cap applied when max_iters == -1
+
+// Device scalar slots (FP64).
+enum { SC_ALPHA = 0, SC_BETA, SC_SUM, SC_OLDZTR, SC_RTR, SC_MEAN, SC_COUNT, SC_RSCALE, SC_AUX0, SC_AUX1,
+       SC_AUX2, SC_AUX3, SC_NUM = 16 };
+// k_finish operations.
+enum { FIN_SUM = 0, FIN_ALPHA, FIN_BETA, FIN_OLDZTR, FIN_RTR, FIN_MEAN, FIN_OUT };
+
+struct Level {
+    int nx, ny, nz;
+    uint32_t n;
+    uint8_t *tag;   // packed tags
+    float *r, *z;   // FP32 storage
+    void *rh, *zh;  // FP16/BF16 storage (mixed precision only)
+};
+
+struct MixedState;
+
+struct flub_mgpcg {
+    int L, prec;
+    Level lv[MAXL];
+    float *x, *p, *Ap;
+    double *sc;     // device scalars
+    double *part;   // reduction partials (2*MAXBLK)
+    double *hsc;    // pinned host mirror of sc
+    int pre_post, bottom;
+    hipStream_t s;     // work stream (caller's; may be the null stream)
+    hipStream_t cap;   // private stream used only to capture graphs
+    bool tags_set;
+    bool no_graph;
+    hipGraphExec_t graph[3];
+    bool graph_ok[3];
+    MixedState *mx;
+};
+
+int mg_dot(flub_mgpcg *h, const float *a, const float *b, int op, double eps);
+int mg_zero_centered(flub_mgpcg *h);
+int mg_precondition(flub_mgpcg *h, int smoother, bool use_graph);
+
+// mixed.hip
+int mg_mixed_setup(flub_mgpcg *h);
+void mg_mixed_teardown(flub_mgpcg *h);
+int mg_vcycle_mixed(flub_mgpcg *h, int smoother);
+int mg_solve_mixed(flub_mgpcg *h, int max_iters, double eps, double abs_tol, double rel_tol, int smoother,
+                   int verbose, int *iters, double *rTr);

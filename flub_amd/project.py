@@ -1,0 +1,114 @@
+"""The projection harness: ``Example.divergence``/``call_mgpcg``/``project`` of
+the reference (example_fluid.py:43-49,84-94,120-147; water_3d.py:58-68),
+restricted to the hot path: from (face velocities, cell tags, rigid face
+velocities) to (projected velocities, pressure).
+
+``Projection.project()`` runs it through one C-ABI call (flub_project), which
+fuses divergence/mask/scale and the pressure scale chain into the grid
+kernels; ``Projection.project_stepwise()`` issues the reference's launch
+sequence one kernel at a time through the drop-in modules.  Both produce the
+same fields.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from . import device as wp
+from .const_param import Field, IntField, MACGrid, field_shape
+from .divergence import divergence_tag, mask_with_tag
+from .launch import launch
+from .multigrid.mgpcg import MGPCG, SMOOTHERS
+from .subtract_gradient import subtract_gradient_tag
+from .utils.util import scale_by_3d
+
+
+def _field(arr, d):
+    return Field(f=wp.from_numpy(np.asarray(arr, np.float32)), d=d)
+
+
+class Projection:
+    """Device state of one projection problem on an (nx, ny, nz) grid."""
+
+    def __init__(self, shape, n_mg_levels=4, pre_and_post_smoothing=2, bottom_smoothing=6,
+                 precision="fp32", h=None):
+        self.shape = tuple(int(s) for s in shape)
+        self.h = float(h) if h is not None else 1.0 / self.shape[0]
+        self.mgpcg = MGPCG(3, self.shape, n_mg_levels, float, precision=precision)
+        self.mgpcg.pre_and_post_smoothing = pre_and_post_smoothing
+        self.mgpcg.bottom_smoothing = bottom_smoothing
+        g = self.shape
+        self.vel0 = MACGrid(*(Field(f=wp.zeros(field_shape(d, g)), d=d) for d in (1, 2, 3)))
+        self.vel_rigid = MACGrid(*(Field(f=wp.zeros(field_shape(d, g)), d=d) for d in (1, 2, 3)))
+        self.tag = IntField(f=wp.zeros(g, int))
+        self.div = Field(f=wp.zeros(g))
+        self.p0 = Field(f=wp.zeros(g))
+        self.iterations = None
+        self.rTr = None
+
+    # ---------------------------------------------------------------- inputs
+    def load(self, scene):
+        """Copy a scene dict (flub_amd.scenes) to the device."""
+        self.set_velocity(scene["u"], scene["v"], scene["w"])
+        self.set_rigid_velocity(scene["ur"], scene["vr"], scene["wr"])
+        self.tag.f.assign(np.asarray(scene["tag"], np.int32))
+
+    def set_velocity(self, u, v, w):
+        self.vel0.u.f.assign(u)
+        self.vel0.v.f.assign(v)
+        self.vel0.w.f.assign(w)
+
+    def set_rigid_velocity(self, u, v, w):
+        self.vel_rigid.u.f.assign(u)
+        self.vel_rigid.v.f.assign(v)
+        self.vel_rigid.w.f.assign(w)
+
+    def velocity(self):
+        return self.vel0.u.f.numpy(), self.vel0.v.f.numpy(), self.vel0.w.f.numpy()
+
+    # ---------------------------------------------------------------- fused
+    def project(self, max_iters=-1, rel_tol=1e-12, abs_tol=0.0, smoother="phases"):
+        it = ctypes.c_int(0)
+        rtr = ctypes.c_double(0.0)
+        v, r = self.vel0, self.vel_rigid
+        _lib.call("flub_project", self.mgpcg.handle, v.u.f.ptr, v.v.f.ptr, v.w.f.ptr, r.u.f.ptr,
+                  r.v.f.ptr, r.w.f.ptr, self.tag.f.ptr, float(self.h), int(max_iters), float(abs_tol),
+                  float(rel_tol), SMOOTHERS[smoother], self.p0.f.ptr, self.div.f.ptr, ctypes.byref(it),
+                  ctypes.byref(rtr), allow=(_lib.ENOTCONV,))
+        self.iterations, self.rTr = it.value, rtr.value
+        return self.iterations
+
+    # ---------------------------------------------------------------- stepwise
+    def divergence(self):
+        """example_fluid.py:84-93 (without the rigid precompute and the print)."""
+        shape = self.shape
+        launch(divergence_tag, dim=shape, inputs=[self.vel0, self.div, self.vel_rigid, self.tag])
+        launch(mask_with_tag, dim=shape, inputs=[self.div, self.tag])
+        launch(scale_by_3d, dim=shape, inputs=[self.div, -1.0 / self.h])
+
+    def call_mgpcg(self, pf, max_iters=-1, rel_tol=1e-12, abs_tol=0.0, smoother="phases", verbose=False):
+        """water_3d.py:58-68."""
+        mg = self.mgpcg
+        mg.set_tags(self.tag.f)
+        mg.init(self.div.f, -self.h)
+        mg.mask_interior()
+        self.iterations = mg.solve(max_iters, abs_tol=abs_tol, rel_tol=rel_tol, verbose=verbose,
+                                   smoother=smoother)
+        self.rTr = mg.last_rTr
+        wp.copy(pf.f, mg.x)
+        launch(scale_by_3d, dim=self.shape, inputs=[pf, self.h])
+
+    def project_stepwise(self, max_iters=-1, rel_tol=1e-12, abs_tol=0.0, smoother="phases", verbose=False):
+        """example_fluid.py:120-145 launch by launch."""
+        self.divergence()
+        self.p0.f.zero_()
+        self.call_mgpcg(self.p0, max_iters, rel_tol, abs_tol, smoother, verbose)
+        launch(scale_by_3d, dim=self.shape, inputs=[self.p0, 1.0 / self.h])
+        g = self.shape
+        launch(subtract_gradient_tag, dim=field_shape(1, g), inputs=[self.vel0, self.vel_rigid, self.p0, self.tag, 1])
+        launch(subtract_gradient_tag, dim=field_shape(2, g), inputs=[self.vel0, self.vel_rigid, self.p0, self.tag, 2])
+        launch(subtract_gradient_tag, dim=field_shape(3, g), inputs=[self.vel0, self.vel_rigid, self.p0, self.tag, 3])
+        launch(scale_by_3d, dim=self.shape, inputs=[self.p0, self.h])
+        return self.iterations

@@ -1,0 +1,20 @@
+"""Grid utilities on the projection path (utils/util.py:20-28 ``sample_3d``,
+:162-165 ``scale_by_3d`` of the reference)."""
+from __future__ import annotations
+
+from .. import _lib
+from ..launch import Kernel
+
+
+def _scale_by_3d(q, scale):
+    _lib.call("flub_scale", q.f.ptr, float(scale), q.f.size, None)
+
+
+scale_by_3d = Kernel("scale_by_3d", _scale_by_3d, lambda q, s: q.f.shape)
+
+
+def sample_3d(qf, u: int, v: int, w: int):
+    """Clamped fetch qf[clamp(u), clamp(v), clamp(w)] on a host (NumPy) array; the
+    HIP kernels inline the same clamp in their stencils."""
+    gx, gy, gz = qf.shape
+    return qf[min(max(u, 0), gx - 1), min(max(v, 0), gy - 1), min(max(w, 0), gz - 1)]

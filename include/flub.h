@@ -1,0 +1,167 @@
+/*
+ * flub.h -- C-ABI of the MI355X pressure-projection engine (libflub.so).
+ *
+ * The boundary replaces the Warp kernels and runtime services that the
+ * reference's projection path launches from Python.  Every entry point takes
+ * plain device pointers and sizes (no framework types), is stream-ordered on
+ * the `stream` argument (NULL = the default stream), and returns an int:
+ *   0 ok; < 0 argument/state error (FLUB_E* in csrc/common.h); > 0 hipError_t.
+ * flub_last_error() returns the text of the last failure of the calling thread.
+ *
+ * Layout (identical to the reference's wp.array3d fields, const_param.py:114-143):
+ *   cell fields  float32 [nx][ny][nz], k fastest
+ *   u faces      float32 [nx+1][ny][nz]; v [nx][ny+1][nz]; w [nx][ny][nz+1]
+ *   tags         int32   [nx][ny][nz], 0 AIR, 1 FLUID, 2 SOLID at the boundary
+ *                (packed to one byte per cell inside the solver handle).
+ * Grid sizes are limited to nx*ny*nz < 2^31 and faces < 2^31.
+ */
+#ifndef FLUB_H
+#define FLUB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct flub_mgpcg flub_mgpcg;
+
+/* Smoother ids of MGPCG.solve / apply_preconditioner (multigrid/mgpcg.py:286-349):
+ * "default" = tag-free _smooth/_restrict/compute_Ap with mean removal,
+ * "phases"  = tagged _smooth_mcadams/_restrict_mcadams/compute_Ap_mcadams,
+ * "fraction"= "phases" plus mean removal (zero_centered) as in solve(). */
+enum { FLUB_SMOOTHER_DEFAULT = 0, FLUB_SMOOTHER_PHASES = 1, FLUB_SMOOTHER_FRACTION = 2 };
+
+/* Storage precision of the V-cycle (the outer CG is always FP32 with FP64 dots). */
+enum { FLUB_PREC_FP32 = 0, FLUB_PREC_FP16 = 1, FLUB_PREC_BF16 = 2 };
+
+/* ---------------------------------------------------------------------------
+ * Runtime services.  Replace Warp's array/runtime API the path relies on:
+ * wp.empty/zeros/copy/from_numpy/.numpy()/.zero_()/.fill_() and
+ * wp.ScopedTimer(synchronize=True) (const_param.py:114-150, water_3d.py:62).
+ * ------------------------------------------------------------------------- */
+const char *flub_last_error(void);
+int flub_device_count(int *count);
+int flub_malloc(void **ptr, size_t bytes);                       /* wp.empty */
+int flub_free(void *ptr);
+int flub_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);   /* wp.from_numpy */
+int flub_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);   /* .numpy() */
+int flub_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);   /* wp.copy */
+int flub_memset(void *ptr, int byte, size_t bytes, void *stream);              /* .zero_() */
+int flub_fill_f32(float *ptr, float value, size_t n, void *stream);           /* .fill_() */
+int flub_fill_i32(int32_t *ptr, int32_t value, size_t n, void *stream);
+int flub_stream_create(void **stream);
+int flub_stream_destroy(void *stream);
+int flub_stream_synchronize(void *stream);
+int flub_device_synchronize(void);
+int flub_event_create(void **event);
+int flub_event_destroy(void *event);
+int flub_event_record(void *event, void *stream);
+int flub_event_elapsed_ms(void *start, void *stop, float *ms);
+
+/* ---------------------------------------------------------------------------
+ * Cell classification (geometry.py:335-344 _tag_dirichlet, :390-397 _tag_neumann).
+ * ------------------------------------------------------------------------- */
+/* tag = phi > 0 ? AIR : FLUID over n cells. */
+int flub_tag_dirichlet(const float *phi, int32_t *tag, size_t n, void *stream);
+/* tag = SOLID where phi_rigid < 0, else unchanged. */
+int flub_tag_neumann(const float *phi_rigid, int32_t *tag, size_t n, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Grid kernels of the projection (launched by example_fluid.py:84-94,120-147).
+ * ------------------------------------------------------------------------- */
+/* divergence.py:44-99 divergence_tag: div = (vr-vl)+(vt-vb)+(vn-vf), a face flux
+ * replaced by the rigid velocity when the neighbour across it is SOLID and 0 on
+ * the domain walls.  Walls are taken per axis (the reference's res=shape[0] is
+ * the same on the cubic grids it supports). */
+int flub_divergence_tag(const float *u, const float *v, const float *w, float *div,
+                        const float *u_rigid, const float *v_rigid, const float *w_rigid,
+                        const int32_t *tag, int nx, int ny, int nz, void *stream);
+/* divergence.py:174-180 mask_with_tag: div = 0 where tag != FLUID. */
+int flub_mask_with_tag(float *div, const int32_t *tag, size_t n, void *stream);
+/* utils/util.py:162-165 scale_by_3d (and mgpcg.py:9-12 _scale_by_3d): q *= s. */
+int flub_scale(float *q, float s, size_t n, void *stream);
+/* subtract_gradient.py:38-68 subtract_gradient_tag for axis d (1=u, 2=v, 3=w):
+ * uf/u_rigid have the face shape of that axis; p and tag are cell fields. */
+int flub_subtract_gradient_tag(float *uf, const float *u_rigid, const float *p, const int32_t *tag,
+                               int nx, int ny, int nz, int d, void *stream);
+/* warp.utils.array_inner (mgpcg.py:5,391,...): *out_dev = sum a*b, accumulated
+ * in FP64 by a deterministic two-stage reduction. */
+int flub_inner(const float *a, const float *b, size_t n, double *out_dev, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * MGPCG solver handle (multigrid/mgpcg.py:234-490).  The handle owns the level
+ * pyramid (r, z, packed tags per level), x, p, Ap and the device scalars; no
+ * allocation happens after create.  One handle per stream; not thread-safe.
+ * ------------------------------------------------------------------------- */
+/* MGPCG.__init__ (mgpcg.py:235-268): level l has shape (n >> l) per axis.
+ * Requires every level extent >= 2 and nz >> (levels-1) even. */
+int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int n_mg_levels, int precision,
+                      void *stream);
+int flub_mgpcg_destroy(flub_mgpcg *h);
+/* Attributes pre_and_post_smoothing / bottom_smoothing (mgpcg.py:238-239). */
+int flub_mgpcg_set_smoothing(flub_mgpcg *h, int pre_and_post_smoothing, int bottom_smoothing);
+/* MGPCG.set_tags (mgpcg.py:360-369): tag (int32, may be NULL to keep level 0)
+ * -> packed tags of every level (restrict_tag, mgpcg.py:162-174). */
+int flub_mgpcg_set_tags(flub_mgpcg *h, const int32_t *tag);
+/* MGPCG.init (mgpcg.py:276-284): zero z0, Ap, p, x; r0 = k * r. */
+int flub_mgpcg_init(flub_mgpcg *h, const float *r, float k);
+/* MGPCG.mask_interior (mgpcg.py:464-465,492-499). */
+int flub_mgpcg_mask_interior(flub_mgpcg *h);
+/* MGPCG.zero_centered (mgpcg.py:270-274,191-205). */
+int flub_mgpcg_zero_centered(flub_mgpcg *h);
+/* MGPCG.apply_preconditioner / graph_precondition (mgpcg.py:286-349); the
+ * V-cycle is captured once per smoother into a HIP graph and replayed. */
+int flub_mgpcg_apply_preconditioner(flub_mgpcg *h, int smoother);
+int flub_mgpcg_graph_precondition(flub_mgpcg *h, int smoother);
+/* compute_Ap_mcadams / compute_Ap (mgpcg.py:30-39,410-414): Ap = -A p. */
+int flub_mgpcg_compute_Ap(flub_mgpcg *h, int smoother);
+/* update_p/r/x (mgpcg.py:207-232,351-358) and update_alpha/beta (:212-222,372-376). */
+int flub_mgpcg_update_p(flub_mgpcg *h);
+int flub_mgpcg_update_r(flub_mgpcg *h);
+int flub_mgpcg_update_x(flub_mgpcg *h);
+int flub_mgpcg_update_alpha(flub_mgpcg *h, double eps);
+int flub_mgpcg_update_beta(flub_mgpcg *h, double eps);
+/* array_inner of two handle buffers into a handle scalar.
+ * Buffers: 0 x, 1 p, 2 Ap, 3 r[0], 4 z[0]; scalars: 0 alpha, 1 beta, 2 sum, 3 old_zTr. */
+int flub_mgpcg_inner(flub_mgpcg *h, int buf_a, int buf_b, int scalar_out);
+/* MGPCG.solve (mgpcg.py:378-462).  max_iters = -1 for no limit; stops when
+ * r.r < max(abs_tol, rel_tol * r0.r0) (squared norms).  *iters receives the
+ * completed-iteration count, *rTr the last r.r.  verbose prints the residual
+ * of every iteration as the reference does. */
+int flub_mgpcg_solve(flub_mgpcg *h, int max_iters, double eps, double abs_tol, double rel_tol,
+                     int smoother, int verbose, int *iters, double *rTr);
+/* Device pointers of the handle's buffers (x, p, Ap, r[l], z[l], tag[l]). */
+int flub_mgpcg_buffer(flub_mgpcg *h, int which, int level, void **ptr);
+/* divd[l] of the reference (mgpcg.py:51-58), computed from the packed tags into
+ * a caller float buffer of the level's shape. */
+int flub_mgpcg_compute_divd(flub_mgpcg *h, int level, float *divd);
+/* The device scalars alpha, beta, sum, old_zTr, rTr (FP64), copied to host. */
+int flub_mgpcg_scalars(flub_mgpcg *h, double out[5]);
+int flub_mgpcg_level_shape(flub_mgpcg *h, int level, int shape[3]);
+/* Measurement hook (the reference's wp.ScopedTimer, water_3d.py:62, at kernel
+ * granularity): replays one kernel of the solve `reps` times on the handle's
+ * stream between two HIP events and returns the mean ms per launch and the
+ * kernel's algorithmic HBM bytes per launch.  Kernel ids (FLUB_PROF_*):
+ * 0 level-0 RB half-sweep, 1 Ap + p.Ap, 2 r/x update + r.r, 3 whole V-cycle,
+ * 4 restriction 0->1, 5 prolongation 1->0, 6 p update.  Leaves the handle's
+ * vectors in an unspecified state (call before init / after reading x). */
+int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_ms, double *bytes);
+
+/* ---------------------------------------------------------------------------
+ * Whole projection (example_fluid.py:120-147 with water_3d.py:58-68 call_mgpcg):
+ * divergence_tag + mask + scale(-1/h), set_tags, init(div, -h), mask_interior,
+ * solve, p = x*h, subtract_gradient_tag d=1,2,3 with p/h, p *= h.
+ * u, v, w are updated in place; p (cell field) receives the stored pressure
+ * x*h; div (cell field, may be NULL) receives the scaled divergence.
+ * ------------------------------------------------------------------------- */
+int flub_project(flub_mgpcg *h, float *u, float *v, float *w, const float *u_rigid,
+                 const float *v_rigid, const float *w_rigid, const int32_t *tag, float h_cell,
+                 int max_iters, double abs_tol, double rel_tol, int smoother, float *p, float *div,
+                 int *iters, double *rTr);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLUB_H */

@@ -1,0 +1,320 @@
+"""TEST INFRASTRUCTURE -- Python face of the CPU oracle (``liboracle.so``).
+
+The oracle restates the reference Warp kernels of the projection path in C
+(``flub_oracle.c``; every routine cites the reference file:line it follows).
+It is the checker for tests/, the smoke check and the CPU-baseline leg of
+bench.py, and nothing else: the shipped package never imports this module.
+
+All arrays are NumPy, C-contiguous, k (z) fastest, as the reference lays out
+its ``wp.array3d`` fields.  ``real`` selects the FP32 restatement or its FP64
+twin.  Smoother ids: 0 = "default", 1 = "phases", 2 = "fraction".
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+SMOOTHERS = {"default": 0, "phases": 1, "fraction": 2}
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_Z = ctypes.c_size_t
+_D = ctypes.c_double
+
+
+def _declare(L):
+    for suf, R in (("_f32", ctypes.c_float), ("_f64", ctypes.c_double)):
+        sig = {
+            "orc_compute_divd": (None, [_P, _P, _I, _I, _I]),
+            "orc_compute_Ap_mcadams": (None, [_P, _P, _P, _I, _I, _I]),
+            "orc_compute_Ap_default": (None, [_P, _P, _I, _I, _I]),
+            "orc_smooth_mcadams": (None, [_P, _P, _P, _P, _I, _I, _I, _I]),
+            "orc_smooth_default": (None, [_P, _P, _I, _I, _I, _I]),
+            "orc_restrict_mcadams_serial": (None, [_P, _P, _P, _P, _P, _I, _I, _I]),
+            "orc_restrict_mcadams": (None, [_P, _P, _P, _P, _P, _I, _I, _I]),
+            "orc_restrict_default": (None, [_P, _P, _P, _I, _I, _I]),
+            "orc_prolongate": (None, [_P, _P, _I, _I, _I]),
+            "orc_divergence_tag": (None, [_P] * 8 + [_I, _I, _I]),
+            "orc_mask_with_tag": (None, [_P, _P, _Z]),
+            "orc_scale": (None, [_P, R, _Z]),
+            "orc_mask_interior": (None, [_P, _P, _Z]),
+            "orc_subtract_gradient_tag": (None, [_P, _P, _P, _P, _I, _I, _I, _I]),
+            "orc_zero_centered": (None, [_P, _P, _Z]),
+            "orc_inner": (_D, [_P, _P, _Z, _I]),
+            "orc_mgpcg_create": (_P, [_I, _I, _I, _I]),
+            "orc_mgpcg_destroy": (None, [_P]),
+            "orc_mgpcg_config": (None, [_P, _I, _I, _I, _I]),
+            "orc_mgpcg_buf": (_P, [_P, _I, _I]),
+            "orc_mgpcg_tag": (_P, [_P, _I]),
+            "orc_restrict_tag": (None, [_P, _P, _I, _I, _I]),
+            "orc_mgpcg_set_tags": (None, [_P, _P]),
+            "orc_mgpcg_init": (None, [_P, _P, R]),
+            "orc_mgpcg_mask_interior": (None, [_P]),
+            "orc_mgpcg_apply_preconditioner": (None, [_P, _I]),
+            "orc_mgpcg_solve": (_I, [_P, _I, _D, _D, _D, _I, _I, ctypes.POINTER(_D)]),
+            "orc_project": (_I, [_P] * 10 + [_D, _I, _D, _D, _D, _I, ctypes.POINTER(_D)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name + suf)
+            f.restype = res
+            f.argtypes = args
+    L.orc_tag_dirichlet.argtypes = [_P, _P, _Z]
+    L.orc_tag_neumann.argtypes = [_P, _P, _Z]
+    L.orc_set_threads.argtypes = [_I]
+    L.orc_set_threads.restype = _I
+
+
+def set_threads(n: int) -> int:
+    """Set the OpenMP thread count of the oracle (0 keeps it); returns it."""
+    return lib().orc_set_threads(n)
+
+
+def _ptr(a: np.ndarray):
+    assert a.flags.c_contiguous
+    return a.ctypes.data
+
+
+def _dt(real):
+    return np.float32 if real in (np.float32, "f32", 32) else np.float64
+
+
+def _suf(real):
+    return "_f32" if _dt(real) == np.float32 else "_f64"
+
+
+def _fn(name, real):
+    return getattr(lib(), name + _suf(real))
+
+
+def _arr(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+# ---------------------------------------------------------------- per kernel
+def tag_dirichlet(phi):
+    phi = _arr(phi, np.float32)
+    tag = np.empty(phi.shape, np.int32)
+    lib().orc_tag_dirichlet(_ptr(phi), _ptr(tag), phi.size)
+    return tag
+
+
+def tag_neumann(phi_rigid, tag):
+    phi_rigid = _arr(phi_rigid, np.float32)
+    tag = _arr(tag, np.int32).copy()
+    lib().orc_tag_neumann(_ptr(phi_rigid), _ptr(tag), tag.size)
+    return tag
+
+
+def restrict_tag(tag):
+    tag = _arr(tag, np.int32)
+    cx, cy, cz = (s // 2 for s in tag.shape)
+    out = np.empty((cx, cy, cz), np.int32)
+    lib().orc_restrict_tag_f32(_ptr(tag), _ptr(out), cx, cy, cz)
+    return out
+
+
+def compute_divd(tag, real=np.float32):
+    tag = _arr(tag, np.int32)
+    out = np.empty(tag.shape, _dt(real))
+    _fn("orc_compute_divd", real)(_ptr(tag), _ptr(out), *tag.shape)
+    return out
+
+
+def compute_Ap(p, tag=None, real=np.float32):
+    p = _arr(p, _dt(real))
+    Ap = np.empty_like(p)
+    if tag is None:
+        _fn("orc_compute_Ap_default", real)(_ptr(Ap), _ptr(p), *p.shape)
+    else:
+        tag = _arr(tag, np.int32)
+        _fn("orc_compute_Ap_mcadams", real)(_ptr(Ap), _ptr(p), _ptr(tag), *p.shape)
+    return Ap
+
+
+def smooth(z, r, tag, divd, phase, real=np.float32):
+    """One _smooth_mcadams launch (tag given) or _smooth (tag None); returns new z."""
+    z = _arr(z, _dt(real)).copy()
+    r = _arr(r, _dt(real))
+    if tag is None:
+        _fn("orc_smooth_default", real)(_ptr(z), _ptr(r), *z.shape, int(phase))
+    else:
+        tag = _arr(tag, np.int32)
+        divd = _arr(divd, _dt(real))
+        _fn("orc_smooth_mcadams", real)(_ptr(z), _ptr(r), _ptr(tag), _ptr(divd), *z.shape, int(phase))
+    return z
+
+
+def restrict(r, z, tag=None, tag_c=None, real=np.float32, serial=False):
+    r = _arr(r, _dt(real))
+    z = _arr(z, _dt(real))
+    out = np.zeros(tuple(s // 2 for s in r.shape), _dt(real))
+    if tag is None:
+        _fn("orc_restrict_default", real)(_ptr(r), _ptr(z), _ptr(out), *r.shape)
+    else:
+        tag = _arr(tag, np.int32)
+        tag_c = _arr(tag_c, np.int32)
+        name = "orc_restrict_mcadams_serial" if serial else "orc_restrict_mcadams"
+        _fn(name, real)(_ptr(r), _ptr(z), _ptr(tag), _ptr(out), _ptr(tag_c), *r.shape)
+    return out
+
+
+def prolongate(z, zc, real=np.float32):
+    z = _arr(z, _dt(real)).copy()
+    zc = _arr(zc, _dt(real))
+    _fn("orc_prolongate", real)(_ptr(z), _ptr(zc), *z.shape)
+    return z
+
+
+def divergence_tag(u, v, w, ur, vr, wr, tag, real=np.float32):
+    dt = _dt(real)
+    u, v, w, ur, vr, wr = (_arr(a, dt) for a in (u, v, w, ur, vr, wr))
+    tag = _arr(tag, np.int32)
+    div = np.empty(tag.shape, dt)
+    _fn("orc_divergence_tag", real)(_ptr(u), _ptr(v), _ptr(w), _ptr(div), _ptr(ur), _ptr(vr),
+                                    _ptr(wr), _ptr(tag), *tag.shape)
+    return div
+
+
+def mask_with_tag(div, tag, real=np.float32):
+    div = _arr(div, _dt(real)).copy()
+    tag = _arr(tag, np.int32)
+    _fn("orc_mask_with_tag", real)(_ptr(div), _ptr(tag), div.size)
+    return div
+
+
+def scale(q, s, real=np.float32):
+    q = _arr(q, _dt(real)).copy()
+    _fn("orc_scale", real)(_ptr(q), s, q.size)
+    return q
+
+
+def subtract_gradient_tag(uf, u_rigid, p, tag, d, real=np.float32):
+    dt = _dt(real)
+    uf = _arr(uf, dt).copy()
+    u_rigid = _arr(u_rigid, dt)
+    p = _arr(p, dt)
+    tag = _arr(tag, np.int32)
+    _fn("orc_subtract_gradient_tag", real)(_ptr(uf), _ptr(u_rigid), _ptr(p), _ptr(tag), *tag.shape, int(d))
+    return uf
+
+
+def zero_centered(q, tag, real=np.float32):
+    q = _arr(q, _dt(real)).copy()
+    tag = _arr(tag, np.int32)
+    _fn("orc_zero_centered", real)(_ptr(q), _ptr(tag), q.size)
+    return q
+
+
+def inner(a, b, real=np.float32, f64=True):
+    a = _arr(a, _dt(real))
+    b = _arr(b, _dt(real))
+    return _fn("orc_inner", real)(_ptr(a), _ptr(b), a.size, int(f64))
+
+
+# ---------------------------------------------------------------- MGPCG
+class MGPCG:
+    """Oracle twin of multigrid/mgpcg.py:234-490 (same levels and control flow)."""
+
+    def __init__(self, shape, n_mg_levels, real=np.float32, pre_and_post_smoothing=6,
+                 bottom_smoothing=6, dot_f64=True, parallel_dot=False):
+        self.real = _dt(real)
+        self.shape = tuple(int(s) for s in shape)
+        self.L = int(n_mg_levels)
+        self._h = _fn("orc_mgpcg_create", self.real)(*self.shape, self.L)
+        if not self._h:
+            raise ValueError("bad level count")
+        _fn("orc_mgpcg_config", self.real)(self._h, int(pre_and_post_smoothing), int(bottom_smoothing),
+                                           int(dot_f64), int(parallel_dot))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _fn("orc_mgpcg_destroy", self.real)(h)
+            self._h = None
+
+    def level_shape(self, l):
+        return tuple(s >> l for s in self.shape)
+
+    def _view(self, which, level, dtype=None):
+        shp = self.level_shape(level)
+        dtype = dtype or self.real
+        if which == "tag":
+            ptr = _fn("orc_mgpcg_tag", self.real)(self._h, level)
+            dtype = np.int32
+        else:
+            code = {"x": 0, "p": 1, "Ap": 2, "r": 3, "z": 4, "divd": 5}[which]
+            ptr = _fn("orc_mgpcg_buf", self.real)(self._h, code, level)
+        n = int(np.prod(shp))
+        buf = (ctypes.c_byte * (n * np.dtype(dtype).itemsize)).from_address(ptr)
+        return np.frombuffer(buf, dtype=dtype).reshape(shp)
+
+    def get(self, which, level=0):
+        return self._view(which, level).copy()
+
+    def put(self, which, arr, level=0):
+        self._view(which, level)[...] = arr
+
+    def set_tags(self, tag=None):
+        if tag is None:
+            _fn("orc_mgpcg_set_tags", self.real)(self._h, None)
+        else:
+            tag = _arr(tag, np.int32)
+            _fn("orc_mgpcg_set_tags", self.real)(self._h, _ptr(tag))
+
+    def init(self, r, k):
+        r = _arr(r, self.real)
+        _fn("orc_mgpcg_init", self.real)(self._h, _ptr(r), k)
+
+    def mask_interior(self):
+        _fn("orc_mgpcg_mask_interior", self.real)(self._h)
+
+    def apply_preconditioner(self, smoother="phases"):
+        _fn("orc_mgpcg_apply_preconditioner", self.real)(self._h, SMOOTHERS[smoother])
+
+    def solve(self, max_iters=1, eps=1e-12, abs_tol=1e-12, rel_tol=1e-12, smoother="default", verbose=False):
+        rtr = ctypes.c_double(0.0)
+        it = _fn("orc_mgpcg_solve", self.real)(self._h, int(max_iters), eps, abs_tol, rel_tol,
+                                              SMOOTHERS[smoother], int(verbose), ctypes.byref(rtr))
+        return it, rtr.value
+
+    def project(self, u, v, w, ur, vr, wr, tag, h, max_iters=-1, abs_tol=0.0, rel_tol=1e-12,
+                smoother="phases", eps=None):
+        """example_fluid.py:120-147 (+ water_3d.py:58-68). Returns (u, v, w, p0, div, iters, rTr)."""
+        dt = self.real
+        u, v, w = (_arr(a, dt).copy() for a in (u, v, w))
+        ur, vr, wr = (_arr(a, dt) for a in (ur, vr, wr))
+        tag = _arr(tag, np.int32)
+        div = np.empty(self.shape, dt)
+        p0 = np.empty(self.shape, dt)
+        rtr = ctypes.c_double(0.0)
+        if eps is None:
+            eps = 1e-12 if dt == np.float32 else 0.0
+        it = _fn("orc_project", dt)(self._h, _ptr(u), _ptr(v), _ptr(w), _ptr(ur), _ptr(vr), _ptr(wr),
+                                    _ptr(tag), _ptr(div), _ptr(p0), float(h), int(max_iters), float(eps),
+                                    float(abs_tol), float(rel_tol), SMOOTHERS[smoother], ctypes.byref(rtr))
+        return u, v, w, p0, div, it, rtr.value

@@ -1,0 +1,222 @@
+"""GPU parity of the HIP path against the CPU oracle (through the C-ABI).
+
+Integer work (tags, coarse masks, divd) must be bit-exact; every FP32 kernel
+follows the reference's per-cell operation order and must be bitwise equal to
+the FP32 oracle on the same inputs; whole solves are compared within stated
+tolerances against the FP32 oracle and the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from flub_amd import device as wp
+from flub_amd import scenes
+from flub_amd.const_param import Field, IntField, MACGrid
+from flub_amd.divergence import divergence_tag, mask_with_tag
+from flub_amd.geometry import tag_dirichlet, tag_neumann
+from flub_amd.launch import launch
+from flub_amd.multigrid.mgpcg import MGPCG
+from flub_amd.project import Projection
+from flub_amd.subtract_gradient import subtract_gradient_tag
+from flub_amd.utils.util import scale_by_3d
+
+pytestmark = pytest.mark.gpu
+
+
+def bits_equal(a, b):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def random_tags(shape, seed, p=(0.25, 0.6, 0.15)):
+    rng = np.random.default_rng(seed)
+    return rng.choice(3, size=shape, p=list(p)).astype(np.int32)
+
+
+SCENE_CASES = [("dam_break", lambda: scenes.dam_break(32, 0)),
+               ("spheres", lambda: scenes.dam_break_spheres(32, 1, n_spheres=8, r_cells=(2, 5))),
+               ("sphere_flow", lambda: scenes.flow_past_sphere(32, 2))]
+
+
+@pytest.mark.parametrize("name,make", SCENE_CASES)
+def test_tag_classification_bitexact(name, make, oracle):
+    s = make()
+    phi = Field(f=wp.from_numpy(s["phi"]))
+    phr = Field(f=wp.from_numpy(s["phi_rigid"]))
+    tag = IntField(f=wp.zeros(s["phi"].shape, int))
+    tag_dirichlet(tag, phi)
+    tag_neumann(tag, phr)
+    ref = oracle.tag_neumann(s["phi_rigid"], oracle.tag_dirichlet(s["phi"]))
+    assert np.array_equal(tag.f.numpy(), ref)
+
+
+@pytest.mark.parametrize("shape,levels,seed", [((16, 16, 16), 4, 0), ((32, 32, 32), 4, 1),
+                                               ((64, 32, 16), 3, 2), ((128, 128, 128), 5, 3)])
+def test_set_tags_pyramid_and_divd_bitexact(shape, levels, seed, oracle):
+    tag = random_tags(shape, seed)
+    mg = MGPCG(3, shape, levels)
+    mg.set_tags(wp.from_numpy(tag))
+    t = tag
+    for l in range(levels):
+        got = mg.tag[l].numpy().astype(np.int32)
+        assert np.array_equal(got, t), f"level {l}"
+        assert bits_equal(mg.divd[l].numpy(), oracle.compute_divd(t)), f"divd level {l}"
+        if l + 1 < levels:
+            t = oracle.restrict_tag(t)
+
+
+def _mac(s):
+    vel = MACGrid(*(Field(f=wp.from_numpy(s[c]), d=d) for d, c in ((1, "u"), (2, "v"), (3, "w"))))
+    rig = MACGrid(*(Field(f=wp.from_numpy(s[c]), d=d) for d, c in ((1, "ur"), (2, "vr"), (3, "wr"))))
+    return vel, rig
+
+
+@pytest.mark.parametrize("name,make", SCENE_CASES)
+def test_divergence_mask_scale_bitexact(name, make, oracle):
+    s = make()
+    rng = np.random.default_rng(5)
+    for c in ("ur", "vr", "wr"):   # non-zero rigid velocities exercise the solid override
+        s[c] = rng.normal(0, 1, s[c].shape).astype(np.float32)
+    vel, rig = _mac(s)
+    tag = IntField(f=wp.from_numpy(s["tag"]))
+    div = Field(f=wp.zeros(s["tag"].shape))
+    shape = s["tag"].shape
+    launch(divergence_tag, dim=shape, inputs=[vel, div, rig, tag])
+    ref = oracle.divergence_tag(s["u"], s["v"], s["w"], s["ur"], s["vr"], s["wr"], s["tag"])
+    assert bits_equal(div.f.numpy(), ref)
+    launch(mask_with_tag, dim=shape, inputs=[div, tag])
+    ref = oracle.mask_with_tag(ref, s["tag"])
+    assert bits_equal(div.f.numpy(), ref)
+    launch(scale_by_3d, dim=shape, inputs=[div, -1.0 / s["n"]])
+    ref = oracle.scale(ref, np.float32(-1.0 / s["n"]))
+    assert bits_equal(div.f.numpy(), ref)
+
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_subtract_gradient_bitexact(d, oracle):
+    s = scenes.flow_past_sphere(32, 2)
+    rng = np.random.default_rng(7)
+    p = rng.normal(0, 1, s["tag"].shape).astype(np.float32)
+    for c in ("ur", "vr", "wr"):
+        s[c] = rng.normal(0, 1, s[c].shape).astype(np.float32)
+    vel, rig = _mac(s)
+    tag = IntField(f=wp.from_numpy(s["tag"]))
+    pf = Field(f=wp.from_numpy(p))
+    comp = {1: "u", 2: "v", 3: "w"}[d]
+    launch(subtract_gradient_tag, dim=s[comp].shape, inputs=[vel, rig, pf, tag, d])
+    got = {1: vel.u, 2: vel.v, 3: vel.w}[d].f.numpy()
+    ref = oracle.subtract_gradient_tag(s[comp], s[comp + "r"], p, s["tag"], d)
+    assert bits_equal(got, ref)
+
+
+def _loaded_mgpcg(tag, levels, pre_post=2, bottom=6):
+    mg = MGPCG(3, tag.shape, levels)
+    mg.pre_and_post_smoothing = pre_post
+    mg.bottom_smoothing = bottom
+    mg.set_tags(wp.from_numpy(tag))
+    return mg
+
+
+@pytest.mark.parametrize("smoother", ["phases", "default"])
+def test_compute_Ap_bitexact(smoother, oracle):
+    s = scenes.dam_break_spheres(32, 1, n_spheres=8, r_cells=(2, 5))
+    tag = s["tag"]
+    mg = _loaded_mgpcg(tag, 3)
+    p = np.random.default_rng(3).normal(0, 1, tag.shape).astype(np.float32)
+    mg.p.assign(p)
+    mg.compute_Ap(smoother)
+    ref = oracle.compute_Ap(p, tag if smoother == "phases" else None)
+    assert bits_equal(mg.Ap.numpy(), ref)
+    # the fused FP64 p.Ap lands in `sum`
+    assert np.isclose(mg.scalars()[2], float(np.dot(p.astype(np.float64).ravel(), ref.astype(np.float64).ravel())),
+                      rtol=1e-12, atol=1e-9)
+
+
+@pytest.mark.parametrize("smoother,shape,levels", [("phases", (32, 32, 32), 3), ("phases", (64, 64, 64), 4),
+                                                   ("phases", (64, 32, 32), 4), ("default", (32, 32, 32), 3)])
+def test_vcycle_bitexact(smoother, shape, levels, oracle):
+    n = shape[0]
+    if smoother == "default":
+        tag = np.ones(shape, np.int32)
+    else:
+        s = scenes.dam_break_spheres(n, 1, n_spheres=8, r_cells=(2, 5)) if shape[1] == n else None
+        tag = s["tag"] if s is not None else random_tags(shape, 9, (0.1, 0.85, 0.05))
+    r = np.random.default_rng(11).normal(0, 1, shape).astype(np.float32)
+    r[tag != 1] = 0.0
+    mg = _loaded_mgpcg(tag, levels)
+    mg.r[0].assign(r)
+    orc = oracle.MGPCG(shape, levels, pre_and_post_smoothing=2, bottom_smoothing=6)
+    orc.set_tags(tag)
+    orc.put("r", r, 0)
+    orc.apply_preconditioner(smoother)
+    mg.apply_preconditioner(smoother)
+    assert bits_equal(mg.z[0].numpy(), orc.get("z", 0))
+    for l in range(1, levels):
+        assert bits_equal(mg.r[l].numpy(), orc.get("r", l)), f"restricted r level {l}"
+    mg.graph_precondition(smoother)          # captured graph replays the same cycle
+    mg.graph_precondition(smoother)
+    assert bits_equal(mg.z[0].numpy(), orc.get("z", 0))
+
+
+def _rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+@pytest.mark.parametrize("name,make,levels", [("dam_break_64", lambda: scenes.dam_break(64, 0), 4),
+                                              ("spheres_64", lambda: scenes.dam_break_spheres(64, 1), 4),
+                                              ("sphere_flow_64", lambda: scenes.flow_past_sphere(64, 2), 4)])
+def test_projection_matches_oracle(name, make, levels, oracle):
+    s = make()
+    n = s["n"]
+    pr = Projection((n, n, n), n_mg_levels=levels, pre_and_post_smoothing=2, bottom_smoothing=6)
+    pr.load(s)
+    it = pr.project(max_iters=-1, rel_tol=1e-12)
+    p_gpu = pr.p0.f.numpy()
+    u, v, w = pr.velocity()
+    o32 = oracle.MGPCG((n, n, n), levels, pre_and_post_smoothing=2, bottom_smoothing=6)
+    ru, rv, rw, rp, rdiv, rit, rrtr = o32.project(s["u"], s["v"], s["w"], s["ur"], s["vr"], s["wr"], s["tag"],
+                                                   1.0 / n, rel_tol=1e-12)
+    o64 = oracle.MGPCG((n, n, n), levels, real=np.float64, pre_and_post_smoothing=2, bottom_smoothing=6)
+    _, _, _, p64, _, _, _ = o64.project(s["u"], s["v"], s["w"], s["ur"], s["vr"], s["wr"], s["tag"], 1.0 / n,
+                                        rel_tol=1e-24)
+    assert bits_equal(pr.div.f.numpy(), rdiv)                 # fused divergence == reference sequence
+    assert abs(it - rit) <= 1
+    assert pr.rTr is not None and pr.rTr < 1.0
+    assert _rel(p_gpu, rp) < 1e-5                             # FP32 GPU vs FP32 oracle
+    assert _rel(p_gpu, p64) < 1e-5                            # FP32 GPU vs FP64 oracle (1e-12 residual)
+    assert _rel(u, ru) < 1e-5 and _rel(v, rv) < 1e-5 and _rel(w, rw) < 1e-5
+    # post-projection divergence on FLUID cells equals the solver's residual (small)
+    post = oracle.divergence_tag(u, v, w, s["ur"], s["vr"], s["wr"], s["tag"], real=np.float64)
+    pre = oracle.divergence_tag(s["u"], s["v"], s["w"], s["ur"], s["vr"], s["wr"], s["tag"], real=np.float64)
+    fl = s["tag"] == 1
+    assert np.linalg.norm(post[fl]) < 2e-5 * np.linalg.norm(pre[fl])
+
+
+def test_stepwise_equals_fused(oracle):
+    s = scenes.dam_break_spheres(64, 1)
+    a = Projection((64,) * 3, 4, 2, 6)
+    a.load(s)
+    a.project()
+    b = Projection((64,) * 3, 4, 2, 6)
+    b.load(s)
+    b.project_stepwise()
+    assert a.iterations == b.iterations
+    assert bits_equal(a.p0.f.numpy(), b.p0.f.numpy())
+    for x, y in zip(a.velocity(), b.velocity()):
+        assert bits_equal(x, y)
+
+
+def test_half_fluid_cos_problem(oracle):
+    """The reference's MGPCG_Test problem (mgpcg.py:501-558) at 64^3, 4 levels."""
+    tag, r = scenes.half_fluid_cos(64)
+    mg = _loaded_mgpcg(tag, 4, pre_post=6, bottom=6)
+    mg.init(wp.from_numpy(r), 1.0)
+    mg.mask_interior()
+    it = mg.solve(400, rel_tol=1e-12, abs_tol=0.0, smoother="phases")
+    o = oracle.MGPCG(tag.shape, 4, real=np.float64)
+    o.set_tags(tag)
+    o.init(r.astype(np.float64), 1.0)
+    o.mask_interior()
+    oit, _ = o.solve(400, eps=0.0, rel_tol=1e-24, abs_tol=0.0, smoother="phases")
+    assert it < 400
+    assert _rel(mg.x.numpy(), o.get("x")) < 1e-5
