@@ -14,6 +14,7 @@
 #include "common.h"
 #include "grid_ops.h"
 #include "mgpcg_internal.h"
+#include "smooth.h"
 
 // ---------------------------------------------------------------- tags
 __global__ void k_pack_tags(const int32_t *__restrict__ in, uint8_t *__restrict__ out, uint32_t n)
@@ -413,6 +414,25 @@ static int half_sweep(flub_mgpcg *h, int l, int tagged, int phase)
         if (rc_) return rc_;      \
     } while (0)
 
+// Bottom level (mgpcg.py:319-336) from z = 0 (zeroed with the restriction at
+// mgpcg.py:312): one workgroup when it is small, else one launch per sweep.
+static void bottom_solve(flub_mgpcg *h, int tagged)
+{
+    const int lb = h->L - 1;
+    Level &B = h->lv[lb];
+    const int nb = h->bottom / 2;
+    if (h->L > 1) k_fill_f32<<<nb256(B.n), 256, 0, h->s>>>(B.z, 0.0f, B.n);
+    if (B.n / 2 <= BOTTOM_ONE_WG_MAX) {
+        if (nb > 0) {
+            if (tagged) k_bottom<true><<<1, 1024, 0, h->s>>>(B.z, B.r, B.tag, B.nx, B.ny, B.nz, nb);
+            else k_bottom<false><<<1, 1024, 0, h->s>>>(B.z, B.r, B.tag, B.nx, B.ny, B.nz, nb);
+        }
+    } else {
+        for (int it = 0; it < nb; ++it) { half_sweep(h, lb, tagged, 0); half_sweep(h, lb, tagged, 1); }
+        for (int it = 0; it < nb; ++it) { half_sweep(h, lb, tagged, 1); half_sweep(h, lb, tagged, 0); }
+    }
+}
+
 // mgpcg.py:296-349, launched kernel by kernel (captured into a graph by the caller).
 static int vcycle_fp32(flub_mgpcg *h, int smoother)
 {
@@ -433,19 +453,7 @@ static int vcycle_fp32(flub_mgpcg *h, int smoother)
         else k_restrict<false><<<nb256(C.n), 256, 0, h->s>>>(F.r, F.z, F.tag, C.r, C.tag, F.nx, F.ny, F.nz);
         FLUB_CHECK_LAUNCH();
     }
-    const int lb = h->L - 1;
-    Level &B = h->lv[lb];
-    const int nb = h->bottom / 2;
-    if (B.n / 2 <= BOTTOM_ONE_WG_MAX) {
-        if (nb > 0) {
-            if (tagged) k_bottom<true><<<1, 1024, 0, h->s>>>(B.z, B.r, B.tag, B.nx, B.ny, B.nz, nb);
-            else k_bottom<false><<<1, 1024, 0, h->s>>>(B.z, B.r, B.tag, B.nx, B.ny, B.nz, nb);
-            FLUB_CHECK_LAUNCH();
-        }
-    } else {
-        for (int it = 0; it < nb; ++it) { TRY(half_sweep(h, lb, tagged, 0)); TRY(half_sweep(h, lb, tagged, 1)); }
-        for (int it = 0; it < nb; ++it) { TRY(half_sweep(h, lb, tagged, 1)); TRY(half_sweep(h, lb, tagged, 0)); }
-    }
+    bottom_solve(h, tagged);
     for (int l = h->L - 2; l >= 0; --l) {
         Level &F = h->lv[l];
         k_prolong<<<nb256(F.n), 256, 0, h->s>>>(F.z, h->lv[l + 1].z, F.nx, F.ny, F.nz);
@@ -458,10 +466,52 @@ static int vcycle_fp32(flub_mgpcg *h, int smoother)
     return FLUB_OK;
 }
 
+static void bottom_solve(flub_mgpcg *h, int tagged);
+
+// The same V-cycle on the quad kernels of smooth.hip, with the fusions listed
+// there.  Pre-smoothing of level l runs in place on zpre (its first sweep
+// starts from z = 0), the restriction reads zpre, the prolongating first
+// post-sweep reads zpre + z[l+1] and writes z[l], the rest runs in place on
+// z[l]; the last level-0 sweep also yields the partials of z.r.
+static int vcycle_fast(flub_mgpcg *h)
+{
+    for (int l = 0; l + 1 < h->L; ++l) {
+        Level &F = h->lv[l];
+        const int s = h->pre_post << l;
+        TRY(fast_sweep(h->s, F, F.zpre, F.zpre, nullptr, 0, SWEEP_ZERO, nullptr, nullptr));
+        TRY(fast_sweep(h->s, F, F.zpre, F.zpre, nullptr, 1, SWEEP_PLAIN, nullptr, nullptr));
+        for (int it = 1; it < s; ++it) {
+            TRY(fast_sweep(h->s, F, F.zpre, F.zpre, nullptr, 0, SWEEP_PLAIN, nullptr, nullptr));
+            TRY(fast_sweep(h->s, F, F.zpre, F.zpre, nullptr, 1, SWEEP_PLAIN, nullptr, nullptr));
+        }
+        TRY(fast_restrict(h->s, F, F.zpre, h->lv[l + 1]));
+    }
+    bottom_solve(h, 1);
+    for (int l = h->L - 2; l >= 0; --l) {
+        Level &F = h->lv[l];
+        const int s = h->pre_post << l;
+        const float *zc = h->lv[l + 1].z;
+        for (int it = 0; it < s; ++it) {
+            const bool last = l == 0 && it + 1 == s;
+            if (it == 0) TRY(fast_sweep(h->s, F, F.zpre, F.z, zc, 1, SWEEP_PROLONG, nullptr, nullptr));
+            else TRY(fast_sweep(h->s, F, F.z, F.z, nullptr, 1, SWEEP_PLAIN, nullptr, nullptr));
+            TRY(fast_sweep(h->s, F, F.z, F.z, nullptr, 0, SWEEP_PLAIN, last ? h->part : nullptr,
+                           last ? &h->vc_dot_nblk : nullptr));
+        }
+    }
+    return FLUB_OK;
+}
+
+static bool use_fast(const flub_mgpcg *h, int smoother)
+{
+    return h->fast && smoother != FLUB_SMOOTHER_DEFAULT && h->pre_post >= 1;
+}
+
 int mg_precondition(flub_mgpcg *h, int smoother, bool use_graph)
 {
     if (h->prec != FLUB_PREC_FP32) return mg_vcycle_mixed(h, smoother);
-    if (!use_graph || h->no_graph) return vcycle_fp32(h, smoother);
+    const bool fast = use_fast(h, smoother);
+    if (!use_graph || h->no_graph) return fast ? vcycle_fast(h) : vcycle_fp32(h, smoother);
     if (!h->graph_ok[smoother]) {
         // Capture on the handle's private stream (the work stream may be the
         // null stream, which cannot be captured); replay on the work stream.
@@ -469,7 +519,7 @@ int mg_precondition(flub_mgpcg *h, int smoother, bool use_graph)
         hipStream_t work = h->s;
         FLUB_CHECK_HIP(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
         h->s = h->cap;
-        int rc = vcycle_fp32(h, smoother);
+        int rc = fast ? vcycle_fast(h) : vcycle_fp32(h, smoother);
         h->s = work;
         hipError_t e = hipStreamEndCapture(h->cap, &g);
         if (rc) { if (e == hipSuccess) hipGraphDestroy(g); return rc; }
@@ -537,6 +587,7 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
             ALLOC(L.tag, L.n);
             ALLOC(L.r, (size_t)L.n * 4);
             ALLOC(L.z, (size_t)L.n * 4);
+            ALLOC(L.zpre, (size_t)L.n * 4);
             if (precision != FLUB_PREC_FP32) {
                 ALLOC(L.rh, (size_t)L.n * 2);
                 ALLOC(L.zh, (size_t)L.n * 2);
@@ -546,6 +597,8 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
         ALLOC(h->x, n0 * 4);
         ALLOC(h->p, n0 * 4);
         ALLOC(h->Ap, n0 * 4);
+        ALLOC(h->p_alt, n0 * 4);
+        h->fast = levels >= 2 && (nz % 4) == 0;
         ALLOC(h->sc, SC_NUM * sizeof(double));
         ALLOC(h->part, 2 * MAXBLK * sizeof(double));
         hipError_t e = hipHostMalloc((void **)&h->hsc, SC_NUM * sizeof(double), hipHostMallocDefault);
@@ -573,9 +626,9 @@ FLUB_API int flub_mgpcg_destroy(flub_mgpcg *h)
     mg_mixed_teardown(h);
     for (int l = 0; l < h->L; ++l) {
         Level &L = h->lv[l];
-        hipFree(L.tag); hipFree(L.r); hipFree(L.z); hipFree(L.rh); hipFree(L.zh);
+        hipFree(L.tag); hipFree(L.r); hipFree(L.z); hipFree(L.zpre); hipFree(L.rh); hipFree(L.zh);
     }
-    hipFree(h->x); hipFree(h->p); hipFree(h->Ap); hipFree(h->sc); hipFree(h->part);
+    hipFree(h->x); hipFree(h->p); hipFree(h->p_alt); hipFree(h->Ap); hipFree(h->sc); hipFree(h->part);
     if (h->hsc) hipHostFree(h->hsc);
     if (h->cap) hipStreamDestroy(h->cap);
     free(h);
@@ -759,6 +812,57 @@ static int read_scalar(flub_mgpcg *h, int idx, double *out)
 
 // mgpcg.py:378-462.  Same control flow; x's update is fused into r's (both
 // use this iteration's alpha and p, which the preconditioner does not touch).
+// mgpcg.py:378-462 for the "phases" smoother on the fused kernels: the z.r
+// partials come out of the V-cycle's last sweep, update_p is folded into the
+// Ap kernel (out of place, alternating p and p_alt), update_x into update_r.
+static int solve_fast(flub_mgpcg *h, int max_iters, double eps, double abs_tol, double rel_tol, int verbose,
+                      int *iters, double *rTr_out)
+{
+    Level &L0 = h->lv[0];
+    const uint32_t n = L0.n;
+    const int nb = red_blocks(n);
+    float *const p_home = h->p;
+    TRY(mg_dot(h, L0.r, L0.r, FIN_RTR, 0.0));
+    double rTr0;
+    TRY(read_scalar(h, SC_RTR, &rTr0));
+    const double tol = fmax(abs_tol, rTr0 * rel_tol);
+    TRY(mg_precondition(h, FLUB_SMOOTHER_PHASES, true));
+    TRY(launch_finish(h, h->vc_dot_nblk, FIN_OLDZTR, 0.0));
+    const int cap = max_iters < 0 ? SOLVE_SAFETY_CAP : max_iters;
+    int it = 0;
+    double rTr = rTr0;
+    int status = FLUB_OK;
+    while (it < cap) {
+        int nbp = 0;
+        TRY(fast_p_Ap(h->s, L0, h->p, h->p_alt, h->Ap, h->sc, h->part, &nbp));
+        float *t = h->p; h->p = h->p_alt; h->p_alt = t;
+        TRY(launch_finish(h, nbp, FIN_ALPHA, eps));
+        k_update_rx_dot<<<nb, RBS, 0, h->s>>>(L0.r, h->x, h->Ap, h->p, h->sc, n, h->part);
+        FLUB_CHECK_LAUNCH();
+        TRY(launch_finish(h, nb, FIN_RTR, 0.0));
+        TRY(read_scalar(h, SC_RTR, &rTr));
+        if (verbose) printf("after iter %d, |residual|_2=%.9g\n", it + 1, sqrt(rTr));
+        if (!isfinite(rTr)) { status = FLUB_ENONFINITE; flub_set_error("non-finite residual at iteration %d", it + 1); break; }
+        if (rTr < tol) break;
+        TRY(mg_precondition(h, FLUB_SMOOTHER_PHASES, true));
+        TRY(launch_finish(h, h->vc_dot_nblk, FIN_BETA, eps));
+        it += 1;
+    }
+    if (h->p != p_home) {   // leave p in the buffer the handle exposes
+        FLUB_CHECK_HIP(hipMemcpyAsync(p_home, h->p, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
+        h->p_alt = h->p;
+        h->p = p_home;
+    }
+    if (status == FLUB_OK && it >= cap && max_iters < 0) {
+        status = FLUB_ENOTCONV;
+        flub_set_error("no convergence within %d iterations", cap);
+    }
+    if (verbose) printf("finished with  %d  iterations\n", it);
+    if (iters) *iters = it;
+    if (rTr_out) *rTr_out = rTr;
+    return status;
+}
+
 static int solve_fp32(flub_mgpcg *h, int max_iters, double eps, double abs_tol, double rel_tol, int smoother,
                       int verbose, int *iters, double *rTr_out)
 {
@@ -819,6 +923,8 @@ FLUB_API int flub_mgpcg_solve(flub_mgpcg *h, int max_iters, double eps, double a
     FLUB_REQUIRE(h->tags_set, FLUB_ESTATE, "solve before set_tags");
     if (h->prec != FLUB_PREC_FP32)
         return mg_solve_mixed(h, max_iters, eps, abs_tol, rel_tol, smoother, verbose, iters, rTr);
+    if (smoother == FLUB_SMOOTHER_PHASES && use_fast(h, smoother))
+        return solve_fast(h, max_iters, eps, abs_tol, rel_tol, verbose, iters, rTr);
     return solve_fp32(h, max_iters, eps, abs_tol, rel_tol, smoother, verbose, iters, rTr);
 }
 
@@ -930,8 +1036,16 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
     const int nb = red_blocks(L0.n);
     auto launch_once = [&]() -> int {
         switch (kernel) {
-        case 0: return half_sweep(h, 0, 1, 0);
-        case 1: return compute_Ap_dot(h, FLUB_SMOOTHER_PHASES, FIN_SUM, 0.0);
+        case 0:
+            if (h->fast) return fast_sweep(h->s, L0, L0.z, L0.z, nullptr, 0, SWEEP_PLAIN, nullptr, nullptr);
+            return half_sweep(h, 0, 1, 0);
+        case 1:
+            if (h->fast) {
+                int nbp = 0;
+                TRY(fast_p_Ap(h->s, L0, h->p, h->p_alt, h->Ap, h->sc, h->part, &nbp));
+                return launch_finish(h, nbp, FIN_SUM, 0.0);
+            }
+            return compute_Ap_dot(h, FLUB_SMOOTHER_PHASES, FIN_SUM, 0.0);
         case 2:
             k_update_rx_dot<<<nb, RBS, 0, h->s>>>(L0.r, h->x, h->Ap, h->p, h->sc, L0.n, h->part);
             FLUB_CHECK_LAUNCH();
@@ -939,11 +1053,13 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
         case 3: return mg_precondition(h, FLUB_SMOOTHER_PHASES, true);
         case 4: {
             Level &C = h->lv[1];
+            if (h->fast) return fast_restrict(h->s, L0, L0.zpre, C);
             k_restrict<true><<<nb256(C.n), 256, 0, h->s>>>(L0.r, L0.z, L0.tag, C.r, C.tag, L0.nx, L0.ny, L0.nz);
             FLUB_CHECK_LAUNCH();
             return FLUB_OK;
         }
         case 5:
+            if (h->fast) return fast_sweep(h->s, L0, L0.zpre, L0.z, h->lv[1].z, 1, SWEEP_PROLONG, nullptr, nullptr);
             k_prolong<<<nb256(L0.n), 256, 0, h->s>>>(L0.z, h->lv[1].z, L0.nx, L0.ny, L0.nz);
             FLUB_CHECK_LAUNCH();
             return FLUB_OK;
@@ -956,11 +1072,11 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
     };
     switch (kernel) {
     case 0: b = N * (2 * v + 1); break;                      // z (7-pt), r (own colour), tags
-    case 1: b = N * (2 * 4 + 1); break;                      // p, tags -> Ap
+    case 1: b = h->fast ? N * (4 * 4 + 1) : N * (2 * 4 + 1); break;   // (z, p_old ->) p, tags -> Ap
     case 2: b = N * 6 * 4; break;                            // r, x, Ap, p -> r, x
     case 3: b = N * 0; break;                                // filled by the caller's model
     case 4: b = N * (2 * v + 1) + N / 8 * (v + 1); break;
-    case 5: b = N * 2 * v + N / 8 * v; break;
+    case 5: b = h->fast ? N * (2.625 * v + 1) : N * 2 * v + N / 8 * v; break;   // fused: zpre, zc, r/2, tags -> z
     case 6: b = N * 3 * 4; break;
     default: break;
     }

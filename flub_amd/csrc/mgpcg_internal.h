@@ -20,6 +20,7 @@ struct Level {
     uint32_t n;
     uint8_t *tag;   // packed tags
     float *r, *z;   // FP32 storage
+    float *zpre;    // pre-smoothed z of the fast V-cycle (post-smoothing lands in z)
     void *rh, *zh;  // FP16/BF16 storage (mixed precision only)
 };
 
@@ -29,7 +30,10 @@ struct flub_mgpcg {
     int L, prec;
     Level lv[MAXL];
     float *x, *p, *Ap;
+    float *p_alt;   // second p buffer: the fused p-update + Ap writes out of place
     double *sc;     // device scalars
+    bool fast;      // quad kernels usable (tagged smoother, levels >= 2, pre_post >= 1)
+    int vc_dot_nblk;  // partial count of the z.r fused into the fast V-cycle (0: none)
     double *part;   // reduction partials (2*MAXBLK)
     double *hsc;    // pinned host mirror of sc
     int pre_post, bottom;
