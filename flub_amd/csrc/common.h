@@ -115,6 +115,24 @@ __device__ __forceinline__ float nsum_stencil(LoadX X, int i, int j, int k, int 
     return ret;
 }
 
+// XCD-aware contiguous work split.  The GPU hands workgroup b to XCD b % 8
+// (each XCD has its own L2); the grid (a multiple of 8 workgroups) is
+// remapped so that XCD x owns the x-th contiguous eighth of the items, and
+// each workgroup a contiguous run of them.  Consecutive sweeps over the same
+// level then find most of their data in the same L2.
+struct Chunk {
+    uint32_t begin, end;
+};
+__device__ __forceinline__ Chunk xcd_chunk(uint32_t total)
+{
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t lb = (b & 7u) * (nb >> 3) + (b >> 3);
+    const uint32_t per = ((total + nb - 1) / nb + 255u) & ~255u;
+    const uint32_t begin = lb * per;
+    const uint32_t end = begin + per < total ? begin + per : total;
+    return {begin, end};
+}
+
 // Deterministic block reduction of a double (fixed shuffle/LDS tree).
 template <int BS>
 __device__ __forceinline__ double block_sum(double v, double *lds)

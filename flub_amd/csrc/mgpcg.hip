@@ -422,6 +422,18 @@ static void bottom_solve(flub_mgpcg *h, int tagged)
     Level &B = h->lv[lb];
     const int nb = h->bottom / 2;
     if (h->L > 1) k_fill_f32<<<nb256(B.n), 256, 0, h->s>>>(B.z, 0.0f, B.n);
+    if (tagged && h->fast && B.nz % 4 == 0 && B.n / 2 > BOTTOM_ONE_WG_MAX) {
+        // quad sweeps over many workgroups (a single workgroup is latency-bound here)
+        for (int it = 0; it < nb; ++it) {
+            fast_sweep(h->s, B, B.z, B.z, nullptr, 0, SWEEP_PLAIN, nullptr, nullptr);
+            fast_sweep(h->s, B, B.z, B.z, nullptr, 1, SWEEP_PLAIN, nullptr, nullptr);
+        }
+        for (int it = 0; it < nb; ++it) {
+            fast_sweep(h->s, B, B.z, B.z, nullptr, 1, SWEEP_PLAIN, nullptr, nullptr);
+            fast_sweep(h->s, B, B.z, B.z, nullptr, 0, SWEEP_PLAIN, nullptr, nullptr);
+        }
+        return;
+    }
     if (B.n / 2 <= BOTTOM_ONE_WG_MAX) {
         if (nb > 0) {
             if (tagged) k_bottom<true><<<1, 1024, 0, h->s>>>(B.z, B.r, B.tag, B.nx, B.ny, B.nz, nb);

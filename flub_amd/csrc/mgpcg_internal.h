@@ -6,7 +6,7 @@
 #define MAXL 12
 #define RBS 256                 // reduction block size
 #define MAXBLK 2048             // max partials per reduction
-#define BOTTOM_ONE_WG_MAX 65536 // bottom levels up to this many cells per colour run in one workgroup
+#define BOTTOM_ONE_WG_MAX 2048  // bottom levels up to this many cells per colour run in one workgroup
 #define SOLVE_SAFETY_CAP 20000  // iteration cap applied when max_iters == -1
 
 // Device scalar slots (FP64).

@@ -117,7 +117,8 @@ __global__ void __launch_bounds__(256) k_rbgs4(const float *zin, float *zout, co
     const int nq = nz >> 2;
     const uint32_t total = (uint32_t)nx * ny * nq;
     double acc = 0.0;
-    for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < total; t += gridDim.x * 256u) {
+    const Chunk ch = xcd_chunk(total);
+    for (uint32_t t = ch.begin + threadIdx.x; t < ch.end; t += 256u) {
         const uint32_t row = t / nq;
         const int k0 = (int)(t - row * nq) * 4;
         const int i = (int)(row / ny), j = (int)(row - (uint32_t)i * ny);
@@ -176,7 +177,8 @@ __global__ void __launch_bounds__(256) k_restrict4(const float *__restrict__ r, 
     const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;
     const int nq = nz >> 2;   // coarse pairs per coarse row
     const uint32_t total = (uint32_t)cx * cy * nq;
-    for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < total; t += gridDim.x * 256u) {
+    const Chunk ch = xcd_chunk(total);
+    for (uint32_t t = ch.begin + threadIdx.x; t < ch.end; t += 256u) {
         const uint32_t crow = t / nq;
         const int kq = (int)(t - crow * nq);
         const int I = (int)(crow / cy), J = (int)(crow - (uint32_t)I * cy);
@@ -257,7 +259,8 @@ __global__ void __launch_bounds__(256) k_p_Ap4(const float *__restrict__ p, floa
         q.v[0] = fmaf(beta, a.x, b.x); q.v[1] = fmaf(beta, a.y, b.y);
         q.v[2] = fmaf(beta, a.z, b.z); q.v[3] = fmaf(beta, a.w, b.w);
     };
-    for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < total; t += gridDim.x * 256u) {
+    const Chunk ch = xcd_chunk(total);
+    for (uint32_t t = ch.begin + threadIdx.x; t < ch.end; t += 256u) {
         const uint32_t row = t / nq;
         const int k0 = (int)(t - row * nq) * 4;
         const int i = (int)(row / ny), j = (int)(row - (uint32_t)i * ny);
@@ -295,10 +298,13 @@ __global__ void __launch_bounds__(256) k_p_Ap4(const float *__restrict__ p, floa
 }
 
 // ------------------------------------------------------------------ host
+// Workgroup count: one item per thread up to MAXBLK workgroups, rounded up to a
+// multiple of 8 (xcd_chunk's remap is a bijection on multiples of 8).
 static unsigned grid_for(uint32_t items)
 {
     unsigned b = blocks_for(items, 256);
-    return b < 1 ? 1 : (b > MAXBLK ? MAXBLK : b);
+    b = b < 1 ? 1 : (b > MAXBLK ? MAXBLK : b);
+    return (b + 7u) & ~7u;
 }
 
 int fast_sweep(hipStream_t s, const Level &L, const float *zin, float *zout, const float *zc, int phase, int mode,

@@ -183,7 +183,11 @@ def test_projection_matches_oracle(name, make, levels, oracle):
     assert abs(it - rit) <= 1
     assert pr.rTr is not None and pr.rTr < 1.0
     assert _rel(p_gpu, rp) < 1e-5                             # FP32 GPU vs FP32 oracle
-    assert _rel(p_gpu, p64) < 1e-5                            # FP32 GPU vs FP64 oracle (1e-12 residual)
+    # vs the converged FP64 solution (residual 1e-12): a 1e-6 residual bounds the
+    # error only up to the conditioning, so the GPU must be as accurate as the
+    # FP32 reference algorithm itself (plus rounding slack)
+    err_ref = _rel(rp, p64)
+    assert _rel(p_gpu, p64) < 1.5 * err_ref + 1e-6, (_rel(p_gpu, p64), err_ref)
     assert _rel(u, ru) < 1e-5 and _rel(v, rv) < 1e-5 and _rel(w, rw) < 1e-5
     # post-projection divergence on FLUID cells equals the solver's residual (small)
     post = oracle.divergence_tag(u, v, w, s["ur"], s["vr"], s["wr"], s["tag"], real=np.float64)
