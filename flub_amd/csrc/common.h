@@ -115,22 +115,21 @@ __device__ __forceinline__ float nsum_stencil(LoadX X, int i, int j, int k, int 
     return ret;
 }
 
-// XCD-aware contiguous work split.  The GPU hands workgroup b to XCD b % 8
-// (each XCD has its own L2); the grid (a multiple of 8 workgroups) is
-// remapped so that XCD x owns the x-th contiguous eighth of the items, and
-// each workgroup a contiguous run of them.  Consecutive sweeps over the same
-// level then find most of their data in the same L2.
+// XCD-aware work split.  The GPU hands workgroup b to XCD b % 8 (each XCD
+// has its own L2); with a grid that is a multiple of 8 workgroups, XCD x owns
+// the x-th contiguous eighth of the items and its workgroups stride through
+// that slab together (so HBM still sees streaming order).  Consecutive sweeps
+// over the same level then find their data in the same XCD's L2.
 struct Chunk {
-    uint32_t begin, end;
+    uint32_t begin, end, first, stride;
 };
 __device__ __forceinline__ Chunk xcd_chunk(uint32_t total)
 {
-    const uint32_t nb = gridDim.x, b = blockIdx.x;
-    const uint32_t lb = (b & 7u) * (nb >> 3) + (b >> 3);
-    const uint32_t per = ((total + nb - 1) / nb + 255u) & ~255u;
-    const uint32_t begin = lb * per;
-    const uint32_t end = begin + per < total ? begin + per : total;
-    return {begin, end};
+    const uint32_t nb8 = gridDim.x >> 3, x = blockIdx.x & 7u, lb = blockIdx.x >> 3;
+    const uint32_t slab = (total + 7u) >> 3;
+    const uint32_t begin = x * slab;
+    const uint32_t end = begin + slab < total ? begin + slab : total;
+    return {begin, end, begin + lb * 256u + threadIdx.x, nb8 * 256u};
 }
 
 // Deterministic block reduction of a double (fixed shuffle/LDS tree).

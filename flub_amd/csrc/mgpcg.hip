@@ -376,6 +376,8 @@ static int launch_finish(flub_mgpcg *h, int nblk, int op, double eps, double *ds
     return FLUB_OK;
 }
 
+int mg_finish_alpha(flub_mgpcg *h, int nblk, double eps) { return launch_finish(h, nblk, FIN_ALPHA, eps); }
+
 int mg_dot(flub_mgpcg *h, const float *a, const float *b, int op, double eps)
 {
     const uint32_t n = h->lv[0].n;
@@ -597,6 +599,7 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
             L.nx = nx >> l; L.ny = ny >> l; L.nz = nz >> l;
             L.n = (uint32_t)L.nx * L.ny * L.nz;
             ALLOC(L.tag, L.n);
+            ALLOC(L.code, (size_t)L.n * 2);
             ALLOC(L.r, (size_t)L.n * 4);
             ALLOC(L.z, (size_t)L.n * 4);
             ALLOC(L.zpre, (size_t)L.n * 4);
@@ -638,7 +641,7 @@ FLUB_API int flub_mgpcg_destroy(flub_mgpcg *h)
     mg_mixed_teardown(h);
     for (int l = 0; l < h->L; ++l) {
         Level &L = h->lv[l];
-        hipFree(L.tag); hipFree(L.r); hipFree(L.z); hipFree(L.zpre); hipFree(L.rh); hipFree(L.zh);
+        hipFree(L.tag); hipFree(L.code); hipFree(L.r); hipFree(L.z); hipFree(L.zpre); hipFree(L.rh); hipFree(L.zh);
     }
     hipFree(h->x); hipFree(h->p); hipFree(h->p_alt); hipFree(h->Ap); hipFree(h->sc); hipFree(h->part);
     if (h->hsc) hipHostFree(h->hsc);
@@ -676,6 +679,7 @@ FLUB_API int flub_mgpcg_set_tags(flub_mgpcg *h, const int32_t *tag)
         k_restrict_tag<<<nb256(C.n), 256, 0, h->s>>>(h->lv[l].tag, C.tag, C.nx, C.ny, C.nz);
         FLUB_CHECK_LAUNCH();
     }
+    for (int l = 0; l < h->L; ++l) TRY(make_codes(h->s, h->lv[l]));
     h->tags_set = true;
     return FLUB_OK;
 }
@@ -719,6 +723,7 @@ FLUB_API int flub_mgpcg_apply_preconditioner(flub_mgpcg *h, int smoother)
 {
     FLUB_REQUIRE(h && smoother_ok(smoother), FLUB_EINVAL, "bad handle or smoother");
     FLUB_REQUIRE(h->tags_set, FLUB_ESTATE, "apply_preconditioner before set_tags");
+    if (h->prec != FLUB_PREC_FP32) TRY(mg_mixed_rscale(h));
     return mg_precondition(h, smoother, false);
 }
 
@@ -726,6 +731,7 @@ FLUB_API int flub_mgpcg_graph_precondition(flub_mgpcg *h, int smoother)
 {
     FLUB_REQUIRE(h && smoother_ok(smoother), FLUB_EINVAL, "bad handle or smoother");
     FLUB_REQUIRE(h->tags_set, FLUB_ESTATE, "graph_precondition before set_tags");
+    if (h->prec != FLUB_PREC_FP32) TRY(mg_mixed_rscale(h));
     return mg_precondition(h, smoother, true);
 }
 

@@ -19,6 +19,7 @@ struct Level {
     int nx, ny, nz;
     uint32_t n;
     uint8_t *tag;   // packed tags
+    uint16_t *code; // per-cell stencil code (neighbour classes, divd, FLUID bit)
     float *r, *z;   // FP32 storage
     float *zpre;    // pre-smoothed z of the fast V-cycle (post-smoothing lands in z)
     void *rh, *zh;  // FP16/BF16 storage (mixed precision only)
@@ -50,7 +51,11 @@ int mg_dot(flub_mgpcg *h, const float *a, const float *b, int op, double eps);
 int mg_zero_centered(flub_mgpcg *h);
 int mg_precondition(flub_mgpcg *h, int smoother, bool use_graph);
 
+// alpha = old_zTr / (sum of the partials + eps) (mgpcg.py:212-216)
+int mg_finish_alpha(flub_mgpcg *h, int nblk, double eps);
+
 // mixed.hip
+int mg_mixed_rscale(flub_mgpcg *h);
 int mg_mixed_setup(flub_mgpcg *h);
 void mg_mixed_teardown(flub_mgpcg *h);
 int mg_vcycle_mixed(flub_mgpcg *h, int smoother);

@@ -1,23 +1,392 @@
-// Mixed-precision V-cycle and flexible CG (no counterpart in the reference).
-// Placeholder until the FP16/BF16 kernels land: handles created with a
-// reduced-precision V-cycle report FLUB_ESTATE from solve.
-#include "mgpcg_internal.h"
+// Mixed-precision solve (no counterpart in the reference): the V-cycle of
+// mgpcg.py:296-349 with z and r stored in FP16 or BF16 (arithmetic in FP32)
+// inside an FP32 flexible CG (Polak-Ribiere beta) with FP64 reductions and an
+// FP64 true-residual check.
+//
+// Storage scaling.  The V-cycle is linear, so each level keeps its (z, r) as
+// c_l times the true values with c_l = s0 * 8^-l: s0 = 2^-e is the power of
+// two that brings max|r| of the CG residual to (0.5, 1] (times 2^-shift of
+// extra FP16 headroom on large grids), and 8^-l undoes the 8x growth of the
+// un-normalised sum restriction (mgpcg.py:99).  Powers of two are exact in
+// FP32, so the scaling changes no rounding; it only keeps FP16 values in
+// range.
+//
+// Level data: rh (scaled r), zph (pre-smoothed z), zh (post-smoothed z), all
+// in the storage type.  The first level-0 sweep reads the FP32 CG residual
+// and writes rh; the last level-0 sweep writes the FP32 z of the CG (divided
+// by s0) and the FP64 partials of z.r and z.q (q = -A p of the last CG step).
+// The sweeps and restrictions are the kernels of smooth.hip instantiated for
+// the storage type.
+#include <math.h>
+#include <stdio.h>
 
-int mg_mixed_setup(flub_mgpcg *h) { (void)h; return FLUB_OK; }
-void mg_mixed_teardown(flub_mgpcg *h) { (void)h; }
+#include "mgpcg_internal.h"
+#include "smooth.h"
+
+// r -= alpha q; x += alpha p; partials of r.r and max|r|.
+__global__ void __launch_bounds__(RBS) k_mx_update(float *__restrict__ r, float *__restrict__ x,
+                                                   const float *__restrict__ q, const float *__restrict__ p,
+                                                   const double *__restrict__ sc, uint32_t n, double *__restrict__ part)
+{
+    __shared__ double lds[RBS / 64];
+    const float alpha = (float)sc[SC_ALPHA];
+    double acc = 0.0;
+    float mx = 0.0f;
+    for (uint32_t t = blockIdx.x * RBS + threadIdx.x; t < n; t += gridDim.x * RBS) {
+        const float rn = fmaf(-alpha, q[t], r[t]);
+        r[t] = rn;
+        x[t] = fmaf(alpha, p[t], x[t]);
+        acc += (double)rn * (double)rn;
+        mx = fmaxf(mx, fabsf(rn));
+    }
+    const double s = block_sum<RBS>(acc, lds);
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_down(mx, off, 64));
+    __shared__ float lm[RBS / 64];
+    if ((threadIdx.x & 63) == 0) lm[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = 0.0f;
+        for (int w = 0; w < RBS / 64; ++w) m = fmaxf(m, lm[w]);
+        part[blockIdx.x] = s;
+        part[gridDim.x + blockIdx.x] = (double)m;
+    }
+}
+
+// r = b - ndt(x) evaluated in FP64 (true residual), stored in FP32; partials
+// of |r|^2 (FP64, from the FP64 values) and max|r|.
+__global__ void __launch_bounds__(RBS) k_true_residual(float *__restrict__ r, const float *__restrict__ b,
+                                                       const float *__restrict__ x, const uint8_t *__restrict__ tag,
+                                                       int nx, int ny, int nz, double *__restrict__ part)
+{
+    __shared__ double lds[RBS / 64];
+    const uint32_t n = (uint32_t)nx * ny * nz;
+    double acc = 0.0, mx = 0.0;
+    for (uint32_t t = blockIdx.x * RBS + threadIdx.x; t < n; t += gridDim.x * RBS) {
+        const int k = t % nz, j = (t / nz) % ny, i = t / ((uint32_t)nz * ny);
+        const int tI = tag[t];
+        double rv = 0.0;
+        if (tI == 1) {
+            const double xI = x[t];
+            double s = 0.0;
+            const int ii[6] = {i + 1, i - 1, i, i, i, i}, jj[6] = {j, j, j + 1, j - 1, j, j},
+                      kk[6] = {k, k, k, k, k + 1, k - 1};
+#pragma unroll
+            for (int d = 0; d < 6; ++d) {
+                if (ii[d] < 0 || ii[d] >= nx || jj[d] < 0 || jj[d] >= ny || kk[d] < 0 || kk[d] >= nz) continue;
+                const uint32_t N = cidx(ii[d], jj[d], kk[d], ny, nz);
+                const int tN = tag[N];
+                if (tN == 2) continue;
+                s += (tN == 0 ? 0.0 : (double)x[N]) - xI;
+            }
+            rv = (double)b[t] - s;
+        }
+        r[t] = (float)rv;
+        acc += rv * rv;
+        mx = fmax(mx, fabs(rv));
+    }
+    const double s = block_sum<RBS>(acc, lds);
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
+    __shared__ double lm[RBS / 64];
+    if ((threadIdx.x & 63) == 0) lm[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int w = 0; w < RBS / 64; ++w) m = fmax(m, lm[w]);
+        part[blockIdx.x] = s;
+        part[gridDim.x + blockIdx.x] = m;
+    }
+}
+
+// Partials of r.r and max|r| without changing r.
+__global__ void __launch_bounds__(RBS) k_mx_rnorm(const float *__restrict__ r, uint32_t n, double *__restrict__ part)
+{
+    __shared__ double lds[RBS / 64];
+    __shared__ float lm[RBS / 64];
+    double acc = 0.0;
+    float mx = 0.0f;
+    for (uint32_t t = blockIdx.x * RBS + threadIdx.x; t < n; t += gridDim.x * RBS) {
+        const float v = r[t];
+        acc += (double)v * (double)v;
+        mx = fmaxf(mx, fabsf(v));
+    }
+    const double s = block_sum<RBS>(acc, lds);
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_down(mx, off, 64));
+    if ((threadIdx.x & 63) == 0) lm[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = 0.0f;
+        for (int w = 0; w < RBS / 64; ++w) m = fmaxf(m, lm[w]);
+        part[blockIdx.x] = s;
+        part[gridDim.x + blockIdx.x] = (double)m;
+    }
+}
+
+// Reduce (sum, max) partials: sc[RTR] = sum, sc[RSCALE] = 2^-(e+shift) with
+// 2^(e-1) < max <= 2^e (2^-shift when max == 0).
+__global__ void __launch_bounds__(RBS) k_mx_finish_r(const double *__restrict__ part, int nblk, double *__restrict__ sc,
+                                                     int shift)
+{
+    __shared__ double lds[RBS / 64];
+    __shared__ double lm[RBS / 64];
+    double acc = 0.0, mx = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += RBS) { acc += part[b]; mx = fmax(mx, part[nblk + b]); }
+    const double s = block_sum<RBS>(acc, lds);
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
+    if ((threadIdx.x & 63) == 0) lm[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int w = 0; w < RBS / 64; ++w) m = fmax(m, lm[w]);
+        sc[SC_RTR] = s;
+        sc[SC_RSCALE] = ldexp((m > 0.0 && isfinite(m)) ? ldexp(1.0, -ilogb(m) - 1) : 1.0, -shift);
+    }
+}
+
+// Polak-Ribiere update from the V-cycle's partials: zr = z.r, zq = z.q;
+// beta = -alpha * zq / (zr_old + eps) (q is -A p of the step, alpha the
+// reference's sign), zr_old = zr.  First call (first=1): only zr_old = zr.
+__global__ void __launch_bounds__(RBS) k_mx_finish_beta(const double *__restrict__ part, int nblk,
+                                                        double *__restrict__ sc, double eps, int first)
+{
+    __shared__ double lds[RBS / 64];
+    double a1 = 0.0, a2 = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += RBS) { a1 += part[b]; a2 += part[nblk + b]; }
+    const double zr = block_sum<RBS>(a1, lds);
+    const double zq = block_sum<RBS>(a2, lds);
+    if (threadIdx.x == 0) {
+        sc[SC_BETA] = first ? 0.0 : -sc[SC_ALPHA] * zq / (sc[SC_OLDZTR] + eps);
+        sc[SC_OLDZTR] = zr;
+        sc[SC_AUX0] = zr;
+    }
+}
+
+// ------------------------------------------------------------------ host
+struct MixedState {
+    float *b;      // FP32 copy of the initial residual (true-residual recompute)
+    void *zph[MAXL];
+    int shift;     // extra 2^-shift on the storage scale (FP16 headroom for z ~ n^2 r)
+};
+
+int mg_mixed_setup(flub_mgpcg *h)
+{
+    if (h->prec == FLUB_PREC_FP32) return FLUB_OK;
+    FLUB_REQUIRE(h->L >= 2 && (h->lv[h->L - 1].nz % 4) == 0, FLUB_ESHAPE,
+                 "mixed precision needs >= 2 levels and z extents divisible by 4 on every level");
+    h->mx = new MixedState();
+    {
+        // z of the V-cycle reaches about 0.4 n^2 max|r| (smoothest mode, with the
+        // unweighted restriction's overshoot); keep it below 2^14 in FP16.
+        int nmax = h->lv[0].nx > h->lv[0].ny ? h->lv[0].nx : h->lv[0].ny;
+        nmax = nmax > h->lv[0].nz ? nmax : h->lv[0].nz;
+        const double zmax = 0.41 * (double)nmax * (double)nmax;
+        h->mx->shift = h->prec == FLUB_PREC_FP16 && zmax > 16384.0 ? (int)ceil(log2(zmax / 16384.0)) : 0;
+    }
+    FLUB_CHECK_HIP(hipMalloc((void **)&h->mx->b, (size_t)h->lv[0].n * 4));
+    for (int l = 0; l < h->L; ++l) FLUB_CHECK_HIP(hipMalloc(&h->mx->zph[l], (size_t)h->lv[l].n * 2));
+    return FLUB_OK;
+}
+
+void mg_mixed_teardown(flub_mgpcg *h)
+{
+    if (!h->mx) return;
+    hipFree(h->mx->b);
+    for (int l = 0; l < h->L; ++l) hipFree(h->mx->zph[l]);
+    delete h->mx;
+    h->mx = nullptr;
+}
+
+static int red_nb(uint32_t n)
+{
+    int nb = (int)blocks_for(n, RBS);
+    return nb > MAXBLK ? MAXBLK : (nb < 1 ? 1 : nb);
+}
+
+// Storage scale (and r.r) of the current r.
+int mg_mixed_rscale(flub_mgpcg *h)
+{
+    Level &L0 = h->lv[0];
+    const int nb = red_nb(L0.n);
+    k_mx_rnorm<<<nb, RBS, 0, h->s>>>(L0.r, L0.n, h->part);
+    FLUB_CHECK_LAUNCH();
+    k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nb, h->sc, h->mx->shift);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+#define TRYM(x)               \
+    do {                      \
+        int rc_ = (x);        \
+        if (rc_) return rc_;  \
+    } while (0)
+
+static int mx_sweep(flub_mgpcg *h, int l, int mode, bool src32, int out, int phase, const void *zin, void *zout,
+                    const void *zc)
+{
+    Level &L = h->lv[l];
+    SweepInfo si{};
+    si.zin = zin; si.zout = zout; si.zc = zc; si.pfac = 8.0f;
+    si.rsrc = L.rh; si.rh_out = L.rh; si.sc = h->sc;
+    si.r32 = h->lv[0].r; si.q32 = h->Ap; si.z32 = h->lv[0].z; si.part = h->part;
+    si.phase = phase; si.mode = mode; si.out = out; si.src32 = src32;
+    int nb = 0;
+    TRYM(sweep(h->s, L, h->prec, si, &nb));
+    if (out == SWEEP_OUT_FINAL) h->vc_dot_nblk = nb;
+    return FLUB_OK;
+}
+
+// One V-cycle (mgpcg.py:296-349 schedule) from the FP32 residual r[0] into
+// the FP32 z[0], with the z.r / z.q partials of the last sweep.
+static int vcycle_mixed(flub_mgpcg *h)
+{
+    const int L = h->L;
+    for (int l = 0; l + 1 < L; ++l) {
+        void *zp = h->mx->zph[l];
+        const int s = h->pre_post << l;
+        TRYM(mx_sweep(h, l, SWEEP_ZERO, l == 0, SWEEP_OUT_STORE, 0, nullptr, zp, nullptr));
+        TRYM(mx_sweep(h, l, SWEEP_PLAIN, false, SWEEP_OUT_STORE, 1, zp, zp, nullptr));
+        for (int it = 1; it < s; ++it) {
+            TRYM(mx_sweep(h, l, SWEEP_PLAIN, false, SWEEP_OUT_STORE, 0, zp, zp, nullptr));
+            TRYM(mx_sweep(h, l, SWEEP_PLAIN, false, SWEEP_OUT_STORE, 1, zp, zp, nullptr));
+        }
+        TRYM(restrict_level(h->s, h->lv[l], h->lv[l].rh, zp, h->lv[l + 1], h->lv[l + 1].rh, h->prec));
+    }
+    {
+        const int lb = L - 1;
+        void *z = h->lv[lb].zh;
+        const int nb = h->bottom / 2;
+        bool started = false;
+        for (int pass = 0; pass < 2; ++pass)
+            for (int it = 0; it < nb; ++it)
+                for (int k = 0; k < 2; ++k) {
+                    const int phase = pass == 0 ? k : 1 - k;
+                    TRYM(mx_sweep(h, lb, started ? SWEEP_PLAIN : SWEEP_ZERO, false, SWEEP_OUT_STORE, phase, z, z,
+                                  nullptr));
+                    started = true;
+                }
+        if (!started) FLUB_CHECK_HIP(hipMemsetAsync(z, 0, (size_t)h->lv[lb].n * 2, h->s));
+    }
+    for (int l = L - 2; l >= 0; --l) {
+        void *zp = h->mx->zph[l];
+        void *z = h->lv[l].zh;
+        const void *zc = h->lv[l + 1].zh;
+        const int s = h->pre_post << l;
+        for (int it = 0; it < s; ++it) {
+            const bool last = l == 0 && it + 1 == s;
+            if (it == 0) TRYM(mx_sweep(h, l, SWEEP_PROLONG, false, SWEEP_OUT_STORE, 1, zp, z, zc));
+            else TRYM(mx_sweep(h, l, SWEEP_PLAIN, false, SWEEP_OUT_STORE, 1, z, z, nullptr));
+            TRYM(mx_sweep(h, l, SWEEP_PLAIN, false, last ? SWEEP_OUT_FINAL : SWEEP_OUT_STORE, 0, z, z, nullptr));
+        }
+    }
+    return FLUB_OK;
+}
 
 int mg_vcycle_mixed(flub_mgpcg *h, int smoother)
 {
-    (void)h; (void)smoother;
-    flub_set_error("mixed-precision V-cycle not built yet");
-    return FLUB_ESTATE;
+    FLUB_REQUIRE(smoother != FLUB_SMOOTHER_DEFAULT, FLUB_EINVAL, "mixed precision supports the tagged smoothers");
+    FLUB_REQUIRE(h->pre_post >= 1, FLUB_EINVAL, "mixed precision needs pre_and_post_smoothing >= 1");
+    if (!h->graph_ok[smoother]) {
+        hipGraph_t g;
+        hipStream_t work = h->s;
+        FLUB_CHECK_HIP(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
+        h->s = h->cap;
+        int rc = vcycle_mixed(h);
+        h->s = work;
+        hipError_t e = hipStreamEndCapture(h->cap, &g);
+        if (rc) { if (e == hipSuccess) hipGraphDestroy(g); return rc; }
+        FLUB_CHECK_HIP(e);
+        FLUB_CHECK_HIP(hipGraphInstantiate(&h->graph[smoother], g, nullptr, nullptr, 0));
+        FLUB_CHECK_HIP(hipGraphDestroy(g));
+        h->graph_ok[smoother] = true;
+    }
+    FLUB_CHECK_HIP(hipGraphLaunch(h->graph[smoother], h->s));
+    return FLUB_OK;
 }
 
-int mg_solve_mixed(flub_mgpcg *h, int max_iters, double eps, double abs_tol, double rel_tol, int smoother,
-                   int verbose, int *iters, double *rTr)
+static int read_sc(flub_mgpcg *h, int idx, double *out)
 {
-    (void)h; (void)max_iters; (void)eps; (void)abs_tol; (void)rel_tol; (void)smoother; (void)verbose;
-    (void)iters; (void)rTr;
-    flub_set_error("mixed-precision solve not built yet");
-    return FLUB_ESTATE;
+    FLUB_CHECK_HIP(hipMemcpyAsync(h->hsc + idx, h->sc + idx, sizeof(double), hipMemcpyDeviceToHost, h->s));
+    FLUB_CHECK_HIP(hipStreamSynchronize(h->s));
+    *out = h->hsc[idx];
+    return FLUB_OK;
+}
+
+static int true_residual(flub_mgpcg *h, int nb)
+{
+    Level &L0 = h->lv[0];
+    k_true_residual<<<nb, RBS, 0, h->s>>>(L0.r, h->mx->b, h->x, L0.tag, L0.nx, L0.ny, L0.nz, h->part);
+    FLUB_CHECK_LAUNCH();
+    k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nb, h->sc, h->mx->shift);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+// Flexible PCG (Polak-Ribiere) in FP32 with FP64 reductions around the
+// reduced-precision V-cycle.  Same stopping rule as mgpcg.py:393,444
+// (squared norms of the recursive residual).  When it passes, the true
+// residual b - ndt(x) is recomputed in FP64 and replaces r; the solve stops
+// unless it exceeds the tolerance by more than the drift allowance (then up
+// to max_replacements more rounds).  *rTr reports that FP64 true residual.
+int mg_solve_mixed(flub_mgpcg *h, int max_iters, double eps, double abs_tol, double rel_tol, int smoother,
+                   int verbose, int *iters, double *rTr_out)
+{
+    FLUB_REQUIRE(smoother == FLUB_SMOOTHER_PHASES, FLUB_EINVAL, "mixed precision solve supports smoother \"phases\"");
+    Level &L0 = h->lv[0];
+    const uint32_t n = L0.n;
+    const int nb = red_nb(n);
+    const double drift = 100.0;       // accepted true/recursive gap in squared norm (10x in norm)
+    const int max_replacements = 3;
+    int replacements = 0;
+    // b = r0 (the right-hand side; x = 0 after init)
+    FLUB_CHECK_HIP(hipMemcpyAsync(h->mx->b, L0.r, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
+    TRYM(mg_mixed_rscale(h));     // r.r and the storage scale of r (x = 0 after init, so r = b)
+    double rTr0;
+    TRYM(read_sc(h, SC_RTR, &rTr0));
+    const double tol = fmax(abs_tol, rTr0 * rel_tol);
+    float *const p_home = h->p;
+    TRYM(mg_vcycle_mixed(h, smoother));
+    k_mx_finish_beta<<<1, RBS, 0, h->s>>>(h->part, h->vc_dot_nblk, h->sc, eps, 1);
+    FLUB_CHECK_LAUNCH();
+    const int cap = max_iters < 0 ? SOLVE_SAFETY_CAP : max_iters;
+    int it = 0, status = FLUB_OK;
+    double rTr = rTr0;
+    while (it < cap) {
+        int nbp = 0;
+        TRYM(fast_p_Ap(h->s, L0, h->p, h->p_alt, h->Ap, h->sc, h->part, &nbp));
+        float *t = h->p; h->p = h->p_alt; h->p_alt = t;
+        TRYM(mg_finish_alpha(h, nbp, eps));   // alpha = zr / (p.q + eps)
+        k_mx_update<<<nb, RBS, 0, h->s>>>(L0.r, h->x, h->Ap, h->p, h->sc, n, h->part);
+        FLUB_CHECK_LAUNCH();
+        k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nb, h->sc, h->mx->shift);
+        FLUB_CHECK_LAUNCH();
+        TRYM(read_sc(h, SC_RTR, &rTr));
+        if (verbose) printf("after iter %d, |residual|_2=%.9g\n", it + 1, sqrt(rTr));
+        if (!isfinite(rTr)) { status = FLUB_ENONFINITE; flub_set_error("non-finite residual at iteration %d", it + 1); break; }
+        if (rTr < tol) {
+            // the reference's test passed on the recursive residual: check the
+            // FP64 true residual for drift (it replaces r either way)
+            TRYM(true_residual(h, nb));
+            double rTrue;
+            TRYM(read_sc(h, SC_RTR, &rTrue));
+            if (verbose) printf("  true |residual|_2=%.9g\n", sqrt(rTrue));
+            if (rTrue < drift * tol || replacements >= max_replacements) { rTr = rTrue; break; }
+            ++replacements;
+        }
+        TRYM(mg_vcycle_mixed(h, smoother));
+        k_mx_finish_beta<<<1, RBS, 0, h->s>>>(h->part, h->vc_dot_nblk, h->sc, eps, 0);
+        FLUB_CHECK_LAUNCH();
+        it += 1;
+    }
+    if (h->p != p_home) {
+        FLUB_CHECK_HIP(hipMemcpyAsync(p_home, h->p, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
+        h->p_alt = h->p;
+        h->p = p_home;
+    }
+    if (status == FLUB_OK && it >= cap && max_iters < 0) {
+        status = FLUB_ENOTCONV;
+        flub_set_error("no convergence within %d iterations", cap);
+    }
+    if (verbose) printf("finished with  %d  iterations\n", it);
+    if (iters) *iters = it;
+    if (rTr_out) *rTr_out = rTr;
+    return status;
 }

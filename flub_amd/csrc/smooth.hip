@@ -1,305 +1,415 @@
-// Fast tagged multigrid kernels (levels with nz % 4 == 0).
+// Fast tagged multigrid kernels for levels with nz % 4 == 0, in FP32 (the
+// reference arithmetic, bitwise) and in FP16/BF16 storage (the mixed solver).
 //
 // Each thread owns a "quad": 4 consecutive cells along k of one (i, j) row,
-// loaded and stored as one 16-byte vector (tags as one 32-bit word), so a
-// wavefront moves whole 256-byte runs of z, r and 64-byte runs of tags.  The
-// i/j neighbours are the quads of the adjacent rows; the k neighbours beyond
-// the quad are two scalar loads.  Per cell the arithmetic is exactly the
-// reference's (mgpcg.py:72-101,118-142), so results are bitwise those of the
-// one-cell-per-thread kernels and of the FP32 oracle.
+// moved as one 16-byte (FP32) or 8-byte (FP16/BF16) vector.  The i/j
+// neighbours are the quads of the adjacent rows; the k neighbours beyond the
+// quad come from the adjacent lanes (lane +-1 holds quad +-1 of the same row)
+// and are loaded only at wavefront edges.
 //
-// Fusions (each bitwise equal to the unfused launch sequence):
-//  * ZERO: the first pre-smoothing half-sweep of a V-cycle level starts from
-//    z = 0 (mgpcg.py:299,312): z is not read, all four cells are written.
-//  * PROLONG: the first post-smoothing half-sweep reads z + z_coarse[parent]
-//    (mgpcg.py:113-116,339) at every stencil point and writes the prolongated
-//    value to the cells it does not update.
-//  * DOT: the last half-sweep of the cycle also accumulates z.r (FP64) of its
-//    final values (mgpcg.py:451).
+// Neighbour classes come from a per-cell 16-bit stencil code built once per
+// set_tags (k_make_codes): for the neighbours i+, i-, j+, j-, k+, k- two bits
+// each (0 = SOLID or outside the domain, 1 = AIR, 2 = FLUID), bits 12-14 the
+// count divd of in-domain non-solid neighbours (mgpcg.py:51-58,144-160), bit
+// 15 set for a FLUID cell.  neighbor_dot_tag (mgpcg.py:118-142) is then, per
+// axis, ret += (a + b) with a = (x'_+ - x_I) for an AIR/FLUID neighbour (x' = 0
+// on AIR) and a = 0 for a SOLID/outside one.  The reference multiplies the
+// latter difference by 0.0 instead; the signed zero that gives never changes
+// ret (ret starts at +0 and is never -0), so results stay bitwise those of the
+// reference kernels and of the FP32 oracle.
+//
+// Fusions (each equal to the unfused launch sequence):
+//  * ZERO: the first pre-smoothing half-sweep of a level starts from z = 0
+//    (mgpcg.py:299,312): z is not read, all four cells are written.
+//  * PROLONG: the first post-smoothing half-sweep reads z + c*z_coarse[parent]
+//    (mgpcg.py:113-116,339; c = 1 in FP32, 8 for the mixed level scaling) at
+//    every stencil point and writes the prolongated value to the cells it does
+//    not update.  Out of place: its neighbours still read the unprolongated z.
+//  * DOT: the last half-sweep of the cycle also accumulates z.r (FP64).
+//  * FINAL (mixed): the last level-0 sweep writes the FP32 z of the CG and the
+//    FP64 partials of z.r and z.q.
 #include "common.h"
 #include "mgpcg_internal.h"
 #include "smooth.h"
 
 namespace {
 
-enum { MODE_PLAIN = 0, MODE_ZERO = 1, MODE_PROLONG = 2 };
+template <typename T> __device__ __forceinline__ float ldv(T v);
+template <> __device__ __forceinline__ float ldv<float>(float v) { return v; }
+template <> __device__ __forceinline__ float ldv<__half>(__half v) { return __half2float(v); }
+template <> __device__ __forceinline__ float ldv<__hip_bfloat16>(__hip_bfloat16 v) { return __bfloat162float(v); }
+template <typename T> __device__ __forceinline__ T stv(float v);
+template <> __device__ __forceinline__ float stv<float>(float v) { return v; }
+template <> __device__ __forceinline__ __half stv<__half>(float v) { return __float2half_rn(v); }
+template <> __device__ __forceinline__ __hip_bfloat16 stv<__hip_bfloat16>(float v) { return __float2bfloat16(v); }
 
-struct Quad {
-    float v[4];
-    int t[4];
+template <typename T>
+__device__ __forceinline__ void ld4(const T *p, float v[4])
+{
+    if constexpr (sizeof(T) == 4) {
+        const float4 a = *reinterpret_cast<const float4 *>(p);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    } else {
+        const uint2 w = *reinterpret_cast<const uint2 *>(p);
+        const T *e = reinterpret_cast<const T *>(&w);
+        v[0] = ldv<T>(e[0]); v[1] = ldv<T>(e[1]); v[2] = ldv<T>(e[2]); v[3] = ldv<T>(e[3]);
+    }
+}
+template <typename T>
+__device__ __forceinline__ void st4(T *p, const float v[4])
+{
+    if constexpr (sizeof(T) == 4) {
+        *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+        uint2 w;
+        T *e = reinterpret_cast<T *>(&w);
+        e[0] = stv<T>(v[0]); e[1] = stv<T>(v[1]); e[2] = stv<T>(v[2]); e[3] = stv<T>(v[3]);
+        *reinterpret_cast<uint2 *>(p) = w;
+    }
+}
+template <typename T>
+__device__ __forceinline__ void ld2(const T *p, float v[2])
+{
+    if constexpr (sizeof(T) == 4) {
+        const float2 a = *reinterpret_cast<const float2 *>(p);
+        v[0] = a.x; v[1] = a.y;
+    } else {
+        const uint32_t w = *reinterpret_cast<const uint32_t *>(p);
+        const T *e = reinterpret_cast<const T *>(&w);
+        v[0] = ldv<T>(e[0]); v[1] = ldv<T>(e[1]);
+    }
+}
+
+__device__ __forceinline__ void codes4(const uint16_t *code, uint32_t base, uint32_t c[4])
+{
+    const uint2 w = *reinterpret_cast<const uint2 *>(code + base);
+    c[0] = w.x & 0xffffu; c[1] = w.x >> 16; c[2] = w.y & 0xffffu; c[3] = w.y >> 16;
+}
+
+// neighbor_dot_tag from the stencil code (the self-FLUID test is the caller's).
+__device__ __forceinline__ float nsum_code(uint32_t cd, float xI, float xip, float xim, float xjp, float xjm,
+                                           float xkp, float xkm)
+{
+    float s = 0.0f;
+    const float xs[6] = {xip, xim, xjp, xjm, xkp, xkm};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t cp = (cd >> (4 * a)) & 3u, cm = (cd >> (4 * a + 2)) & 3u;
+        const float ta = cp == 0 ? 0.0f : ((cp == 2 ? xs[2 * a] : 0.0f) - xI);
+        const float tb = cm == 0 ? 0.0f : ((cm == 2 ? xs[2 * a + 1] : 0.0f) - xI);
+        s = s + (ta + tb);
+    }
+    return s;
+}
+
+__device__ __forceinline__ int code_divd(uint32_t cd) { return (int)((cd >> 12) & 7u); }
+__device__ __forceinline__ bool code_fluid(uint32_t cd) { return (cd >> 15) != 0; }
+
+struct Idx {
+    int nx, ny, nz, nq, lq, lny;
+    __device__ __forceinline__ void decode(uint32_t t, int &i, int &j, int &k0) const
+    {
+        uint32_t row, kq;
+        if (lq >= 0) { row = t >> lq; kq = t & ((1u << lq) - 1u); }
+        else { row = t / (uint32_t)nq; kq = t - row * (uint32_t)nq; }
+        if (lny >= 0) { i = (int)(row >> lny); j = (int)(row & ((1u << lny) - 1u)); }
+        else { i = (int)(row / (uint32_t)ny); j = (int)(row - (uint32_t)i * ny); }
+        k0 = (int)kq * 4;
+    }
 };
-
-__device__ __forceinline__ void unpack_tags(uint32_t w, int t[4])
-{
-    t[0] = w & 0xff; t[1] = (w >> 8) & 0xff; t[2] = (w >> 16) & 0xff; t[3] = w >> 24;
-}
-
-// The reference's neighbor_dot_tag for one cell from explicit neighbour
-// values; an out-of-domain neighbour is given as the cell itself (clamp) and
-// excluded from divd.
-__device__ __forceinline__ float ndt7(float xI, int tI, float xip, int tip, bool ip, float xim, int tim,
-                                      bool im, float xjp, int tjp, bool jp, float xjm, int tjm, bool jm,
-                                      float xkp, int tkp, bool kp, float xkm, int tkm, bool km, int *divd)
-{
-    float ret = 0.0f;
-    int d = 0;
-    {
-        const float a = (tip == 0 ? 0.0f : xip) - xI, b = (tim == 0 ? 0.0f : xim) - xI;
-        ret = ret + (a * (tip < 2 ? 1.0f : 0.0f) + b * (tim < 2 ? 1.0f : 0.0f));
-        d += (ip && tip < 2) + (im && tim < 2);
-    }
-    {
-        const float a = (tjp == 0 ? 0.0f : xjp) - xI, b = (tjm == 0 ? 0.0f : xjm) - xI;
-        ret = ret + (a * (tjp < 2 ? 1.0f : 0.0f) + b * (tjm < 2 ? 1.0f : 0.0f));
-        d += (jp && tjp < 2) + (jm && tjm < 2);
-    }
-    {
-        const float a = (tkp == 0 ? 0.0f : xkp) - xI, b = (tkm == 0 ? 0.0f : xkm) - xI;
-        ret = ret + (a * (tkp < 2 ? 1.0f : 0.0f) + b * (tkm < 2 ? 1.0f : 0.0f));
-        d += (kp && tkp < 2) + (km && tkm < 2);
-    }
-    if (tI != 1) ret = 0.0f;
-    *divd = tI == 1 ? d : 0;
-    return ret;
-}
-
-struct Geo {
-    int nx, ny, nz;
-    int cny, cnz;  // coarse dims (PROLONG)
-};
-
-// Load the quad of row (i, j) starting at k0 (z values, prolongated when
-// PROLONG, zero when ZERO) and its tags.
-template <int MODE>
-__device__ __forceinline__ void load_quad(const float *z, const float *__restrict__ zc,
-                                          const uint8_t *__restrict__ tag, const Geo &g, int i, int j,
-                                          int k0, Quad &q)
-{
-    const uint32_t base = cidx(i, j, k0, g.ny, g.nz);
-    unpack_tags(*reinterpret_cast<const uint32_t *>(tag + base), q.t);
-    if (MODE == MODE_ZERO) {
-        q.v[0] = q.v[1] = q.v[2] = q.v[3] = 0.0f;
-        return;
-    }
-    const float4 a = *reinterpret_cast<const float4 *>(z + base);
-    q.v[0] = a.x; q.v[1] = a.y; q.v[2] = a.z; q.v[3] = a.w;
-    if (MODE == MODE_PROLONG) {
-        const uint32_t cb = cidx(i >> 1, j >> 1, k0 >> 1, g.cny, g.cnz);
-        const float2 c = *reinterpret_cast<const float2 *>(zc + cb);
-        q.v[0] = q.v[0] + c.x; q.v[1] = q.v[1] + c.x;
-        q.v[2] = q.v[2] + c.y; q.v[3] = q.v[3] + c.y;
-    }
-}
-
-template <int MODE>
-__device__ __forceinline__ float load_one(const float *z, const float *__restrict__ zc,
-                                          const Geo &g, int i, int j, int k)
-{
-    if (MODE == MODE_ZERO) return 0.0f;
-    float v = z[cidx(i, j, k, g.ny, g.nz)];
-    if (MODE == MODE_PROLONG) v = v + zc[cidx(i >> 1, j >> 1, k >> 1, g.cny, g.cnz)];
-    return v;
-}
 
 }  // namespace
 
-// One red-black half-sweep (phase p updates (i+j+k)&1 == p^1) over quads.
-// z is read from zin and the quad written to zout: in place (zin == zout) for
-// PLAIN and ZERO, where a thread rewrites the cells it does not update with
-// the bits it read; out of place for PROLONG, whose rewritten cells change.
-template <int MODE, bool DOT>
-__global__ void __launch_bounds__(256) k_rbgs4(const float *zin, float *zout, const float *__restrict__ r,
-                                               const uint8_t *__restrict__ tag, const float *__restrict__ zc,
-                                               int nx, int ny, int nz, int phase, double *__restrict__ part)
+struct SweepArgs {
+    const void *zin;      // T: z read (unused for ZERO)
+    void *zout;           // T: z written (OUT 0/1)
+    const void *zc;       // T: coarse z (PROLONG)
+    float pfac;           // prolongation factor (1 FP32, 8 mixed)
+    const void *rsrc;     // T: level r, or the FP32 CG residual when SRC32
+    void *rh_out;         // T: SRC32 stores the scaled r here
+    const uint16_t *code;
+    const double *sc;     // device scalars (SC_RSCALE for SRC32 / FINAL)
+    const float *r32, *q32;
+    float *z32;           // FINAL output
+    double *part;
+    Idx g;
+    int phase;
+};
+
+enum { OUT_STORE = 0, OUT_DOT = 1, OUT_FINAL = 2 };
+
+template <typename T, int MODE>
+__device__ __forceinline__ void row4(const SweepArgs &a, int i, int j, int k0, float v[4])
 {
-    __shared__ double lds[4];
-    const Geo g{nx, ny, nz, ny >> 1, nz >> 1};
-    const int nq = nz >> 2;
-    const uint32_t total = (uint32_t)nx * ny * nq;
-    double acc = 0.0;
-    const Chunk ch = xcd_chunk(total);
-    for (uint32_t t = ch.begin + threadIdx.x; t < ch.end; t += 256u) {
-        const uint32_t row = t / nq;
-        const int k0 = (int)(t - row * nq) * 4;
-        const int i = (int)(row / ny), j = (int)(row - (uint32_t)i * ny);
-        Quad c, xp, xm, yp, ym;
-        load_quad<MODE>(zin, zc, tag, g, i, j, k0, c);
-        const bool ip = i + 1 < nx, im = i > 0, jp = j + 1 < ny, jm = j > 0;
-        if (ip) load_quad<MODE>(zin, zc, tag, g, i + 1, j, k0, xp); else xp = c;
-        if (im) load_quad<MODE>(zin, zc, tag, g, i - 1, j, k0, xm); else xm = c;
-        if (jp) load_quad<MODE>(zin, zc, tag, g, i, j + 1, k0, yp); else yp = c;
-        if (jm) load_quad<MODE>(zin, zc, tag, g, i, j - 1, k0, ym); else ym = c;
-        const bool kpo = k0 + 4 < nz, kmo = k0 > 0;
-        const uint32_t base = cidx(i, j, k0, ny, nz);
-        const float zk4 = kpo ? load_one<MODE>(zin, zc, g, i, j, k0 + 4) : 0.0f;
-        const int tk4 = kpo ? tag[base + 4] : 0;
-        const float zkm = kmo ? load_one<MODE>(zin, zc, g, i, j, k0 - 1) : 0.0f;
-        const int tkm = kmo ? tag[base - 1] : 0;
-        const float4 r4 = *reinterpret_cast<const float4 *>(r + base);
-        const float rv[4] = {r4.x, r4.y, r4.z, r4.w};
-        float out[4];
-        const int upd = ((i + j + k0) & 1) ^ phase ^ 1;   // quad offset parity that is updated
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            out[q] = c.v[q];
-            if ((q & 1) != upd) continue;
-            const bool kp = q < 3 ? true : kpo, km = q > 0 ? true : kmo;
-            const float xkp = q < 3 ? c.v[q + 1] : (kpo ? zk4 : c.v[q]);
-            const int tkp = q < 3 ? c.t[q + 1] : (kpo ? tk4 : c.t[q]);
-            const float xkm = q > 0 ? c.v[q - 1] : (kmo ? zkm : c.v[q]);
-            const int tkmv = q > 0 ? c.t[q - 1] : (kmo ? tkm : c.t[q]);
-            int divd;
-            const float nd = ndt7(c.v[q], c.t[q], xp.v[q], xp.t[q], ip, xm.v[q], xm.t[q], im, yp.v[q], yp.t[q], jp,
-                                  ym.v[q], ym.t[q], jm, xkp, tkp, kp, xkm, tkmv, km, &divd);
-            out[q] = divd != 0 ? c.v[q] + (rv[q] + nd) / (float)divd : 0.0f;
-        }
-        *reinterpret_cast<float4 *>(zout + base) = make_float4(out[0], out[1], out[2], out[3]);
-        if (DOT) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) acc += (double)out[q] * (double)rv[q];
-        }
-    }
-    if (DOT) {
-        const double s = block_sum<256>(acc, lds);
-        if (threadIdx.x == 0) part[blockIdx.x] = s;
+    if (MODE == SWEEP_ZERO) { v[0] = v[1] = v[2] = v[3] = 0.0f; return; }
+    ld4<T>(reinterpret_cast<const T *>(a.zin) + cidx(i, j, k0, a.g.ny, a.g.nz), v);
+    if (MODE == SWEEP_PROLONG) {
+        float c[2];
+        ld2<T>(reinterpret_cast<const T *>(a.zc) + cidx(i >> 1, j >> 1, k0 >> 1, a.g.ny >> 1, a.g.nz >> 1), c);
+        v[0] = v[0] + a.pfac * c[0]; v[1] = v[1] + a.pfac * c[0];
+        v[2] = v[2] + a.pfac * c[1]; v[3] = v[3] + a.pfac * c[1];
     }
 }
 
-// Gather restriction (mgpcg.py:94-101) for two coarse cells per thread: the
-// fine quad k0..k0+3 of the four child rows (2I+a, 2J+b).  Residual
-// r - (-ndt(z)) per fine cell; a FLUID parent sums its 8 children in the
-// serial launch order (ii, jj, kk), others get 0.
-__global__ void __launch_bounds__(256) k_restrict4(const float *__restrict__ r, const float *__restrict__ z,
-                                                   const uint8_t *__restrict__ tag, float *__restrict__ rc,
-                                                   const uint8_t *__restrict__ tagc, int nx, int ny, int nz)
+template <typename T, int MODE>
+__device__ __forceinline__ float one(const SweepArgs &a, int i, int j, int k)
 {
-    const Geo g{nx, ny, nz, 0, 0};
-    const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;
-    const int nq = nz >> 2;   // coarse pairs per coarse row
-    const uint32_t total = (uint32_t)cx * cy * nq;
+    if (MODE == SWEEP_ZERO) return 0.0f;
+    float v = ldv<T>(reinterpret_cast<const T *>(a.zin)[cidx(i, j, k, a.g.ny, a.g.nz)]);
+    if (MODE == SWEEP_PROLONG)
+        v = v + a.pfac * ldv<T>(reinterpret_cast<const T *>(a.zc)[cidx(i >> 1, j >> 1, k >> 1, a.g.ny >> 1, a.g.nz >> 1)]);
+    return v;
+}
+
+// One red-black half-sweep (phase p updates (i+j+k)&1 == p^1, mgpcg.py:72-86).
+template <typename T, int MODE, bool SRC32, int OUT>
+__global__ void __launch_bounds__(256) k_sweep(SweepArgs a)
+{
+    __shared__ double lds[4];
+    const Idx g = a.g;
+    const uint32_t total = (uint32_t)g.nx * g.ny * g.nq;
+    const int lane = threadIdx.x & 63;
+    const float s0 = (SRC32 || OUT == OUT_FINAL) ? (float)a.sc[SC_RSCALE] : 1.0f;
+    double acc1 = 0.0, acc2 = 0.0;
     const Chunk ch = xcd_chunk(total);
-    for (uint32_t t = ch.begin + threadIdx.x; t < ch.end; t += 256u) {
-        const uint32_t crow = t / nq;
-        const int kq = (int)(t - crow * nq);
-        const int I = (int)(crow / cy), J = (int)(crow - (uint32_t)I * cy);
-        const int k0 = kq * 4;
-        const uint32_t cb = cidx(I, J, kq * 2, cy, cz);
-        const uint32_t tc2 = *reinterpret_cast<const uint16_t *>(tagc + cb);
-        const int tcp[2] = {(int)(tc2 & 0xff), (int)(tc2 >> 8)};
+    for (uint32_t t = ch.first; t < ch.end; t += ch.stride) {
+        int i, j, k0;
+        g.decode(t, i, j, k0);
+        const uint32_t base = cidx(i, j, k0, g.ny, g.nz);
+        float c[4], xp[4], xm[4], yp[4], ym[4];
+        row4<T, MODE>(a, i, j, k0, c);
+        if (i + 1 < g.nx) row4<T, MODE>(a, i + 1, j, k0, xp); else { xp[0] = c[0]; xp[1] = c[1]; xp[2] = c[2]; xp[3] = c[3]; }
+        if (i > 0) row4<T, MODE>(a, i - 1, j, k0, xm); else { xm[0] = c[0]; xm[1] = c[1]; xm[2] = c[2]; xm[3] = c[3]; }
+        if (j + 1 < g.ny) row4<T, MODE>(a, i, j + 1, k0, yp); else { yp[0] = c[0]; yp[1] = c[1]; yp[2] = c[2]; yp[3] = c[3]; }
+        if (j > 0) row4<T, MODE>(a, i, j - 1, k0, ym); else { ym[0] = c[0]; ym[1] = c[1]; ym[2] = c[2]; ym[3] = c[3]; }
+        // k neighbours from the adjacent lanes (same row), loaded at wavefront edges
+        const float left = __shfl_up(c[3], 1, 64), right = __shfl_down(c[0], 1, 64);
+        float zkm = c[0], zk4 = c[3];
+        if (k0 > 0) zkm = lane > 0 ? left : one<T, MODE>(a, i, j, k0 - 1);
+        if (k0 + 4 < g.nz) zk4 = (lane < 63 && t + 1 < ch.end) ? right : one<T, MODE>(a, i, j, k0 + 4);
+        uint32_t cd[4];
+        codes4(a.code, base, cd);
+        float rv[4];
+        if (SRC32) {
+            const float4 r4 = *reinterpret_cast<const float4 *>(a.r32 + base);
+            rv[0] = r4.x * s0; rv[1] = r4.y * s0; rv[2] = r4.z * s0; rv[3] = r4.w * s0;
+            st4<T>(reinterpret_cast<T *>(a.rh_out) + base, rv);
+            if constexpr (sizeof(T) != 4) {     // continue with the stored (rounded) values
+                rv[0] = ldv<T>(stv<T>(rv[0])); rv[1] = ldv<T>(stv<T>(rv[1]));
+                rv[2] = ldv<T>(stv<T>(rv[2])); rv[3] = ldv<T>(stv<T>(rv[3]));
+            }
+        } else {
+            ld4<T>(reinterpret_cast<const T *>(a.rsrc) + base, rv);
+        }
+        float out[4];
+        const int upd = ((i + j + k0) & 1) ^ a.phase ^ 1;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            out[q] = c[q];
+            if ((q & 1) != upd) continue;
+            const float xkp = q < 3 ? c[q + 1] : zk4;
+            const float xkm = q > 0 ? c[q - 1] : zkm;
+            const float s = nsum_code(cd[q], c[q], xp[q], xm[q], yp[q], ym[q], xkp, xkm);
+            const int divd = code_divd(cd[q]);
+            out[q] = divd != 0 ? c[q] + (rv[q] + s) / (float)divd : 0.0f;
+        }
+        if (OUT == OUT_FINAL) {
+            const float inv = 1.0f / s0;
+            const float4 r4 = *reinterpret_cast<const float4 *>(a.r32 + base);
+            const float4 q4 = *reinterpret_cast<const float4 *>(a.q32 + base);
+            const float o0 = out[0] * inv, o1 = out[1] * inv, o2 = out[2] * inv, o3 = out[3] * inv;
+            *reinterpret_cast<float4 *>(a.z32 + base) = make_float4(o0, o1, o2, o3);
+            acc1 += (double)o0 * r4.x + (double)o1 * r4.y + (double)o2 * r4.z + (double)o3 * r4.w;
+            acc2 += (double)o0 * q4.x + (double)o1 * q4.y + (double)o2 * q4.z + (double)o3 * q4.w;
+        } else {
+            st4<T>(reinterpret_cast<T *>(a.zout) + base, out);
+            if (OUT == OUT_DOT) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc1 += (double)out[q] * (double)rv[q];
+            }
+        }
+    }
+    if (OUT != OUT_STORE) {
+        const double s1 = block_sum<256>(acc1, lds);
+        if (OUT == OUT_FINAL) {
+            const double s2 = block_sum<256>(acc2, lds);
+            if (threadIdx.x == 0) a.part[gridDim.x + blockIdx.x] = s2;
+        }
+        if (threadIdx.x == 0) a.part[blockIdx.x] = s1;
+    }
+}
+
+// Gather restriction (mgpcg.py:94-101), two coarse cells per thread: the fine
+// quad of the four child rows (2I+a, 2J+b).  Residual r - (-ndt(z)) per fine
+// cell; a FLUID parent sums its 8 children in the serial launch order
+// (ii, jj, kk), others get 0.  The coarse value is stored times cscale
+// (1 in FP32; 1/8 for the mixed level scaling).
+template <typename T>
+__global__ void __launch_bounds__(256) k_restrict(const T *__restrict__ r, const T *__restrict__ z,
+                                                  const uint16_t *__restrict__ code, T *__restrict__ rc,
+                                                  const uint8_t *__restrict__ tagc, Idx g, float cscale)
+{
+    // g describes the coarse grid in units of coarse pairs (nq = cz/2 = fine quads per row)
+    const int nx = g.nx * 2, ny = g.ny * 2, nz = g.nz * 2;
+    const int cy = g.ny, cz = g.nz;
+    const uint32_t total = (uint32_t)g.nx * g.ny * g.nq;
+    const int lane = threadIdx.x & 63;
+    const Chunk ch = xcd_chunk(total);
+    for (uint32_t t = ch.first; t < ch.end; t += ch.stride) {
+        int I, J, kc;
+        g.decode(t, I, J, kc);   // kc = 4 * pair index = fine k0
+        const int k0 = kc;
         float res[2][2][4];
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+        for (int aa = 0; aa < 2; ++aa)
 #pragma unroll
             for (int b = 0; b < 2; ++b) {
-                const int i = 2 * I + a, j = 2 * J + b;
-                Quad c, xp, xm, yp, ym;
-                load_quad<MODE_PLAIN>(z, nullptr, tag, g, i, j, k0, c);
-                const bool ip = i + 1 < nx, im = i > 0, jp = j + 1 < ny, jm = j > 0;
-                if (ip) load_quad<MODE_PLAIN>(z, nullptr, tag, g, i + 1, j, k0, xp); else xp = c;
-                if (im) load_quad<MODE_PLAIN>(z, nullptr, tag, g, i - 1, j, k0, xm); else xm = c;
-                if (jp) load_quad<MODE_PLAIN>(z, nullptr, tag, g, i, j + 1, k0, yp); else yp = c;
-                if (jm) load_quad<MODE_PLAIN>(z, nullptr, tag, g, i, j - 1, k0, ym); else ym = c;
+                const int i = 2 * I + aa, j = 2 * J + b;
                 const uint32_t base = cidx(i, j, k0, ny, nz);
-                const bool kpo = k0 + 4 < nz, kmo = k0 > 0;
-                const float zk4 = kpo ? z[base + 4] : 0.0f;
-                const int tk4 = kpo ? tag[base + 4] : 0;
-                const float zkm = kmo ? z[base - 1] : 0.0f;
-                const int tkm = kmo ? tag[base - 1] : 0;
-                const float4 r4 = *reinterpret_cast<const float4 *>(r + base);
-                const float rv[4] = {r4.x, r4.y, r4.z, r4.w};
+                float c[4], xp[4], xm[4], yp[4], ym[4];
+                ld4<T>(z + base, c);
+                if (i + 1 < nx) ld4<T>(z + cidx(i + 1, j, k0, ny, nz), xp); else { xp[0] = c[0]; xp[1] = c[1]; xp[2] = c[2]; xp[3] = c[3]; }
+                if (i > 0) ld4<T>(z + cidx(i - 1, j, k0, ny, nz), xm); else { xm[0] = c[0]; xm[1] = c[1]; xm[2] = c[2]; xm[3] = c[3]; }
+                if (j + 1 < ny) ld4<T>(z + cidx(i, j + 1, k0, ny, nz), yp); else { yp[0] = c[0]; yp[1] = c[1]; yp[2] = c[2]; yp[3] = c[3]; }
+                if (j > 0) ld4<T>(z + cidx(i, j - 1, k0, ny, nz), ym); else { ym[0] = c[0]; ym[1] = c[1]; ym[2] = c[2]; ym[3] = c[3]; }
+                const float left = __shfl_up(c[3], 1, 64), right = __shfl_down(c[0], 1, 64);
+                float zkm = c[0], zk4 = c[3];
+                if (k0 > 0) zkm = lane > 0 ? left : ldv<T>(z[base - 1]);
+                if (k0 + 4 < nz) zk4 = (lane < 63 && t + 1 < ch.end) ? right : ldv<T>(z[base + 4]);
+                uint32_t cd[4];
+                codes4(code, base, cd);
+                float rv[4];
+                ld4<T>(r + base, rv);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const bool kp = q < 3 ? true : kpo, km = q > 0 ? true : kmo;
-                    const float xkp = q < 3 ? c.v[q + 1] : (kpo ? zk4 : c.v[q]);
-                    const int tkp = q < 3 ? c.t[q + 1] : (kpo ? tk4 : c.t[q]);
-                    const float xkm = q > 0 ? c.v[q - 1] : (kmo ? zkm : c.v[q]);
-                    const int tkmv = q > 0 ? c.t[q - 1] : (kmo ? tkm : c.t[q]);
-                    int divd;
-                    const float nd = ndt7(c.v[q], c.t[q], xp.v[q], xp.t[q], ip, xm.v[q], xm.t[q], im, yp.v[q],
-                                          yp.t[q], jp, ym.v[q], ym.t[q], jm, xkp, tkp, kp, xkm, tkmv, km, &divd);
-                    res[a][b][q] = rv[q] - (-nd);
+                    const float xkp = q < 3 ? c[q + 1] : zk4;
+                    const float xkm = q > 0 ? c[q - 1] : zkm;
+                    float nd = nsum_code(cd[q], c[q], xp[q], xm[q], yp[q], ym[q], xkp, xkm);
+                    if (!code_fluid(cd[q])) nd = 0.0f;
+                    res[aa][b][q] = rv[q] - (-nd);
                 }
             }
-        float outv[2];
+        const uint32_t cb = cidx(I, J, k0 >> 1, cy, cz);
+        const uint32_t tc2 = *reinterpret_cast<const uint16_t *>(tagc + cb);
+        float o[2];
 #pragma unroll
         for (int cc = 0; cc < 2; ++cc) {
             float s = 0.0f;
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+            for (int aa = 0; aa < 2; ++aa)
 #pragma unroll
                 for (int b = 0; b < 2; ++b)
 #pragma unroll
-                    for (int kk = 0; kk < 2; ++kk) s += res[a][b][2 * cc + kk];
-            outv[cc] = tcp[cc] == 1 ? s : 0.0f;
+                    for (int kk = 0; kk < 2; ++kk) s += res[aa][b][2 * cc + kk];
+            const int tp = cc ? (int)(tc2 >> 8) : (int)(tc2 & 0xff);
+            o[cc] = tp == 1 ? s * cscale : 0.0f;
         }
-        *reinterpret_cast<float2 *>(rc + cb) = make_float2(outv[0], outv[1]);
+        if constexpr (sizeof(T) == 4) {
+            *reinterpret_cast<float2 *>(rc + cb) = make_float2(o[0], o[1]);
+        } else {
+            rc[cb] = stv<T>(o[0]);
+            rc[cb + 1] = stv<T>(o[1]);
+        }
     }
 }
 
 // p_new = z + beta*p_old at every stencil point (mgpcg.py:229-232), then
 // Ap = ndt(p_new) (mgpcg.py:36-39) and the FP64 partial of p_new.Ap; p_new is
-// stored to pout (a different buffer from p: neighbours still read p_old).
-__global__ void __launch_bounds__(256) k_p_Ap4(const float *__restrict__ p, float *__restrict__ pout,
-                                               float *__restrict__ Ap,
-                                               const float *__restrict__ z, const uint8_t *__restrict__ tag,
-                                               const double *__restrict__ sc, int nx, int ny, int nz,
-                                               double *__restrict__ part)
+// stored to pout (not p: the neighbours still read p_old).
+__global__ void __launch_bounds__(256) k_p_Ap(const float *__restrict__ p, float *__restrict__ pout,
+                                              float *__restrict__ Ap, const float *__restrict__ z,
+                                              const uint16_t *__restrict__ code, const double *__restrict__ sc,
+                                              Idx g, double *__restrict__ part)
 {
     __shared__ double lds[4];
     const float beta = (float)sc[SC_BETA];
-    const Geo g{nx, ny, nz, 0, 0};
-    const int nq = nz >> 2;
-    const uint32_t total = (uint32_t)nx * ny * nq;
+    const uint32_t total = (uint32_t)g.nx * g.ny * g.nq;
+    const int lane = threadIdx.x & 63;
     double acc = 0.0;
-    auto pq = [&](int i, int j, int k0, Quad &q) {
-        const uint32_t base = cidx(i, j, k0, g.ny, g.nz);
-        unpack_tags(*reinterpret_cast<const uint32_t *>(tag + base), q.t);
-        const float4 a = *reinterpret_cast<const float4 *>(p + base);
-        const float4 b = *reinterpret_cast<const float4 *>(z + base);
-        q.v[0] = fmaf(beta, a.x, b.x); q.v[1] = fmaf(beta, a.y, b.y);
-        q.v[2] = fmaf(beta, a.z, b.z); q.v[3] = fmaf(beta, a.w, b.w);
+    auto pq = [&](int i, int j, int k0, float v[4]) {
+        const uint32_t b = cidx(i, j, k0, g.ny, g.nz);
+        const float4 a = *reinterpret_cast<const float4 *>(p + b);
+        const float4 zz = *reinterpret_cast<const float4 *>(z + b);
+        v[0] = fmaf(beta, a.x, zz.x); v[1] = fmaf(beta, a.y, zz.y);
+        v[2] = fmaf(beta, a.z, zz.z); v[3] = fmaf(beta, a.w, zz.w);
     };
     const Chunk ch = xcd_chunk(total);
-    for (uint32_t t = ch.begin + threadIdx.x; t < ch.end; t += 256u) {
-        const uint32_t row = t / nq;
-        const int k0 = (int)(t - row * nq) * 4;
-        const int i = (int)(row / ny), j = (int)(row - (uint32_t)i * ny);
-        Quad c, xp, xm, yp, ym;
+    for (uint32_t t = ch.first; t < ch.end; t += ch.stride) {
+        int i, j, k0;
+        g.decode(t, i, j, k0);
+        const uint32_t base = cidx(i, j, k0, g.ny, g.nz);
+        float c[4], xp[4], xm[4], yp[4], ym[4];
         pq(i, j, k0, c);
-        const bool ip = i + 1 < nx, im = i > 0, jp = j + 1 < ny, jm = j > 0;
-        if (ip) pq(i + 1, j, k0, xp); else xp = c;
-        if (im) pq(i - 1, j, k0, xm); else xm = c;
-        if (jp) pq(i, j + 1, k0, yp); else yp = c;
-        if (jm) pq(i, j - 1, k0, ym); else ym = c;
-        const uint32_t base = cidx(i, j, k0, ny, nz);
-        const bool kpo = k0 + 4 < nz, kmo = k0 > 0;
-        const float zk4 = kpo ? fmaf(beta, p[base + 4], z[base + 4]) : 0.0f;
-        const int tk4 = kpo ? tag[base + 4] : 0;
-        const float zkm = kmo ? fmaf(beta, p[base - 1], z[base - 1]) : 0.0f;
-        const int tkm = kmo ? tag[base - 1] : 0;
+        if (i + 1 < g.nx) pq(i + 1, j, k0, xp); else { xp[0] = c[0]; xp[1] = c[1]; xp[2] = c[2]; xp[3] = c[3]; }
+        if (i > 0) pq(i - 1, j, k0, xm); else { xm[0] = c[0]; xm[1] = c[1]; xm[2] = c[2]; xm[3] = c[3]; }
+        if (j + 1 < g.ny) pq(i, j + 1, k0, yp); else { yp[0] = c[0]; yp[1] = c[1]; yp[2] = c[2]; yp[3] = c[3]; }
+        if (j > 0) pq(i, j - 1, k0, ym); else { ym[0] = c[0]; ym[1] = c[1]; ym[2] = c[2]; ym[3] = c[3]; }
+        const float left = __shfl_up(c[3], 1, 64), right = __shfl_down(c[0], 1, 64);
+        float zkm = c[0], zk4 = c[3];
+        if (k0 > 0) zkm = lane > 0 ? left : fmaf(beta, p[base - 1], z[base - 1]);
+        if (k0 + 4 < g.nz) zk4 = (lane < 63 && t + 1 < ch.end) ? right : fmaf(beta, p[base + 4], z[base + 4]);
+        uint32_t cd[4];
+        codes4(code, base, cd);
         float out[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const bool kp = q < 3 ? true : kpo, km = q > 0 ? true : kmo;
-            const float xkp = q < 3 ? c.v[q + 1] : (kpo ? zk4 : c.v[q]);
-            const int tkp = q < 3 ? c.t[q + 1] : (kpo ? tk4 : c.t[q]);
-            const float xkm = q > 0 ? c.v[q - 1] : (kmo ? zkm : c.v[q]);
-            const int tkmv = q > 0 ? c.t[q - 1] : (kmo ? tkm : c.t[q]);
-            int divd;
-            out[q] = ndt7(c.v[q], c.t[q], xp.v[q], xp.t[q], ip, xm.v[q], xm.t[q], im, yp.v[q], yp.t[q], jp,
-                          ym.v[q], ym.t[q], jm, xkp, tkp, kp, xkm, tkmv, km, &divd);
-            acc += (double)c.v[q] * (double)out[q];
+            const float xkp = q < 3 ? c[q + 1] : zk4;
+            const float xkm = q > 0 ? c[q - 1] : zkm;
+            const float s = nsum_code(cd[q], c[q], xp[q], xm[q], yp[q], ym[q], xkp, xkm);
+            out[q] = code_fluid(cd[q]) ? s : 0.0f;
+            acc += (double)c[q] * (double)out[q];
         }
-        *reinterpret_cast<float4 *>(pout + base) = make_float4(c.v[0], c.v[1], c.v[2], c.v[3]);
+        *reinterpret_cast<float4 *>(pout + base) = make_float4(c[0], c[1], c[2], c[3]);
         *reinterpret_cast<float4 *>(Ap + base) = make_float4(out[0], out[1], out[2], out[3]);
     }
     const double s = block_sum<256>(acc, lds);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// Stencil codes of a level from its one-byte tags.
+__global__ void k_make_codes(const uint8_t *__restrict__ tag, uint16_t *__restrict__ code, int nx, int ny, int nz)
+{
+    const uint32_t n = (uint32_t)nx * ny * nz;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int k = t % nz, j = (t / nz) % ny, i = t / ((uint32_t)nz * ny);
+    const int self = tag[t];
+    uint32_t cd = 0, divd = 0;
+    const int ni[6] = {i + 1, i - 1, i, i, i, i}, nj[6] = {j, j, j + 1, j - 1, j, j}, nk[6] = {k, k, k, k, k + 1, k - 1};
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        uint32_t cls = 0;
+        if (ni[d] >= 0 && ni[d] < nx && nj[d] >= 0 && nj[d] < ny && nk[d] >= 0 && nk[d] < nz) {
+            const int tn = tag[cidx(ni[d], nj[d], nk[d], ny, nz)];
+            cls = tn == 0 ? 1u : (tn == 1 ? 2u : 0u);
+            divd += tn < 2;
+        }
+        cd |= cls << (2 * d);
+    }
+    if (self == 1) cd |= (divd << 12) | 0x8000u;
+    code[t] = (uint16_t)cd;
+}
+
 // ------------------------------------------------------------------ host
+static int ilog2_exact(int v)
+{
+    if (v <= 0 || (v & (v - 1))) return -1;
+    int l = 0;
+    while ((1 << l) < v) ++l;
+    return l;
+}
+
+static Idx make_idx(int nx, int ny, int nz)
+{
+    Idx g;
+    g.nx = nx; g.ny = ny; g.nz = nz; g.nq = nz / 4;
+    g.lq = ilog2_exact(g.nq);
+    g.lny = ilog2_exact(ny);
+    return g;
+}
+
 // Workgroup count: one item per thread up to MAXBLK workgroups, rounded up to a
-// multiple of 8 (xcd_chunk's remap is a bijection on multiples of 8).
+// multiple of 8 (xcd_chunk's remap needs it).
 static unsigned grid_for(uint32_t items)
 {
     unsigned b = blocks_for(items, 256);
@@ -307,28 +417,79 @@ static unsigned grid_for(uint32_t items)
     return (b + 7u) & ~7u;
 }
 
-int fast_sweep(hipStream_t s, const Level &L, const float *zin, float *zout, const float *zc, int phase, int mode,
-               double *part, int *nblk)
+int make_codes(hipStream_t s, const Level &L)
 {
-    const uint32_t quads = L.n / 4;
-    const unsigned nb = grid_for(quads);
-    if (nblk) *nblk = (int)nb;
-    const bool dot = part != nullptr;
-#define SW(M, D) k_rbgs4<M, D><<<nb, 256, 0, s>>>(zin, zout, L.r, L.tag, zc, L.nx, L.ny, L.nz, phase, part)
-    if (mode == MODE_ZERO) { if (dot) SW(MODE_ZERO, true); else SW(MODE_ZERO, false); }
-    else if (mode == MODE_PROLONG) { if (dot) SW(MODE_PROLONG, true); else SW(MODE_PROLONG, false); }
-    else { if (dot) SW(MODE_PLAIN, true); else SW(MODE_PLAIN, false); }
-#undef SW
+    k_make_codes<<<blocks_for(L.n, 256), 256, 0, s>>>(L.tag, L.code, L.nx, L.ny, L.nz);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
 }
 
-int fast_restrict(hipStream_t s, const Level &F, const float *zf, const Level &C)
+template <typename T, int MODE, bool SRC32, int OUT>
+static void launch_k(hipStream_t s, unsigned nb, const SweepArgs &a)
 {
-    const uint32_t items = (uint32_t)C.nx * C.ny * (F.nz / 4);
-    k_restrict4<<<grid_for(items), 256, 0, s>>>(F.r, zf, F.tag, C.r, C.tag, F.nx, F.ny, F.nz);
+    k_sweep<T, MODE, SRC32, OUT><<<nb, 256, 0, s>>>(a);
+}
+
+template <typename T>
+static int sweep_t(hipStream_t s, const Level &L, const SweepInfo &si, int *nblk)
+{
+    SweepArgs a{};
+    a.zin = si.zin; a.zout = si.zout; a.zc = si.zc; a.pfac = si.pfac;
+    a.rsrc = si.rsrc; a.rh_out = si.rh_out; a.code = L.code; a.sc = si.sc;
+    a.r32 = si.r32; a.q32 = si.q32; a.z32 = si.z32; a.part = si.part;
+    a.g = make_idx(L.nx, L.ny, L.nz);
+    a.phase = si.phase;
+    const unsigned nb = grid_for(L.n / 4);
+    if (nblk) *nblk = (int)nb;
+    const int m = si.mode, o = si.out;
+    if (m == SWEEP_ZERO && si.src32) launch_k<T, SWEEP_ZERO, true, OUT_STORE>(s, nb, a);
+    else if (m == SWEEP_ZERO) launch_k<T, SWEEP_ZERO, false, OUT_STORE>(s, nb, a);
+    else if (m == SWEEP_PROLONG) launch_k<T, SWEEP_PROLONG, false, OUT_STORE>(s, nb, a);
+    else if (o == OUT_DOT) launch_k<T, SWEEP_PLAIN, false, OUT_DOT>(s, nb, a);
+    else if (o == OUT_FINAL) launch_k<T, SWEEP_PLAIN, false, OUT_FINAL>(s, nb, a);
+    else launch_k<T, SWEEP_PLAIN, false, OUT_STORE>(s, nb, a);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
+}
+
+int sweep(hipStream_t s, const Level &L, int prec, const SweepInfo &si, int *nblk)
+{
+    if (prec == FLUB_PREC_FP16) return sweep_t<__half>(s, L, si, nblk);
+    if (prec == FLUB_PREC_BF16) return sweep_t<__hip_bfloat16>(s, L, si, nblk);
+    return sweep_t<float>(s, L, si, nblk);
+}
+
+int restrict_level(hipStream_t s, const Level &F, const void *rf, const void *zf, const Level &C, void *rc, int prec)
+{
+    const Idx g = make_idx(C.nx, C.ny, C.nz);   // nq = cz/2 = fine quads per row
+    Idx gc = g;
+    gc.nq = F.nz / 4;
+    gc.lq = ilog2_exact(gc.nq);
+    const uint32_t items = (uint32_t)C.nx * C.ny * gc.nq;
+    const unsigned nb = grid_for(items);
+    if (prec == FLUB_PREC_FP16)
+        k_restrict<__half><<<nb, 256, 0, s>>>((const __half *)rf, (const __half *)zf, F.code, (__half *)rc, C.tag, gc, 0.125f);
+    else if (prec == FLUB_PREC_BF16)
+        k_restrict<__hip_bfloat16><<<nb, 256, 0, s>>>((const __hip_bfloat16 *)rf, (const __hip_bfloat16 *)zf, F.code,
+                                                      (__hip_bfloat16 *)rc, C.tag, gc, 0.125f);
+    else
+        k_restrict<float><<<nb, 256, 0, s>>>((const float *)rf, (const float *)zf, F.code, (float *)rc, C.tag, gc, 1.0f);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+int fast_sweep(hipStream_t s, const Level &L, const float *zin, float *zout, const float *zc, int phase, int mode,
+               double *part, int *nblk)
+{
+    SweepInfo si{};
+    si.zin = zin; si.zout = zout; si.zc = zc; si.pfac = 1.0f; si.rsrc = L.r;
+    si.phase = phase; si.mode = mode; si.out = part ? OUT_DOT : OUT_STORE; si.part = part;
+    return sweep(s, L, FLUB_PREC_FP32, si, nblk);
+}
+
+int fast_restrict(hipStream_t s, const Level &F, const float *zf, const Level &C)
+{
+    return restrict_level(s, F, F.r, zf, C, C.r, FLUB_PREC_FP32);
 }
 
 int fast_p_Ap(hipStream_t s, const Level &L0, const float *p, float *pout, float *Ap, const double *sc,
@@ -336,7 +497,7 @@ int fast_p_Ap(hipStream_t s, const Level &L0, const float *p, float *pout, float
 {
     const unsigned nb = grid_for(L0.n / 4);
     *nblk = (int)nb;
-    k_p_Ap4<<<nb, 256, 0, s>>>(p, pout, Ap, L0.z, L0.tag, sc, L0.nx, L0.ny, L0.nz, part);
+    k_p_Ap<<<nb, 256, 0, s>>>(p, pout, Ap, L0.z, L0.code, sc, make_idx(L0.nx, L0.ny, L0.nz), part);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
 }

@@ -70,6 +70,7 @@ def _declare(L):
             "orc_mgpcg_destroy": (None, [_P]),
             "orc_mgpcg_config": (None, [_P, _I, _I, _I, _I]),
             "orc_mgpcg_buf": (_P, [_P, _I, _I]),
+            "orc_mgpcg_growth": (None, [_P, _I]),
             "orc_mgpcg_tag": (_P, [_P, _I]),
             "orc_restrict_tag": (None, [_P, _P, _I, _I, _I]),
             "orc_mgpcg_set_tags": (None, [_P, _P]),
@@ -256,6 +257,10 @@ class MGPCG:
         if h:
             _fn("orc_mgpcg_destroy", self.real)(h)
             self._h = None
+
+    def set_growth(self, growth):
+        """Smoothing count growth per level (1: the reference's << l; 0: constant)."""
+        _fn("orc_mgpcg_growth", self.real)(self._h, int(growth))
 
     def level_shape(self, l):
         return tuple(s >> l for s in self.shape)

@@ -335,7 +335,7 @@ static double FN(inner_par)(const REAL *a, const REAL *b, size_t n)
  * MGPCG (mgpcg.py:234-490), same levels, buffers and control flow.
  * ------------------------------------------------------------------------- */
 typedef struct {
-    int L, pre_post, bottom;
+    int L, pre_post, bottom, growth;   /* sweeps at level l: pre_post << (growth * l) */
     int nx[ORC_MAXL], ny[ORC_MAXL], nz[ORC_MAXL];
     size_t n[ORC_MAXL];
     REAL *r[ORC_MAXL], *z[ORC_MAXL], *divd[ORC_MAXL];
@@ -353,6 +353,7 @@ FN(orc_mgpcg) *FN(orc_mgpcg_create)(int nx, int ny, int nz, int levels)
     m->L = levels;
     m->pre_post = 6;
     m->bottom = 6;
+    m->growth = 1;
     m->dot_f64 = 1;
     for (int l = 0; l < levels; ++l) {
         m->nx[l] = nx >> l; m->ny[l] = ny >> l; m->nz[l] = nz >> l;
@@ -379,6 +380,10 @@ void FN(orc_mgpcg_config)(FN(orc_mgpcg) *m, int pre_post, int bottom, int dot_f6
 {
     m->pre_post = pre_post; m->bottom = bottom; m->dot_f64 = dot_f64; m->parallel_dot = parallel_dot;
 }
+
+/* Smoothing growth per level: 1 = the reference's pre_and_post_smoothing << l
+ * (mgpcg.py:304,343), 0 = the same count on every level. */
+void FN(orc_mgpcg_growth)(FN(orc_mgpcg) *m, int growth) { m->growth = growth; }
 
 REAL *FN(orc_mgpcg_buf)(FN(orc_mgpcg) *m, int which, int level)
 {
@@ -453,7 +458,7 @@ void FN(orc_mgpcg_apply_preconditioner)(FN(orc_mgpcg) *m, int smoother)
 {
     memset(m->z[0], 0, m->n[0] * sizeof(REAL));
     for (int l = 0; l + 1 < m->L; ++l) {
-        for (int it = 0; it < (m->pre_post << l); ++it) {
+        for (int it = 0; it < (m->pre_post << (m->growth * l)); ++it) {
             FN(half_sweep)(m, l, smoother, 0);
             FN(half_sweep)(m, l, smoother, 1);
         }
@@ -476,7 +481,7 @@ void FN(orc_mgpcg_apply_preconditioner)(FN(orc_mgpcg) *m, int smoother)
     }
     for (int l = m->L - 2; l >= 0; --l) {
         FN(orc_prolongate)(m->z[l], m->z[l + 1], m->nx[l], m->ny[l], m->nz[l]);
-        for (int it = 0; it < (m->pre_post << l); ++it) {
+        for (int it = 0; it < (m->pre_post << (m->growth * l)); ++it) {
             FN(half_sweep)(m, l, smoother, 1);
             FN(half_sweep)(m, l, smoother, 0);
         }
