@@ -1114,3 +1114,27 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
     if (bytes) *bytes = b;
     return rc;
 }
+
+// ---------------------------------------------------------------- caller buffers
+FLUB_API int flub_mgpcg_apply_A(flub_mgpcg *h, const float *x, float *y)
+{
+    FLUB_REQUIRE(h && x && y, FLUB_EINVAL, "flub_mgpcg_apply_A: null argument");
+    FLUB_REQUIRE(h->tags_set, FLUB_ESTATE, "apply_A before set_tags");
+    Level &L0 = h->lv[0];
+    const int nb = red_blocks(L0.n);
+    k_Ap_dot<true><<<nb, RBS, 0, h->s>>>(y, x, L0.tag, L0.nx, L0.ny, L0.nz, h->part);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+FLUB_API int flub_mgpcg_precondition_into(flub_mgpcg *h, const float *r, float *z, int smoother)
+{
+    FLUB_REQUIRE(h && r && z && smoother_ok(smoother), FLUB_EINVAL, "flub_mgpcg_precondition_into: bad argument");
+    FLUB_REQUIRE(h->tags_set, FLUB_ESTATE, "precondition before set_tags");
+    Level &L0 = h->lv[0];
+    FLUB_CHECK_HIP(hipMemcpyAsync(L0.r, r, (size_t)L0.n * 4, hipMemcpyDeviceToDevice, h->s));
+    if (h->prec != FLUB_PREC_FP32) TRY(mg_mixed_rscale(h));
+    TRY(mg_precondition(h, smoother, true));
+    FLUB_CHECK_HIP(hipMemcpyAsync(z, L0.z, (size_t)L0.n * 4, hipMemcpyDeviceToDevice, h->s));
+    return FLUB_OK;
+}

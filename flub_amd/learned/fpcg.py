@@ -1,0 +1,107 @@
+"""Flexible PCG with the learned preconditioner and MGPCG as fallback and judge.
+
+Solves neighbor_dot_tag(x) = b (the reference's system, mgpcg.py:378-462,
+same sign conventions: q = neighbor_dot_tag(p) = -A p, the preconditioner
+approximates A^-1) with the Polak-Ribiere beta of flexible CG, FP64
+reductions and the reference's squared-norm stopping test r.r < max(abs_tol,
+rel_tol * r0.r0).
+
+Preconditioner per iteration:
+  * "net"    -- z = s * U-Net([r/s, FLUID]) (s = RMS of r on FLUID cells);
+  * "vcycle" -- the library's V-cycle (the MGPCG preconditioner).
+The solve starts on the network and falls back to the V-cycle for good as
+soon as one iteration reduces |r| by less than the factor `stagnation`
+(MGPCG as automatic fallback).  At the end the FP64 true residual of x is
+recomputed with the library's operator (MGPCG as judge of the error).
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+
+import torch
+
+from .. import _lib
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class LearnedPCG:
+    def __init__(self, mg, net, stagnation=0.5, max_net_iters=1 << 30):
+        self.mg = mg
+        self.net = net
+        self.stagnation = stagnation
+        self.max_net_iters = max_net_iters
+
+    def apply_A(self, x, y):
+        _lib.call("flub_mgpcg_apply_A", self.mg.handle, _p(x), _p(y))
+
+    def vcycle(self, r, z):
+        _lib.call("flub_mgpcg_precondition_into", self.mg.handle, _p(r), _p(z), 1)
+
+    def net_precondition(self, r, fluid):
+        s = torch.sqrt((r[fluid].double() ** 2).mean()).clamp_min(1e-300).float()
+        inp = torch.stack([torch.where(fluid, r / s, torch.zeros_like(r)), fluid.float()], dim=-1)[None]
+        z = self.net.infer(inp)[0] * s
+        return torch.where(fluid, z, torch.zeros_like(z)).contiguous()
+
+    @staticmethod
+    def dot(a, b):
+        return float(torch.dot(a.view(-1).double(), b.view(-1).double()))
+
+    def solve(self, b, fluid, max_iters=-1, rel_tol=1e-12, abs_tol=0.0, eps=1e-12, use_net=True):
+        """b: FP32 right-hand side (FLUID-masked); fluid: bool mask.  Returns
+        (x, stats)."""
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x = torch.zeros_like(b)
+        r = b.clone()
+        q = torch.empty_like(b)
+        z = torch.empty_like(b)
+        rr0 = self.dot(r, r)
+        tol = max(abs_tol, rr0 * rel_tol)
+        mode = "net" if use_net and self.net is not None else "vcycle"
+        stats = {"net_iters": 0, "vcycle_iters": 0, "fallback_at": None}
+
+        def precond():
+            nonlocal z
+            if mode == "net":
+                z = self.net_precondition(r, fluid)
+                stats["net_iters"] += 1
+            else:
+                self.vcycle(r, z)
+                stats["vcycle_iters"] += 1
+
+        precond()
+        p = z.clone()
+        zr = self.dot(z, r)
+        it = 0
+        rr = rr0
+        cap = max_iters if max_iters >= 0 else 20000
+        while it < cap:
+            self.apply_A(p, q)
+            alpha = zr / (self.dot(p, q) + eps)
+            x.add_(p, alpha=alpha)
+            r.add_(q, alpha=-alpha)
+            rr_new = self.dot(r, r)
+            if not (rr_new == rr_new) or rr_new in (float("inf"),):
+                raise FloatingPointError(f"non-finite residual at iteration {it + 1}")
+            if mode == "net" and (rr_new > self.stagnation ** 2 * rr or stats["net_iters"] >= self.max_net_iters):
+                mode = "vcycle"
+                stats["fallback_at"] = it + 1
+            rr = rr_new
+            if rr < tol:
+                break
+            precond()
+            beta = -alpha * self.dot(z, q) / (zr + eps)
+            zr = self.dot(z, r)
+            p = z + beta * p
+            it += 1
+        # judge: FP64 true residual with the library's operator
+        self.apply_A(x, q)
+        true_rr = float(((b.double() - q.double()) ** 2).sum())
+        torch.cuda.synchronize()
+        stats.update(iterations=it, rTr=rr, true_rTr=true_rr, rTr0=rr0, ms=(time.perf_counter() - t0) * 1e3)
+        return x, stats

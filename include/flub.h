@@ -132,6 +132,12 @@ int flub_mgpcg_inner(flub_mgpcg *h, int buf_a, int buf_b, int scalar_out);
  * of every iteration as the reference does. */
 int flub_mgpcg_solve(flub_mgpcg *h, int max_iters, double eps, double abs_tol, double rel_tol,
                      int smoother, int verbose, int *iters, double *rTr);
+/* compute_Ap_mcadams(Ap, p, tag) (mgpcg.py:36-39) on caller buffers of the
+ * level-0 shape with the handle's tags: y = neighbor_dot_tag(x) (= -A x). */
+int flub_mgpcg_apply_A(flub_mgpcg *h, const float *x, float *y);
+/* One preconditioner application on caller buffers: r[0] <- r, V-cycle
+ * (apply_preconditioner, mgpcg.py:296-349), z <- z[0]. */
+int flub_mgpcg_precondition_into(flub_mgpcg *h, const float *r, float *z, int smoother);
 /* Device pointers of the handle's buffers (x, p, Ap, r[l], z[l], tag[l]). */
 int flub_mgpcg_buffer(flub_mgpcg *h, int which, int level, void **ptr);
 /* divd[l] of the reference (mgpcg.py:51-58), computed from the packed tags into
@@ -160,6 +166,26 @@ int flub_project(flub_mgpcg *h, float *u, float *v, float *w, const float *u_rig
                  const float *v_rigid, const float *w_rigid, const int32_t *tag, float h_cell,
                  int max_iters, double abs_tol, double rel_tol, int smoother, float *p, float *div,
                  int *iters, double *rTr);
+
+/* ---------------------------------------------------------------------------
+ * Learned preconditioner kernels (no counterpart in the reference).  BF16
+ * channels-last tensors x[n][d][h][w][c]; weights wt[cout][27*cin] (taps
+ * (dz,dy,dx) row-major, cin fastest); FP32 accumulation on the matrix cores.
+ * Supported (cin, cout): (16,16) (16,32) (32,32) (32,64) (64,64) (96,32)
+ * (48,16) and the swapped pairs used by the data gradient.
+ * ------------------------------------------------------------------------- */
+/* 3x3x3 "same" convolution, optional bias (FP32, may be NULL) and ReLU; writes
+ * BF16 y, or FP32 y32 when y32 is not NULL.  The data gradient is this call
+ * on dY with the taps flipped and the channel roles swapped. */
+int flub_conv3d_bf16(const void *x, const void *wt, const float *bias, void *y, float *y32, int nb, int d, int h,
+                     int w, int cin, int cout, int relu, void *stream);
+/* Weight gradient dw[cout][27*cin] (FP32) = sum over voxels of dy (x) x,
+ * reduced deterministically through the caller's FP32 workspace
+ * (work_floats >= 4 * cout * 27 * cin). */
+int flub_conv3d_wgrad_bf16(const void *x, const void *dy, float *dw, float *work, size_t work_floats, int nb, int d,
+                           int h, int w, int cin, int cout, void *stream);
+/* 1x1 output head with one channel: y[v] = b + sum_c x[v][c] w[c] (FP32 out). */
+int flub_head_fwd(const void *x, const float *w, float b, float *y, size_t nvox, int c, void *stream);
 
 #ifdef __cplusplus
 }
