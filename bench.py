@@ -1,18 +1,27 @@
 """Benchmark: ms per pressure projection to 1e-6 relative residual on one MI355X.
 
 Workload (default): flow past a sphere on a 256^3 MAC grid -- solid sphere of
-radius 0.15 at the centre, inflow u=1 through a SOLID ghost layer at x-min,
+radius 0.15 at the centre, inflow u = 1 through a SOLID ghost layer at x-min,
 outflow through an AIR layer at x-max, u = 1 + N(0, 0.1), v, w = N(0, 0.1)
-(seed 2).  One step = one full projection (divergence, MGPCG solve to
-||r||/||r0|| < 1e-6, gradient subtraction) with inputs resident in HBM; the
-velocity inputs are restored by a device copy outside the timed region
-before every step.  Prints one JSON line.
+(seed 2).  One step = one full projection (divergence, pressure solve to
+||r|| / ||r0|| < 1e-6, gradient subtraction) with the inputs resident in HBM;
+the velocity inputs are restored by a device copy outside the timed region
+before every step.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--solver fp32]
+Solvers, each timed over the same steps: "fp32" (the reference MGPCG:
+4-level V-cycle, 2 RB-GS pre/post sweeps with the reference's doubling per
+level, 6 bottom sweeps), "fp16" / "bf16" (the same V-cycle with FP16/BF16
+storage inside FP32 flexible CG with FP64 reductions) and "learned" (U-Net
+preconditioner in flexible PCG with V-cycle fallback, weights
+flub_amd/learned/weights/unet_128.npz).  `value` is the fastest solver's
+ms per projection; every solver's figures are in "solvers".
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--solvers fp32 fp16 bf16 learned]
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -24,6 +33,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+# HBM bytes per launch of the level-0 FP32 half-sweep at 256^3, from two
+# rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of `bench.py --kernels-only`,
+# calibrated on a 67.4 MB device copy (profiles/rb_half_sweep_l0_traffic_256.md)
+MEASURED_TRAFFIC_L0 = {256: 242.7e6}
 
 
 def log(*a):
@@ -37,9 +51,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--levels", type=int, default=4)
-    ap.add_argument("--solver", default="fp32", choices=["fp32", "fp16", "bf16"])
+    ap.add_argument("--solvers", nargs="+", default=["fp32", "fp16", "bf16", "learned"])
+    ap.add_argument("--learned-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only replay the level-0 half-sweep (for counter collection)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
@@ -53,42 +70,88 @@ def main():
     t0 = time.time()
     s = scenes.flow_past_sphere(n, 2)
     log(f"scene {n}^3 built in {time.time() - t0:.1f}s; fluid cells {(s['tag'] == 1).sum()}")
-    pr = Projection((n, n, n), n_mg_levels=L, pre_and_post_smoothing=2, bottom_smoothing=6,
-                    precision=args.solver)
-    pr.load(s)
-    # pristine copies of the inputs in HBM
     src = [device.from_numpy(s[c]) for c in ("u", "v", "w")]
-    dst = [pr.vel0.u.f, pr.vel0.v.f, pr.vel0.w.f]
 
-    def reset():
-        for a, b in zip(dst, src):
-            device.copy(a, b)
-
-    iters = []
-    for i in range(args.warmup):
-        reset()
-        pr.project(max_iters=-1, rel_tol=1e-12)
-        log(f"warmup {i}: {pr.iterations} iterations")
-    times = []
-    for i in range(args.steps):
-        reset()
+    if args.kernels_only:
+        pr = Projection((n, n, n), n_mg_levels=L, pre_and_post_smoothing=2, bottom_smoothing=6)
+        pr.load(s)
+        pr.mgpcg.set_tags(pr.tag.f)
+        # a copy of known size (calibrates FETCH_SIZE / WRITE_SIZE)
+        device.copy(pr.vel0.u.f, src[0])
         device.synchronize()
-        a = time.perf_counter()
-        pr.project(max_iters=-1, rel_tol=1e-12)
-        device.synchronize()
-        times.append((time.perf_counter() - a) * 1e3)
-        iters.append(pr.iterations)
-        log(f"step {i}: {times[-1]:.3f} ms, {pr.iterations} iterations, rTr {pr.rTr:.3e}")
-    ms = float(np.mean(times))
+        avg, byt = ctypes.c_float(), ctypes.c_double()
+        _lib.call("flub_mgpcg_profile", pr.mgpcg.handle, 0, 20, ctypes.byref(avg), ctypes.byref(byt))
+        print(json.dumps({"kernel": "rb_half_sweep_l0", "ms": avg.value, "bytes": byt.value,
+                          "copy_bytes": src[0].nbytes}))
+        return
 
-    # accuracy of the last projection against the initial residual
-    p = pr.p0.f.numpy()
-
-    # per-kernel timing with HIP events on the solver's stream
+    solvers = {}
     kernels = {}
-    names = {0: "rb_half_sweep_l0", 1: "Ap_dot", 2: "update_rx_dot", 4: "restrict_l0", 5: "prolong_l0",
-             6: "update_p", 3: "vcycle"}
-    import ctypes
+    last_pr = None
+    for name in args.solvers:
+        if name == "learned":
+            continue
+        pr = Projection((n, n, n), n_mg_levels=L, pre_and_post_smoothing=2, bottom_smoothing=6, precision=name)
+        pr.load(s)
+        dst = [pr.vel0.u.f, pr.vel0.v.f, pr.vel0.w.f]
+        times, iters = [], []
+        for i in range(args.warmup + args.steps):
+            for a, b in zip(dst, src):
+                device.copy(a, b)
+            device.synchronize()
+            t = time.perf_counter()
+            pr.project(max_iters=-1, rel_tol=1e-12)
+            device.synchronize()
+            if i >= args.warmup:
+                times.append((time.perf_counter() - t) * 1e3)
+                iters.append(pr.iterations)
+                log(f"{name} step {i - args.warmup}: {times[-1]:.3f} ms, {pr.iterations} iterations, "
+                    f"rTr {pr.rTr:.3e}")
+        solvers[name] = {"ms": round(float(np.mean(times)), 4), "ms_min": round(float(np.min(times)), 4),
+                         "iterations": int(np.median(iters))}
+        if name == "fp32":
+            kernels = profile_kernels(pr)
+        last_pr = pr
+
+    if "learned" in args.solvers:
+        solvers["learned"] = learned_leg(s, n, L, src, args.learned_steps, args.warmup)
+
+    best = min(solvers, key=lambda k: solvers[k]["ms"])
+    ms = solvers[best]["ms"]
+    dom = kernels.get("rb_half_sweep_l0")
+    roofline = None
+    if dom:
+        roofline = {"bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC_L0.get(n),
+                    "kernel": "k_sweep<float> level-0 red-black half-sweep (9 B/cell algorithmic)",
+                    "bytes_per_launch": dom["bytes"]}
+    cpu = None if args.no_cpu else cpu_baseline(s, n, L, args.cpu_threads)
+    out = {
+        "metric": "ms per pressure projection to 1e-6 rel. residual @256^3: learned/mixed vs MGPCG",
+        "value": ms, "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if best in ("fp32",) else ("bf16+f32" if best in ("bf16", "learned") else "f16+f32"),
+        "data": "synthetic (seeded scene)",
+        "config": {"workload": f"flow past sphere {n}^3: divergence, pressure solve to 1e-6 rel. residual, "
+                               f"gradient subtraction; MGPCG {L} levels, 2 RB-GS pre/post (doubling per level), "
+                               f"6 bottom", "grid": n, "best_solver": best},
+        "solvers": solvers,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    if "fp32" in solvers:
+        out["speedup_vs_mgpcg_fp32"] = {k: round(solvers["fp32"]["ms"] / v["ms"], 3) for k, v in solvers.items()}
+    print(json.dumps(out), flush=True)
+
+
+def profile_kernels(pr):
+    """Mean duration of single kernels of the FP32 solve, replayed with HIP
+    events on the solver's stream, and their algorithmic bytes."""
+    from flub_amd import _lib
+    names = {0: "rb_half_sweep_l0", 5: "prolong_sweep_l0", 4: "restrict_l0", 1: "p_update_Ap_dot",
+             2: "update_rx_dot", 6: "update_p", 3: "vcycle"}
+    kernels = {}
     for kid, name in names.items():
         avg = ctypes.c_float()
         byt = ctypes.c_double()
@@ -98,30 +161,50 @@ def main():
             ent["GBs"] = round(byt.value / (avg.value * 1e-3) / 1e9, 1)
         kernels[name] = ent
         log(f"kernel {name}: {ent}")
-    dom = kernels["rb_half_sweep_l0"]
-    roofline = {"bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_smooth_mc (level-0 red-black half-sweep)",
-                "bytes_per_launch": dom["bytes"]}
+    return kernels
 
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(s, n, L, args.cpu_threads)
 
-    out = {
-        "metric": "ms per pressure projection to 1e-6 rel. residual @256^3: learned/mixed vs MGPCG",
-        "value": round(ms, 4), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(ms, 4), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32" if args.solver == "fp32" else f"{args.solver}+f32",
-        "data": "synthetic (seeded scene)",
-        "config": {"workload": f"flow past sphere {n}^3, {args.solver} MGPCG ({L} levels, 2 pre/post RBGS, "
-                               f"bottom 6), rel. residual 1e-6", "grid": n, "levels": L, "solver": args.solver,
-                   "iterations": int(np.median(iters))},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "kernels": kernels,
-    }
-    print(json.dumps(out), flush=True)
+def learned_leg(s, n, L, src, steps, warmup):
+    import torch
+    from flub_amd import device
+    from flub_amd.learned.evaluate import load_net
+    from flub_amd.learned.fpcg import LearnedPCG
+    from flub_amd.learned.unet import flops_per_voxel
+    from flub_amd.project import Projection
+
+    path = os.path.join(ROOT, "flub_amd", "learned", "weights", "unet_128.npz")
+    net = load_net(path)
+    pr = Projection((n, n, n), n_mg_levels=L, pre_and_post_smoothing=2, bottom_smoothing=6)
+    pr.load(s)
+    fluid = torch.from_numpy(s["tag"] == 1).cuda()
+    solver = LearnedPCG(pr.mgpcg, net)
+    dst = [pr.vel0.u.f, pr.vel0.v.f, pr.vel0.w.f]
+    times, st = [], None
+    for i in range(max(1, warmup) + steps):
+        for a, b in zip(dst, src):
+            device.copy(a, b)
+        device.synchronize()
+        t = time.perf_counter()
+        st = pr.project_learned(solver, fluid)
+        device.synchronize()
+        if i >= max(1, warmup):
+            times.append((time.perf_counter() - t) * 1e3)
+            log(f"learned step: {times[-1]:.2f} ms, {st['iterations']} iterations "
+                f"({st['net_iters']} net, {st['vcycle_iters']} V-cycle, fallback at {st['fallback_at']})")
+    # one network application, timed alone
+    r = torch.randn((1, n, n, n, 2), device="cuda")
+    net.infer(r)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    net.infer(r)
+    torch.cuda.synchronize()
+    net_ms = (time.perf_counter() - t) * 1e3
+    tflops = flops_per_voxel() * n ** 3 / (net_ms * 1e-3) / 1e12
+    return {"ms": round(float(np.mean(times)), 3), "iterations": st["iterations"], "net_iters": st["net_iters"],
+            "vcycle_iters": st["vcycle_iters"], "fallback_at": st["fallback_at"],
+            "true_rel_res": float(np.sqrt(st["true_rTr"] / st["rTr0"])),
+            "unet_forward_ms": round(net_ms, 3), "unet_tflops": round(tflops, 2),
+            "unet_mfma_frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4)}
 
 
 def cpu_baseline(s, n, L, threads):

@@ -95,6 +95,15 @@ def lib():
     """Load libflub.so once; raise if it has not been built."""
     global _lib
     if _lib is None:
+        # One HIP runtime per process: PyTorch-ROCm ships its own libamdhip64
+        # (same soname as the system one libflub links).  Loading PyTorch first
+        # makes libflub bind to that already-loaded runtime, so both share one
+        # device context; loading libflub first would bring in a second runtime
+        # and PyTorch would then find no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise ImportError(
                 f"{LIB_PATH} is missing: build the HIP library first (python -m flub_amd.build)")

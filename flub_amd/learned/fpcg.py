@@ -11,8 +11,10 @@ Preconditioner per iteration:
   * "vcycle" -- the library's V-cycle (the MGPCG preconditioner).
 The solve starts on the network and falls back to the V-cycle for good as
 soon as one iteration reduces |r| by less than the factor `stagnation`
-(MGPCG as automatic fallback).  At the end the FP64 true residual of x is
-recomputed with the library's operator (MGPCG as judge of the error).
+(MGPCG as automatic fallback).  When the recursive residual passes the test,
+the FP64 true residual of x is recomputed with the library's operator (MGPCG
+as judge of the error); if it exceeds the tolerance by more than 4x it
+replaces r and the solve continues on the V-cycle.
 """
 from __future__ import annotations
 
@@ -29,11 +31,12 @@ def _p(t):
 
 
 class LearnedPCG:
-    def __init__(self, mg, net, stagnation=0.5, max_net_iters=1 << 30):
+    def __init__(self, mg, net, stagnation=0.5, max_net_iters=1 << 30, max_replacements=3):
         self.mg = mg
         self.net = net
         self.stagnation = stagnation
         self.max_net_iters = max_net_iters
+        self.max_replacements = max_replacements
 
     def apply_A(self, x, y):
         _lib.call("flub_mgpcg_apply_A", self.mg.handle, _p(x), _p(y))
@@ -44,7 +47,7 @@ class LearnedPCG:
     def net_precondition(self, r, fluid):
         s = torch.sqrt((r[fluid].double() ** 2).mean()).clamp_min(1e-300).float()
         inp = torch.stack([torch.where(fluid, r / s, torch.zeros_like(r)), fluid.float()], dim=-1)[None]
-        z = self.net.infer(inp)[0] * s
+        z = self.net.infer(inp)[0] * (s * self.net.out_scale)
         return torch.where(fluid, z, torch.zeros_like(z)).contiguous()
 
     @staticmethod
@@ -59,6 +62,7 @@ class LearnedPCG:
         x = torch.zeros_like(b)
         r = b.clone()
         q = torch.empty_like(b)
+        ax = torch.empty_like(b)
         z = torch.empty_like(b)
         rr0 = self.dot(r, r)
         tol = max(abs_tol, rr0 * rel_tol)
@@ -93,15 +97,31 @@ class LearnedPCG:
                 stats["fallback_at"] = it + 1
             rr = rr_new
             if rr < tol:
-                break
+                # the judge: FP64 true residual with the library's operator; on
+                # drift (> 4x the tolerance) replace r by it and go on with the
+                # V-cycle (at most `max_replacements` times)
+                self.apply_A(x, ax)
+                true_r = b.double() - ax.double()
+                true_rr = float((true_r ** 2).sum())
+                if true_rr <= 4.0 * tol or stats.setdefault("replacements", 0) >= self.max_replacements:
+                    break
+                stats["replacements"] += 1
+                r.copy_(true_r.float())
+                rr = true_rr
+                mode = "vcycle"
+                precond()              # restart the directions from the replaced residual
+                zr = self.dot(z, r)
+                p = z.clone()
+                it += 1
+                continue
             precond()
             beta = -alpha * self.dot(z, q) / (zr + eps)
             zr = self.dot(z, r)
             p = z + beta * p
             it += 1
         # judge: FP64 true residual with the library's operator
-        self.apply_A(x, q)
-        true_rr = float(((b.double() - q.double()) ** 2).sum())
+        self.apply_A(x, ax)
+        true_rr = float(((b.double() - ax.double()) ** 2).sum())
         torch.cuda.synchronize()
         stats.update(iterations=it, rTr=rr, true_rTr=true_rr, rTr0=rr0, ms=(time.perf_counter() - t0) * 1e3)
         return x, stats

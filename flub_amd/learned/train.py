@@ -28,13 +28,17 @@ def rel_loss(pred, target, fluid):
     return (num / den).mean()
 
 
-def train(n=128, pairs=2000, steps=1500, batch=2, lr=2e-3, seed=0, first_seed=1000, log=print, out=None):
+def train(n=128, pairs=2000, steps=1500, batch=2, lr=1e-3, seed=0, first_seed=1000, log=print, out=None):
     torch.manual_seed(seed)
     dev = "cuda"
     t0 = time.time()
     X, Y = make_pairs(n, list(range(first_seed, first_seed + pairs)), log=log)
     log(f"generated {pairs} pairs at {n}^3 in {time.time() - t0:.1f}s")
+    # fixed output scale: the network predicts Y / yscale (O(1) values)
+    yscale = float(torch.sqrt((Y.float() ** 2).sum() / X[..., 1].float().sum()))
+    log(f"target scale {yscale:.3f}")
     net = UNet(seed).to(dev)
+    net.out_scale = yscale
     opt = torch.optim.Adam(net.parameters(), lr=lr)
     sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, T_max=steps, eta_min=lr * 0.05)
     g = torch.Generator().manual_seed(seed)
@@ -46,18 +50,19 @@ def train(n=128, pairs=2000, steps=1500, batch=2, lr=2e-3, seed=0, first_seed=10
         yb = Y[idx]
         fl = xb[..., 1]
         pred = net(xb) * fl
-        loss = rel_loss(pred, yb, fl)
+        loss = rel_loss(pred, yb / yscale, fl)
         opt.zero_grad(set_to_none=True)
         loss.backward()
+        torch.nn.utils.clip_grad_norm_(net.parameters(), 1.0)
         opt.step()
         sched.step()
-        hist.append(float(loss))
+        hist.append(loss.item())
         if step % 50 == 0 or step == steps - 1:
             log(f"step {step}: loss {np.mean(hist[-50:]):.4f} ({(time.time() - t1) / (step + 1) * 1e3:.1f} ms/step)")
     if out:
         os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
         np.savez(out, **net.state_numpy(), n=np.int32(n), pairs=np.int32(pairs), steps=np.int32(steps),
-                 final_loss=np.float32(np.mean(hist[-50:])))
+                 final_loss=np.float32(np.mean(hist[-50:])), loss_history=np.float32(hist))
         log(f"saved {out}")
     return net, hist
 
@@ -68,7 +73,7 @@ def main():
     ap.add_argument("--pairs", type=int, default=2000)
     ap.add_argument("--steps", type=int, default=1500)
     ap.add_argument("--batch", type=int, default=2)
-    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--out", default=os.path.join(HERE, "weights", "unet_128.npz"))
     a = ap.parse_args()
     train(a.n, a.pairs, a.steps, a.batch, a.lr, out=a.out, log=lambda s: print(s, flush=True))

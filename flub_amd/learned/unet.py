@@ -10,11 +10,13 @@ Forward cost per voxel: 2*27*(16*16 + 16*16 + (16*32 + 32*32)/8 + (32*64 +
 """
 from __future__ import annotations
 
+import ctypes
 import math
 
 import numpy as np
 import torch
 
+from .. import _lib
 from .conv import Conv3dFn, conv3d
 
 # (name, cin, cout); every conv is followed by ReLU
@@ -59,6 +61,7 @@ class UNet(torch.nn.Module):
             self.b[name] = torch.nn.Parameter(torch.zeros(cout))
         self.head_w = torch.nn.Parameter(torch.randn(16, generator=g) * math.sqrt(1.0 / 16))
         self.head_b = torch.nn.Parameter(torch.zeros(1))
+        self.out_scale = 1.0   # the network predicts target / out_scale
 
     def _c(self, x, name):
         return Conv3dFn.apply(x, self.w[name], self.b[name], True)
@@ -93,13 +96,19 @@ class UNet(torch.nn.Module):
         d1 = c3(c3(torch.cat([up2(e2), e1], dim=-1), "d1a"), "d1b")
         del e1, e2
         d0 = c3(c3(torch.cat([up2(d1), e0], dim=-1), "d0a"), "d0b")
-        return (d0.float() @ self.head_w.detach()) + self.head_b.detach()
+        y = torch.empty((n, d, h, w), device=x.device, dtype=torch.float32)
+        hw = self.head_w.detach().float().contiguous()
+        _lib.call("flub_head_fwd", ctypes.c_void_p(d0.data_ptr()), ctypes.c_void_p(hw.data_ptr()),
+                  float(self.head_b.detach()[0]), ctypes.c_void_p(y.data_ptr()), y.numel(), d0.shape[-1],
+                  ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+        return y
 
     def state_numpy(self):
         out = {f"w_{k}": v.detach().cpu().numpy() for k, v in self.w.items()}
         out.update({f"b_{k}": v.detach().cpu().numpy() for k, v in self.b.items()})
         out["head_w"] = self.head_w.detach().cpu().numpy()
         out["head_b"] = self.head_b.detach().cpu().numpy()
+        out["out_scale"] = np.float64(self.out_scale)
         return out
 
     def load_numpy(self, d):
@@ -109,3 +118,4 @@ class UNet(torch.nn.Module):
                 self.b[k].copy_(torch.from_numpy(np.asarray(d[f"b_{k}"])))
             self.head_w.copy_(torch.from_numpy(np.asarray(d["head_w"])))
             self.head_b.copy_(torch.from_numpy(np.asarray(d["head_b"])))
+        self.out_scale = float(d.get("out_scale", 1.0))

@@ -100,6 +100,32 @@ class Projection:
         wp.copy(pf.f, mg.x)
         launch(scale_by_3d, dim=self.shape, inputs=[pf, self.h])
 
+    def project_learned(self, solver, fluid, rel_tol=1e-12, abs_tol=0.0):
+        """project() with the pressure solve done by a flub_amd.learned.fpcg.LearnedPCG
+        (the U-Net preconditioner with V-cycle fallback) on this projection's
+        MGPCG handle; fluid is a torch bool mask of the FLUID cells.  Returns
+        the solver statistics."""
+        import torch
+
+        self.divergence()
+        mg = self.mgpcg
+        mg.set_tags(self.tag.f)
+        mg.init(self.div.f, -self.h)
+        mg.mask_interior()
+        b = torch.empty(self.shape, device="cuda", dtype=torch.float32)
+        _lib.call("flub_memcpy_d2d", b.data_ptr(), mg.r[0].ptr, mg.r[0].nbytes, None)
+        x, st = solver.solve(b, fluid, rel_tol=rel_tol, abs_tol=abs_tol)
+        _lib.call("flub_memcpy_d2d", mg.x.ptr, x.data_ptr(), mg.x.nbytes, None)
+        self.iterations, self.rTr = st["iterations"], st["rTr"]
+        wp.copy(self.p0.f, mg.x)
+        launch(scale_by_3d, dim=self.shape, inputs=[self.p0, self.h])
+        launch(scale_by_3d, dim=self.shape, inputs=[self.p0, 1.0 / self.h])
+        g = self.shape
+        for d in (1, 2, 3):
+            launch(subtract_gradient_tag, dim=field_shape(d, g), inputs=[self.vel0, self.vel_rigid, self.p0, self.tag, d])
+        launch(scale_by_3d, dim=self.shape, inputs=[self.p0, self.h])
+        return st
+
     def project_stepwise(self, max_iters=-1, rel_tol=1e-12, abs_tol=0.0, smoother="phases", verbose=False):
         """example_fluid.py:120-145 launch by launch."""
         self.divergence()
