@@ -149,13 +149,17 @@ def profile_kernels(pr):
     """Mean duration of single kernels of the FP32 solve, replayed with HIP
     events on the solver's stream, and their algorithmic bytes."""
     from flub_amd import _lib
-    names = {0: "rb_half_sweep_l0", 5: "prolong_sweep_l0", 4: "restrict_l0", 1: "p_update_Ap_dot",
-             2: "update_rx_dot", 6: "update_p", 3: "vcycle"}
+    names = {0: "rb_half_sweep_l0", 7: "blocked4_l0", 5: "prolong_sweep_l0", 4: "restrict_l0",
+             1: "p_update_Ap_dot", 2: "update_rx_dot", 6: "update_p", 3: "vcycle"}
     kernels = {}
     for kid, name in names.items():
         avg = ctypes.c_float()
         byt = ctypes.c_double()
-        _lib.call("flub_mgpcg_profile", pr.mgpcg.handle, kid, 20, ctypes.byref(avg), ctypes.byref(byt))
+        try:
+            _lib.call("flub_mgpcg_profile", pr.mgpcg.handle, kid, 20, ctypes.byref(avg), ctypes.byref(byt))
+        except _lib.FlubError as e:   # e.g. no blocked smoothing for this grid
+            log(f"kernel {name}: {e}")
+            continue
         ent = {"ms": round(avg.value, 5), "bytes": byt.value}
         if byt.value > 0:
             ent["GBs"] = round(byt.value / (avg.value * 1e-3) / 1e9, 1)

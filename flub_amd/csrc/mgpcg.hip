@@ -489,22 +489,49 @@ static void bottom_solve(flub_mgpcg *h, int tagged);
 // z[l]; the last level-0 sweep also yields the partials of z.r.
 static int vcycle_fast(flub_mgpcg *h)
 {
+    // Levels 1.. whose shape fits a tile run their smoothing temporally
+    // blocked (blocked.hip); the pre-smoothed z then lands in the buffer
+    // that the first post-smoothing launch does not write.
+    const void *pre_z[MAXL];
+    uint8_t ph[MG_MAX_PHASES];
     for (int l = 0; l + 1 < h->L; ++l) {
         Level &F = h->lv[l];
         const int s = h->pre_post << l;
-        TRY(fast_sweep(h->s, F, F.zpre, F.zpre, nullptr, 0, SWEEP_ZERO, nullptr, nullptr));
-        TRY(fast_sweep(h->s, F, F.zpre, F.zpre, nullptr, 1, SWEEP_PLAIN, nullptr, nullptr));
-        for (int it = 1; it < s; ++it) {
-            TRY(fast_sweep(h->s, F, F.zpre, F.zpre, nullptr, 0, SWEEP_PLAIN, nullptr, nullptr));
+        if (mg_use_tb(h, l, 2 * s, true)) {
+            const int n = mg_phase_pairs(ph, s, 0);
+            float *X = (tb_launches(n) & 1) ? F.zpre : F.z;
+            TRY(tb_chain(h->s, h->tb_order, F, FLUB_PREC_FP32, SWEEP_ZERO, nullptr, nullptr, 1.0f, F.r, ph, n, X,
+                         X == F.z ? F.zpre : F.z));
+            pre_z[l] = X;
+        } else {
+            TRY(fast_sweep(h->s, F, F.zpre, F.zpre, nullptr, 0, SWEEP_ZERO, nullptr, nullptr));
             TRY(fast_sweep(h->s, F, F.zpre, F.zpre, nullptr, 1, SWEEP_PLAIN, nullptr, nullptr));
+            for (int it = 1; it < s; ++it) {
+                TRY(fast_sweep(h->s, F, F.zpre, F.zpre, nullptr, 0, SWEEP_PLAIN, nullptr, nullptr));
+                TRY(fast_sweep(h->s, F, F.zpre, F.zpre, nullptr, 1, SWEEP_PLAIN, nullptr, nullptr));
+            }
+            pre_z[l] = F.zpre;
         }
-        TRY(fast_restrict(h->s, F, F.zpre, h->lv[l + 1]));
+        TRY(fast_restrict(h->s, F, (const float *)pre_z[l], h->lv[l + 1]));
     }
-    bottom_solve(h, 1);
+    const int lb = h->L - 1;
+    if (h->L > 1 && mg_use_tb(h, lb, 2 * (h->bottom / 2) * 2)) {
+        Level &B = h->lv[lb];
+        const int n = mg_bottom_phases(ph, h->bottom / 2);
+        TRY(tb_chain(h->s, h->tb_order, B, FLUB_PREC_FP32, SWEEP_ZERO, nullptr, nullptr, 1.0f, B.r, ph, n, B.z, B.zpre));
+    } else {
+        bottom_solve(h, 1);
+    }
     for (int l = h->L - 2; l >= 0; --l) {
         Level &F = h->lv[l];
         const int s = h->pre_post << l;
         const float *zc = h->lv[l + 1].z;
+        if (mg_use_tb(h, l, 2 * s, true)) {
+            const int n = mg_phase_pairs(ph, s, 1);
+            TRY(tb_chain(h->s, h->tb_order, F, FLUB_PREC_FP32, SWEEP_PROLONG, pre_z[l], zc, 1.0f, F.r, ph, n, F.z, F.zpre,
+                         l == 0 ? h->part : nullptr, l == 0 ? &h->vc_dot_nblk : nullptr));
+            continue;
+        }
         for (int it = 0; it < s; ++it) {
             const bool last = l == 0 && it + 1 == s;
             if (it == 0) TRY(fast_sweep(h->s, F, F.zpre, F.z, zc, 1, SWEEP_PROLONG, nullptr, nullptr));
@@ -576,6 +603,8 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
     h->pre_post = 6;
     h->bottom = 6;
     h->no_graph = getenv("FLUB_NO_GRAPH") != nullptr;
+    h->no_tb = getenv("FLUB_NO_TB") != nullptr;
+    h->no_tb_l0 = getenv("FLUB_NO_TB_L0") != nullptr;
     int rc = FLUB_OK;
 #define ALLOC(ptr, bytes)                                              \
     do {                                                               \
@@ -615,7 +644,7 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
         ALLOC(h->p_alt, n0 * 4);
         h->fast = levels >= 2 && (nz % 4) == 0;
         ALLOC(h->sc, SC_NUM * sizeof(double));
-        ALLOC(h->part, 2 * MAXBLK * sizeof(double));
+        ALLOC(h->part, 2 * MAXPART * sizeof(double));
         hipError_t e = hipHostMalloc((void **)&h->hsc, SC_NUM * sizeof(double), hipHostMallocDefault);
         if (e != hipSuccess) { flub_set_error("hipHostMalloc: %s", hipGetErrorString(e)); rc = (int)e; goto fail; }
         e = hipMemsetAsync(h->sc, 0, SC_NUM * sizeof(double), h->s);
@@ -624,6 +653,10 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
         if (e != hipSuccess) { flub_set_error("init: %s", hipGetErrorString(e)); rc = (int)e; goto fail; }
         rc = mg_mixed_setup(h);
         if (rc) goto fail;
+        if (!h->no_tb) {
+            rc = tb_tables_create(&h->tb_order);
+            if (rc) goto fail;
+        }
     }
 #undef ALLOC
     *out = h;
@@ -644,6 +677,7 @@ FLUB_API int flub_mgpcg_destroy(flub_mgpcg *h)
         hipFree(L.tag); hipFree(L.code); hipFree(L.r); hipFree(L.z); hipFree(L.zpre); hipFree(L.rh); hipFree(L.zh);
     }
     hipFree(h->x); hipFree(h->p); hipFree(h->p_alt); hipFree(h->Ap); hipFree(h->sc); hipFree(h->part);
+    hipFree(h->tb_order);
     if (h->hsc) hipHostFree(h->hsc);
     if (h->cap) hipStreamDestroy(h->cap);
     free(h);
@@ -1085,6 +1119,12 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
             k_update_p<<<nb256(L0.n), 256, 0, h->s>>>(h->p, L0.z, h->sc, L0.n);
             FLUB_CHECK_LAUNCH();
             return FLUB_OK;
+        case 7: {   // one blocked launch of 4 level-0 half-sweeps (zpre -> z)
+            const uint8_t ph[4] = {1, 0, 1, 0};
+            FLUB_REQUIRE(h->tb_order && tb_supported(L0), FLUB_ESTATE, "no blocked smoothing for this grid");
+            return tb_chain(h->s, h->tb_order, L0, FLUB_PREC_FP32, SWEEP_PLAIN, L0.zpre, nullptr, 1.0f, L0.r, ph, 4,
+                            L0.z, L0.zpre);
+        }
         default: flub_set_error("unknown kernel id %d", kernel); return FLUB_EINVAL;
         }
     };
@@ -1096,6 +1136,7 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
     case 4: b = N * (2 * v + 1) + N / 8 * (v + 1); break;
     case 5: b = h->fast ? N * (2.625 * v + 1) : N * 2 * v + N / 8 * v; break;   // fused: zpre, zc, r/2, tags -> z
     case 6: b = N * 3 * 4; break;
+    case 7: b = 4 * N * (2 * v + 1); break;                  // four half-sweeps
     default: break;
     }
     int rc = launch_once();   // warm-up (captures the graph for id 3)

@@ -6,6 +6,7 @@
 #define MAXL 12
 #define RBS 256                 // reduction block size
 #define MAXBLK 2048             // max partials per reduction
+#define MAXPART 8192            // partial buffer size (the blocked level-0 sweep writes one per tile)
 #define BOTTOM_ONE_WG_MAX 2048  // bottom levels up to this many cells per colour run in one workgroup
 #define SOLVE_SAFETY_CAP 20000  // iteration cap applied when max_iters == -1
 
@@ -35,13 +36,16 @@ struct flub_mgpcg {
     double *sc;     // device scalars
     bool fast;      // quad kernels usable (tagged smoother, levels >= 2, pre_post >= 1)
     int vc_dot_nblk;  // partial count of the z.r fused into the fast V-cycle (0: none)
-    double *part;   // reduction partials (2*MAXBLK)
+    double *part;   // reduction partials (2*MAXPART)
     double *hsc;    // pinned host mirror of sc
     int pre_post, bottom;
     hipStream_t s;     // work stream (caller's; may be the null stream)
     hipStream_t cap;   // private stream used only to capture graphs
     bool tags_set;
     bool no_graph;
+    bool no_tb;        // launch-per-half-sweep smoothing on every level (no temporal blocking)
+    bool no_tb_l0;     // no temporal blocking on level 0
+    uint32_t *tb_order;  // update-order tables of the blocked smoother
     hipGraphExec_t graph[3];
     bool graph_ok[3];
     MixedState *mx;

@@ -234,42 +234,77 @@ static int mx_sweep(flub_mgpcg *h, int l, int mode, bool src32, int out, int pha
     return FLUB_OK;
 }
 
+// Zero fill of 32-bit words (a kernel, not a memset: it is captured in graphs).
+__global__ void k_mx_zero(uint32_t *__restrict__ p, uint32_t n)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0u;
+}
+
 // One V-cycle (mgpcg.py:296-349 schedule) from the FP32 residual r[0] into
-// the FP32 z[0], with the z.r / z.q partials of the last sweep.
+// the FP32 z[0], with the z.r / z.q partials of the last sweep.  Levels 1..
+// that fit a tile are smoothed temporally blocked (blocked.hip, FP32 in LDS
+// between the half-sweeps of one launch), as in the FP32 V-cycle.
 static int vcycle_mixed(flub_mgpcg *h)
 {
     const int L = h->L;
+    const void *pre_z[MAXL];
+    uint8_t ph[MG_MAX_PHASES];
     for (int l = 0; l + 1 < L; ++l) {
         void *zp = h->mx->zph[l];
         const int s = h->pre_post << l;
-        TRYM(mx_sweep(h, l, SWEEP_ZERO, l == 0, SWEEP_OUT_STORE, 0, nullptr, zp, nullptr));
-        TRYM(mx_sweep(h, l, SWEEP_PLAIN, false, SWEEP_OUT_STORE, 1, zp, zp, nullptr));
-        for (int it = 1; it < s; ++it) {
-            TRYM(mx_sweep(h, l, SWEEP_PLAIN, false, SWEEP_OUT_STORE, 0, zp, zp, nullptr));
+        if (mg_use_tb(h, l, 2 * s)) {
+            Level &F = h->lv[l];
+            const int n = mg_phase_pairs(ph, s, 0);
+            void *X = (tb_launches(n) & 1) ? zp : F.zh;
+            TRYM(tb_chain(h->s, h->tb_order, F, h->prec, SWEEP_ZERO, nullptr, nullptr, 8.0f, F.rh, ph, n, X, X == zp ? F.zh : zp));
+            pre_z[l] = X;
+        } else {
+            TRYM(mx_sweep(h, l, SWEEP_ZERO, l == 0, SWEEP_OUT_STORE, 0, nullptr, zp, nullptr));
             TRYM(mx_sweep(h, l, SWEEP_PLAIN, false, SWEEP_OUT_STORE, 1, zp, zp, nullptr));
+            for (int it = 1; it < s; ++it) {
+                TRYM(mx_sweep(h, l, SWEEP_PLAIN, false, SWEEP_OUT_STORE, 0, zp, zp, nullptr));
+                TRYM(mx_sweep(h, l, SWEEP_PLAIN, false, SWEEP_OUT_STORE, 1, zp, zp, nullptr));
+            }
+            pre_z[l] = zp;
         }
-        TRYM(restrict_level(h->s, h->lv[l], h->lv[l].rh, zp, h->lv[l + 1], h->lv[l + 1].rh, h->prec));
+        TRYM(restrict_level(h->s, h->lv[l], h->lv[l].rh, pre_z[l], h->lv[l + 1], h->lv[l + 1].rh, h->prec));
     }
     {
         const int lb = L - 1;
-        void *z = h->lv[lb].zh;
+        Level &B = h->lv[lb];
+        void *z = B.zh;
         const int nb = h->bottom / 2;
-        bool started = false;
-        for (int pass = 0; pass < 2; ++pass)
-            for (int it = 0; it < nb; ++it)
-                for (int k = 0; k < 2; ++k) {
-                    const int phase = pass == 0 ? k : 1 - k;
-                    TRYM(mx_sweep(h, lb, started ? SWEEP_PLAIN : SWEEP_ZERO, false, SWEEP_OUT_STORE, phase, z, z,
-                                  nullptr));
-                    started = true;
-                }
-        if (!started) FLUB_CHECK_HIP(hipMemsetAsync(z, 0, (size_t)h->lv[lb].n * 2, h->s));
+        if (mg_use_tb(h, lb, 4 * nb)) {
+            const int n = mg_bottom_phases(ph, nb);
+            TRYM(tb_chain(h->s, h->tb_order, B, h->prec, SWEEP_ZERO, nullptr, nullptr, 8.0f, B.rh, ph, n, z, h->mx->zph[lb]));
+        } else {
+            bool started = false;
+            for (int pass = 0; pass < 2; ++pass)
+                for (int it = 0; it < nb; ++it)
+                    for (int k = 0; k < 2; ++k) {
+                        const int phase = pass == 0 ? k : 1 - k;
+                        TRYM(mx_sweep(h, lb, started ? SWEEP_PLAIN : SWEEP_ZERO, false, SWEEP_OUT_STORE, phase, z, z,
+                                      nullptr));
+                        started = true;
+                    }
+            if (!started) {
+                const uint32_t words = B.n / 2;   // 16-bit values, n even
+                k_mx_zero<<<blocks_for(words, 256), 256, 0, h->s>>>((uint32_t *)z, words);
+                FLUB_CHECK_LAUNCH();
+            }
+        }
     }
     for (int l = L - 2; l >= 0; --l) {
         void *zp = h->mx->zph[l];
         void *z = h->lv[l].zh;
         const void *zc = h->lv[l + 1].zh;
         const int s = h->pre_post << l;
+        if (mg_use_tb(h, l, 2 * s)) {
+            const int n = mg_phase_pairs(ph, s, 1);
+            TRYM(tb_chain(h->s, h->tb_order, h->lv[l], h->prec, SWEEP_PROLONG, pre_z[l], zc, 8.0f, h->lv[l].rh, ph, n, z, zp));
+            continue;
+        }
         for (int it = 0; it < s; ++it) {
             const bool last = l == 0 && it + 1 == s;
             if (it == 0) TRYM(mx_sweep(h, l, SWEEP_PROLONG, false, SWEEP_OUT_STORE, 1, zp, z, zc));

@@ -36,3 +36,53 @@ int fast_restrict(hipStream_t s, const Level &F, const float *zf, const Level &C
 // pout = z + beta*p; Ap = ndt(pout); partials of pout.Ap.
 int fast_p_Ap(hipStream_t s, const Level &L0, const float *p, float *pout, float *Ap, const double *sc,
               double *part, int *nblk);
+
+// Temporally blocked smoothing (blocked.hip): up to TB_G half-sweeps per launch.
+#define TB_G 4
+// The level's shape is a multiple of one of the tile shapes.
+bool tb_supported(const Level &L);
+// Launches needed for nsteps half-sweeps.
+int tb_launches(int nsteps);
+// nsteps half-sweeps (phase of step s: phases[s]) of level L in the storage
+// type of prec; the first starts as mode0 (SWEEP_ZERO, SWEEP_PLAIN from zin,
+// SWEEP_PROLONG from zin + pfac * zc[parent]).  The launches alternate
+// between the buffers dst and other, the last one writing dst; the first one
+// must not write zin (zin != the first buffer written, which is dst when
+// tb_launches(nsteps) is odd, else other), and each launch (TB_G steps)
+// must contain both phases.
+// Device copy of the tiles' update-order tables (one per handle).
+int tb_tables_create(uint32_t **dev);
+// With part (FP32; the last launch not a ZERO one) the last launch also writes
+// per-workgroup FP64 partials of z.r over the level into part and their count
+// into *nblk.
+int tb_chain(hipStream_t s, const uint32_t *tables, const Level &L, int prec, int mode0, const void *zin,
+             const void *zc, float pfac, const void *r, const uint8_t *phases, int nsteps, void *dst, void *other,
+             double *part = nullptr, int *nblk = nullptr);
+
+#define MG_MAX_PHASES (4 * 4096)
+// Levels above this many cells are bandwidth-bound: one launch per
+// half-sweep streams them better than the blocked kernel, whose halo
+// recomputation then costs more than the launches it saves.
+#define TB_MAX_CELLS (1u << 21)
+// Blocked smoothing for nsteps half-sweeps of level l (level 0 only when the
+// caller allows it and it is not switched off).
+static inline bool mg_use_tb(const flub_mgpcg *h, int l, int nsteps, bool level0 = false)
+{
+    return !h->no_tb && h->tb_order && (l >= 1 || (level0 && !h->no_tb_l0)) && nsteps >= 1 &&
+           nsteps <= MG_MAX_PHASES && h->lv[l].n <= TB_MAX_CELLS && tb_supported(h->lv[l]);
+}
+// ph = first, 1-first repeated `pairs` times (pre-smoothing mgpcg.py:303-309
+// with first 0, post-smoothing mgpcg.py:342-348 with first 1); returns the
+// step count.
+static inline int mg_phase_pairs(uint8_t *ph, int pairs, int first)
+{
+    for (int i = 0; i < pairs; ++i) { ph[2 * i] = (uint8_t)first; ph[2 * i + 1] = (uint8_t)(1 - first); }
+    return 2 * pairs;
+}
+// Bottom schedule (mgpcg.py:319-336): (0, 1) x nb, then (1, 0) x nb.
+static inline int mg_bottom_phases(uint8_t *ph, int nb)
+{
+    mg_phase_pairs(ph, nb, 0);
+    mg_phase_pairs(ph + 2 * nb, nb, 1);
+    return 4 * nb;
+}

@@ -7,16 +7,9 @@
 // quad come from the adjacent lanes (lane +-1 holds quad +-1 of the same row)
 // and are loaded only at wavefront edges.
 //
-// Neighbour classes come from a per-cell 16-bit stencil code built once per
-// set_tags (k_make_codes): for the neighbours i+, i-, j+, j-, k+, k- two bits
-// each (0 = SOLID or outside the domain, 1 = AIR, 2 = FLUID), bits 12-14 the
-// count divd of in-domain non-solid neighbours (mgpcg.py:51-58,144-160), bit
-// 15 set for a FLUID cell.  neighbor_dot_tag (mgpcg.py:118-142) is then, per
-// axis, ret += (a + b) with a = (x'_+ - x_I) for an AIR/FLUID neighbour (x' = 0
-// on AIR) and a = 0 for a SOLID/outside one.  The reference multiplies the
-// latter difference by 0.0 instead; the signed zero that gives never changes
-// ret (ret starts at +0 and is never -0), so results stay bitwise those of the
-// reference kernels and of the FP32 oracle.
+// Neighbour classes come from the per-cell 16-bit stencil code described in
+// stencil.h, so results stay bitwise those of the reference kernels and of
+// the FP32 oracle.
 //
 // Fusions (each equal to the unfused launch sequence):
 //  * ZERO: the first pre-smoothing half-sweep of a level starts from z = 0
@@ -31,79 +24,9 @@
 #include "common.h"
 #include "mgpcg_internal.h"
 #include "smooth.h"
+#include "stencil.h"
 
 namespace {
-
-template <typename T> __device__ __forceinline__ float ldv(T v);
-template <> __device__ __forceinline__ float ldv<float>(float v) { return v; }
-template <> __device__ __forceinline__ float ldv<__half>(__half v) { return __half2float(v); }
-template <> __device__ __forceinline__ float ldv<__hip_bfloat16>(__hip_bfloat16 v) { return __bfloat162float(v); }
-template <typename T> __device__ __forceinline__ T stv(float v);
-template <> __device__ __forceinline__ float stv<float>(float v) { return v; }
-template <> __device__ __forceinline__ __half stv<__half>(float v) { return __float2half_rn(v); }
-template <> __device__ __forceinline__ __hip_bfloat16 stv<__hip_bfloat16>(float v) { return __float2bfloat16(v); }
-
-template <typename T>
-__device__ __forceinline__ void ld4(const T *p, float v[4])
-{
-    if constexpr (sizeof(T) == 4) {
-        const float4 a = *reinterpret_cast<const float4 *>(p);
-        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-    } else {
-        const uint2 w = *reinterpret_cast<const uint2 *>(p);
-        const T *e = reinterpret_cast<const T *>(&w);
-        v[0] = ldv<T>(e[0]); v[1] = ldv<T>(e[1]); v[2] = ldv<T>(e[2]); v[3] = ldv<T>(e[3]);
-    }
-}
-template <typename T>
-__device__ __forceinline__ void st4(T *p, const float v[4])
-{
-    if constexpr (sizeof(T) == 4) {
-        *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
-    } else {
-        uint2 w;
-        T *e = reinterpret_cast<T *>(&w);
-        e[0] = stv<T>(v[0]); e[1] = stv<T>(v[1]); e[2] = stv<T>(v[2]); e[3] = stv<T>(v[3]);
-        *reinterpret_cast<uint2 *>(p) = w;
-    }
-}
-template <typename T>
-__device__ __forceinline__ void ld2(const T *p, float v[2])
-{
-    if constexpr (sizeof(T) == 4) {
-        const float2 a = *reinterpret_cast<const float2 *>(p);
-        v[0] = a.x; v[1] = a.y;
-    } else {
-        const uint32_t w = *reinterpret_cast<const uint32_t *>(p);
-        const T *e = reinterpret_cast<const T *>(&w);
-        v[0] = ldv<T>(e[0]); v[1] = ldv<T>(e[1]);
-    }
-}
-
-__device__ __forceinline__ void codes4(const uint16_t *code, uint32_t base, uint32_t c[4])
-{
-    const uint2 w = *reinterpret_cast<const uint2 *>(code + base);
-    c[0] = w.x & 0xffffu; c[1] = w.x >> 16; c[2] = w.y & 0xffffu; c[3] = w.y >> 16;
-}
-
-// neighbor_dot_tag from the stencil code (the self-FLUID test is the caller's).
-__device__ __forceinline__ float nsum_code(uint32_t cd, float xI, float xip, float xim, float xjp, float xjm,
-                                           float xkp, float xkm)
-{
-    float s = 0.0f;
-    const float xs[6] = {xip, xim, xjp, xjm, xkp, xkm};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const uint32_t cp = (cd >> (4 * a)) & 3u, cm = (cd >> (4 * a + 2)) & 3u;
-        const float ta = cp == 0 ? 0.0f : ((cp == 2 ? xs[2 * a] : 0.0f) - xI);
-        const float tb = cm == 0 ? 0.0f : ((cm == 2 ? xs[2 * a + 1] : 0.0f) - xI);
-        s = s + (ta + tb);
-    }
-    return s;
-}
-
-__device__ __forceinline__ int code_divd(uint32_t cd) { return (int)((cd >> 12) & 7u); }
-__device__ __forceinline__ bool code_fluid(uint32_t cd) { return (cd >> 15) != 0; }
 
 struct Idx {
     int nx, ny, nz, nq, lq, lny;
@@ -209,9 +132,7 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgs a)
             if ((q & 1) != upd) continue;
             const float xkp = q < 3 ? c[q + 1] : zk4;
             const float xkm = q > 0 ? c[q - 1] : zkm;
-            const float s = nsum_code(cd[q], c[q], xp[q], xm[q], yp[q], ym[q], xkp, xkm);
-            const int divd = code_divd(cd[q]);
-            out[q] = divd != 0 ? c[q] + (rv[q] + s) / (float)divd : 0.0f;
+            out[q] = gs_update(cd[q], c[q], rv[q], xp[q], xm[q], yp[q], ym[q], xkp, xkm);
         }
         if (OUT == OUT_FINAL) {
             const float inv = 1.0f / s0;

@@ -151,7 +151,8 @@ int flub_mgpcg_level_shape(flub_mgpcg *h, int level, int shape[3]);
  * stream between two HIP events and returns the mean ms per launch and the
  * kernel's algorithmic HBM bytes per launch.  Kernel ids (FLUB_PROF_*):
  * 0 level-0 RB half-sweep, 1 Ap + p.Ap, 2 r/x update + r.r, 3 whole V-cycle,
- * 4 restriction 0->1, 5 prolongation 1->0, 6 p update.  Leaves the handle's
+ * 4 restriction 0->1, 5 prolongation 1->0, 6 p update, 7 one temporally
+ * blocked launch of four level-0 half-sweeps.  Leaves the handle's
  * vectors in an unspecified state (call before init / after reading x). */
 int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_ms, double *bytes);
 

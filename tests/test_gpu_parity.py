@@ -133,7 +133,8 @@ def test_compute_Ap_bitexact(smoother, oracle):
 
 
 @pytest.mark.parametrize("smoother,shape,levels", [("phases", (32, 32, 32), 3), ("phases", (64, 64, 64), 4),
-                                                   ("phases", (64, 32, 32), 4), ("default", (32, 32, 32), 3)])
+                                                   ("phases", (64, 32, 32), 4), ("default", (32, 32, 32), 3),
+                                                   ("phases", (128, 128, 128), 4), ("phases", (256, 256, 256), 4)])
 def test_vcycle_bitexact(smoother, shape, levels, oracle):
     n = shape[0]
     if smoother == "default":
