@@ -37,7 +37,7 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 # HBM bytes per launch of the level-0 FP32 half-sweep at 256^3, from two
 # rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of `bench.py --kernels-only`,
 # calibrated on a 67.4 MB device copy (profiles/rb_half_sweep_l0_traffic_256.md)
-MEASURED_TRAFFIC_L0 = {256: 242.7e6}
+MEASURED_TRAFFIC_L0 = {("rb_half_sweep_l0", 256): 242.7e6}
 
 
 def log(*a):
@@ -57,6 +57,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--kernels-only", action="store_true",
                     help="only replay the level-0 half-sweep (for counter collection)")
+    ap.add_argument("--kernel", choices=["split", "reference-layout"], default="split",
+                    help="which level-0 half-sweep --kernels-only replays")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
@@ -80,9 +82,9 @@ def main():
         device.copy(pr.vel0.u.f, src[0])
         device.synchronize()
         avg, byt = ctypes.c_float(), ctypes.c_double()
-        _lib.call("flub_mgpcg_profile", pr.mgpcg.handle, 0, 20, ctypes.byref(avg), ctypes.byref(byt))
-        print(json.dumps({"kernel": "rb_half_sweep_l0", "ms": avg.value, "bytes": byt.value,
-                          "copy_bytes": src[0].nbytes}))
+        kid, kname = (8, "rb_half_sweep_l0_split") if args.kernel == "split" else (0, "rb_half_sweep_l0")
+        _lib.call("flub_mgpcg_profile", pr.mgpcg.handle, kid, 20, ctypes.byref(avg), ctypes.byref(byt))
+        print(json.dumps({"kernel": kname, "ms": avg.value, "bytes": byt.value, "copy_bytes": src[0].nbytes}))
         return
 
     solvers = {}
@@ -118,12 +120,16 @@ def main():
 
     best = min(solvers, key=lambda k: solvers[k]["ms"])
     ms = solvers[best]["ms"]
-    dom = kernels.get("rb_half_sweep_l0")
+    # the dominant kernel: the level-0 half-sweep the solve runs (colour-split when available)
+    dom_name = "rb_half_sweep_l0_split" if "rb_half_sweep_l0_split" in kernels else "rb_half_sweep_l0"
+    dom = kernels.get(dom_name)
     roofline = None
     if dom:
         roofline = {"bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC_L0.get(n),
-                    "kernel": "k_sweep<float> level-0 red-black half-sweep (9 B/cell algorithmic)",
+                    "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC_L0.get((dom_name, n)),
+                    "kernel": ("k_ssweep level-0 red-black half-sweep, colour-split layout"
+                               if dom_name.endswith("split") else "k_sweep<float> level-0 red-black half-sweep")
+                              + " (9 B/cell algorithmic)",
                     "bytes_per_launch": dom["bytes"]}
     cpu = None if args.no_cpu else cpu_baseline(s, n, L, args.cpu_threads)
     out = {
@@ -149,7 +155,8 @@ def profile_kernels(pr):
     """Mean duration of single kernels of the FP32 solve, replayed with HIP
     events on the solver's stream, and their algorithmic bytes."""
     from flub_amd import _lib
-    names = {0: "rb_half_sweep_l0", 7: "blocked4_l0", 5: "prolong_sweep_l0", 4: "restrict_l0",
+    names = {8: "rb_half_sweep_l0_split", 9: "restrict_l0_split", 10: "p_update_Ap_dot_split",
+             0: "rb_half_sweep_l0", 7: "blocked4_l0", 5: "prolong_sweep_l0", 4: "restrict_l0",
              1: "p_update_Ap_dot", 2: "update_rx_dot", 6: "update_p", 3: "vcycle"}
     kernels = {}
     for kid, name in names.items():

@@ -487,16 +487,29 @@ static void bottom_solve(flub_mgpcg *h, int tagged);
 // starts from z = 0), the restriction reads zpre, the prolongating first
 // post-sweep reads zpre + z[l+1] and writes z[l], the rest runs in place on
 // z[l]; the last level-0 sweep also yields the partials of z.r.
-static int vcycle_fast(flub_mgpcg *h)
+static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
 {
-    // Levels 1.. whose shape fits a tile run their smoothing temporally
-    // blocked (blocked.hip); the pre-smoothed z then lands in the buffer
-    // that the first post-smoothing launch does not write.
+    // Levels whose shape fits a tile run their smoothing temporally blocked
+    // (blocked.hip); the pre-smoothed z then lands in the buffer that the
+    // first post-smoothing launch does not write.  With `split`, level 0 is
+    // the colour-split copy (split.hip): r in sp.r, pre-smoothing in place on
+    // sp.zpre, post-smoothing into sp.z[zslot].
     const void *pre_z[MAXL];
     uint8_t ph[MG_MAX_PHASES];
+    const SplitBuf &S = h->sp;
     for (int l = 0; l + 1 < h->L; ++l) {
         Level &F = h->lv[l];
         const int s = h->pre_post << l;
+        if (l == 0 && split) {
+            TRY(split_sweep(h->s, F, nullptr, S.zpre, nullptr, S.r, S.code, 0, SWEEP_ZERO, nullptr, nullptr));
+            TRY(split_sweep(h->s, F, S.zpre, S.zpre, nullptr, S.r, S.code, 1, SWEEP_PLAIN, nullptr, nullptr));
+            for (int it = 1; it < s; ++it) {
+                TRY(split_sweep(h->s, F, S.zpre, S.zpre, nullptr, S.r, S.code, 0, SWEEP_PLAIN, nullptr, nullptr));
+                TRY(split_sweep(h->s, F, S.zpre, S.zpre, nullptr, S.r, S.code, 1, SWEEP_PLAIN, nullptr, nullptr));
+            }
+            TRY(split_restrict(h->s, F, S.r, S.zpre, S.code, h->lv[1]));
+            continue;
+        }
         if (mg_use_tb(h, l, 2 * s, true)) {
             const int n = mg_phase_pairs(ph, s, 0);
             float *X = (tb_launches(n) & 1) ? F.zpre : F.z;
@@ -526,6 +539,17 @@ static int vcycle_fast(flub_mgpcg *h)
         Level &F = h->lv[l];
         const int s = h->pre_post << l;
         const float *zc = h->lv[l + 1].z;
+        if (l == 0 && split) {
+            float *Z = S.z[zslot];
+            for (int it = 0; it < s; ++it) {
+                const bool last = it + 1 == s;
+                if (it == 0) TRY(split_sweep(h->s, F, S.zpre, Z, zc, S.r, S.code, 1, SWEEP_PROLONG, nullptr, nullptr));
+                else TRY(split_sweep(h->s, F, Z, Z, nullptr, S.r, S.code, 1, SWEEP_PLAIN, nullptr, nullptr));
+                TRY(split_sweep(h->s, F, Z, Z, nullptr, S.r, S.code, 0, SWEEP_PLAIN, last ? h->part : nullptr,
+                                last ? &h->vc_dot_nblk : nullptr));
+            }
+            continue;
+        }
         if (mg_use_tb(h, l, 2 * s, true)) {
             const int n = mg_phase_pairs(ph, s, 1);
             TRY(tb_chain(h->s, h->tb_order, F, FLUB_PREC_FP32, SWEEP_PROLONG, pre_z[l], zc, 1.0f, F.r, ph, n, F.z, F.zpre,
@@ -552,7 +576,7 @@ int mg_precondition(flub_mgpcg *h, int smoother, bool use_graph)
 {
     if (h->prec != FLUB_PREC_FP32) return mg_vcycle_mixed(h, smoother);
     const bool fast = use_fast(h, smoother);
-    if (!use_graph || h->no_graph) return fast ? vcycle_fast(h) : vcycle_fp32(h, smoother);
+    if (!use_graph || h->no_graph) return fast ? vcycle_fast(h, false) : vcycle_fp32(h, smoother);
     if (!h->graph_ok[smoother]) {
         // Capture on the handle's private stream (the work stream may be the
         // null stream, which cannot be captured); replay on the work stream.
@@ -560,7 +584,7 @@ int mg_precondition(flub_mgpcg *h, int smoother, bool use_graph)
         hipStream_t work = h->s;
         FLUB_CHECK_HIP(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
         h->s = h->cap;
-        int rc = fast ? vcycle_fast(h) : vcycle_fp32(h, smoother);
+        int rc = fast ? vcycle_fast(h, false) : vcycle_fp32(h, smoother);
         h->s = work;
         hipError_t e = hipStreamEndCapture(h->cap, &g);
         if (rc) { if (e == hipSuccess) hipGraphDestroy(g); return rc; }
@@ -573,9 +597,33 @@ int mg_precondition(flub_mgpcg *h, int smoother, bool use_graph)
     return FLUB_OK;
 }
 
+// The V-cycle on the colour-split level 0 (sp.r -> sp.z[zslot]), as a graph
+// in slot 3 + zslot.
+static int mg_precondition_split(flub_mgpcg *h, int zslot)
+{
+    if (h->no_graph) return vcycle_fast(h, true, zslot);
+    const int gs = 3 + zslot;
+    if (!h->graph_ok[gs]) {
+        hipGraph_t g;
+        hipStream_t work = h->s;
+        FLUB_CHECK_HIP(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
+        h->s = h->cap;
+        int rc = vcycle_fast(h, true, zslot);
+        h->s = work;
+        hipError_t e = hipStreamEndCapture(h->cap, &g);
+        if (rc) { if (e == hipSuccess) hipGraphDestroy(g); return rc; }
+        FLUB_CHECK_HIP(e);
+        FLUB_CHECK_HIP(hipGraphInstantiate(&h->graph[gs], g, nullptr, nullptr, 0));
+        FLUB_CHECK_HIP(hipGraphDestroy(g));
+        h->graph_ok[gs] = true;
+    }
+    FLUB_CHECK_HIP(hipGraphLaunch(h->graph[gs], h->s));
+    return FLUB_OK;
+}
+
 static void drop_graphs(flub_mgpcg *h)
 {
-    for (int s = 0; s < 3; ++s)
+    for (int s = 0; s < 5; ++s)
         if (h->graph_ok[s]) { hipGraphExecDestroy(h->graph[s]); h->graph_ok[s] = false; }
 }
 
@@ -657,6 +705,18 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
             rc = tb_tables_create(&h->tb_order);
             if (rc) goto fail;
         }
+        e = hipEventCreateWithFlags(&h->ev, hipEventDisableTiming);
+        if (e != hipSuccess) { flub_set_error("hipEventCreate: %s", hipGetErrorString(e)); rc = (int)e; goto fail; }
+        h->split = precision == FLUB_PREC_FP32 && h->fast && split_supported(h->lv[0]) &&
+                   getenv("FLUB_NO_SPLIT") == nullptr;
+        if (h->split) {
+            ALLOC(h->sp.r, n0 * 4);
+            ALLOC(h->sp.z[0], n0 * 4);
+            ALLOC(h->sp.z[1], n0 * 4);
+            ALLOC(h->sp.zpre, n0 * 4);
+            ALLOC(h->sp.x, n0 * 4);
+            ALLOC(h->sp.code, n0 * 2);
+        }
     }
 #undef ALLOC
     *out = h;
@@ -678,6 +738,9 @@ FLUB_API int flub_mgpcg_destroy(flub_mgpcg *h)
     }
     hipFree(h->x); hipFree(h->p); hipFree(h->p_alt); hipFree(h->Ap); hipFree(h->sc); hipFree(h->part);
     hipFree(h->tb_order);
+    hipFree(h->sp.r); hipFree(h->sp.z[0]); hipFree(h->sp.z[1]); hipFree(h->sp.zpre); hipFree(h->sp.x);
+    hipFree(h->sp.code);
+    if (h->ev) hipEventDestroy(h->ev);
     if (h->hsc) hipHostFree(h->hsc);
     if (h->cap) hipStreamDestroy(h->cap);
     free(h);
@@ -714,6 +777,7 @@ FLUB_API int flub_mgpcg_set_tags(flub_mgpcg *h, const int32_t *tag)
         FLUB_CHECK_LAUNCH();
     }
     for (int l = 0; l < h->L; ++l) TRY(make_codes(h->s, h->lv[l]));
+    if (h->split) TRY(split_convert(h->s, L0, L0.code, h->sp.code, 2, true));
     h->tags_set = true;
     return FLUB_OK;
 }
@@ -874,33 +938,77 @@ static int solve_fast(flub_mgpcg *h, int max_iters, double eps, double abs_tol, 
     const uint32_t n = L0.n;
     const int nb = red_blocks(n);
     float *const p_home = h->p;
-    TRY(mg_dot(h, L0.r, L0.r, FIN_RTR, 0.0));
+    // With the split layout, r, x, z, p and Ap of level 0 are colour-split
+    // for the whole solve and converted back at the end.  The V-cycle is then
+    // queued before the host reads the residual norm, into the z buffer the
+    // iteration does not use: the GPU keeps working while the host tests for
+    // convergence.  If the test stops the solve, that speculative z is not
+    // used (x, r, z and the scalars are those of the reference sequence; the
+    // coarse levels hold the speculative cycle's values).
+    const bool split = h->split;
+    int zs = 0;
+    float *R = L0.r, *X = h->x, *Z = L0.z;
+    if (split) {
+        TRY(split_convert(h->s, L0, L0.r, h->sp.r, 4, true));
+        TRY(split_convert(h->s, L0, h->x, h->sp.x, 4, true));
+        R = h->sp.r; X = h->sp.x; Z = h->sp.z[zs];
+    }
+    TRY(mg_dot(h, R, R, FIN_RTR, 0.0));
     double rTr0;
     TRY(read_scalar(h, SC_RTR, &rTr0));
     const double tol = fmax(abs_tol, rTr0 * rel_tol);
-    TRY(mg_precondition(h, FLUB_SMOOTHER_PHASES, true));
+    TRY(split ? mg_precondition_split(h, zs) : mg_precondition(h, FLUB_SMOOTHER_PHASES, true));
     TRY(launch_finish(h, h->vc_dot_nblk, FIN_OLDZTR, 0.0));
     const int cap = max_iters < 0 ? SOLVE_SAFETY_CAP : max_iters;
     int it = 0;
-    double rTr = rTr0;
+    double rTr = rTr0, r_prev = rTr0, rate = 1.0;   // rate: last reduction factor of r.r
     int status = FLUB_OK;
     while (it < cap) {
         int nbp = 0;
-        TRY(fast_p_Ap(h->s, L0, h->p, h->p_alt, h->Ap, h->sc, h->part, &nbp));
+        // speculate unless this test is predicted to stop the solve (a wasted
+        // speculative cycle would delay the end of the solve)
+        const bool speculate = split && it + 1 < cap && r_prev * rate > 16.0 * tol;
+        if (split) TRY(split_p_Ap(h->s, L0, h->p, h->p_alt, h->Ap, Z, h->sp.code, h->sc, h->part, &nbp));
+        else TRY(fast_p_Ap(h->s, L0, h->p, h->p_alt, h->Ap, h->sc, h->part, &nbp));
         float *t = h->p; h->p = h->p_alt; h->p_alt = t;
         TRY(launch_finish(h, nbp, FIN_ALPHA, eps));
-        k_update_rx_dot<<<nb, RBS, 0, h->s>>>(L0.r, h->x, h->Ap, h->p, h->sc, n, h->part);
+        k_update_rx_dot<<<nb, RBS, 0, h->s>>>(R, X, h->Ap, h->p, h->sc, n, h->part);
         FLUB_CHECK_LAUNCH();
         TRY(launch_finish(h, nb, FIN_RTR, 0.0));
-        TRY(read_scalar(h, SC_RTR, &rTr));
+        if (speculate) {
+            FLUB_CHECK_HIP(hipMemcpyAsync(h->hsc + SC_RTR, h->sc + SC_RTR, sizeof(double), hipMemcpyDeviceToHost, h->s));
+            FLUB_CHECK_HIP(hipEventRecord(h->ev, h->s));
+            TRY(mg_precondition_split(h, zs ^ 1));
+            FLUB_CHECK_HIP(hipEventSynchronize(h->ev));
+            rTr = h->hsc[SC_RTR];
+        } else {
+            TRY(read_scalar(h, SC_RTR, &rTr));
+        }
         if (verbose) printf("after iter %d, |residual|_2=%.9g\n", it + 1, sqrt(rTr));
         if (!isfinite(rTr)) { status = FLUB_ENONFINITE; flub_set_error("non-finite residual at iteration %d", it + 1); break; }
         if (rTr < tol) break;
-        TRY(mg_precondition(h, FLUB_SMOOTHER_PHASES, true));
+        rate = r_prev > 0.0 ? fmin(rTr / r_prev, 1.0) : 1.0;
+        r_prev = rTr;
+        if (split) {
+            if (!speculate) TRY(mg_precondition_split(h, zs ^ 1));
+            zs ^= 1;
+            Z = h->sp.z[zs];
+        } else {
+            TRY(mg_precondition(h, FLUB_SMOOTHER_PHASES, true));
+        }
         TRY(launch_finish(h, h->vc_dot_nblk, FIN_BETA, eps));
         it += 1;
     }
-    if (h->p != p_home) {   // leave p in the buffer the handle exposes
+    if (split) {   // back to the reference layout (sp.zpre is free scratch now)
+        TRY(split_convert(h->s, L0, X, h->x, 4, false));
+        TRY(split_convert(h->s, L0, R, L0.r, 4, false));
+        TRY(split_convert(h->s, L0, Z, L0.z, 4, false));
+        TRY(split_convert(h->s, L0, h->Ap, h->sp.zpre, 4, false));
+        FLUB_CHECK_HIP(hipMemcpyAsync(h->Ap, h->sp.zpre, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
+        TRY(split_convert(h->s, L0, h->p, h->sp.zpre, 4, false));
+        FLUB_CHECK_HIP(hipMemcpyAsync(p_home, h->sp.zpre, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
+        if (h->p != p_home) { h->p_alt = h->p; h->p = p_home; }
+    } else if (h->p != p_home) {   // leave p in the buffer the handle exposes
         FLUB_CHECK_HIP(hipMemcpyAsync(p_home, h->p, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
         h->p_alt = h->p;
         h->p = p_home;
@@ -1102,7 +1210,7 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
             k_update_rx_dot<<<nb, RBS, 0, h->s>>>(L0.r, h->x, h->Ap, h->p, h->sc, L0.n, h->part);
             FLUB_CHECK_LAUNCH();
             return launch_finish(h, nb, FIN_RTR, 0.0);
-        case 3: return mg_precondition(h, FLUB_SMOOTHER_PHASES, true);
+        case 3: return h->split ? mg_precondition_split(h, 0) : mg_precondition(h, FLUB_SMOOTHER_PHASES, true);
         case 4: {
             Level &C = h->lv[1];
             if (h->fast) return fast_restrict(h->s, L0, L0.zpre, C);
@@ -1125,6 +1233,19 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
             return tb_chain(h->s, h->tb_order, L0, FLUB_PREC_FP32, SWEEP_PLAIN, L0.zpre, nullptr, 1.0f, L0.r, ph, 4,
                             L0.z, L0.zpre);
         }
+        case 8:   // level-0 half-sweep in the colour-split layout
+            FLUB_REQUIRE(h->split, FLUB_ESTATE, "no colour-split level 0 for this handle");
+            return split_sweep(h->s, L0, h->sp.z[0], h->sp.z[0], nullptr, h->sp.r, h->sp.code, 0, SWEEP_PLAIN,
+                               nullptr, nullptr);
+        case 9:
+            FLUB_REQUIRE(h->split, FLUB_ESTATE, "no colour-split level 0 for this handle");
+            return split_restrict(h->s, L0, h->sp.r, h->sp.zpre, h->sp.code, h->lv[1]);
+        case 10: {
+            FLUB_REQUIRE(h->split, FLUB_ESTATE, "no colour-split level 0 for this handle");
+            int nbp = 0;
+            TRY(split_p_Ap(h->s, L0, h->p, h->p_alt, h->Ap, h->sp.z[0], h->sp.code, h->sc, h->part, &nbp));
+            return launch_finish(h, nbp, FIN_SUM, 0.0);
+        }
         default: flub_set_error("unknown kernel id %d", kernel); return FLUB_EINVAL;
         }
     };
@@ -1137,6 +1258,9 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
     case 5: b = h->fast ? N * (2.625 * v + 1) : N * 2 * v + N / 8 * v; break;   // fused: zpre, zc, r/2, tags -> z
     case 6: b = N * 3 * 4; break;
     case 7: b = 4 * N * (2 * v + 1); break;                  // four half-sweeps
+    case 8: b = N * (2 * v + 1); break;
+    case 9: b = N * (2 * v + 1) + N / 8 * (v + 1); break;
+    case 10: b = N * (4 * 4 + 1); break;
     default: break;
     }
     int rc = launch_once();   // warm-up (captures the graph for id 3)
@@ -1173,6 +1297,11 @@ FLUB_API int flub_mgpcg_precondition_into(flub_mgpcg *h, const float *r, float *
     FLUB_REQUIRE(h && r && z && smoother_ok(smoother), FLUB_EINVAL, "flub_mgpcg_precondition_into: bad argument");
     FLUB_REQUIRE(h->tags_set, FLUB_ESTATE, "precondition before set_tags");
     Level &L0 = h->lv[0];
+    if (h->split && smoother == FLUB_SMOOTHER_PHASES) {
+        TRY(split_convert(h->s, L0, r, h->sp.r, 4, true));
+        TRY(mg_precondition_split(h, 0));
+        return split_convert(h->s, L0, h->sp.z[0], z, 4, false);
+    }
     FLUB_CHECK_HIP(hipMemcpyAsync(L0.r, r, (size_t)L0.n * 4, hipMemcpyDeviceToDevice, h->s));
     if (h->prec != FLUB_PREC_FP32) TRY(mg_mixed_rscale(h));
     TRY(mg_precondition(h, smoother, true));

@@ -28,6 +28,13 @@ struct Level {
 
 struct MixedState;
 
+// Level 0 in the colour-split layout (split.hip), used inside the FP32 solve.
+struct SplitBuf {
+    float *r, *zpre, *x;
+    float *z[2];    // two z buffers: the solve preconditions speculatively into the spare one
+    uint16_t *code;
+};
+
 struct flub_mgpcg {
     int L, prec;
     Level lv[MAXL];
@@ -46,8 +53,11 @@ struct flub_mgpcg {
     bool no_tb;        // launch-per-half-sweep smoothing on every level (no temporal blocking)
     bool no_tb_l0;     // no temporal blocking on level 0
     uint32_t *tb_order;  // update-order tables of the blocked smoother
-    hipGraphExec_t graph[3];
-    bool graph_ok[3];
+    hipGraphExec_t graph[5];   // per smoother; [3], [4]: the V-cycle on the split level 0 into sp.z[0], sp.z[1]
+    bool graph_ok[5];
+    hipEvent_t ev;     // marks the residual norm's copy to the host
+    bool split;        // level 0 of the FP32 solve runs in the colour-split layout
+    SplitBuf sp;
     MixedState *mx;
 };
 

@@ -86,3 +86,18 @@ static inline int mg_bottom_phases(uint8_t *ph, int nb)
     mg_phase_pairs(ph + 2 * nb, nb, 1);
     return 4 * nb;
 }
+
+// Colour-split level 0 (split.hip).
+bool split_supported(const Level &L0);
+// Reference layout <-> split layout of a level-0 array of 4- or 2-byte values.
+int split_convert(hipStream_t s, const Level &L0, const void *src, void *dst, int elem_bytes, bool to_split);
+// One half-sweep (mode SWEEP_ZERO / PLAIN / PROLONG from zin + zc[parent]);
+// with part, also the partials of z.r (count in *nblk).
+int split_sweep(hipStream_t s, const Level &L0, const float *zin, float *zout, const float *zc, const float *r,
+                const uint16_t *code, int phase, int mode, double *part, int *nblk);
+// Restriction of the split level 0 (r, z) into level 1's r.
+int split_restrict(hipStream_t s, const Level &F, const float *r, const float *z, const uint16_t *code,
+                   const Level &C);
+// pout = z + beta*p; Ap = ndt(pout); partials of pout.Ap (all split).
+int split_p_Ap(hipStream_t s, const Level &L0, const float *p, float *pout, float *Ap, const float *z,
+               const uint16_t *code, const double *sc, double *part, int *nblk);

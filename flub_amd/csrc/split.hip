@@ -1,0 +1,400 @@
+// Colour-split storage of level 0 for the fused FP32 solve.
+//
+// A red-black half-sweep (mgpcg.py:72-86) updates the cells of one colour
+// from the other colour's values.  In the reference's layout (k fastest)
+// every cache line holds both colours, so a half-sweep streams all of z and
+// r although it writes half of z.  Inside the solve, level 0 is therefore
+// stored split by colour: an array of N values holds colour c = (i+j+k)&1 in
+// its half c, cell (i, j, k) at index ((i*ny + j) * nz/2 + (k >> 1)) of that
+// half.  Cell k of row (i, j) with colour c sits at k = 2m + o, o =
+// (c - i - j) & 1; its k-neighbours (the other colour) are at m + o and
+// m + o - 1 of the same row of the other half, its i/j neighbours at index m
+// of the neighbouring rows of the other half.  A half-sweep then reads the
+// updated colour's z, r and codes and the other colour's z: 9 bytes per cell
+// of the level instead of the 14 its cache lines move in the reference
+// layout.  The coarse levels (1..) keep the reference layout; the coarse
+// index of a fine cell (i, j, k) is (i/2, j/2, k/2) = (i/2, j/2, m).
+//
+// The per-cell arithmetic and its order are those of the reference-layout
+// kernels (gs_update, the residual of the restriction, p = z + beta*p and
+// ndt(p)), so the results are bitwise the same; only the FP64 reduction
+// order of the dot products differs.
+//
+// Threads own quads: 4 consecutive cells of one half-row (m0 .. m0+3); the
+// k-neighbour beyond the quad comes from the adjacent lane (which holds the
+// adjacent quad of the same half-row) or, at a wavefront edge, from memory.
+#include "common.h"
+#include "mgpcg_internal.h"
+#include "smooth.h"
+#include "stencil.h"
+
+namespace {
+
+struct SGeo {
+    int nx, ny, nh, nq;   // nh = nz/2 cells per half-row, nq = nh/4 quads
+    int lq, lny;          // log2 of nq and ny (-1: not a power of two)
+    uint32_t half;        // N/2
+    __device__ __forceinline__ void decode(uint32_t t, int &i, int &j, int &m0) const
+    {
+        uint32_t row, q;
+        if (lq >= 0) { row = t >> lq; q = t & ((1u << lq) - 1u); }
+        else { row = t / (uint32_t)nq; q = t - row * (uint32_t)nq; }
+        if (lny >= 0) { i = (int)(row >> lny); j = (int)(row & ((1u << lny) - 1u)); }
+        else { i = (int)(row / (uint32_t)ny); j = (int)(row - (uint32_t)i * ny); }
+        m0 = 4 * (int)q;
+    }
+    __device__ __forceinline__ uint32_t row(int i, int j) const { return ((uint32_t)i * ny + j) * (uint32_t)nh; }
+};
+
+__device__ __forceinline__ void f4(const float *p, float v[4])
+{
+    const float4 a = *reinterpret_cast<const float4 *>(p);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+__device__ __forceinline__ void st_f4(float *p, const float v[4])
+{
+    *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+
+struct SSweepArgs {
+    const float *zin;      // split z (PLAIN, PROLONG)
+    float *zout;           // split z written
+    const float *zc;       // PROLONG: z of level 1 (reference layout)
+    const float *r;        // split r
+    const uint16_t *code;  // split stencil codes
+    double *part;          // DOT: partials of z.r
+    SGeo g;
+    int cu;                // colour updated (the half-sweep phase ^ 1)
+};
+
+// One red-black half-sweep of level 0 in the split layout.  ZERO starts from
+// z = 0 and also writes the other colour's zeros; PROLONG reads z +
+// z_coarse[parent] everywhere and also writes the other colour's prolongated
+// values (out of place, zin != zout); DOT accumulates z.r over both colours.
+template <int MODE, bool DOT>
+__global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
+{
+    __shared__ double lds[4];
+    const SGeo g = a.g;
+    const uint32_t total = (uint32_t)g.nx * g.ny * g.nq;
+    const int lane = threadIdx.x & 63;
+    const uint32_t own = (uint32_t)a.cu * g.half, oth = (uint32_t)(a.cu ^ 1) * g.half;
+    double acc = 0.0;
+    const Chunk ch = xcd_chunk(total);
+    for (uint32_t t = ch.first; t < ch.end; t += ch.stride) {
+        int i, j, m0;
+        g.decode(t, i, j, m0);
+        const int o = (a.cu - i - j) & 1;   // the updated colour sits at k = 2m + o
+        auto zq = [&](uint32_t h, int ii, int jj, float v[4]) {
+            if (MODE == SWEEP_ZERO) { v[0] = v[1] = v[2] = v[3] = 0.0f; return; }
+            f4(a.zin + h + g.row(ii, jj) + m0, v);
+            if (MODE == SWEEP_PROLONG) {
+                float p[4];
+                f4(a.zc + cidx(ii >> 1, jj >> 1, m0, g.ny >> 1, g.nh), p);
+                v[0] = v[0] + p[0]; v[1] = v[1] + p[1]; v[2] = v[2] + p[2]; v[3] = v[3] + p[3];
+            }
+        };
+        auto z1 = [&](uint32_t h, int ii, int jj, int m) -> float {
+            if (MODE == SWEEP_ZERO) return 0.0f;
+            float v = a.zin[h + g.row(ii, jj) + m];
+            if (MODE == SWEEP_PROLONG) v = v + a.zc[cidx(ii >> 1, jj >> 1, m, g.ny >> 1, g.nh)];
+            return v;
+        };
+        float c[4], w[4], xp[4], xm[4], yp[4], ym[4];
+        zq(own, i, j, c);
+        zq(oth, i, j, w);
+        if (i + 1 < g.nx) zq(oth, i + 1, j, xp); else { xp[0] = xp[1] = xp[2] = xp[3] = 0.0f; }
+        if (i > 0) zq(oth, i - 1, j, xm); else { xm[0] = xm[1] = xm[2] = xm[3] = 0.0f; }
+        if (j + 1 < g.ny) zq(oth, i, j + 1, yp); else { yp[0] = yp[1] = yp[2] = yp[3] = 0.0f; }
+        if (j > 0) zq(oth, i, j - 1, ym); else { ym[0] = ym[1] = ym[2] = ym[3] = 0.0f; }
+        // the k-neighbour beyond the quad: other colour at m0-1 (o = 0) or m0+4 (o = 1)
+        const float left = __shfl_up(w[3], 1, 64), right = __shfl_down(w[0], 1, 64);
+        float edge = 0.0f;   // outside the domain: stencil class 0, unused
+        if (o == 0) {
+            if (m0 > 0) edge = lane > 0 ? left : z1(oth, i, j, m0 - 1);
+        } else {
+            if (m0 + 4 < g.nh) edge = (lane < 63 && t + 1 < ch.end) ? right : z1(oth, i, j, m0 + 4);
+        }
+        const uint32_t base = g.row(i, j) + m0;
+        uint32_t cd[4];
+        codes4(a.code, own + base, cd);
+        float rv[4];
+        f4(a.r + own + base, rv);
+        float out[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            float kp, km;
+            if (o == 0) { kp = w[e]; km = e > 0 ? w[e - 1] : edge; }
+            else { km = w[e]; kp = e < 3 ? w[e + 1] : edge; }
+            out[e] = gs_update(cd[e], c[e], rv[e], xp[e], xm[e], yp[e], ym[e], kp, km);
+        }
+        st_f4(a.zout + own + base, out);
+        if (MODE != SWEEP_PLAIN) st_f4(a.zout + oth + base, w);   // zeros / prolongated values
+        if (DOT) {
+            float ro[4];
+            f4(a.r + oth + base, ro);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc += (double)out[e] * (double)rv[e] + (double)w[e] * (double)ro[e];
+        }
+    }
+    if (DOT) {
+        const double s = block_sum<256>(acc, lds);
+        if (threadIdx.x == 0) a.part[blockIdx.x] = s;
+    }
+}
+
+// Gather restriction (mgpcg.py:94-101) from split level 0 into level 1
+// (reference layout), four coarse cells per thread: the children of coarse
+// (I, J, K) are, per child row (2I+a, 2J+b), the cells m = K of both halves.
+// Residual r - (-ndt(z)) per fine cell (0 unless FLUID); a FLUID parent sums
+// its children in the serial launch order (a, b, kk), others get 0.
+__global__ void __launch_bounds__(256) k_srestrict(const float *__restrict__ r, const float *__restrict__ z,
+                                                   const uint16_t *__restrict__ code, float *__restrict__ rc,
+                                                   const uint8_t *__restrict__ tagc, SGeo g, SGeo cg)
+{
+    // cg: the coarse grid in quads (nx, ny coarse; nh = coarse nz = fine nz/2; nq = nh/4)
+    const uint32_t total = (uint32_t)cg.nx * cg.ny * cg.nq;
+    const int lane = threadIdx.x & 63;
+    const Chunk ch = xcd_chunk(total);
+    for (uint32_t t = ch.first; t < ch.end; t += ch.stride) {
+        int I, J, K0;
+        cg.decode(t, I, J, K0);
+        float res[2][2][2][4];   // [a][b][kk][e]: child (2I+a, 2J+b, 2(K0+e)+kk)
+#pragma unroll
+        for (int aa = 0; aa < 2; ++aa)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int i = 2 * I + aa, j = 2 * J + b;
+                const uint32_t base = g.row(i, j) + K0;
+                float zz[2][4];
+                f4(z + base, zz[0]);
+                f4(z + g.half + base, zz[1]);
+#pragma unroll
+                for (int cc = 0; cc < 2; ++cc) {
+                    const uint32_t own = (uint32_t)cc * g.half, oth = (uint32_t)(cc ^ 1) * g.half;
+                    const int o = (cc - i - j) & 1;   // this colour's child is kk = o
+                    const float *w = zz[cc ^ 1];
+                    float xp[4], xm[4], yp[4], ym[4];
+                    if (i + 1 < g.nx) f4(z + oth + g.row(i + 1, j) + K0, xp); else { xp[0] = xp[1] = xp[2] = xp[3] = 0.0f; }
+                    if (i > 0) f4(z + oth + g.row(i - 1, j) + K0, xm); else { xm[0] = xm[1] = xm[2] = xm[3] = 0.0f; }
+                    if (j + 1 < g.ny) f4(z + oth + g.row(i, j + 1) + K0, yp); else { yp[0] = yp[1] = yp[2] = yp[3] = 0.0f; }
+                    if (j > 0) f4(z + oth + g.row(i, j - 1) + K0, ym); else { ym[0] = ym[1] = ym[2] = ym[3] = 0.0f; }
+                    const float left = __shfl_up(w[3], 1, 64), right = __shfl_down(w[0], 1, 64);
+                    float edge = 0.0f;
+                    if (o == 0) {
+                        if (K0 > 0) edge = lane > 0 ? left : z[oth + base - 1];
+                    } else {
+                        if (K0 + 4 < g.nh) edge = (lane < 63 && t + 1 < ch.end) ? right : z[oth + base + 4];
+                    }
+                    uint32_t cd[4];
+                    codes4(code, own + base, cd);
+                    float rv[4];
+                    f4(r + own + base, rv);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        float kp, km;
+                        if (o == 0) { kp = w[e]; km = e > 0 ? w[e - 1] : edge; }
+                        else { km = w[e]; kp = e < 3 ? w[e + 1] : edge; }
+                        float nd = nsum_code(cd[e], zz[cc][e], xp[e], xm[e], yp[e], ym[e], kp, km);
+                        if (!code_fluid(cd[e])) nd = 0.0f;
+                        res[aa][b][o][e] = rv[e] - (-nd);
+                    }
+                }
+            }
+        const uint32_t cb = cidx(I, J, K0, cg.ny, cg.nh);
+        const uint32_t t4 = *reinterpret_cast<const uint32_t *>(tagc + cb);
+        float out[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            float s = 0.0f;
+#pragma unroll
+            for (int aa = 0; aa < 2; ++aa)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int kk = 0; kk < 2; ++kk) s += res[aa][b][kk][e];
+            out[e] = ((t4 >> (8 * e)) & 0xffu) == 1u ? s : 0.0f;
+        }
+        st_f4(rc + cb, out);
+    }
+}
+
+// p_new = z + beta*p_old at every stencil point (mgpcg.py:229-232), then
+// Ap = ndt(p_new) (mgpcg.py:36-39) and the FP64 partial of p_new.Ap, in the
+// split layout; each thread does the quads of both colours at one position.
+// p_new is stored to pout (the neighbours still read p_old).
+__global__ void __launch_bounds__(256) k_sp_Ap(const float *__restrict__ p, float *__restrict__ pout,
+                                               float *__restrict__ Ap, const float *__restrict__ z,
+                                               const uint16_t *__restrict__ code, const double *__restrict__ sc,
+                                               SGeo g, double *__restrict__ part)
+{
+    __shared__ double lds[4];
+    const float beta = (float)sc[SC_BETA];
+    const uint32_t total = (uint32_t)g.nx * g.ny * g.nq;
+    const int lane = threadIdx.x & 63;
+    double acc = 0.0;
+    const Chunk ch = xcd_chunk(total);
+    for (uint32_t t = ch.first; t < ch.end; t += ch.stride) {
+        int i, j, m0;
+        g.decode(t, i, j, m0);
+        auto pq = [&](uint32_t h, int ii, int jj, float v[4]) {
+            const uint32_t b = h + g.row(ii, jj) + m0;
+            float a[4], zz[4];
+            f4(p + b, a);
+            f4(z + b, zz);
+            v[0] = fmaf(beta, a[0], zz[0]); v[1] = fmaf(beta, a[1], zz[1]);
+            v[2] = fmaf(beta, a[2], zz[2]); v[3] = fmaf(beta, a[3], zz[3]);
+        };
+        const uint32_t base = g.row(i, j) + m0;
+        float pn[2][4];
+        pq(0, i, j, pn[0]);
+        pq(g.half, i, j, pn[1]);
+        const float l0 = __shfl_up(pn[0][3], 1, 64), r0 = __shfl_down(pn[0][0], 1, 64);
+        const float l1 = __shfl_up(pn[1][3], 1, 64), r1 = __shfl_down(pn[1][0], 1, 64);
+#pragma unroll
+        for (int cc = 0; cc < 2; ++cc) {
+            const uint32_t own = (uint32_t)cc * g.half, oth = (uint32_t)(cc ^ 1) * g.half;
+            const int o = (cc - i - j) & 1;
+            const float *w = pn[cc ^ 1];
+            float xp[4], xm[4], yp[4], ym[4];
+            if (i + 1 < g.nx) pq(oth, i + 1, j, xp); else { xp[0] = xp[1] = xp[2] = xp[3] = 0.0f; }
+            if (i > 0) pq(oth, i - 1, j, xm); else { xm[0] = xm[1] = xm[2] = xm[3] = 0.0f; }
+            if (j + 1 < g.ny) pq(oth, i, j + 1, yp); else { yp[0] = yp[1] = yp[2] = yp[3] = 0.0f; }
+            if (j > 0) pq(oth, i, j - 1, ym); else { ym[0] = ym[1] = ym[2] = ym[3] = 0.0f; }
+            float edge = 0.0f;
+            if (o == 0) {
+                if (m0 > 0) edge = lane > 0 ? (cc ? l0 : l1) : fmaf(beta, p[oth + base - 1], z[oth + base - 1]);
+            } else {
+                if (m0 + 4 < g.nh)
+                    edge = (lane < 63 && t + 1 < ch.end) ? (cc ? r0 : r1)
+                                                         : fmaf(beta, p[oth + base + 4], z[oth + base + 4]);
+            }
+            uint32_t cd[4];
+            codes4(code, own + base, cd);
+            float out[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                float kp, km;
+                if (o == 0) { kp = w[e]; km = e > 0 ? w[e - 1] : edge; }
+                else { km = w[e]; kp = e < 3 ? w[e + 1] : edge; }
+                const float s = nsum_code(cd[e], pn[cc][e], xp[e], xm[e], yp[e], ym[e], kp, km);
+                out[e] = code_fluid(cd[e]) ? s : 0.0f;
+                acc += (double)pn[cc][e] * (double)out[e];
+            }
+            st_f4(pout + own + base, pn[cc]);
+            st_f4(Ap + own + base, out);
+        }
+    }
+    const double s = block_sum<256>(acc, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// Reference layout <-> split layout, one reference quad (4 cells along k)
+// per thread: cells k0, k0+2 go to one half, k0+1, k0+3 to the other, both
+// at m = k0/2, k0/2 + 1.
+template <typename T, bool TO_SPLIT>
+__global__ void __launch_bounds__(256) k_split_conv(const T *__restrict__ src, T *__restrict__ dst, SGeo g)
+{
+    const uint32_t nzq = (uint32_t)g.nh / 2;   // reference quads per row
+    const uint32_t total = (uint32_t)g.nx * g.ny * nzq;
+    for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < total; t += gridDim.x * 256u) {
+        const uint32_t row = t / nzq, kq = t - row * nzq;
+        const int i = (int)(row / (uint32_t)g.ny), j = (int)(row - (uint32_t)i * g.ny);
+        const int k0 = 4 * (int)kq;
+        const uint32_t nat = row * (uint32_t)(2 * g.nh) + (uint32_t)k0;
+        const uint32_t c0 = (uint32_t)((i + j + k0) & 1);   // colour of cells k0, k0+2
+        const uint32_t s0 = c0 * g.half + row * (uint32_t)g.nh + (uint32_t)(k0 >> 1);
+        const uint32_t s1 = (c0 ^ 1u) * g.half + row * (uint32_t)g.nh + (uint32_t)(k0 >> 1);
+        if (TO_SPLIT) {
+            const T v0 = src[nat], v1 = src[nat + 1], v2 = src[nat + 2], v3 = src[nat + 3];
+            dst[s0] = v0; dst[s0 + 1] = v2;
+            dst[s1] = v1; dst[s1 + 1] = v3;
+        } else {
+            const T v0 = src[s0], v2 = src[s0 + 1], v1 = src[s1], v3 = src[s1 + 1];
+            dst[nat] = v0; dst[nat + 1] = v1; dst[nat + 2] = v2; dst[nat + 3] = v3;
+        }
+    }
+}
+
+int ilog2_or_neg(int v)
+{
+    if (v <= 0 || (v & (v - 1))) return -1;
+    int l = 0;
+    while ((1 << l) < v) ++l;
+    return l;
+}
+
+SGeo make_sgeo(int nx, int ny, int nz)
+{
+    SGeo g;
+    g.nx = nx; g.ny = ny; g.nh = nz / 2; g.nq = g.nh / 4;
+    g.lq = ilog2_or_neg(g.nq);
+    g.lny = ilog2_or_neg(ny);
+    g.half = (uint32_t)nx * ny * (uint32_t)g.nh;
+    return g;
+}
+
+unsigned grid8(uint32_t items)
+{
+    unsigned b = blocks_for(items, 256);
+    b = b < 1 ? 1 : (b > MAXBLK ? MAXBLK : b);
+    return (b + 7u) & ~7u;   // xcd_chunk needs a multiple of 8
+}
+
+}  // namespace
+
+bool split_supported(const Level &L0) { return L0.nz % 8 == 0 && L0.nx % 2 == 0 && L0.ny % 2 == 0; }
+
+int split_convert(hipStream_t s, const Level &L0, const void *src, void *dst, int elem_bytes, bool to_split)
+{
+    const SGeo g = make_sgeo(L0.nx, L0.ny, L0.nz);
+    const unsigned nb = blocks_for(L0.n / 4, 256) < 4096 ? blocks_for(L0.n / 4, 256) : 4096;
+    if (elem_bytes == 4) {
+        if (to_split) k_split_conv<float, true><<<nb, 256, 0, s>>>((const float *)src, (float *)dst, g);
+        else k_split_conv<float, false><<<nb, 256, 0, s>>>((const float *)src, (float *)dst, g);
+    } else {
+        if (to_split) k_split_conv<uint16_t, true><<<nb, 256, 0, s>>>((const uint16_t *)src, (uint16_t *)dst, g);
+        else k_split_conv<uint16_t, false><<<nb, 256, 0, s>>>((const uint16_t *)src, (uint16_t *)dst, g);
+    }
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+int split_sweep(hipStream_t s, const Level &L0, const float *zin, float *zout, const float *zc, const float *r,
+                const uint16_t *code, int phase, int mode, double *part, int *nblk)
+{
+    SSweepArgs a{};
+    a.zin = zin; a.zout = zout; a.zc = zc; a.r = r; a.code = code; a.part = part;
+    a.g = make_sgeo(L0.nx, L0.ny, L0.nz);
+    a.cu = phase ^ 1;
+    const unsigned nb = grid8((uint32_t)L0.nx * L0.ny * a.g.nq);
+    if (nblk) *nblk = (int)nb;
+    if (mode == SWEEP_ZERO) k_ssweep<SWEEP_ZERO, false><<<nb, 256, 0, s>>>(a);
+    else if (mode == SWEEP_PROLONG) k_ssweep<SWEEP_PROLONG, false><<<nb, 256, 0, s>>>(a);
+    else if (part) k_ssweep<SWEEP_PLAIN, true><<<nb, 256, 0, s>>>(a);
+    else k_ssweep<SWEEP_PLAIN, false><<<nb, 256, 0, s>>>(a);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+int split_restrict(hipStream_t s, const Level &F, const float *r, const float *z, const uint16_t *code,
+                   const Level &C)
+{
+    const SGeo g = make_sgeo(F.nx, F.ny, F.nz);
+    SGeo cg = make_sgeo(C.nx, C.ny, 2 * C.nz);   // nh = C.nz
+    const unsigned nb = grid8((uint32_t)C.nx * C.ny * cg.nq);
+    k_srestrict<<<nb, 256, 0, s>>>(r, z, code, C.r, C.tag, g, cg);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+int split_p_Ap(hipStream_t s, const Level &L0, const float *p, float *pout, float *Ap, const float *z,
+               const uint16_t *code, const double *sc, double *part, int *nblk)
+{
+    const SGeo g = make_sgeo(L0.nx, L0.ny, L0.nz);
+    const unsigned nb = grid8((uint32_t)L0.nx * L0.ny * g.nq);
+    *nblk = (int)nb;
+    k_sp_Ap<<<nb, 256, 0, s>>>(p, pout, Ap, z, code, sc, g, part);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}

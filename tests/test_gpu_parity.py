@@ -159,6 +159,46 @@ def test_vcycle_bitexact(smoother, shape, levels, oracle):
     assert bits_equal(mg.z[0].numpy(), orc.get("z", 0))
 
 
+@pytest.mark.parametrize("shape,levels", [((64, 64, 64), 4), ((128, 128, 128), 4), ((64, 32, 48), 3),
+                                          ((256, 256, 256), 4)])
+def test_split_vcycle_bitexact(shape, levels, oracle):
+    """The V-cycle the fused FP32 solve runs (level 0 in the colour-split
+    layout, blocked coarse levels), reached through precondition_into."""
+    import ctypes
+    from flub_amd import _lib
+    n = shape[0]
+    s = scenes.dam_break_spheres(n, 1, n_spheres=8, r_cells=(2, 5)) if len(set(shape)) == 1 else None
+    tag = s["tag"] if s is not None else random_tags(shape, 4, (0.1, 0.85, 0.05))
+    r = np.random.default_rng(12).normal(0, 1, shape).astype(np.float32)
+    r[tag != 1] = 0.0
+    mg = _loaded_mgpcg(tag, levels)
+    rd, zd = wp.from_numpy(r), wp.zeros(shape)
+    _lib.call("flub_mgpcg_precondition_into", mg.handle, ctypes.c_void_p(rd.ptr), ctypes.c_void_p(zd.ptr), 1)
+    orc = oracle.MGPCG(shape, levels, pre_and_post_smoothing=2, bottom_smoothing=6)
+    orc.set_tags(tag)
+    orc.put("r", r, 0)
+    orc.apply_preconditioner("phases")
+    assert bits_equal(zd.numpy(), orc.get("z", 0))
+
+
+def test_split_solve_matches_reference_layout(monkeypatch):
+    """The fused solve on the split level 0 and on the reference layout take
+    the same iterations to the same solution (only FP64 dot orders differ)."""
+    s = scenes.dam_break_spheres(64, 1)
+    res = []
+    for env in (None, "1"):
+        if env:
+            monkeypatch.setenv("FLUB_NO_SPLIT", env)
+        pr = Projection((64, 64, 64), n_mg_levels=4, pre_and_post_smoothing=2, bottom_smoothing=6)
+        pr.load(s)
+        it = pr.project(max_iters=-1, rel_tol=1e-12)
+        res.append((it, pr.p0.f.numpy(), pr.rTr))
+    (i0, p0, r0), (i1, p1, r1) = res
+    assert abs(i0 - i1) <= 1
+    assert _rel(p0, p1) < 1e-6
+    assert r0 < 1.0 and r1 < 1.0
+
+
 def _rel(a, b):
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
 
