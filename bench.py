@@ -34,10 +34,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 MFMA_BF16_PEAK_TFLOPS = 2500.0
-# HBM bytes per launch of the level-0 FP32 half-sweep at 256^3, from two
-# rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of `bench.py --kernels-only`,
-# calibrated on a 67.4 MB device copy (profiles/rb_half_sweep_l0_traffic_256.md)
-MEASURED_TRAFFIC_L0 = {("rb_half_sweep_l0", 256): 242.7e6}
+# HBM bytes per launch of the level-0 FP32 half-sweeps at 256^3, from two
+# rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of `bench.py --kernels-only
+# [--kernel ...]`, calibrated on a 67.4 MB device copy
+# (profiles/rb_half_sweep_l0_split_traffic_256.md, profiles/rb_half_sweep_l0_traffic_256.md)
+MEASURED_TRAFFIC_L0 = {("rb_half_sweep_l0_split", 256): 158.6e6, ("rb_half_sweep_l0", 256): 242.7e6}
 
 
 def log(*a):
