@@ -4,8 +4,8 @@
 // few microseconds of dispatch and memory latency for very little data, so a
 // V-cycle that launches one kernel per half-sweep (mgpcg.py:296-349: 4*s*2^l
 // of them on level l) is bound by launch latency there.  This kernel runs up
-// to TB_G consecutive half-sweeps in one launch.  Each workgroup owns a core
-// tile of CI x CJ x CK cells, loads z on the core plus a halo of TB_G cells
+// to G consecutive half-sweeps in one launch.  Each workgroup owns a core
+// tile of CI x CJ x CK cells, loads z on the core plus a halo of G cells
 // into LDS, and applies the half-sweeps in LDS: step s (of n) updates the
 // cells of its colour within distance n-1-s of the core (the 7-point
 // dependency cone), so after n steps the core holds exactly the values the
@@ -40,9 +40,9 @@ struct TbArgs {
     float pfac;
     int nx, ny, nz;
     int ti, tj, tk;       // tiles per axis
-    int nsteps;           // 1..TB_G
+    int nsteps;           // 1..G (the configuration's halo width)
     uint32_t phases;      // bit s: phase of step s (updates (i+j+k)&1 == phase^1)
-    int cnt[TB_G];        // cnt[d]: pairs within distance d of the core
+    int cnt[TB_GMAX];     // cnt[d]: pairs within distance d of the core
 };
 
 // x / d for d in 1..6 (y = RN(1/d)): q0 = x*y, one FMA residual step.  For
@@ -82,13 +82,12 @@ __device__ __forceinline__ float tb_update(uint32_t cd, float z, float r, float 
 }
 
 // Kernel layout of a pair entry in `order`: LDS index of its even cell (bits
-// 0-15) and the pair's distance to the core (bits 16-18).  Register layout of
+// 0-15) and the pair's distance to the core (bits 16-19).  Register layout of
 // an owned pair: LDS index (0-15), parity of the even cell (16), distance of
-// the even / odd cell (17-19 / 20-22; 7 for a pair outside the domain).
-template <typename T, int CI, int CJ, int CK, int NT, int MODE, bool DOT>
+// the even / odd cell (17-20 / 21-24; 15 for a pair outside the domain).
+template <typename T, int CI, int CJ, int CK, int NT, int G, int MODE, bool DOT>
 __global__ void __launch_bounds__(NT) k_tb(TbArgs a)
 {
-    constexpr int G = TB_G;
     constexpr int BI = CI + 2 * G, BJ = CJ + 2 * G, BK = CK + 2 * G;
     constexpr int SJ = BK, SI = BJ * BK;
     constexpr int BP = BK / 2;                  // cell pairs per LDS row
@@ -146,7 +145,7 @@ __global__ void __launch_bounds__(NT) k_tb(TbArgs a)
 #pragma unroll
     for (int u = 0; u < NU; ++u) {
         const int q = threadIdx.x + u * NT;
-        r0[u] = 0.0f; r1[u] = 0.0f; cw[u] = 0u; pk[u] = 0x7e0000u;   // distances 7: never updated
+        r0[u] = 0.0f; r1[u] = 0.0f; cw[u] = 0u; pk[u] = 0x1fe0000u;   // distances 15: never updated
         if (q < NOWN) {
             const uint32_t x0 = a.order[q] & 0xffffu;
             const int li = (int)(x0 / SI), lj = (int)((x0 / BK) % BJ), lk = (int)(x0 % BK);
@@ -160,7 +159,7 @@ __global__ void __launch_bounds__(NT) k_tb(TbArgs a)
                 auto dist = [](int l, int c) { return l < G ? G - l : (l >= G + c ? l - (G + c - 1) : 0); };
                 const int dij = max(dist(li, CI), dist(lj, CJ));
                 const uint32_t d0 = (uint32_t)max(dij, dist(lk, CK)), d1 = (uint32_t)max(dij, dist(lk + 1, CK));
-                pk[u] = x0 | ((uint32_t)((gi + gj + gk) & 1) << 16) | (d0 << 17) | (d1 << 20);
+                pk[u] = x0 | ((uint32_t)((gi + gj + gk) & 1) << 16) | (d0 << 17) | (d1 << 21);
             }
         }
     }
@@ -209,7 +208,7 @@ __global__ void __launch_bounds__(NT) k_tb(TbArgs a)
                 if (u >= NU || u * NT >= cnt) continue;
                 const uint32_t p = pk[u];
                 const uint32_t sel = ((p >> 16) & 1u) ^ phase ^ 1u;   // the pair's cell of colour phase^1
-                const uint32_t d = (p >> (17 + 3 * sel)) & 7u;
+                const uint32_t d = (p >> (17 + 4 * sel)) & 15u;
                 const bool act = d <= lim;
                 const int x = act ? (int)(p & 0xffffu) + (int)sel : XDUMMY;
                 const uint32_t cd = (cw[u] >> (16 * sel)) & 0xffffu;
@@ -243,7 +242,7 @@ __global__ void __launch_bounds__(NT) k_tb(TbArgs a)
         for (int u = 0; u < NU; ++u) {
             if (u * NT >= a.cnt[0]) break;
             const uint32_t p = pk[u];
-            if (((p >> 17) & 0x3fu) != 0u) continue;
+            if (((p >> 17) & 0xffu) != 0u) continue;
             const int x = (int)(p & 0xffffu);
             acc += (double)zs[x] * (double)r0[u] + (double)zs[x + 1] * (double)r1[u];
         }
@@ -262,28 +261,37 @@ __global__ void __launch_bounds__(NT) k_tb(TbArgs a)
     }
 }
 
-// Tile configurations.
-enum { TB_CFG_16 = 0, TB_CFG_8x16 = 1, TB_CFG_8 = 2, TB_NCFG = 3 };
+// Tile configurations: core CI x CJ x CK, threads, halo width G (half-sweeps
+// per launch).  The G = 8 configurations (8 half-sweeps per launch on levels
+// of at most 64^3 cells) are optional: measured on a 256^3 solve, their larger
+// halos cost more than the launches they save.
+enum { TB_CFG_16 = 0, TB_CFG_8x16 = 1, TB_CFG_8 = 2, TB_CFG_8x16_G8 = 3, TB_CFG_8_G8 = 4, TB_NCFG = 5 };
+struct TbGeom {
+    int ci, cj, ck, nt, g;
+};
+const TbGeom kGeom[TB_NCFG] = {{16, 16, 16, 1024, 4}, {8, 8, 16, 1024, 4}, {8, 8, 8, 1024, 4},
+                               {8, 8, 16, 512, 8}, {8, 8, 8, 512, 8}};
+#define TB_G8_MAX_CELLS (1u << 18)
 
-int tb_cfg(const Level &L)
+bool fits(const Level &L, int c) { return L.nx % kGeom[c].ci == 0 && L.ny % kGeom[c].cj == 0 && L.nz % kGeom[c].ck == 0; }
+
+int tb_cfg(const Level &L, bool allow_g8)
 {
-    if (L.nx % 16 == 0 && L.ny % 16 == 0 && L.nz % 16 == 0 && L.n >= (1u << 21)) return TB_CFG_16;
-    if (L.nx % 8 == 0 && L.ny % 8 == 0 && L.nz % 16 == 0) return TB_CFG_8x16;
-    if (L.nx % 8 == 0 && L.ny % 8 == 0 && L.nz % 8 == 0) return TB_CFG_8;
+    if (allow_g8 && L.n <= TB_G8_MAX_CELLS) {
+        if (fits(L, TB_CFG_8x16_G8)) return TB_CFG_8x16_G8;
+        if (fits(L, TB_CFG_8_G8)) return TB_CFG_8_G8;
+    }
+    if (L.n >= (1u << 21) && fits(L, TB_CFG_16)) return TB_CFG_16;
+    if (fits(L, TB_CFG_8x16)) return TB_CFG_8x16;
+    if (fits(L, TB_CFG_8)) return TB_CFG_8;
     return -1;
 }
-
-// Tile geometry of the configurations: core CI x CJ x CK, threads.
-struct TbGeom {
-    int ci, cj, ck, nt;
-};
-const TbGeom kGeom[TB_NCFG] = {{16, 16, 16, 512}, {8, 8, 16, 256}, {8, 8, 8, 256}};
 
 // Order tables (host copy): the updatable pairs of a tile sorted by their
 // distance to the core (stable), and the counts within each distance.
 struct TbTable {
     std::vector<uint32_t> order;
-    int cnt[TB_G];
+    int cnt[TB_GMAX];
     size_t offset;   // in the concatenated device table
 };
 
@@ -294,7 +302,7 @@ const std::vector<TbTable> &tb_tables()
         size_t off = 0;
         for (int c = 0; c < TB_NCFG; ++c) {
             const TbGeom &g = kGeom[c];
-            const int G = TB_G, BI = g.ci + 2 * G, BJ = g.cj + 2 * G, BK = g.ck + 2 * G;
+            const int G = g.g, BI = g.ci + 2 * G, BJ = g.cj + 2 * G, BK = g.ck + 2 * G;
             auto dist = [G](int l, int n) { return l < G ? G - l : (l >= G + n ? l - (G + n - 1) : 0); };
             std::vector<std::pair<int, uint32_t>> e;
             for (int li = 1; li < BI - 1; ++li)
@@ -329,9 +337,11 @@ void tb_launch(hipStream_t s, int cfg, TbArgs a)
     const TbGeom &g = kGeom[cfg];
     a.ti = a.nx / g.ci; a.tj = a.ny / g.cj; a.tk = a.nz / g.ck;
     const unsigned nb = (unsigned)(a.ti * a.tj * a.tk);
-    if (cfg == TB_CFG_16) k_tb<T, 16, 16, 16, 512, MODE, DOT><<<nb, 512, 0, s>>>(a);
-    else if (cfg == TB_CFG_8x16) k_tb<T, 8, 8, 16, 256, MODE, DOT><<<nb, 256, 0, s>>>(a);
-    else k_tb<T, 8, 8, 8, 256, MODE, DOT><<<nb, 256, 0, s>>>(a);
+    if (cfg == TB_CFG_16) k_tb<T, 16, 16, 16, 1024, 4, MODE, DOT><<<nb, 1024, 0, s>>>(a);
+    else if (cfg == TB_CFG_8x16) k_tb<T, 8, 8, 16, 1024, 4, MODE, DOT><<<nb, 1024, 0, s>>>(a);
+    else if (cfg == TB_CFG_8) k_tb<T, 8, 8, 8, 1024, 4, MODE, DOT><<<nb, 1024, 0, s>>>(a);
+    else if (cfg == TB_CFG_8x16_G8) k_tb<T, 8, 8, 16, 512, 8, MODE, DOT><<<nb, 512, 0, s>>>(a);
+    else k_tb<T, 8, 8, 8, 512, 8, MODE, DOT><<<nb, 512, 0, s>>>(a);
 }
 
 template <typename T>
@@ -351,9 +361,13 @@ void tb_dispatch(hipStream_t s, int cfg, int mode, const TbArgs &a)
 
 }  // namespace
 
-bool tb_supported(const Level &L) { return tb_cfg(L) >= 0; }
+bool tb_supported(const Level &L) { return tb_cfg(L, true) >= 0; }
 
-int tb_launches(int nsteps) { return (nsteps + TB_G - 1) / TB_G; }
+int tb_launches(const Level &L, int nsteps, bool allow_g8)
+{
+    const int cfg = tb_cfg(L, allow_g8);
+    return cfg < 0 ? 0 : (nsteps + kGeom[cfg].g - 1) / kGeom[cfg].g;
+}
 
 int tb_tables_create(uint32_t **dev)
 {
@@ -373,14 +387,15 @@ int tb_tables_create(uint32_t **dev)
 
 int tb_chain(hipStream_t s, const uint32_t *tables, const Level &L, int prec, int mode0, const void *zin,
              const void *zc, float pfac, const void *r, const uint8_t *phases, int nsteps, void *dst, void *other,
-             double *part, int *nblk)
+             double *part, int *nblk, bool allow_g8)
 {
-    const int cfg = tb_cfg(L);
+    const int cfg = tb_cfg(L, allow_g8);
     FLUB_REQUIRE(tables, FLUB_ESTATE, "blocked smoothing: order tables missing");
     FLUB_REQUIRE(cfg >= 0, FLUB_ESHAPE, "blocked smoothing: level %dx%dx%d is not a multiple of a tile", L.nx, L.ny,
                  L.nz);
     FLUB_REQUIRE(nsteps >= 1 && dst && other && dst != other && r, FLUB_EINVAL, "blocked smoothing: bad arguments");
-    const int m = tb_launches(nsteps);
+    const int G = cfg < 0 ? 1 : kGeom[cfg].g;
+    const int m = (nsteps + G - 1) / G;
     void *w = (m & 1) ? dst : other;    // the last launch writes dst
     FLUB_REQUIRE(mode0 == SWEEP_ZERO || (zin && zin != w), FLUB_EINVAL,
                  "blocked smoothing: the first launch must not write its input");
@@ -392,11 +407,11 @@ int tb_chain(hipStream_t s, const uint32_t *tables, const Level &L, int prec, in
     const TbTable &tab = tb_tables()[cfg];
     const void *in = zin;
     for (int c = 0; c < m; ++c) {
-        const int st = c * TB_G, cnt = nsteps - st < TB_G ? nsteps - st : TB_G;
+        const int st = c * G, cnt = nsteps - st < G ? nsteps - st : G;
         TbArgs a{};
         a.zin = in; a.zout = w; a.zc = zc; a.r = r; a.code = L.code; a.pfac = pfac;
         a.order = tables + tab.offset;
-        for (int d = 0; d < TB_G; ++d) a.cnt[d] = tab.cnt[d];
+        for (int d = 0; d < G; ++d) a.cnt[d] = tab.cnt[d];
         a.nx = L.nx; a.ny = L.ny; a.nz = L.nz;
         a.nsteps = cnt;
         uint32_t seen = 0;

@@ -512,9 +512,9 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
         }
         if (mg_use_tb(h, l, 2 * s, true)) {
             const int n = mg_phase_pairs(ph, s, 0);
-            float *X = (tb_launches(n) & 1) ? F.zpre : F.z;
+            float *X = (tb_launches(F, n, h->tb_g8) & 1) ? F.zpre : F.z;
             TRY(tb_chain(h->s, h->tb_order, F, FLUB_PREC_FP32, SWEEP_ZERO, nullptr, nullptr, 1.0f, F.r, ph, n, X,
-                         X == F.z ? F.zpre : F.z));
+                         X == F.z ? F.zpre : F.z, nullptr, nullptr, h->tb_g8));
             pre_z[l] = X;
         } else {
             TRY(fast_sweep(h->s, F, F.zpre, F.zpre, nullptr, 0, SWEEP_ZERO, nullptr, nullptr));
@@ -531,7 +531,7 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
     if (h->L > 1 && mg_use_tb(h, lb, 2 * (h->bottom / 2) * 2)) {
         Level &B = h->lv[lb];
         const int n = mg_bottom_phases(ph, h->bottom / 2);
-        TRY(tb_chain(h->s, h->tb_order, B, FLUB_PREC_FP32, SWEEP_ZERO, nullptr, nullptr, 1.0f, B.r, ph, n, B.z, B.zpre));
+        TRY(tb_chain(h->s, h->tb_order, B, FLUB_PREC_FP32, SWEEP_ZERO, nullptr, nullptr, 1.0f, B.r, ph, n, B.z, B.zpre, nullptr, nullptr, h->tb_g8));
     } else {
         bottom_solve(h, 1);
     }
@@ -553,7 +553,7 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
         if (mg_use_tb(h, l, 2 * s, true)) {
             const int n = mg_phase_pairs(ph, s, 1);
             TRY(tb_chain(h->s, h->tb_order, F, FLUB_PREC_FP32, SWEEP_PROLONG, pre_z[l], zc, 1.0f, F.r, ph, n, F.z, F.zpre,
-                         l == 0 ? h->part : nullptr, l == 0 ? &h->vc_dot_nblk : nullptr));
+                         l == 0 ? h->part : nullptr, l == 0 ? &h->vc_dot_nblk : nullptr, h->tb_g8));
             continue;
         }
         for (int it = 0; it < s; ++it) {
@@ -653,6 +653,7 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
     h->no_graph = getenv("FLUB_NO_GRAPH") != nullptr;
     h->no_tb = getenv("FLUB_NO_TB") != nullptr;
     h->no_tb_l0 = getenv("FLUB_NO_TB_L0") != nullptr;
+    h->tb_g8 = getenv("FLUB_TB_G8") != nullptr;
     int rc = FLUB_OK;
 #define ALLOC(ptr, bytes)                                              \
     do {                                                               \
@@ -1231,7 +1232,7 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
             const uint8_t ph[4] = {1, 0, 1, 0};
             FLUB_REQUIRE(h->tb_order && tb_supported(L0), FLUB_ESTATE, "no blocked smoothing for this grid");
             return tb_chain(h->s, h->tb_order, L0, FLUB_PREC_FP32, SWEEP_PLAIN, L0.zpre, nullptr, 1.0f, L0.r, ph, 4,
-                            L0.z, L0.zpre);
+                            L0.z, L0.zpre, nullptr, nullptr, h->tb_g8);
         }
         case 8:   // level-0 half-sweep in the colour-split layout
             FLUB_REQUIRE(h->split, FLUB_ESTATE, "no colour-split level 0 for this handle");

@@ -52,6 +52,7 @@ struct flub_mgpcg {
     bool no_graph;
     bool no_tb;        // launch-per-half-sweep smoothing on every level (no temporal blocking)
     bool no_tb_l0;     // no temporal blocking on level 0
+    bool tb_g8;        // blocked smoothing may take 8 half-sweeps per launch on small levels
     uint32_t *tb_order;  // update-order tables of the blocked smoother
     hipGraphExec_t graph[5];   // per smoother; [3], [4]: the V-cycle on the split level 0 into sp.z[0], sp.z[1]
     bool graph_ok[5];

@@ -256,8 +256,8 @@ static int vcycle_mixed(flub_mgpcg *h)
         if (mg_use_tb(h, l, 2 * s)) {
             Level &F = h->lv[l];
             const int n = mg_phase_pairs(ph, s, 0);
-            void *X = (tb_launches(n) & 1) ? zp : F.zh;
-            TRYM(tb_chain(h->s, h->tb_order, F, h->prec, SWEEP_ZERO, nullptr, nullptr, 8.0f, F.rh, ph, n, X, X == zp ? F.zh : zp));
+            void *X = (tb_launches(F, n, h->tb_g8) & 1) ? zp : F.zh;
+            TRYM(tb_chain(h->s, h->tb_order, F, h->prec, SWEEP_ZERO, nullptr, nullptr, 8.0f, F.rh, ph, n, X, X == zp ? F.zh : zp, nullptr, nullptr, h->tb_g8));
             pre_z[l] = X;
         } else {
             TRYM(mx_sweep(h, l, SWEEP_ZERO, l == 0, SWEEP_OUT_STORE, 0, nullptr, zp, nullptr));
@@ -277,7 +277,7 @@ static int vcycle_mixed(flub_mgpcg *h)
         const int nb = h->bottom / 2;
         if (mg_use_tb(h, lb, 4 * nb)) {
             const int n = mg_bottom_phases(ph, nb);
-            TRYM(tb_chain(h->s, h->tb_order, B, h->prec, SWEEP_ZERO, nullptr, nullptr, 8.0f, B.rh, ph, n, z, h->mx->zph[lb]));
+            TRYM(tb_chain(h->s, h->tb_order, B, h->prec, SWEEP_ZERO, nullptr, nullptr, 8.0f, B.rh, ph, n, z, h->mx->zph[lb], nullptr, nullptr, h->tb_g8));
         } else {
             bool started = false;
             for (int pass = 0; pass < 2; ++pass)
@@ -302,7 +302,7 @@ static int vcycle_mixed(flub_mgpcg *h)
         const int s = h->pre_post << l;
         if (mg_use_tb(h, l, 2 * s)) {
             const int n = mg_phase_pairs(ph, s, 1);
-            TRYM(tb_chain(h->s, h->tb_order, h->lv[l], h->prec, SWEEP_PROLONG, pre_z[l], zc, 8.0f, h->lv[l].rh, ph, n, z, zp));
+            TRYM(tb_chain(h->s, h->tb_order, h->lv[l], h->prec, SWEEP_PROLONG, pre_z[l], zc, 8.0f, h->lv[l].rh, ph, n, z, zp, nullptr, nullptr, h->tb_g8));
             continue;
         }
         for (int it = 0; it < s; ++it) {

@@ -37,19 +37,20 @@ int fast_restrict(hipStream_t s, const Level &F, const float *zf, const Level &C
 int fast_p_Ap(hipStream_t s, const Level &L0, const float *p, float *pout, float *Ap, const double *sc,
               double *part, int *nblk);
 
-// Temporally blocked smoothing (blocked.hip): up to TB_G half-sweeps per launch.
-#define TB_G 4
+// Temporally blocked smoothing (blocked.hip): up to 4 or 8 half-sweeps per
+// launch, depending on the level's tile configuration.
+#define TB_GMAX 8
 // The level's shape is a multiple of one of the tile shapes.
 bool tb_supported(const Level &L);
-// Launches needed for nsteps half-sweeps.
-int tb_launches(int nsteps);
+// Launches needed for nsteps half-sweeps of level L.
+int tb_launches(const Level &L, int nsteps, bool allow_g8);
 // nsteps half-sweeps (phase of step s: phases[s]) of level L in the storage
 // type of prec; the first starts as mode0 (SWEEP_ZERO, SWEEP_PLAIN from zin,
 // SWEEP_PROLONG from zin + pfac * zc[parent]).  The launches alternate
 // between the buffers dst and other, the last one writing dst; the first one
 // must not write zin (zin != the first buffer written, which is dst when
-// tb_launches(nsteps) is odd, else other), and each launch (TB_G steps)
-// must contain both phases.
+// tb_launches(L, nsteps, allow_g8) is odd, else other), and each launch
+// must contain both phases.  allow_g8 admits the 8-step configurations.
 // Device copy of the tiles' update-order tables (one per handle).
 int tb_tables_create(uint32_t **dev);
 // With part (FP32; the last launch not a ZERO one) the last launch also writes
@@ -57,7 +58,7 @@ int tb_tables_create(uint32_t **dev);
 // into *nblk.
 int tb_chain(hipStream_t s, const uint32_t *tables, const Level &L, int prec, int mode0, const void *zin,
              const void *zc, float pfac, const void *r, const uint8_t *phases, int nsteps, void *dst, void *other,
-             double *part = nullptr, int *nblk = nullptr);
+             double *part = nullptr, int *nblk = nullptr, bool allow_g8 = true);
 
 #define MG_MAX_PHASES (4 * 4096)
 // Levels above this many cells are bandwidth-bound: one launch per
