@@ -710,6 +710,7 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
         if (e != hipSuccess) { flub_set_error("hipEventCreate: %s", hipGetErrorString(e)); rc = (int)e; goto fail; }
         h->split = precision == FLUB_PREC_FP32 && h->fast && split_supported(h->lv[0]) &&
                    getenv("FLUB_NO_SPLIT") == nullptr;
+        h->fused_pAp = getenv("FLUB_FUSED_PAP") != nullptr;
         if (h->split) {
             ALLOC(h->sp.r, n0 * 4);
             ALLOC(h->sp.z[0], n0 * 4);
@@ -969,9 +970,15 @@ static int solve_fast(flub_mgpcg *h, int max_iters, double eps, double abs_tol, 
         // speculate unless this test is predicted to stop the solve (a wasted
         // speculative cycle would delay the end of the solve)
         const bool speculate = split && it + 1 < cap && r_prev * rate > 16.0 * tol;
-        if (split) TRY(split_p_Ap(h->s, L0, h->p, h->p_alt, h->Ap, Z, h->sp.code, h->sc, h->part, &nbp));
-        else TRY(fast_p_Ap(h->s, L0, h->p, h->p_alt, h->Ap, h->sc, h->part, &nbp));
-        float *t = h->p; h->p = h->p_alt; h->p_alt = t;
+        if (split && !h->fused_pAp) {   // p = z + beta*p, then Ap = ndt(p)
+            k_update_p<<<nb256(n), 256, 0, h->s>>>(h->p, Z, h->sc, n);
+            FLUB_CHECK_LAUNCH();
+            TRY(split_p_Ap(h->s, L0, h->p, nullptr, h->Ap, nullptr, h->sp.code, h->sc, h->part, &nbp));
+        } else {
+            if (split) TRY(split_p_Ap(h->s, L0, h->p, h->p_alt, h->Ap, Z, h->sp.code, h->sc, h->part, &nbp));
+            else TRY(fast_p_Ap(h->s, L0, h->p, h->p_alt, h->Ap, h->sc, h->part, &nbp));
+            float *t = h->p; h->p = h->p_alt; h->p_alt = t;
+        }
         TRY(launch_finish(h, nbp, FIN_ALPHA, eps));
         k_update_rx_dot<<<nb, RBS, 0, h->s>>>(R, X, h->Ap, h->p, h->sc, n, h->part);
         FLUB_CHECK_LAUNCH();
@@ -1247,6 +1254,14 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
             TRY(split_p_Ap(h->s, L0, h->p, h->p_alt, h->Ap, h->sp.z[0], h->sp.code, h->sc, h->part, &nbp));
             return launch_finish(h, nbp, FIN_SUM, 0.0);
         }
+        case 11: {   // the unfused pair: p update, then Ap of p (split)
+            FLUB_REQUIRE(h->split, FLUB_ESTATE, "no colour-split level 0 for this handle");
+            int nbp = 0;
+            k_update_p<<<nb256(L0.n), 256, 0, h->s>>>(h->p, h->sp.z[0], h->sc, L0.n);
+            FLUB_CHECK_LAUNCH();
+            TRY(split_p_Ap(h->s, L0, h->p, nullptr, h->Ap, nullptr, h->sp.code, h->sc, h->part, &nbp));
+            return launch_finish(h, nbp, FIN_SUM, 0.0);
+        }
         default: flub_set_error("unknown kernel id %d", kernel); return FLUB_EINVAL;
         }
     };
@@ -1262,6 +1277,7 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
     case 8: b = N * (2 * v + 1); break;
     case 9: b = N * (2 * v + 1) + N / 8 * (v + 1); break;
     case 10: b = N * (4 * 4 + 1); break;
+    case 11: b = N * (4 * 4 + 1); break;
     default: break;
     }
     int rc = launch_once();   // warm-up (captures the graph for id 3)

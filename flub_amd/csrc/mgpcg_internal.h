@@ -58,6 +58,7 @@ struct flub_mgpcg {
     bool graph_ok[5];
     hipEvent_t ev;     // marks the residual norm's copy to the host
     bool split;        // level 0 of the FP32 solve runs in the colour-split layout
+    bool fused_pAp;    // split solve: p update fused into the Ap kernel (else an elementwise pass first)
     SplitBuf sp;
     MixedState *mx;
 };

@@ -99,6 +99,7 @@ int split_sweep(hipStream_t s, const Level &L0, const float *zin, float *zout, c
 // Restriction of the split level 0 (r, z) into level 1's r.
 int split_restrict(hipStream_t s, const Level &F, const float *r, const float *z, const uint16_t *code,
                    const Level &C);
-// pout = z + beta*p; Ap = ndt(pout); partials of pout.Ap (all split).
+// pout = z + beta*p; Ap = ndt(pout); partials of pout.Ap (all split).  With
+// pout == nullptr: Ap = ndt(p) and the partials of p.Ap (z unused).
 int split_p_Ap(hipStream_t s, const Level &L0, const float *p, float *pout, float *Ap, const float *z,
                const uint16_t *code, const double *sc, double *part, int *nblk);

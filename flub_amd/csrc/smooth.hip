@@ -100,11 +100,14 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgs a)
         g.decode(t, i, j, k0);
         const uint32_t base = cidx(i, j, k0, g.ny, g.nz);
         float c[4], xp[4], xm[4], yp[4], ym[4];
+        // neighbour rows clamped to the domain (a wall neighbour reads the cell's
+        // own row, as the reference's clamped sample does; its class is 0):
+        // no branches, so all loads issue at once
         row4<T, MODE>(a, i, j, k0, c);
-        if (i + 1 < g.nx) row4<T, MODE>(a, i + 1, j, k0, xp); else { xp[0] = c[0]; xp[1] = c[1]; xp[2] = c[2]; xp[3] = c[3]; }
-        if (i > 0) row4<T, MODE>(a, i - 1, j, k0, xm); else { xm[0] = c[0]; xm[1] = c[1]; xm[2] = c[2]; xm[3] = c[3]; }
-        if (j + 1 < g.ny) row4<T, MODE>(a, i, j + 1, k0, yp); else { yp[0] = c[0]; yp[1] = c[1]; yp[2] = c[2]; yp[3] = c[3]; }
-        if (j > 0) row4<T, MODE>(a, i, j - 1, k0, ym); else { ym[0] = c[0]; ym[1] = c[1]; ym[2] = c[2]; ym[3] = c[3]; }
+        row4<T, MODE>(a, min(i + 1, g.nx - 1), j, k0, xp);
+        row4<T, MODE>(a, max(i - 1, 0), j, k0, xm);
+        row4<T, MODE>(a, i, min(j + 1, g.ny - 1), k0, yp);
+        row4<T, MODE>(a, i, max(j - 1, 0), k0, ym);
         // k neighbours from the adjacent lanes (same row), loaded at wavefront edges
         const float left = __shfl_up(c[3], 1, 64), right = __shfl_down(c[0], 1, 64);
         float zkm = c[0], zk4 = c[3];
@@ -188,11 +191,11 @@ __global__ void __launch_bounds__(256) k_restrict(const T *__restrict__ r, const
                 const int i = 2 * I + aa, j = 2 * J + b;
                 const uint32_t base = cidx(i, j, k0, ny, nz);
                 float c[4], xp[4], xm[4], yp[4], ym[4];
-                ld4<T>(z + base, c);
-                if (i + 1 < nx) ld4<T>(z + cidx(i + 1, j, k0, ny, nz), xp); else { xp[0] = c[0]; xp[1] = c[1]; xp[2] = c[2]; xp[3] = c[3]; }
-                if (i > 0) ld4<T>(z + cidx(i - 1, j, k0, ny, nz), xm); else { xm[0] = c[0]; xm[1] = c[1]; xm[2] = c[2]; xm[3] = c[3]; }
-                if (j + 1 < ny) ld4<T>(z + cidx(i, j + 1, k0, ny, nz), yp); else { yp[0] = c[0]; yp[1] = c[1]; yp[2] = c[2]; yp[3] = c[3]; }
-                if (j > 0) ld4<T>(z + cidx(i, j - 1, k0, ny, nz), ym); else { ym[0] = c[0]; ym[1] = c[1]; ym[2] = c[2]; ym[3] = c[3]; }
+                ld4<T>(z + base, c);   // neighbour rows clamped to the domain (walls: class 0)
+                ld4<T>(z + cidx(min(i + 1, nx - 1), j, k0, ny, nz), xp);
+                ld4<T>(z + cidx(max(i - 1, 0), j, k0, ny, nz), xm);
+                ld4<T>(z + cidx(i, min(j + 1, ny - 1), k0, ny, nz), yp);
+                ld4<T>(z + cidx(i, max(j - 1, 0), k0, ny, nz), ym);
                 const float left = __shfl_up(c[3], 1, 64), right = __shfl_down(c[0], 1, 64);
                 float zkm = c[0], zk4 = c[3];
                 if (k0 > 0) zkm = lane > 0 ? left : ldv<T>(z[base - 1]);
@@ -260,11 +263,11 @@ __global__ void __launch_bounds__(256) k_p_Ap(const float *__restrict__ p, float
         g.decode(t, i, j, k0);
         const uint32_t base = cidx(i, j, k0, g.ny, g.nz);
         float c[4], xp[4], xm[4], yp[4], ym[4];
-        pq(i, j, k0, c);
-        if (i + 1 < g.nx) pq(i + 1, j, k0, xp); else { xp[0] = c[0]; xp[1] = c[1]; xp[2] = c[2]; xp[3] = c[3]; }
-        if (i > 0) pq(i - 1, j, k0, xm); else { xm[0] = c[0]; xm[1] = c[1]; xm[2] = c[2]; xm[3] = c[3]; }
-        if (j + 1 < g.ny) pq(i, j + 1, k0, yp); else { yp[0] = c[0]; yp[1] = c[1]; yp[2] = c[2]; yp[3] = c[3]; }
-        if (j > 0) pq(i, j - 1, k0, ym); else { ym[0] = c[0]; ym[1] = c[1]; ym[2] = c[2]; ym[3] = c[3]; }
+        pq(i, j, k0, c);   // neighbour rows clamped to the domain (walls: class 0)
+        pq(min(i + 1, g.nx - 1), j, k0, xp);
+        pq(max(i - 1, 0), j, k0, xm);
+        pq(i, min(j + 1, g.ny - 1), k0, yp);
+        pq(i, max(j - 1, 0), k0, ym);
         const float left = __shfl_up(c[3], 1, 64), right = __shfl_down(c[0], 1, 64);
         float zkm = c[0], zk4 = c[3];
         if (k0 > 0) zkm = lane > 0 ? left : fmaf(beta, p[base - 1], z[base - 1]);

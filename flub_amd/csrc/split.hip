@@ -100,13 +100,15 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
             if (MODE == SWEEP_PROLONG) v = v + a.zc[cidx(ii >> 1, jj >> 1, m, g.ny >> 1, g.nh)];
             return v;
         };
+        // neighbour rows clamped to the domain (a wall neighbour has stencil
+        // class 0 and is never used): no branches, so all loads issue at once
         float c[4], w[4], xp[4], xm[4], yp[4], ym[4];
         zq(own, i, j, c);
         zq(oth, i, j, w);
-        if (i + 1 < g.nx) zq(oth, i + 1, j, xp); else { xp[0] = xp[1] = xp[2] = xp[3] = 0.0f; }
-        if (i > 0) zq(oth, i - 1, j, xm); else { xm[0] = xm[1] = xm[2] = xm[3] = 0.0f; }
-        if (j + 1 < g.ny) zq(oth, i, j + 1, yp); else { yp[0] = yp[1] = yp[2] = yp[3] = 0.0f; }
-        if (j > 0) zq(oth, i, j - 1, ym); else { ym[0] = ym[1] = ym[2] = ym[3] = 0.0f; }
+        zq(oth, min(i + 1, g.nx - 1), j, xp);
+        zq(oth, max(i - 1, 0), j, xm);
+        zq(oth, i, min(j + 1, g.ny - 1), yp);
+        zq(oth, i, max(j - 1, 0), ym);
         // the k-neighbour beyond the quad: other colour at m0-1 (o = 0) or m0+4 (o = 1)
         const float left = __shfl_up(w[3], 1, 64), right = __shfl_down(w[0], 1, 64);
         float edge = 0.0f;   // outside the domain: stencil class 0, unused
@@ -174,11 +176,11 @@ __global__ void __launch_bounds__(256) k_srestrict(const float *__restrict__ r, 
                     const uint32_t own = (uint32_t)cc * g.half, oth = (uint32_t)(cc ^ 1) * g.half;
                     const int o = (cc - i - j) & 1;   // this colour's child is kk = o
                     const float *w = zz[cc ^ 1];
-                    float xp[4], xm[4], yp[4], ym[4];
-                    if (i + 1 < g.nx) f4(z + oth + g.row(i + 1, j) + K0, xp); else { xp[0] = xp[1] = xp[2] = xp[3] = 0.0f; }
-                    if (i > 0) f4(z + oth + g.row(i - 1, j) + K0, xm); else { xm[0] = xm[1] = xm[2] = xm[3] = 0.0f; }
-                    if (j + 1 < g.ny) f4(z + oth + g.row(i, j + 1) + K0, yp); else { yp[0] = yp[1] = yp[2] = yp[3] = 0.0f; }
-                    if (j > 0) f4(z + oth + g.row(i, j - 1) + K0, ym); else { ym[0] = ym[1] = ym[2] = ym[3] = 0.0f; }
+                    float xp[4], xm[4], yp[4], ym[4];   // rows clamped to the domain (walls: class 0)
+                    f4(z + oth + g.row(min(i + 1, g.nx - 1), j) + K0, xp);
+                    f4(z + oth + g.row(max(i - 1, 0), j) + K0, xm);
+                    f4(z + oth + g.row(i, min(j + 1, g.ny - 1)) + K0, yp);
+                    f4(z + oth + g.row(i, max(j - 1, 0)) + K0, ym);
                     const float left = __shfl_up(w[3], 1, 64), right = __shfl_down(w[0], 1, 64);
                     float edge = 0.0f;
                     if (o == 0) {
@@ -219,10 +221,12 @@ __global__ void __launch_bounds__(256) k_srestrict(const float *__restrict__ r, 
     }
 }
 
-// p_new = z + beta*p_old at every stencil point (mgpcg.py:229-232), then
-// Ap = ndt(p_new) (mgpcg.py:36-39) and the FP64 partial of p_new.Ap, in the
-// split layout; each thread does the quads of both colours at one position.
-// p_new is stored to pout (the neighbours still read p_old).
+// FUSED: p_new = z + beta*p_old at every stencil point (mgpcg.py:229-232),
+// stored to pout (the neighbours still read p_old); otherwise p_new = p (the
+// p update ran before).  Then Ap = ndt(p_new) (mgpcg.py:36-39) and the FP64
+// partial of p_new.Ap, in the split layout; each thread does the quads of
+// both colours at one position.
+template <bool FUSED>
 __global__ void __launch_bounds__(256) k_sp_Ap(const float *__restrict__ p, float *__restrict__ pout,
                                                float *__restrict__ Ap, const float *__restrict__ z,
                                                const uint16_t *__restrict__ code, const double *__restrict__ sc,
@@ -239,12 +243,14 @@ __global__ void __launch_bounds__(256) k_sp_Ap(const float *__restrict__ p, floa
         g.decode(t, i, j, m0);
         auto pq = [&](uint32_t h, int ii, int jj, float v[4]) {
             const uint32_t b = h + g.row(ii, jj) + m0;
+            if (!FUSED) { f4(p + b, v); return; }
             float a[4], zz[4];
             f4(p + b, a);
             f4(z + b, zz);
             v[0] = fmaf(beta, a[0], zz[0]); v[1] = fmaf(beta, a[1], zz[1]);
             v[2] = fmaf(beta, a[2], zz[2]); v[3] = fmaf(beta, a[3], zz[3]);
         };
+        auto p1 = [&](uint32_t b) -> float { return FUSED ? fmaf(beta, p[b], z[b]) : p[b]; };
         const uint32_t base = g.row(i, j) + m0;
         float pn[2][4];
         pq(0, i, j, pn[0]);
@@ -256,18 +262,16 @@ __global__ void __launch_bounds__(256) k_sp_Ap(const float *__restrict__ p, floa
             const uint32_t own = (uint32_t)cc * g.half, oth = (uint32_t)(cc ^ 1) * g.half;
             const int o = (cc - i - j) & 1;
             const float *w = pn[cc ^ 1];
-            float xp[4], xm[4], yp[4], ym[4];
-            if (i + 1 < g.nx) pq(oth, i + 1, j, xp); else { xp[0] = xp[1] = xp[2] = xp[3] = 0.0f; }
-            if (i > 0) pq(oth, i - 1, j, xm); else { xm[0] = xm[1] = xm[2] = xm[3] = 0.0f; }
-            if (j + 1 < g.ny) pq(oth, i, j + 1, yp); else { yp[0] = yp[1] = yp[2] = yp[3] = 0.0f; }
-            if (j > 0) pq(oth, i, j - 1, ym); else { ym[0] = ym[1] = ym[2] = ym[3] = 0.0f; }
+            float xp[4], xm[4], yp[4], ym[4];   // rows clamped to the domain (walls: class 0)
+            pq(oth, min(i + 1, g.nx - 1), j, xp);
+            pq(oth, max(i - 1, 0), j, xm);
+            pq(oth, i, min(j + 1, g.ny - 1), yp);
+            pq(oth, i, max(j - 1, 0), ym);
             float edge = 0.0f;
             if (o == 0) {
-                if (m0 > 0) edge = lane > 0 ? (cc ? l0 : l1) : fmaf(beta, p[oth + base - 1], z[oth + base - 1]);
+                if (m0 > 0) edge = lane > 0 ? (cc ? l0 : l1) : p1(oth + base - 1);
             } else {
-                if (m0 + 4 < g.nh)
-                    edge = (lane < 63 && t + 1 < ch.end) ? (cc ? r0 : r1)
-                                                         : fmaf(beta, p[oth + base + 4], z[oth + base + 4]);
+                if (m0 + 4 < g.nh) edge = (lane < 63 && t + 1 < ch.end) ? (cc ? r0 : r1) : p1(oth + base + 4);
             }
             uint32_t cd[4];
             codes4(code, own + base, cd);
@@ -281,7 +285,7 @@ __global__ void __launch_bounds__(256) k_sp_Ap(const float *__restrict__ p, floa
                 out[e] = code_fluid(cd[e]) ? s : 0.0f;
                 acc += (double)pn[cc][e] * (double)out[e];
             }
-            st_f4(pout + own + base, pn[cc]);
+            if (FUSED) st_f4(pout + own + base, pn[cc]);
             st_f4(Ap + own + base, out);
         }
     }
@@ -394,7 +398,8 @@ int split_p_Ap(hipStream_t s, const Level &L0, const float *p, float *pout, floa
     const SGeo g = make_sgeo(L0.nx, L0.ny, L0.nz);
     const unsigned nb = grid8((uint32_t)L0.nx * L0.ny * g.nq);
     *nblk = (int)nb;
-    k_sp_Ap<<<nb, 256, 0, s>>>(p, pout, Ap, z, code, sc, g, part);
+    if (pout) k_sp_Ap<true><<<nb, 256, 0, s>>>(p, pout, Ap, z, code, sc, g, part);
+    else k_sp_Ap<false><<<nb, 256, 0, s>>>(p, nullptr, Ap, nullptr, code, sc, g, part);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
 }

@@ -153,8 +153,9 @@ int flub_mgpcg_level_shape(flub_mgpcg *h, int level, int shape[3]);
  * 0 level-0 RB half-sweep, 1 Ap + p.Ap, 2 r/x update + r.r, 3 whole V-cycle,
  * 4 restriction 0->1, 5 prolongation 1->0, 6 p update, 7 one temporally
  * blocked launch of four level-0 half-sweeps, 8/9/10 the half-sweep,
- * restriction and Ap of the colour-split level 0 the FP32 solve runs on
- * (FLUB_ESTATE when the grid has none).  Leaves the handle's
+ * restriction and fused p update + Ap of the colour-split level 0 the FP32
+ * solve runs on, 11 the p update and Ap as two kernels (FLUB_ESTATE when the
+ * grid has no split level 0).  Leaves the handle's
  * vectors in an unspecified state (call before init / after reading x). */
 int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_ms, double *bytes);
 
