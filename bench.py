@@ -111,7 +111,7 @@ def main():
                 log(f"{name} step {i - args.warmup}: {times[-1]:.3f} ms, {pr.iterations} iterations, "
                     f"rTr {pr.rTr:.3e}")
         solvers[name] = {"ms": round(float(np.mean(times)), 4), "ms_min": round(float(np.min(times)), 4),
-                         "iterations": int(np.median(iters))}
+                         "iterations": int(np.median(iters)), "true_rel_res": true_rel_residual(pr, s["tag"])}
         if name == "fp32":
             kernels = profile_kernels(pr)
         last_pr = pr
@@ -150,6 +150,19 @@ def main():
     if "fp32" in solvers:
         out["speedup_vs_mgpcg_fp32"] = {k: round(solvers["fp32"]["ms"] / v["ms"], 3) for k, v in solvers.items()}
     print(json.dumps(out), flush=True)
+
+
+def true_rel_residual(pr, tag):
+    """||b - ndt(x)|| / ||b|| of the last solve, evaluated in FP64 on the GPU;
+    b = -h * div on FLUID cells, the right-hand side the solve started from."""
+    from flub_amd import _lib, device
+    div = pr.div.f.numpy()
+    b = np.where(tag == 1, div * np.float32(-pr.h), np.float32(0)).astype(np.float32)
+    bd = device.from_numpy(b)
+    rr = ctypes.c_double()
+    _lib.call("flub_mgpcg_residual_norm", pr.mgpcg.handle, pr.mgpcg.x.ptr, bd.ptr, ctypes.byref(rr))
+    bb = float(np.dot(b.astype(np.float64).ravel(), b.astype(np.float64).ravel()))
+    return float(np.sqrt(rr.value / bb)) if bb > 0 else 0.0
 
 
 def profile_kernels(pr):

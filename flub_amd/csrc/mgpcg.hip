@@ -500,6 +500,19 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
     for (int l = 0; l + 1 < h->L; ++l) {
         Level &F = h->lv[l];
         const int s = h->pre_post << l;
+        if (l == 0 && split && h->prec != FLUB_PREC_FP32) {
+            // FP16/BF16 pre-smoothing (z and r), the first two sweeps converting r
+            SplitMixedSweep m{};
+            m.code = S.code; m.rh = S.rh; m.r32 = S.r; m.sc = h->sc; m.zin = S.zh; m.zout = S.zh;
+            for (int it = 0; it < s; ++it)
+                for (int ph = 0; ph < 2; ++ph) {
+                    m.phase = ph;
+                    m.kind = it == 0 ? (ph == 0 ? SMX_PRE_ZERO_SRC : SMX_PRE_SRC) : SMX_PRE;
+                    TRY(split_sweep_mixed(h->s, F, h->prec, m));
+                }
+            TRY(split_restrict_mixed(h->s, F, h->prec, S.rh, S.zh, S.code, h->lv[1]));
+            continue;
+        }
         if (l == 0 && split) {
             TRY(split_sweep(h->s, F, nullptr, S.zpre, nullptr, S.r, S.code, 0, SWEEP_ZERO, nullptr, nullptr));
             TRY(split_sweep(h->s, F, S.zpre, S.zpre, nullptr, S.r, S.code, 1, SWEEP_PLAIN, nullptr, nullptr));
@@ -539,6 +552,24 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
         Level &F = h->lv[l];
         const int s = h->pre_post << l;
         const float *zc = h->lv[l + 1].z;
+        if (l == 0 && split && h->prec != FLUB_PREC_FP32) {
+            // FP32 post-smoothing from the FP16/BF16 pre-smoothed z; the last
+            // sweep writes the CG's z (unscaled) and the partials of z.r, z.q
+            SplitMixedSweep m{};
+            m.code = S.code; m.rh = S.rh; m.r32 = S.r; m.q32 = h->Ap; m.sc = h->sc;
+            for (int it = 0; it < s; ++it)
+                for (int ph = 1; ph >= 0; --ph) {
+                    const bool last = it + 1 == s && ph == 0;
+                    m.phase = ph;
+                    m.zin = it == 0 && ph == 1 ? S.zh : (const void *)S.zw;
+                    m.zout = S.zw;
+                    m.zc = h->lv[1].z;
+                    m.kind = it == 0 && ph == 1 ? SMX_POST_PROLONG : (last ? SMX_FINAL : SMX_POST);
+                    if (last) { m.z32 = S.z[zslot]; m.part = h->part; m.nblk = &h->vc_dot_nblk; }
+                    TRY(split_sweep_mixed(h->s, F, h->prec, m));
+                }
+            continue;
+        }
         if (l == 0 && split) {
             float *Z = S.z[zslot];
             for (int it = 0; it < s; ++it) {
@@ -599,7 +630,7 @@ int mg_precondition(flub_mgpcg *h, int smoother, bool use_graph)
 
 // The V-cycle on the colour-split level 0 (sp.r -> sp.z[zslot]), as a graph
 // in slot 3 + zslot.
-static int mg_precondition_split(flub_mgpcg *h, int zslot)
+int mg_precondition_split(flub_mgpcg *h, int zslot)
 {
     if (h->no_graph) return vcycle_fast(h, true, zslot);
     const int gs = 3 + zslot;
@@ -708,8 +739,7 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
         }
         e = hipEventCreateWithFlags(&h->ev, hipEventDisableTiming);
         if (e != hipSuccess) { flub_set_error("hipEventCreate: %s", hipGetErrorString(e)); rc = (int)e; goto fail; }
-        h->split = precision == FLUB_PREC_FP32 && h->fast && split_supported(h->lv[0]) &&
-                   getenv("FLUB_NO_SPLIT") == nullptr;
+        h->split = h->fast && split_supported(h->lv[0]) && getenv("FLUB_NO_SPLIT") == nullptr;
         h->fused_pAp = getenv("FLUB_FUSED_PAP") != nullptr;
         if (h->split) {
             ALLOC(h->sp.r, n0 * 4);
@@ -718,6 +748,12 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
             ALLOC(h->sp.zpre, n0 * 4);
             ALLOC(h->sp.x, n0 * 4);
             ALLOC(h->sp.code, n0 * 2);
+            if (precision != FLUB_PREC_FP32) {
+                ALLOC(h->sp.rh, n0 * 2);
+                ALLOC(h->sp.zh, n0 * 2);
+                ALLOC(h->sp.zw, n0 * 4);
+                ALLOC(h->sp.b, n0 * 4);
+            }
         }
     }
 #undef ALLOC
@@ -741,7 +777,7 @@ FLUB_API int flub_mgpcg_destroy(flub_mgpcg *h)
     hipFree(h->x); hipFree(h->p); hipFree(h->p_alt); hipFree(h->Ap); hipFree(h->sc); hipFree(h->part);
     hipFree(h->tb_order);
     hipFree(h->sp.r); hipFree(h->sp.z[0]); hipFree(h->sp.z[1]); hipFree(h->sp.zpre); hipFree(h->sp.x);
-    hipFree(h->sp.code);
+    hipFree(h->sp.code); hipFree(h->sp.rh); hipFree(h->sp.zh); hipFree(h->sp.zw); hipFree(h->sp.b);
     if (h->ev) hipEventDestroy(h->ev);
     if (h->hsc) hipHostFree(h->hsc);
     if (h->cap) hipStreamDestroy(h->cap);
@@ -1316,6 +1352,7 @@ FLUB_API int flub_mgpcg_precondition_into(flub_mgpcg *h, const float *r, float *
     Level &L0 = h->lv[0];
     if (h->split && smoother == FLUB_SMOOTHER_PHASES) {
         TRY(split_convert(h->s, L0, r, h->sp.r, 4, true));
+        if (h->prec != FLUB_PREC_FP32) TRY(mg_mixed_rscale_of(h, h->sp.r));
         TRY(mg_precondition_split(h, 0));
         return split_convert(h->s, L0, h->sp.z[0], z, 4, false);
     }

@@ -33,6 +33,11 @@ struct SplitBuf {
     float *r, *zpre, *x;
     float *z[2];    // two z buffers: the solve preconditions speculatively into the spare one
     uint16_t *code;
+    // mixed precision: r and pre-smoothed z of the V-cycle in FP16/BF16, the
+    // post-smoothing's FP32 working z
+    void *rh, *zh;
+    float *zw;
+    float *b;       // mixed precision: the right-hand side (split), for the true residual
 };
 
 struct flub_mgpcg {
@@ -66,12 +71,16 @@ struct flub_mgpcg {
 int mg_dot(flub_mgpcg *h, const float *a, const float *b, int op, double eps);
 int mg_zero_centered(flub_mgpcg *h);
 int mg_precondition(flub_mgpcg *h, int smoother, bool use_graph);
+// The V-cycle on the colour-split level 0 (sp.r -> sp.z[zslot]; FP32 or the
+// handle's mixed precision), replayed from a captured graph.
+int mg_precondition_split(flub_mgpcg *h, int zslot);
 
 // alpha = old_zTr / (sum of the partials + eps) (mgpcg.py:212-216)
 int mg_finish_alpha(flub_mgpcg *h, int nblk, double eps);
 
 // mixed.hip
 int mg_mixed_rscale(flub_mgpcg *h);
+int mg_mixed_rscale_of(flub_mgpcg *h, const float *r);
 int mg_mixed_setup(flub_mgpcg *h);
 void mg_mixed_teardown(flub_mgpcg *h);
 int mg_vcycle_mixed(flub_mgpcg *h, int smoother);

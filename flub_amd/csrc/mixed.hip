@@ -97,6 +97,24 @@ __global__ void __launch_bounds__(RBS) k_true_residual(float *__restrict__ r, co
     }
 }
 
+// Sum of nblk partials (fixed order) into out[0].
+__global__ void __launch_bounds__(RBS) k_sum_partials(const double *__restrict__ part, int nblk, double *__restrict__ out)
+{
+    __shared__ double lds[RBS / 64];
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += RBS) acc += part[b];
+    const double s = block_sum<RBS>(acc, lds);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+// p = z + beta*p (mgpcg.py:229-232), elementwise (any layout).
+__global__ void k_update_p_mx(float *__restrict__ p, const float *__restrict__ z, const double *__restrict__ sc,
+                              uint32_t n)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = fmaf((float)sc[SC_BETA], p[t], z[t]);
+}
+
 // Partials of r.r and max|r| without changing r.
 __global__ void __launch_bounds__(RBS) k_mx_rnorm(const float *__restrict__ r, uint32_t n, double *__restrict__ part)
 {
@@ -209,6 +227,19 @@ int mg_mixed_rscale(flub_mgpcg *h)
     k_mx_rnorm<<<nb, RBS, 0, h->s>>>(L0.r, L0.n, h->part);
     FLUB_CHECK_LAUNCH();
     k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nb, h->sc, h->mx->shift);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+// The same for the split V-cycle (its reduced-precision z stays O(max|r|):
+// no extra headroom shift).
+int mg_mixed_rscale_of(flub_mgpcg *h, const float *r)
+{
+    const uint32_t n = h->lv[0].n;
+    const int nb = red_nb(n);
+    k_mx_rnorm<<<nb, RBS, 0, h->s>>>(r, n, h->part);
+    FLUB_CHECK_LAUNCH();
+    k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nb, h->sc, 0);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
 }
@@ -361,10 +392,108 @@ static int true_residual(flub_mgpcg *h, int nb)
 // residual b - ndt(x) is recomputed in FP64 and replaces r; the solve stops
 // unless it exceeds the tolerance by more than the drift allowance (then up
 // to max_replacements more rounds).  *rTr reports that FP64 true residual.
+// The same flexible PCG with level 0 in the colour-split layout (split.hip):
+// the V-cycle stores r and the pre-smoothed z of level 0 in FP16/BF16 and
+// keeps the post-smoothing (where z carries the coarse correction) and the
+// coarse levels in FP32.  As in the FP32 solve, the next V-cycle is queued
+// before the host tests the residual, into the spare z buffer, unless this
+// test is predicted to stop the solve.
+static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double abs_tol, double rel_tol, int verbose,
+                             int *iters, double *rTr_out)
+{
+    Level &L0 = h->lv[0];
+    const uint32_t n = L0.n;
+    const int nb = red_nb(n);
+    // The check catches drift of the recursive residual beyond what the FP32
+    // reference algorithm itself shows (about 10x in norm at 256^3).
+    const double drift = 400.0;   // squared norms: 20x in norm
+    const int max_replacements = 3;
+    int replacements = 0;
+    SplitBuf &S = h->sp;
+    TRYM(split_convert(h->s, L0, L0.r, S.r, 4, true));
+    FLUB_CHECK_HIP(hipMemcpyAsync(S.b, S.r, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
+    TRYM(split_convert(h->s, L0, h->x, S.x, 4, true));
+    TRYM(mg_mixed_rscale_of(h, S.r));
+    double rTr0;
+    TRYM(read_sc(h, SC_RTR, &rTr0));
+    const double tol = fmax(abs_tol, rTr0 * rel_tol);
+    float *const p_home = h->p;
+    int zs = 0;
+    TRYM(mg_precondition_split(h, zs));
+    k_mx_finish_beta<<<1, RBS, 0, h->s>>>(h->part, h->vc_dot_nblk, h->sc, eps, 1);
+    FLUB_CHECK_LAUNCH();
+    const int cap = max_iters < 0 ? SOLVE_SAFETY_CAP : max_iters;
+    int it = 0, status = FLUB_OK;
+    double rTr = rTr0, r_prev = rTr0, rate = 1.0;
+    while (it < cap) {
+        const bool speculate = it + 1 < cap && r_prev * rate > 16.0 * tol;
+        // p = z + beta p, then q = ndt(p) and p.q
+        k_update_p_mx<<<blocks_for(n, 256), 256, 0, h->s>>>(h->p, S.z[zs], h->sc, n);
+        FLUB_CHECK_LAUNCH();
+        int nbp = 0;
+        TRYM(split_p_Ap(h->s, L0, h->p, nullptr, h->Ap, nullptr, S.code, h->sc, h->part, &nbp));
+        TRYM(mg_finish_alpha(h, nbp, eps));
+        k_mx_update<<<nb, RBS, 0, h->s>>>(S.r, S.x, h->Ap, h->p, h->sc, n, h->part);
+        FLUB_CHECK_LAUNCH();
+        k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nb, h->sc, 0);
+        FLUB_CHECK_LAUNCH();
+        if (speculate) {
+            FLUB_CHECK_HIP(hipMemcpyAsync(h->hsc + SC_RTR, h->sc + SC_RTR, sizeof(double), hipMemcpyDeviceToHost, h->s));
+            FLUB_CHECK_HIP(hipEventRecord(h->ev, h->s));
+            TRYM(mg_precondition_split(h, zs ^ 1));
+            FLUB_CHECK_HIP(hipEventSynchronize(h->ev));
+            rTr = h->hsc[SC_RTR];
+        } else {
+            TRYM(read_sc(h, SC_RTR, &rTr));
+        }
+        if (verbose) printf("after iter %d, |residual|_2=%.9g\n", it + 1, sqrt(rTr));
+        if (!isfinite(rTr)) { status = FLUB_ENONFINITE; flub_set_error("non-finite residual at iteration %d", it + 1); break; }
+        bool redo = false;   // the speculative cycle used an r that was then replaced
+        if (rTr < tol) {
+            // the reference's test passed on the recursive residual: check the
+            // FP64 true residual for drift (it replaces r either way)
+            int nbt = 0;
+            TRYM(split_true_residual(h->s, L0, S.r, S.b, S.x, S.code, h->part, &nbt));
+            k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nbt, h->sc, 0);
+            FLUB_CHECK_LAUNCH();
+            double rTrue;
+            TRYM(read_sc(h, SC_RTR, &rTrue));
+            if (verbose) printf("  true |residual|_2=%.9g\n", sqrt(rTrue));
+            if (rTrue < drift * tol || replacements >= max_replacements) { rTr = rTrue; break; }
+            ++replacements;
+            redo = true;
+        }
+        rate = r_prev > 0.0 ? fmin(rTr / r_prev, 1.0) : 1.0;
+        r_prev = rTr;
+        if (!speculate || redo) TRYM(mg_precondition_split(h, zs ^ 1));
+        zs ^= 1;
+        k_mx_finish_beta<<<1, RBS, 0, h->s>>>(h->part, h->vc_dot_nblk, h->sc, eps, 0);
+        FLUB_CHECK_LAUNCH();
+        it += 1;
+    }
+    // back to the reference layout (S.zw is free scratch now)
+    TRYM(split_convert(h->s, L0, S.x, h->x, 4, false));
+    TRYM(split_convert(h->s, L0, S.r, L0.r, 4, false));
+    TRYM(split_convert(h->s, L0, S.z[zs], L0.z, 4, false));
+    TRYM(split_convert(h->s, L0, h->Ap, S.zw, 4, false));
+    FLUB_CHECK_HIP(hipMemcpyAsync(h->Ap, S.zw, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
+    TRYM(split_convert(h->s, L0, h->p, S.zw, 4, false));
+    FLUB_CHECK_HIP(hipMemcpyAsync(p_home, S.zw, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
+    if (status == FLUB_OK && it >= cap && max_iters < 0) {
+        status = FLUB_ENOTCONV;
+        flub_set_error("no convergence within %d iterations", cap);
+    }
+    if (verbose) printf("finished with  %d  iterations\n", it);
+    if (iters) *iters = it;
+    if (rTr_out) *rTr_out = rTr;
+    return status;
+}
+
 int mg_solve_mixed(flub_mgpcg *h, int max_iters, double eps, double abs_tol, double rel_tol, int smoother,
                    int verbose, int *iters, double *rTr_out)
 {
     FLUB_REQUIRE(smoother == FLUB_SMOOTHER_PHASES, FLUB_EINVAL, "mixed precision solve supports smoother \"phases\"");
+    if (h->split) return solve_mixed_split(h, max_iters, eps, abs_tol, rel_tol, verbose, iters, rTr_out);
     Level &L0 = h->lv[0];
     const uint32_t n = L0.n;
     const int nb = red_nb(n);
@@ -424,4 +553,21 @@ int mg_solve_mixed(flub_mgpcg *h, int max_iters, double eps, double abs_tol, dou
     if (iters) *iters = it;
     if (rTr_out) *rTr_out = rTr;
     return status;
+}
+
+FLUB_API int flub_mgpcg_residual_norm(flub_mgpcg *h, const float *x, const float *b, double *rr)
+{
+    FLUB_REQUIRE(h && x && b && rr, FLUB_EINVAL, "flub_mgpcg_residual_norm: null argument");
+    FLUB_REQUIRE(h->tags_set, FLUB_ESTATE, "residual_norm before set_tags");
+    Level &L0 = h->lv[0];
+    const int nb = red_nb(L0.n);
+    double *out = h->part + 2 * MAXPART - 1;   // beyond the 2*nb partials
+    k_true_residual<<<nb, RBS, 0, h->s>>>(h->p_alt, b, x, L0.tag, L0.nx, L0.ny, L0.nz, h->part);
+    FLUB_CHECK_LAUNCH();
+    k_sum_partials<<<1, RBS, 0, h->s>>>(h->part, nb, out);
+    FLUB_CHECK_LAUNCH();
+    FLUB_CHECK_HIP(hipMemcpyAsync(h->hsc + SC_AUX3, out, sizeof(double), hipMemcpyDeviceToHost, h->s));
+    FLUB_CHECK_HIP(hipStreamSynchronize(h->s));
+    *rr = h->hsc[SC_AUX3];
+    return FLUB_OK;
 }

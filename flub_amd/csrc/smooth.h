@@ -96,6 +96,29 @@ int split_convert(hipStream_t s, const Level &L0, const void *src, void *dst, in
 // with part, also the partials of z.r (count in *nblk).
 int split_sweep(hipStream_t s, const Level &L0, const float *zin, float *zout, const float *zc, const float *r,
                 const uint16_t *code, int phase, int mode, double *part, int *nblk);
+// Mixed-precision level-0 sweeps on the split layout (T = FP16 or BF16 by
+// prec): pre-smoothing stores z and r as T (the first two sweeps take r from
+// the FP32 CG residual times the storage scale), post-smoothing keeps z in
+// FP32; the last one writes the CG's z = z / scale and the partials of z.r and
+// z.q.
+enum { SMX_PRE_ZERO_SRC = 0, SMX_PRE_SRC, SMX_PRE, SMX_POST_PROLONG, SMX_POST, SMX_FINAL };
+struct SplitMixedSweep {
+    int kind, phase;
+    const void *zin;       // T (pre, post-prolong) or FP32 (post)
+    void *zout;            // T (pre) or FP32 (post)
+    const float *zc;       // level-1 z (post-prolong)
+    void *rh;              // T split r
+    const uint16_t *code;
+    const float *r32, *q32;
+    float *z32;            // FINAL output
+    const double *sc;
+    double *part;
+    int *nblk;
+};
+int split_sweep_mixed(hipStream_t s, const Level &L0, int prec, const SplitMixedSweep &m);
+// Restriction of the split level 0 (r, z stored as T) into level 1's FP32 r.
+int split_restrict_mixed(hipStream_t s, const Level &F, int prec, const void *rh, const void *zh,
+                         const uint16_t *code, const Level &C);
 // Restriction of the split level 0 (r, z) into level 1's r.
 int split_restrict(hipStream_t s, const Level &F, const float *r, const float *z, const uint16_t *code,
                    const Level &C);
@@ -103,3 +126,6 @@ int split_restrict(hipStream_t s, const Level &F, const float *r, const float *z
 // pout == nullptr: Ap = ndt(p) and the partials of p.Ap (z unused).
 int split_p_Ap(hipStream_t s, const Level &L0, const float *p, float *pout, float *Ap, const float *z,
                const uint16_t *code, const double *sc, double *part, int *nblk);
+// r = b - ndt(x) in FP64 (stored FP32), partials of r.r and max|r| (split).
+int split_true_residual(hipStream_t s, const Level &L0, float *r, const float *b, const float *x,
+                        const uint16_t *code, double *part, int *nblk);

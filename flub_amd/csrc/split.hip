@@ -57,21 +57,30 @@ __device__ __forceinline__ void st_f4(float *p, const float v[4])
 }
 
 struct SSweepArgs {
-    const float *zin;      // split z (PLAIN, PROLONG)
-    float *zout;           // split z written
-    const float *zc;       // PROLONG: z of level 1 (reference layout)
-    const float *r;        // split r
+    const void *zin;       // TZI split z (PLAIN, PROLONG)
+    void *zout;            // TZO split z written (STORE / DOT)
+    const float *zc;       // PROLONG: z of level 1 (reference layout, FP32)
+    void *r;               // TR split r of the V-cycle (SRC32: written)
     const uint16_t *code;  // split stencil codes
-    double *part;          // DOT: partials of z.r
+    const float *r32;      // SRC32: the FP32 CG residual (split); FINAL: for z.r
+    const float *q32;      // FINAL: q = -A p of the CG step (split), for z.q
+    float *z32;            // FINAL: the CG's z = z / scale (split, both colours)
+    const double *sc;      // SC_RSCALE: storage scale of r (SRC32, FINAL)
+    double *part;          // DOT: partials of z.r; FINAL: z.r then z.q
     SGeo g;
     int cu;                // colour updated (the half-sweep phase ^ 1)
 };
 
-// One red-black half-sweep of level 0 in the split layout.  ZERO starts from
-// z = 0 and also writes the other colour's zeros; PROLONG reads z +
-// z_coarse[parent] everywhere and also writes the other colour's prolongated
-// values (out of place, zin != zout); DOT accumulates z.r over both colours.
-template <int MODE, bool DOT>
+// One red-black half-sweep of level 0 in the split layout, with z read as
+// TZI, written as TZO and r stored as TR (float, __half or __hip_bfloat16;
+// arithmetic in FP32).  ZERO starts from z = 0 and also writes the other
+// colour's zeros; PROLONG reads z + z_coarse[parent] everywhere and also
+// writes the other colour's prolongated values (out of place, zin != zout).
+// SRC32 takes r of the updated colour from the FP32 CG residual times the
+// storage scale and stores it (rounded to TR, and continues with the rounded
+// values).  OUT: STORE; DOT accumulates z.r over both colours; FINAL writes
+// z / scale of both colours to z32 and accumulates z.r32 and z.q32.
+template <int MODE, int OUT, typename TZI, typename TZO, typename TR, bool SRC32>
 __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
 {
     __shared__ double lds[4];
@@ -79,7 +88,9 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
     const uint32_t total = (uint32_t)g.nx * g.ny * g.nq;
     const int lane = threadIdx.x & 63;
     const uint32_t own = (uint32_t)a.cu * g.half, oth = (uint32_t)(a.cu ^ 1) * g.half;
-    double acc = 0.0;
+    const float s0 = (SRC32 || OUT == SWEEP_OUT_FINAL) ? (float)a.sc[SC_RSCALE] : 1.0f;
+    const TZI *zin = reinterpret_cast<const TZI *>(a.zin);
+    double acc1 = 0.0, acc2 = 0.0;
     const Chunk ch = xcd_chunk(total);
     for (uint32_t t = ch.first; t < ch.end; t += ch.stride) {
         int i, j, m0;
@@ -87,7 +98,7 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
         const int o = (a.cu - i - j) & 1;   // the updated colour sits at k = 2m + o
         auto zq = [&](uint32_t h, int ii, int jj, float v[4]) {
             if (MODE == SWEEP_ZERO) { v[0] = v[1] = v[2] = v[3] = 0.0f; return; }
-            f4(a.zin + h + g.row(ii, jj) + m0, v);
+            ld4<TZI>(zin + h + g.row(ii, jj) + m0, v);
             if (MODE == SWEEP_PROLONG) {
                 float p[4];
                 f4(a.zc + cidx(ii >> 1, jj >> 1, m0, g.ny >> 1, g.nh), p);
@@ -96,7 +107,7 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
         };
         auto z1 = [&](uint32_t h, int ii, int jj, int m) -> float {
             if (MODE == SWEEP_ZERO) return 0.0f;
-            float v = a.zin[h + g.row(ii, jj) + m];
+            float v = ldv<TZI>(zin[h + g.row(ii, jj) + m]);
             if (MODE == SWEEP_PROLONG) v = v + a.zc[cidx(ii >> 1, jj >> 1, m, g.ny >> 1, g.nh)];
             return v;
         };
@@ -121,7 +132,17 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
         uint32_t cd[4];
         codes4(a.code, own + base, cd);
         float rv[4];
-        f4(a.r + own + base, rv);
+        if (SRC32) {
+            f4(a.r32 + own + base, rv);
+            rv[0] = rv[0] * s0; rv[1] = rv[1] * s0; rv[2] = rv[2] * s0; rv[3] = rv[3] * s0;
+            st4<TR>(reinterpret_cast<TR *>(a.r) + own + base, rv);
+            if constexpr (sizeof(TR) != 4) {   // continue with the stored (rounded) values
+                rv[0] = ldv<TR>(stv<TR>(rv[0])); rv[1] = ldv<TR>(stv<TR>(rv[1]));
+                rv[2] = ldv<TR>(stv<TR>(rv[2])); rv[3] = ldv<TR>(stv<TR>(rv[3]));
+            }
+        } else {
+            ld4<TR>(reinterpret_cast<const TR *>(a.r) + own + base, rv);
+        }
         float out[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -130,18 +151,38 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
             else { km = w[e]; kp = e < 3 ? w[e + 1] : edge; }
             out[e] = gs_update(cd[e], c[e], rv[e], xp[e], xm[e], yp[e], ym[e], kp, km);
         }
-        st_f4(a.zout + own + base, out);
-        if (MODE != SWEEP_PLAIN) st_f4(a.zout + oth + base, w);   // zeros / prolongated values
-        if (DOT) {
-            float ro[4];
-            f4(a.r + oth + base, ro);
+        if (OUT == SWEEP_OUT_FINAL) {
+            const float inv = 1.0f / s0;   // a power of two: exact
+            float zo[4], zw[4], ro[4], rw[4], qo[4], qw[4];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) acc += (double)out[e] * (double)rv[e] + (double)w[e] * (double)ro[e];
+            for (int e = 0; e < 4; ++e) { zo[e] = out[e] * inv; zw[e] = w[e] * inv; }
+            st_f4(a.z32 + own + base, zo);
+            st_f4(a.z32 + oth + base, zw);
+            f4(a.r32 + own + base, ro); f4(a.r32 + oth + base, rw);
+            f4(a.q32 + own + base, qo); f4(a.q32 + oth + base, qw);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                acc1 += (double)zo[e] * (double)ro[e] + (double)zw[e] * (double)rw[e];
+                acc2 += (double)zo[e] * (double)qo[e] + (double)zw[e] * (double)qw[e];
+            }
+        } else {
+            st4<TZO>(reinterpret_cast<TZO *>(a.zout) + own + base, out);
+            if (MODE != SWEEP_PLAIN) st4<TZO>(reinterpret_cast<TZO *>(a.zout) + oth + base, w);   // zeros / prolongated
+            if (OUT == SWEEP_OUT_DOT) {
+                float ro[4];
+                ld4<TR>(reinterpret_cast<const TR *>(a.r) + oth + base, ro);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc1 += (double)out[e] * (double)rv[e] + (double)w[e] * (double)ro[e];
+            }
         }
     }
-    if (DOT) {
-        const double s = block_sum<256>(acc, lds);
-        if (threadIdx.x == 0) a.part[blockIdx.x] = s;
+    if (OUT != SWEEP_OUT_STORE) {
+        const double s1 = block_sum<256>(acc1, lds);
+        if (OUT == SWEEP_OUT_FINAL) {
+            const double s2 = block_sum<256>(acc2, lds);
+            if (threadIdx.x == 0) a.part[gridDim.x + blockIdx.x] = s2;
+        }
+        if (threadIdx.x == 0) a.part[blockIdx.x] = s1;
     }
 }
 
@@ -150,7 +191,8 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
 // (I, J, K) are, per child row (2I+a, 2J+b), the cells m = K of both halves.
 // Residual r - (-ndt(z)) per fine cell (0 unless FLUID); a FLUID parent sums
 // its children in the serial launch order (a, b, kk), others get 0.
-__global__ void __launch_bounds__(256) k_srestrict(const float *__restrict__ r, const float *__restrict__ z,
+template <typename TR, typename TZ>
+__global__ void __launch_bounds__(256) k_srestrict(const TR *__restrict__ r, const TZ *__restrict__ z,
                                                    const uint16_t *__restrict__ code, float *__restrict__ rc,
                                                    const uint8_t *__restrict__ tagc, SGeo g, SGeo cg)
 {
@@ -169,29 +211,29 @@ __global__ void __launch_bounds__(256) k_srestrict(const float *__restrict__ r, 
                 const int i = 2 * I + aa, j = 2 * J + b;
                 const uint32_t base = g.row(i, j) + K0;
                 float zz[2][4];
-                f4(z + base, zz[0]);
-                f4(z + g.half + base, zz[1]);
+                ld4<TZ>(z + base, zz[0]);
+                ld4<TZ>(z + g.half + base, zz[1]);
 #pragma unroll
                 for (int cc = 0; cc < 2; ++cc) {
                     const uint32_t own = (uint32_t)cc * g.half, oth = (uint32_t)(cc ^ 1) * g.half;
                     const int o = (cc - i - j) & 1;   // this colour's child is kk = o
                     const float *w = zz[cc ^ 1];
                     float xp[4], xm[4], yp[4], ym[4];   // rows clamped to the domain (walls: class 0)
-                    f4(z + oth + g.row(min(i + 1, g.nx - 1), j) + K0, xp);
-                    f4(z + oth + g.row(max(i - 1, 0), j) + K0, xm);
-                    f4(z + oth + g.row(i, min(j + 1, g.ny - 1)) + K0, yp);
-                    f4(z + oth + g.row(i, max(j - 1, 0)) + K0, ym);
+                    ld4<TZ>(z + oth + g.row(min(i + 1, g.nx - 1), j) + K0, xp);
+                    ld4<TZ>(z + oth + g.row(max(i - 1, 0), j) + K0, xm);
+                    ld4<TZ>(z + oth + g.row(i, min(j + 1, g.ny - 1)) + K0, yp);
+                    ld4<TZ>(z + oth + g.row(i, max(j - 1, 0)) + K0, ym);
                     const float left = __shfl_up(w[3], 1, 64), right = __shfl_down(w[0], 1, 64);
                     float edge = 0.0f;
                     if (o == 0) {
-                        if (K0 > 0) edge = lane > 0 ? left : z[oth + base - 1];
+                        if (K0 > 0) edge = lane > 0 ? left : ldv<TZ>(z[oth + base - 1]);
                     } else {
-                        if (K0 + 4 < g.nh) edge = (lane < 63 && t + 1 < ch.end) ? right : z[oth + base + 4];
+                        if (K0 + 4 < g.nh) edge = (lane < 63 && t + 1 < ch.end) ? right : ldv<TZ>(z[oth + base + 4]);
                     }
                     uint32_t cd[4];
                     codes4(code, own + base, cd);
                     float rv[4];
-                    f4(r + own + base, rv);
+                    ld4<TR>(r + own + base, rv);
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
                         float kp, km;
@@ -293,6 +335,82 @@ __global__ void __launch_bounds__(256) k_sp_Ap(const float *__restrict__ p, floa
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// True residual r = b - ndt(x) with the stencil evaluated in FP64 (from the
+// FP32 x), stored as FP32 r; 0 off FLUID cells.  Partials: sum of r^2 (FP64
+// values) in part[block], max |r| in part[nblk + block].  Split layout, both
+// colours at each position.
+__global__ void __launch_bounds__(256) k_strue_residual(float *__restrict__ r, const float *__restrict__ b,
+                                                        const float *__restrict__ x,
+                                                        const uint16_t *__restrict__ code, SGeo g,
+                                                        double *__restrict__ part)
+{
+    __shared__ double lds[4];
+    __shared__ double lm[4];
+    const uint32_t total = (uint32_t)g.nx * g.ny * g.nq;
+    const int lane = threadIdx.x & 63;
+    double acc = 0.0, mx = 0.0;
+    const Chunk ch = xcd_chunk(total);
+    for (uint32_t t = ch.first; t < ch.end; t += ch.stride) {
+        int i, j, m0;
+        g.decode(t, i, j, m0);
+        const uint32_t base = g.row(i, j) + m0;
+        float xs[2][4];
+        f4(x + base, xs[0]);
+        f4(x + g.half + base, xs[1]);
+        const float l0 = __shfl_up(xs[0][3], 1, 64), r0 = __shfl_down(xs[0][0], 1, 64);
+        const float l1 = __shfl_up(xs[1][3], 1, 64), r1 = __shfl_down(xs[1][0], 1, 64);
+#pragma unroll
+        for (int cc = 0; cc < 2; ++cc) {
+            const uint32_t own = (uint32_t)cc * g.half, oth = (uint32_t)(cc ^ 1) * g.half;
+            const int o = (cc - i - j) & 1;
+            const float *w = xs[cc ^ 1];
+            float xp[4], xm[4], yp[4], ym[4];   // rows clamped to the domain (walls: class 0)
+            f4(x + oth + g.row(min(i + 1, g.nx - 1), j) + m0, xp);
+            f4(x + oth + g.row(max(i - 1, 0), j) + m0, xm);
+            f4(x + oth + g.row(i, min(j + 1, g.ny - 1)) + m0, yp);
+            f4(x + oth + g.row(i, max(j - 1, 0)) + m0, ym);
+            float edge = 0.0f;
+            if (o == 0) {
+                if (m0 > 0) edge = lane > 0 ? (cc ? l0 : l1) : x[oth + base - 1];
+            } else {
+                if (m0 + 4 < g.nh) edge = (lane < 63 && t + 1 < ch.end) ? (cc ? r0 : r1) : x[oth + base + 4];
+            }
+            uint32_t cd[4];
+            codes4(code, own + base, cd);
+            float bv[4];
+            f4(b + own + base, bv);
+            float out[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                float kp, km;
+                if (o == 0) { kp = w[e]; km = e > 0 ? w[e - 1] : edge; }
+                else { km = w[e]; kp = e < 3 ? w[e + 1] : edge; }
+                const double xI = xs[cc][e];
+                const float nbv[6] = {xp[e], xm[e], yp[e], ym[e], kp, km};
+                double sacc = 0.0;
+#pragma unroll
+                for (int d = 0; d < 6; ++d) {
+                    const uint32_t cls = (cd[e] >> (2 * d)) & 3u;
+                    if (cls != 0u) sacc += (cls == 2u ? (double)nbv[d] : 0.0) - xI;
+                }
+                const double rv = code_fluid(cd[e]) ? (double)bv[e] - sacc : 0.0;
+                out[e] = (float)rv;
+                acc += rv * rv;
+                mx = fmax(mx, fabs(rv));
+            }
+            st_f4(r + own + base, out);
+        }
+    }
+    const double s = block_sum<256>(acc, lds);
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
+    if (lane == 0) lm[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = s;
+        part[gridDim.x + blockIdx.x] = fmax(fmax(lm[0], lm[1]), fmax(lm[2], lm[3]));
+    }
+}
+
 // Reference layout <-> split layout, one reference quad (4 cells along k)
 // per thread: cells k0, k0+2 go to one half, k0+1, k0+3 to the other, both
 // at m = k0/2, k0/2 + 1.
@@ -345,6 +463,37 @@ unsigned grid8(uint32_t items)
     return (b + 7u) & ~7u;   // xcd_chunk needs a multiple of 8
 }
 
+template <int MODE, int OUT, typename TZI, typename TZO, typename TR, bool SRC32>
+void sweep_launch(hipStream_t s, unsigned nb, const SSweepArgs &a)
+{
+    k_ssweep<MODE, OUT, TZI, TZO, TR, SRC32><<<nb, 256, 0, s>>>(a);
+}
+
+// The mixed-precision level-0 sweeps: pre-smoothing with T storage of z and
+// r (the first two convert r from the FP32 CG residual), post-smoothing with
+// FP32 z and T r, the last one FINAL.
+template <typename T>
+int split_sweep_mixed_t(hipStream_t s, const Level &L0, const SplitMixedSweep &m)
+{
+    SSweepArgs a{};
+    a.zin = m.zin; a.zout = m.zout; a.zc = m.zc; a.r = m.rh; a.code = m.code;
+    a.r32 = m.r32; a.q32 = m.q32; a.z32 = m.z32; a.sc = m.sc; a.part = m.part;
+    a.g = make_sgeo(L0.nx, L0.ny, L0.nz);
+    a.cu = m.phase ^ 1;
+    const unsigned nb = grid8((uint32_t)L0.nx * L0.ny * a.g.nq);
+    if (m.nblk) *m.nblk = (int)nb;
+    switch (m.kind) {
+    case SMX_PRE_ZERO_SRC: sweep_launch<SWEEP_ZERO, SWEEP_OUT_STORE, T, T, T, true>(s, nb, a); break;
+    case SMX_PRE_SRC: sweep_launch<SWEEP_PLAIN, SWEEP_OUT_STORE, T, T, T, true>(s, nb, a); break;
+    case SMX_PRE: sweep_launch<SWEEP_PLAIN, SWEEP_OUT_STORE, T, T, T, false>(s, nb, a); break;
+    case SMX_POST_PROLONG: sweep_launch<SWEEP_PROLONG, SWEEP_OUT_STORE, T, float, T, false>(s, nb, a); break;
+    case SMX_POST: sweep_launch<SWEEP_PLAIN, SWEEP_OUT_STORE, float, float, T, false>(s, nb, a); break;
+    default: sweep_launch<SWEEP_PLAIN, SWEEP_OUT_FINAL, float, float, T, false>(s, nb, a); break;
+    }
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
 }  // namespace
 
 bool split_supported(const Level &L0) { return L0.nz % 8 == 0 && L0.nx % 2 == 0 && L0.ny % 2 == 0; }
@@ -368,17 +517,23 @@ int split_sweep(hipStream_t s, const Level &L0, const float *zin, float *zout, c
                 const uint16_t *code, int phase, int mode, double *part, int *nblk)
 {
     SSweepArgs a{};
-    a.zin = zin; a.zout = zout; a.zc = zc; a.r = r; a.code = code; a.part = part;
+    a.zin = zin; a.zout = zout; a.zc = zc; a.r = const_cast<float *>(r); a.code = code; a.part = part;
     a.g = make_sgeo(L0.nx, L0.ny, L0.nz);
     a.cu = phase ^ 1;
     const unsigned nb = grid8((uint32_t)L0.nx * L0.ny * a.g.nq);
     if (nblk) *nblk = (int)nb;
-    if (mode == SWEEP_ZERO) k_ssweep<SWEEP_ZERO, false><<<nb, 256, 0, s>>>(a);
-    else if (mode == SWEEP_PROLONG) k_ssweep<SWEEP_PROLONG, false><<<nb, 256, 0, s>>>(a);
-    else if (part) k_ssweep<SWEEP_PLAIN, true><<<nb, 256, 0, s>>>(a);
-    else k_ssweep<SWEEP_PLAIN, false><<<nb, 256, 0, s>>>(a);
+    if (mode == SWEEP_ZERO) sweep_launch<SWEEP_ZERO, SWEEP_OUT_STORE, float, float, float, false>(s, nb, a);
+    else if (mode == SWEEP_PROLONG) sweep_launch<SWEEP_PROLONG, SWEEP_OUT_STORE, float, float, float, false>(s, nb, a);
+    else if (part) sweep_launch<SWEEP_PLAIN, SWEEP_OUT_DOT, float, float, float, false>(s, nb, a);
+    else sweep_launch<SWEEP_PLAIN, SWEEP_OUT_STORE, float, float, float, false>(s, nb, a);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
+}
+
+int split_sweep_mixed(hipStream_t s, const Level &L0, int prec, const SplitMixedSweep &m)
+{
+    if (prec == FLUB_PREC_BF16) return split_sweep_mixed_t<__hip_bfloat16>(s, L0, m);
+    return split_sweep_mixed_t<__half>(s, L0, m);
 }
 
 int split_restrict(hipStream_t s, const Level &F, const float *r, const float *z, const uint16_t *code,
@@ -387,7 +542,22 @@ int split_restrict(hipStream_t s, const Level &F, const float *r, const float *z
     const SGeo g = make_sgeo(F.nx, F.ny, F.nz);
     SGeo cg = make_sgeo(C.nx, C.ny, 2 * C.nz);   // nh = C.nz
     const unsigned nb = grid8((uint32_t)C.nx * C.ny * cg.nq);
-    k_srestrict<<<nb, 256, 0, s>>>(r, z, code, C.r, C.tag, g, cg);
+    k_srestrict<float, float><<<nb, 256, 0, s>>>(r, z, code, C.r, C.tag, g, cg);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+int split_restrict_mixed(hipStream_t s, const Level &F, int prec, const void *rh, const void *zh,
+                         const uint16_t *code, const Level &C)
+{
+    const SGeo g = make_sgeo(F.nx, F.ny, F.nz);
+    SGeo cg = make_sgeo(C.nx, C.ny, 2 * C.nz);
+    const unsigned nb = grid8((uint32_t)C.nx * C.ny * cg.nq);
+    if (prec == FLUB_PREC_BF16)
+        k_srestrict<__hip_bfloat16, __hip_bfloat16><<<nb, 256, 0, s>>>(
+            (const __hip_bfloat16 *)rh, (const __hip_bfloat16 *)zh, code, C.r, C.tag, g, cg);
+    else
+        k_srestrict<__half, __half><<<nb, 256, 0, s>>>((const __half *)rh, (const __half *)zh, code, C.r, C.tag, g, cg);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
 }
@@ -400,6 +570,17 @@ int split_p_Ap(hipStream_t s, const Level &L0, const float *p, float *pout, floa
     *nblk = (int)nb;
     if (pout) k_sp_Ap<true><<<nb, 256, 0, s>>>(p, pout, Ap, z, code, sc, g, part);
     else k_sp_Ap<false><<<nb, 256, 0, s>>>(p, nullptr, Ap, nullptr, code, sc, g, part);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+int split_true_residual(hipStream_t s, const Level &L0, float *r, const float *b, const float *x,
+                        const uint16_t *code, double *part, int *nblk)
+{
+    const SGeo g = make_sgeo(L0.nx, L0.ny, L0.nz);
+    const unsigned nb = grid8((uint32_t)L0.nx * L0.ny * g.nq);
+    *nblk = (int)nb;
+    k_strue_residual<<<nb, 256, 0, s>>>(r, b, x, code, g, part);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
 }

@@ -135,6 +135,11 @@ int flub_mgpcg_solve(flub_mgpcg *h, int max_iters, double eps, double abs_tol, d
 /* compute_Ap_mcadams(Ap, p, tag) (mgpcg.py:36-39) on caller buffers of the
  * level-0 shape with the handle's tags: y = neighbor_dot_tag(x) (= -A x). */
 int flub_mgpcg_apply_A(flub_mgpcg *h, const float *x, float *y);
+/* Accuracy check, no counterpart in the reference (MGPCG.solve reports only
+ * the recursive residual, mgpcg.py:393): *rr = ||b - neighbor_dot_tag(x)||^2
+ * over the FLUID cells, the stencil evaluated in FP64 from the FP32 x and b
+ * (caller buffers of the level-0 shape).  Synchronizes the handle's stream. */
+int flub_mgpcg_residual_norm(flub_mgpcg *h, const float *x, const float *b, double *rr);
 /* One preconditioner application on caller buffers: r[0] <- r, V-cycle
  * (apply_preconditioner, mgpcg.py:296-349), z <- z[0]. */
 int flub_mgpcg_precondition_into(flub_mgpcg *h, const float *r, float *z, int smoother);

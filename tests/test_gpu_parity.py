@@ -265,3 +265,20 @@ def test_half_fluid_cos_problem(oracle):
     oit, _ = o.solve(400, eps=0.0, rel_tol=1e-24, abs_tol=0.0, smoother="phases")
     assert it < 400
     assert _rel(mg.x.numpy(), o.get("x")) < 1e-5
+
+
+def test_residual_norm_fp64(oracle):
+    """flub_mgpcg_residual_norm against the FP64 oracle operator."""
+    import ctypes
+    from flub_amd import _lib
+    tag = random_tags((32, 32, 32), 21, (0.2, 0.7, 0.1))
+    rng = np.random.default_rng(22)
+    x = rng.normal(0, 1, tag.shape).astype(np.float32)
+    b = rng.normal(0, 1, tag.shape).astype(np.float32)
+    mg = _loaded_mgpcg(tag, 3)
+    xd, bd = wp.from_numpy(x), wp.from_numpy(b)
+    rr = ctypes.c_double()
+    _lib.call("flub_mgpcg_residual_norm", mg.handle, xd.ptr, bd.ptr, ctypes.byref(rr))
+    ndt = oracle.compute_Ap(x.astype(np.float64), tag, real=np.float64)   # neighbor_dot_tag(x), 0 off FLUID
+    res = np.where(tag == 1, b.astype(np.float64) - ndt, 0.0)
+    assert np.isclose(rr.value, float(np.sum(res * res)), rtol=1e-12)
