@@ -55,6 +55,74 @@ __device__ __forceinline__ void ld2(const T *p, float v[2])
     }
 }
 
+// N consecutive values of type T (N * sizeof(T) = 4, 8, 16 or 32 bytes,
+// aligned to that size up to 16) as floats, and back.
+template <typename T, int N>
+__device__ __forceinline__ void ldn(const T *p, float v[N])
+{
+    constexpr int B = N * (int)sizeof(T), PER = 16 / (int)sizeof(T);
+    if constexpr (B >= 16) {
+#pragma unroll
+        for (int c = 0; c < B / 16; ++c) {
+            const uint4 w = reinterpret_cast<const uint4 *>(p)[c];
+            const T *e = reinterpret_cast<const T *>(&w);
+#pragma unroll
+            for (int q = 0; q < PER; ++q) v[c * PER + q] = ldv<T>(e[q]);
+        }
+    } else if constexpr (B == 8) {
+        const uint2 w = *reinterpret_cast<const uint2 *>(p);
+        const T *e = reinterpret_cast<const T *>(&w);
+#pragma unroll
+        for (int q = 0; q < N; ++q) v[q] = ldv<T>(e[q]);
+    } else {
+        const uint32_t w = *reinterpret_cast<const uint32_t *>(p);
+        const T *e = reinterpret_cast<const T *>(&w);
+#pragma unroll
+        for (int q = 0; q < N; ++q) v[q] = ldv<T>(e[q]);
+    }
+}
+template <typename T, int N>
+__device__ __forceinline__ void stn(T *p, const float v[N])
+{
+    constexpr int B = N * (int)sizeof(T), PER = 16 / (int)sizeof(T);
+    if constexpr (B >= 16) {
+#pragma unroll
+        for (int c = 0; c < B / 16; ++c) {
+            uint4 w;
+            T *e = reinterpret_cast<T *>(&w);
+#pragma unroll
+            for (int q = 0; q < PER; ++q) e[q] = stv<T>(v[c * PER + q]);
+            reinterpret_cast<uint4 *>(p)[c] = w;
+        }
+    } else if constexpr (B == 8) {
+        uint2 w;
+        T *e = reinterpret_cast<T *>(&w);
+#pragma unroll
+        for (int q = 0; q < N; ++q) e[q] = stv<T>(v[q]);
+        *reinterpret_cast<uint2 *>(p) = w;
+    } else {
+        uint32_t w;
+        T *e = reinterpret_cast<T *>(&w);
+#pragma unroll
+        for (int q = 0; q < N; ++q) e[q] = stv<T>(v[q]);
+        *reinterpret_cast<uint32_t *>(p) = w;
+    }
+}
+// N consecutive 16-bit stencil codes (N = 4 or 8).
+template <int N>
+__device__ __forceinline__ void codesn(const uint16_t *code, uint32_t base, uint32_t c[N])
+{
+    if constexpr (N == 8) {
+        const uint4 w = *reinterpret_cast<const uint4 *>(code + base);
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { c[2 * q] = ws[q] & 0xffffu; c[2 * q + 1] = ws[q] >> 16; }
+    } else {
+        const uint2 w = *reinterpret_cast<const uint2 *>(code + base);
+        c[0] = w.x & 0xffffu; c[1] = w.x >> 16; c[2] = w.y & 0xffffu; c[3] = w.y >> 16;
+    }
+}
+
 __device__ __forceinline__ void codes4(const uint16_t *code, uint32_t base, uint32_t c[4])
 {
     const uint2 w = *reinterpret_cast<const uint2 *>(code + base);
