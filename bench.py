@@ -17,6 +17,16 @@ flub_amd/learned/weights/unet_128.npz).  `value` is the fastest solver's
 ms per projection; every solver's figures are in "solvers".
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--solvers fp32 fp16 bf16 learned]
+
+Second workload (--workload droplets_box, default 512^3): the dam-break block
+plus 200 droplets of radius 2-12 cells (seed 3) and a solid 0.1^3 box at a
+random pose moving with N(0, 0.3) linear and angular velocity; the face
+fractions of the box weight the right-hand side (divergence_fractional_3d).
+One step = geometry (face fractions, rigid face velocities), fractional
+divergence, solve, gradient subtraction, for the plain FP32 MGPCG, the BF16
+mixed solver and the combined learned + BF16 solver; the line reports the
+MGPCG fallback of the learned solver, times, HBM GB/s of the geometry and
+divergence kernels and the MFMA utilisation of the U-Net.
 """
 from __future__ import annotations
 
@@ -50,7 +60,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--workload", choices=["flow_past_sphere", "droplets_box"], default="flow_past_sphere")
+    ap.add_argument("--n", type=int, default=0, help="grid size (default 256, or 512 for droplets_box)")
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--solvers", nargs="+", default=["fp32", "fp16", "bf16", "learned"])
     ap.add_argument("--learned-steps", type=int, default=2)
@@ -69,7 +80,9 @@ def main():
     from flub_amd.project import Projection
 
     _lib.lib()
-    n, L = args.n, args.levels
+    if args.workload == "droplets_box":
+        return droplets_box_main(args)
+    n, L = args.n or 256, args.levels
     t0 = time.time()
     s = scenes.flow_past_sphere(n, 2)
     log(f"scene {n}^3 built in {time.time() - t0:.1f}s; fluid cells {(s['tag'] == 1).sum()}")
@@ -231,6 +244,90 @@ def learned_leg(s, n, L, src, steps, warmup):
             "true_rel_res": float(np.sqrt(st["true_rTr"] / st["rTr0"])),
             "unet_forward_ms": round(net_ms, 3), "unet_tflops": round(tflops, 2),
             "unet_mfma_frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4)}
+
+
+def _timed(fn):
+    from flub_amd import device
+    device.synchronize()
+    t = time.perf_counter()
+    out = fn()
+    device.synchronize()
+    return (time.perf_counter() - t) * 1e3, out
+
+
+def droplets_box_main(args):
+    import torch
+    from flub_amd import device, scenes
+    from flub_amd.learned.evaluate import load_net
+    from flub_amd.learned.fpcg import LearnedPCG
+    from flub_amd.learned.unet import flops_per_voxel
+    from flub_amd.project import Projection
+
+    n, L = args.n or 512, args.levels
+    t0 = time.time()
+    s = scenes.dam_break_droplets_box(n, 3)
+    tag = s["tag"]
+    log(f"scene {n}^3 built in {time.time() - t0:.1f}s; fluid {(tag == 1).mean():.3f}, solid cells {(tag == 2).sum()}")
+    src = [device.from_numpy(s[c]) for c in ("u", "v", "w")]
+    fluid = torch.from_numpy(tag == 1).cuda()
+    net = load_net(os.path.join(ROOT, "flub_amd", "learned", "weights", "unet_128.npz"))
+    out = {}
+    steps = max(1, args.steps)
+    for name, prec, learned in (("fp32", "fp32", False), ("bf16", "bf16", False), ("learned+bf16", "bf16", True)):
+        pr = Projection((n, n, n), n_mg_levels=L, pre_and_post_smoothing=2, bottom_smoothing=6, precision=prec)
+        pr.load(s)
+        pr.set_bodies(s["bodies"])
+        solver = LearnedPCG(pr.mgpcg, net) if learned else None
+        dst = [pr.vel0.u.f, pr.vel0.v.f, pr.vel0.w.f]
+        times, st = [], None
+        for i in range(max(1, args.warmup) + steps):
+            for a, b in zip(dst, src):
+                device.copy(a, b)
+            ms, st = _timed(lambda: pr.project_fractional(solver, fluid if learned else None, rel_tol=1e-12))
+            if i >= max(1, args.warmup):
+                times.append(ms)
+                log(f"{name} step: {ms:.2f} ms, " + (f"{st['iterations']} iterations ({st['net_iters']} net, "
+                    f"{st['vcycle_iters']} MGPCG, fallback at {st['fallback_at']})" if learned
+                    else f"{st} iterations"))
+        ent = {"ms": round(float(np.mean(times)), 3), "iterations": st["iterations"] if learned else st,
+               "true_rel_res": true_rel_residual(pr, tag)}
+        if learned:
+            ent.update(net_iters=st["net_iters"], mgpcg_iters=st["vcycle_iters"], fallbacks=int(st["fallback_at"] is not None),
+                       fallback_at=st["fallback_at"], handovers=st["handovers"], replacements=st["replacements"])
+        out[name] = ent
+        if name == "fp32":
+            # the geometry and divergence kernels alone
+            g_ms, _ = _timed(pr.precompute_geometry)
+            d_ms, _ = _timed(pr.divergence_fractional)
+            faces = 3 * (n + 1) * n * n
+            # fractions + owners written (8 B/face), solid velocities read owner + write (8 B/face)
+            out["geometry"] = {"ms": round(g_ms, 3), "GBs": round(16 * faces / (g_ms * 1e-3) / 1e9, 1),
+                               "bytes": 16 * faces}
+            # fractional divergence: 9 face arrays read + div written (40 B/cell); mask_with_tag
+            # (div and int32 tag read, div written: 12 B/cell); scale_by_3d (8 B/cell)
+            dbytes = (40 + 12 + 8) * n ** 3
+            out["divergence_fractional"] = {"ms": round(d_ms, 3), "GBs": round(dbytes / (d_ms * 1e-3) / 1e9, 1),
+                                            "bytes": dbytes}
+            out["kernels"] = profile_kernels(pr)
+        del pr, solver
+    # one network application at this size
+    r = torch.randn((1, n, n, n, 2), device="cuda")
+    net.infer(r)
+    net_ms, _ = _timed(lambda: net.infer(r))
+    tflops = flops_per_voxel() * n ** 3 / (net_ms * 1e-3) / 1e12
+    out["unet"] = {"forward_ms": round(net_ms, 2), "tflops": round(tflops, 1),
+                   "mfma_frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4)}
+    best = min(("fp32", "bf16", "learned+bf16"), key=lambda k: out[k]["ms"])
+    print(json.dumps({"metric": f"ms per pressure projection to 1e-6 rel. residual @{n}^3 with a moving solid box "
+                                f"(face-fraction RHS): learned+mixed vs MGPCG",
+                      "value": out[best]["ms"], "unit": "ms", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+                      "higher_is_better": False, "data": "synthetic (seeded scene)",
+                      "config": {"workload": f"dam break + 200 droplets + rotated moving box {n}^3: face fractions, "
+                                             f"rigid face velocities, fractional divergence, solve, gradient "
+                                             f"subtraction", "grid": n, "best_solver": best},
+                      "solvers": {k: out[k] for k in ("fp32", "bf16", "learned+bf16")},
+                      "geometry": out["geometry"], "divergence_fractional": out["divergence_fractional"],
+                      "unet": out["unet"], "kernels": out["kernels"]}), flush=True)
 
 
 def cpu_baseline(s, n, L, threads):

@@ -45,6 +45,11 @@ SIGNATURES = {
     "flub_tag_dirichlet": [_P, _P, _Z, _P],
     "flub_tag_neumann": [_P, _P, _Z, _P],
     "flub_divergence_tag": [_P] * 8 + [_I, _I, _I, _P],
+    "flub_phi_prims": [_P, _P, _I, _I, _I, _F, _F, _F, _F, _I, _P, _P, _P, _P, _P],
+    "flub_phi_to_fraction": [_P, _P, _I, _I, _I, _I, _P],
+    "flub_face_fractions": [_P] * 6 + [_I, _I, _I, _F, _I, _P, _P, _P, _P, _P],
+    "flub_solid_vel": [_P, _P, _I, _I, _I, _I, _F, _P, _P, _P, _P],
+    "flub_divergence_fractional_3d": [_P] * 10 + [_I, _I, _I, _P],
     "flub_mask_with_tag": [_P, _P, _Z, _P],
     "flub_scale": [_P, _F, _Z, _P],
     "flub_subtract_gradient_tag": [_P, _P, _P, _P, _I, _I, _I, _I, _P],

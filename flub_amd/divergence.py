@@ -1,8 +1,13 @@
-"""MAC-grid divergence (divergence.py:44-99 ``divergence_tag`` and :174-180
-``mask_with_tag`` of the reference), launched on the HIP C-ABI:
+"""MAC-grid divergence (divergence.py:5-41 ``divergence_fractional_3d``,
+:44-99 ``divergence_tag`` and :174-180 ``mask_with_tag`` of the reference),
+launched on the HIP C-ABI:
 
     launch(divergence_tag, dim=shape, inputs=[vel, div, vel_rigid, tag])
+    launch(divergence_fractional_3d, dim=shape,
+           inputs=[u, v, w, div, fu, fv, fw, u_rigid, v_rigid, w_rigid])
     launch(mask_with_tag, dim=shape, inputs=[div, tag])
+
+(divergence_fractional_3d takes the bare arrays, as the reference's kernel does.)
 """
 from __future__ import annotations
 
@@ -38,3 +43,18 @@ def _mask_with_tag(div, tag):
 
 divergence_tag = Kernel("divergence_tag", _divergence_tag, lambda vels, div, vr, tag: tag.f.shape)
 mask_with_tag = Kernel("mask_with_tag", _mask_with_tag, lambda div, tag: div.f.shape)
+
+
+def _divergence_fractional_3d(uf, vf, wf, velocity_divs, fu, fv, fw, u_rigid, v_rigid, w_rigid):
+    nx, ny, nz = velocity_divs.shape
+    want = {"u": (nx + 1, ny, nz), "v": (nx, ny + 1, nz), "w": (nx, ny, nz + 1)}
+    for name, arrs in (("u", (uf, fu, u_rigid)), ("v", (vf, fv, v_rigid)), ("w", (wf, fw, w_rigid))):
+        for a in arrs:
+            if a.shape != want[name]:
+                raise ValueError(f"divergence_fractional_3d: {name} array has shape {a.shape}, expected {want[name]}")
+    _lib.call("flub_divergence_fractional_3d", uf.ptr, vf.ptr, wf.ptr, velocity_divs.ptr, fu.ptr, fv.ptr, fw.ptr,
+              u_rigid.ptr, v_rigid.ptr, w_rigid.ptr, nx, ny, nz, None)
+
+
+divergence_fractional_3d = Kernel("divergence_fractional_3d", _divergence_fractional_3d,
+                                  lambda u, v, w, div, *rest: div.shape)

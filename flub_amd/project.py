@@ -102,12 +102,74 @@ class Projection:
 
     def project_learned(self, solver, fluid, rel_tol=1e-12, abs_tol=0.0):
         """project() with the pressure solve done by a flub_amd.learned.fpcg.LearnedPCG
-        (the U-Net preconditioner with V-cycle fallback) on this projection's
+        (the U-Net preconditioner with MGPCG fallback) on this projection's
         MGPCG handle; fluid is a torch bool mask of the FLUID cells.  Returns
         the solver statistics."""
+        self.divergence()
+        return self._solve_learned_and_subtract(solver, fluid, rel_tol, abs_tol)
+
+    # ---------------------------------------------------------- solid bodies
+    def set_bodies(self, bodies):
+        """Rigid bodies of the scene (dict of x, q, gtype, radius, v, omega
+        arrays) for the face fractions and rigid face velocities."""
+        from .const_param import make_mac_grid
+        from .geometry import GeometryType3D, PrecomputeFaceFraction
+        from .rigid_coupling import RigidBodies
+
+        gt = np.asarray(bodies["gtype"], np.int32).reshape(-1)
+        rad = np.asarray(bodies["radius"], np.float32).reshape(-1, 3)
+        geoms = [GeometryType3D(int(k), tuple(float(v) for v in r)) for k, r in zip(gt, rad)]
+        self.rc = RigidBodies(bodies["x"], bodies["q"], geoms, bodies.get("v"), bodies.get("omega"), h0=self.h)
+        self.pff = PrecomputeFaceFraction(h=self.h)
+        self.frac = make_mac_grid(float, self.shape)
+        self.vel_owner = make_mac_grid(int, self.shape)
+
+    def precompute_geometry(self):
+        """The solid inputs of a projection: face fractions with their owning
+        bodies (example_fluid.py:109, geometry.py:438-462) and the rigid face
+        velocities (example_fluid.py:87, rigid_coupling.py:225-228)."""
+        rc = self.rc
+        self.pff.precompute_fij_with_cache(self.frac, self.vel_owner, rc)
+        rc.precompute_solid_vel_3d(rc.x_pair.cur, rc.theta_pair.cur, rc.v, rc.omega, rc.rigid_geometry,
+                                   self.vel_rigid, self.vel_owner)
+
+    def divergence_fractional(self):
+        """divergence() with the fraction-weighted fluxes of divergence.py:5-41
+        (the launch example_fluid.py:88 names), then mask and scale."""
+        from .divergence import divergence_fractional_3d
+
+        v, f, r = self.vel0, self.frac, self.vel_rigid
+        launch(divergence_fractional_3d, dim=self.shape,
+               inputs=[v.u.f, v.v.f, v.w.f, self.div.f, f.u.f, f.v.f, f.w.f, r.u.f, r.v.f, r.w.f])
+        launch(mask_with_tag, dim=self.shape, inputs=[self.div, self.tag])
+        launch(scale_by_3d, dim=self.shape, inputs=[self.div, -1.0 / self.h])
+
+    def project_fractional(self, solver=None, fluid=None, max_iters=-1, rel_tol=1e-12, abs_tol=0.0):
+        """A projection of a scene with solid bodies: geometry, fraction-weighted
+        divergence, the pressure solve (this handle's MGPCG, or a LearnedPCG
+        `solver` with the torch FLUID mask `fluid`) and the gradient subtraction.
+        Returns the iteration count, or the learned solver's statistics."""
+        self.precompute_geometry()
+        self.divergence_fractional()
+        if solver is not None:
+            return self._solve_learned_and_subtract(solver, fluid, rel_tol, abs_tol)
+        self.p0.f.zero_()
+        self.call_mgpcg(self.p0, max_iters, rel_tol, abs_tol)
+        self._subtract_gradient()
+        return self.iterations
+
+    def _subtract_gradient(self):
+        """example_fluid.py:134-145 after the solve: p0 / h, the three face
+        updates, p0 * h."""
+        launch(scale_by_3d, dim=self.shape, inputs=[self.p0, 1.0 / self.h])
+        g = self.shape
+        for d in (1, 2, 3):
+            launch(subtract_gradient_tag, dim=field_shape(d, g), inputs=[self.vel0, self.vel_rigid, self.p0, self.tag, d])
+        launch(scale_by_3d, dim=self.shape, inputs=[self.p0, self.h])
+
+    def _solve_learned_and_subtract(self, solver, fluid, rel_tol, abs_tol):
         import torch
 
-        self.divergence()
         mg = self.mgpcg
         mg.set_tags(self.tag.f)
         mg.init(self.div.f, -self.h)
@@ -119,11 +181,7 @@ class Projection:
         self.iterations, self.rTr = st["iterations"], st["rTr"]
         wp.copy(self.p0.f, mg.x)
         launch(scale_by_3d, dim=self.shape, inputs=[self.p0, self.h])
-        launch(scale_by_3d, dim=self.shape, inputs=[self.p0, 1.0 / self.h])
-        g = self.shape
-        for d in (1, 2, 3):
-            launch(subtract_gradient_tag, dim=field_shape(d, g), inputs=[self.vel0, self.vel_rigid, self.p0, self.tag, d])
-        launch(scale_by_3d, dim=self.shape, inputs=[self.p0, self.h])
+        self._subtract_gradient()
         return st
 
     def project_stepwise(self, max_iters=-1, rel_tol=1e-12, abs_tol=0.0, smoother="phases", verbose=False):
@@ -131,10 +189,5 @@ class Projection:
         self.divergence()
         self.p0.f.zero_()
         self.call_mgpcg(self.p0, max_iters, rel_tol, abs_tol, smoother, verbose)
-        launch(scale_by_3d, dim=self.shape, inputs=[self.p0, 1.0 / self.h])
-        g = self.shape
-        launch(subtract_gradient_tag, dim=field_shape(1, g), inputs=[self.vel0, self.vel_rigid, self.p0, self.tag, 1])
-        launch(subtract_gradient_tag, dim=field_shape(2, g), inputs=[self.vel0, self.vel_rigid, self.p0, self.tag, 2])
-        launch(subtract_gradient_tag, dim=field_shape(3, g), inputs=[self.vel0, self.vel_rigid, self.p0, self.tag, 3])
-        launch(scale_by_3d, dim=self.shape, inputs=[self.p0, self.h])
+        self._subtract_gradient()
         return self.iterations

@@ -181,6 +181,82 @@ def random_pool(n=128, seed=1000):
     return _pack(n, phi, phi_rigid, u, v, w, ur, vr, wr)
 
 
+def _sphere_phi_local(n, centres_xyz, radii, phi):
+    """_sphere_phi restricted to each sphere's bounding box (one cell of
+    margin): phi is min(phi, |c - x| - r) there and unchanged elsewhere, so its
+    sign (the tag) equals the full-grid min for spheres of any size."""
+    c1 = centres(n)
+    for (cx, cy, cz), r in zip(centres_xyz, radii):
+        lo = [max(int(np.floor((c - r) * n)) - 1, 0) for c in (cx, cy, cz)]
+        hi = [min(int(np.ceil((c + r) * n)) + 1, n) for c in (cx, cy, cz)]
+        if any(a >= b for a, b in zip(lo, hi)):
+            continue
+        x = c1[lo[0]:hi[0], None, None]
+        y = c1[None, lo[1]:hi[1], None]
+        z = c1[None, None, lo[2]:hi[2]]
+        d = np.sqrt((x - F32(cx)) ** 2 + (y - F32(cy)) ** 2 + (z - F32(cz)) ** 2) - F32(r)
+        sub = phi[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
+        np.minimum(sub, d.astype(F32), out=sub)
+    return phi
+
+
+def _random_quat(rng):
+    q = rng.normal(size=4)
+    return (q / np.linalg.norm(q)).astype(F32)
+
+
+def _box_sdf_host(n, centre, q, half):
+    """Signed distance of a box (half-extents `half`) at `centre` rotated by the
+    unit quaternion q = (x, y, z, w), at the cell centres, in float64."""
+    qx, qy, qz, qw = (float(a) for a in q)
+    # rotation matrix of q; the body frame coordinates are R^T (p - c)
+    R = np.array([[1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qz * qw), 2 * (qx * qz + qy * qw)],
+                  [2 * (qx * qy + qz * qw), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qx * qw)],
+                  [2 * (qx * qz - qy * qw), 2 * (qy * qz + qx * qw), 1 - 2 * (qx * qx + qy * qy)]])
+    c = (np.arange(n, dtype=np.float64) + 0.5) / n
+    out = np.empty((n, n, n), F32)
+    yy, zz = np.meshgrid(c - centre[1], c - centre[2], indexing="ij")
+    for i in range(n):
+        p = np.stack([np.full_like(yy, c[i] - centre[0]), yy, zz], axis=-1) @ R   # (R^T p) per point
+        m = np.abs(p) - np.asarray(half, np.float64)
+        mx = m.max(axis=-1)
+        out[i] = np.where(mx < 0.0, mx, np.linalg.norm(np.maximum(m, 0.0), axis=-1))
+    return out
+
+
+def dam_break_droplets_box(n=512, seed=3, n_drops=200, r_cells=(2.0, 12.0), box=0.1, sigma_box=0.3, sigma=0.5):
+    """The combined scene: the dam-break block (x < 0.4, y < 0.6) union n_drops
+    fluid droplets (radius U[r_cells] cells, centres U[0,1]^3), and a solid box
+    of side `box` at a random pose (centre U[0.3, 0.7]^3, uniformly random
+    orientation) whose linear and angular velocities are N(0, sigma_box) per
+    component.  Fluid faces N(0, sigma).  Besides the usual arrays the scene
+    carries the body ('bodies': x, q, gtype, radius, v, omega) from which the
+    face fractions and rigid face velocities are computed on the device."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.0, 1.0, (n_drops, 3))
+    r = rng.uniform(r_cells[0], r_cells[1], n_drops) / n
+    bc = rng.uniform(0.3, 0.7, 3).astype(F32)
+    bq = _random_quat(rng)
+    bv = rng.normal(0.0, sigma_box, 3).astype(F32)
+    bw = rng.normal(0.0, sigma_box, 3).astype(F32)
+    phi = _sphere_phi_local(n, c, r, _box_phi(n, 0.4, 0.6).copy())
+    half = np.full(3, box / 2.0, F32)
+    phi_rigid = _box_sdf_host(n, bc.astype(np.float64), bq, half)
+    u = rng.normal(0.0, sigma, (n + 1, n, n)).astype(F32)
+    v = rng.normal(0.0, sigma, (n, n + 1, n)).astype(F32)
+    w = rng.normal(0.0, sigma, (n, n, n + 1)).astype(F32)
+    tag = tags_from_phi(phi, phi_rigid)
+    mu, mv, mw = fluid_face_masks(tag)
+    u[~mu] = 0.0
+    v[~mv] = 0.0
+    w[~mw] = 0.0
+    del mu, mv, mw
+    ur, vr, wr = _faces_zero(n)
+    bodies = dict(x=bc.reshape(1, 3), q=bq.reshape(1, 4), gtype=np.array([1], np.int32), radius=half.reshape(1, 3),
+                  v=bv.reshape(1, 3), omega=bw.reshape(1, 3))
+    return _pack(n, phi, phi_rigid, u, v, w, ur, vr, wr, bodies=bodies)
+
+
 def half_fluid_cos(n=32):
     """The reference's MGPCG_Test problem (mgpcg.py:501-513,531-558): tag FLUID
     for j < n/2 and AIR above, r = cos(5 pi i/n) cos(5 pi j/n) cos(5 pi k/n) on
@@ -198,4 +274,5 @@ SCENES = {
     "dam_break_64": lambda: dam_break(64, 0),
     "dam_break_spheres_128": lambda: dam_break_spheres(128, 1),
     "flow_past_sphere_256": lambda: flow_past_sphere(256, 2),
+    "droplets_box_512": lambda: dam_break_droplets_box(512, 3),
 }

@@ -1,9 +1,20 @@
-"""Grid utilities on the projection path (utils/util.py:20-28 ``sample_3d``,
-:162-165 ``scale_by_3d`` of the reference)."""
+"""Grid utilities on the projection path (utils/util.py:4-10 ``TexPair``,
+:20-28 ``sample_3d``, :162-165 ``scale_by_3d`` of the reference)."""
 from __future__ import annotations
 
 from .. import _lib
 from ..launch import Kernel
+
+
+class TexPair:
+    """A current/next pair of buffers (utils/util.py:4-10)."""
+
+    def __init__(self, cur, nxt):
+        self.cur = cur
+        self.nxt = nxt
+
+    def swap(self):
+        self.cur, self.nxt = self.nxt, self.cur
 
 
 def _scale_by_3d(q, scale):

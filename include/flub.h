@@ -69,6 +69,38 @@ int flub_tag_dirichlet(const float *phi, int32_t *tag, size_t n, void *stream);
 int flub_tag_neumann(const float *phi_rigid, int32_t *tag, size_t n, void *stream);
 
 /* ---------------------------------------------------------------------------
+ * Solid geometry: primitive signed distances, face fractions, rigid face
+ * velocities (geometry.py:17-290,401-462; utils/util.py:168-235;
+ * rigid_coupling.py:51-70).  Bodies are structure-of-arrays in device memory:
+ * x[3n], q[4n] (quaternion x, y, z, w), gtype[n] (0 none, 1 quad, 2 circle,
+ * 3 rect, 4 "x", 5 mesh: distance 2.0, resolved from cached level sets by the
+ * caller), radius[3n], v[3n], omega[3n].  Arrays are limited to 2^32 points.
+ * ------------------------------------------------------------------------- */
+/* geometry.py:225-239 precompute_grid_point_phi_prims over an sx x sy x sz array:
+ * phi at ((i + st_x) h, (j + st_y) h, (k + st_z) h) is the smallest body distance
+ * below h (else 2.0, geometry.py:148-169); owner (may be NULL; the reference
+ * writes it for staggered arrays) gets that body's index or -1. */
+int flub_phi_prims(float *phi, int32_t *owner, int sx, int sy, int sz, float stx, float sty, float stz,
+                   float h, int n_bodies, const float *x, const float *q, const int32_t *gtype,
+                   const float *radius, void *stream);
+/* geometry.py:264-290 phi_to_fraction for axis dim (1 u, 2 v, 3 w) of an nx x ny x nz
+ * grid: phi holds the corner distances ((nx+1) x (ny+1) x (nz+1)), frac the face-centre
+ * distances on entry and the face fractions (utils/util.py:168-228) on return. */
+int flub_phi_to_fraction(const float *phi, float *frac, int nx, int ny, int nz, int dim, void *stream);
+/* geometry.py:440-462 PrecomputeFaceFraction.precompute_fij_with_cache for primitive
+ * bodies: face fractions and owners of all three axes, one pass per axis with the
+ * corner and centre distances evaluated in registers (bitwise the staged sequence
+ * phi_prims -> phi_to_fraction, without its (nx+1)(ny+1)(nz+1) corner array). */
+int flub_face_fractions(float *fu, float *fv, float *fw, int32_t *owner_u, int32_t *owner_v, int32_t *owner_w,
+                        int nx, int ny, int nz, float h, int n_bodies, const float *x, const float *q,
+                        const int32_t *gtype, const float *radius, void *stream);
+/* rigid_coupling.py:51-70 precompute_solid_vel for axis d (1 u, 2 v, 3 w) over its
+ * face array (sx, sy, sz): vel = (v + omega x (p - x))[d-1] of the owning body where
+ * h0/100 < i*h0 < 1 - h0/100 and owner != -1, else 0. */
+int flub_solid_vel(float *vel, const int32_t *owner, int sx, int sy, int sz, int d, float h0,
+                   const float *x, const float *v, const float *omega, void *stream);
+
+/* ---------------------------------------------------------------------------
  * Grid kernels of the projection (launched by example_fluid.py:84-94,120-147).
  * ------------------------------------------------------------------------- */
 /* divergence.py:44-99 divergence_tag: div = (vr-vl)+(vt-vb)+(vn-vf), a face flux
@@ -78,6 +110,12 @@ int flub_tag_neumann(const float *phi_rigid, int32_t *tag, size_t n, void *strea
 int flub_divergence_tag(const float *u, const float *v, const float *w, float *div,
                         const float *u_rigid, const float *v_rigid, const float *w_rigid,
                         const int32_t *tag, int nx, int ny, int nz, void *stream);
+/* divergence.py:5-41 divergence_fractional_3d: the face fluxes weighted by the
+ * open fractions fu/fv/fw plus the rigid fluxes weighted by (1 - fraction). */
+int flub_divergence_fractional_3d(const float *u, const float *v, const float *w, float *div,
+                                  const float *fu, const float *fv, const float *fw, const float *u_rigid,
+                                  const float *v_rigid, const float *w_rigid, int nx, int ny, int nz,
+                                  void *stream);
 /* divergence.py:174-180 mask_with_tag: div = 0 where tag != FLUID. */
 int flub_mask_with_tag(float *div, const int32_t *tag, size_t n, void *stream);
 /* utils/util.py:162-165 scale_by_3d (and mgpcg.py:9-12 _scale_by_3d): q *= s. */

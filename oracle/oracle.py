@@ -88,6 +88,20 @@ def _declare(L):
     L.orc_tag_neumann.argtypes = [_P, _P, _Z]
     L.orc_set_threads.argtypes = [_I]
     L.orc_set_threads.restype = _I
+    _F = ctypes.c_float
+    L.orc_sdf_cube.argtypes = [_P, _P, _P]
+    L.orc_sdf_cube.restype = _F
+    L.orc_sdf_atom_prims.argtypes = [_F, _F, _F, _P, _P, _I, _P]
+    L.orc_sdf_atom_prims.restype = _F
+    L.orc_sdf_argmin_prims.argtypes = [_F, _F, _F, _I, _P, _P, _P, _P, _F, _P]
+    L.orc_sdf_argmin_prims.restype = _F
+    L.orc_phi_prims.argtypes = [_P, _P, _I, _I, _I, _F, _F, _F, _F, _I, _P, _P, _P, _P]
+    for name, n in (("orc_fraction_edge", 2), ("orc_fraction_triangle", 3), ("orc_fraction_quad", 5)):
+        getattr(L, name).argtypes = [_F] * n
+        getattr(L, name).restype = _F
+    L.orc_phi_to_fraction.argtypes = [_P, _P, _I, _I, _I, _I]
+    L.orc_solid_vel.argtypes = [_P, _P, _I, _I, _I, _I, _F, _F, _F, _F, _P, _P, _P]
+    L.orc_divergence_fractional.argtypes = [_P] * 10 + [_I, _I, _I]
 
 
 def set_threads(n: int) -> int:
@@ -323,3 +337,98 @@ class MGPCG:
                                     _ptr(tag), _ptr(div), _ptr(p0), float(h), int(max_iters), float(eps),
                                     float(abs_tol), float(rel_tol), SMOOTHERS[smoother], ctypes.byref(rtr))
         return u, v, w, p0, div, it, rtr.value
+
+
+# ------------------------------------------------ solid geometry (FP32 only)
+STAGGER = {0: (0.5, 0.5, 0.5), 1: (0.0, 0.5, 0.5), 2: (0.5, 0.0, 0.5), 3: (0.5, 0.5, 0.0)}
+
+
+class Rigids:
+    """Rigid bodies as arrays: x (n,3), q (n,4) as (x, y, z, w), gtype (n,),
+    radius (n,3), v (n,3), omega (n,3); all float32 / int32."""
+
+    def __init__(self, x, q, gtype, radius, v=None, omega=None):
+        n = len(gtype)
+        self.x = _arr(np.reshape(x, (n, 3)), np.float32)
+        self.q = _arr(np.reshape(q, (n, 4)), np.float32)
+        self.gtype = _arr(gtype, np.int32)
+        self.radius = _arr(np.reshape(radius, (n, 3)), np.float32)
+        self.v = _arr(np.zeros((n, 3)) if v is None else np.reshape(v, (n, 3)), np.float32)
+        self.omega = _arr(np.zeros((n, 3)) if omega is None else np.reshape(omega, (n, 3)), np.float32)
+        self.n = n
+
+
+def sdf_cube(p, radius, q):
+    """geometry.py:17-32 at one point."""
+    p, radius, q = (_arr(a, np.float32) for a in (p, radius, q))
+    return float(lib().orc_sdf_cube(_ptr(p), _ptr(radius), _ptr(q)))
+
+
+def sdf_atom_prims(p, rb: Rigids, b: int):
+    """geometry.py:82-113 for body b at point p."""
+    return float(lib().orc_sdf_atom_prims(float(p[0]), float(p[1]), float(p[2]), _ptr(rb.x[b]), _ptr(rb.q[b]),
+                                          int(rb.gtype[b]), _ptr(rb.radius[b])))
+
+
+def phi_prims(shape, stagger, h, rb: Rigids, owner=True):
+    """geometry.py:225-239 over an array of ``shape``; returns (phi, owner or None)."""
+    phi = np.empty(shape, np.float32)
+    own = np.empty(shape, np.int32) if owner else None
+    lib().orc_phi_prims(_ptr(phi), _ptr(own) if owner else None, *shape, *(float(s) for s in stagger), float(h),
+                        rb.n, _ptr(rb.x), _ptr(rb.q), _ptr(rb.gtype), _ptr(rb.radius))
+    return phi, own
+
+
+def fraction_edge(yl, yr):
+    return float(lib().orc_fraction_edge(yl, yr))
+
+
+def fraction_triangle(ya, yb, yc):
+    return float(lib().orc_fraction_triangle(ya, yb, yc))
+
+
+def fraction_quad(ya, yb, yc, yd, ycen):
+    return float(lib().orc_fraction_quad(ya, yb, yc, yd, ycen))
+
+
+def phi_to_fraction(phi_corner, frac_centre, dim):
+    """geometry.py:264-290; frac_centre holds the face-centre phi; returns the fractions."""
+    phi_corner = _arr(phi_corner, np.float32)
+    frac = _arr(frac_centre, np.float32).copy()
+    nx, ny, nz = (s - 1 for s in phi_corner.shape)
+    lib().orc_phi_to_fraction(_ptr(phi_corner), _ptr(frac), nx, ny, nz, int(dim))
+    return frac
+
+
+def face_fractions(shape, h, rb: Rigids):
+    """geometry.py:440-462 precompute_fij_with_cache for primitive bodies:
+    returns ((fu, fv, fw), (owner_u, owner_v, owner_w))."""
+    nx, ny, nz = shape
+    corner, _ = phi_prims((nx + 1, ny + 1, nz + 1), (0.0, 0.0, 0.0), h, rb, owner=False)
+    fr, ow = [], []
+    for d, fs in ((1, (nx + 1, ny, nz)), (2, (nx, ny + 1, nz)), (3, (nx, ny, nz + 1))):
+        cen, own = phi_prims(fs, STAGGER[d], h, rb)
+        fr.append(phi_to_fraction(corner, cen, d))
+        ow.append(own)
+    return tuple(fr), tuple(ow)
+
+
+def solid_vel(owner, d, h0, rb: Rigids):
+    """rigid_coupling.py:51-70 precompute_solid_vel for axis d over the face array owner."""
+    owner = _arr(owner, np.int32)
+    vel = np.empty(owner.shape, np.float32)
+    st = STAGGER[d]
+    lib().orc_solid_vel(_ptr(vel), _ptr(owner), *owner.shape, int(d), *(float(s) for s in st), float(h0),
+                        _ptr(rb.x), _ptr(rb.v), _ptr(rb.omega))
+    return vel
+
+
+def divergence_fractional(u, v, w, fu, fv, fw, ur, vr, wr):
+    """divergence.py:5-41 divergence_fractional_3d; returns div."""
+    arrs = [_arr(a, np.float32) for a in (u, v, w, fu, fv, fw, ur, vr, wr)]
+    u, v, w, fu, fv, fw, ur, vr, wr = arrs
+    nx, ny, nz = v.shape[0], u.shape[1], u.shape[2]
+    div = np.empty((nx, ny, nz), np.float32)
+    lib().orc_divergence_fractional(_ptr(u), _ptr(v), _ptr(w), _ptr(div), _ptr(fu), _ptr(fv), _ptr(fw), _ptr(ur),
+                                    _ptr(vr), _ptr(wr), nx, ny, nz)
+    return div
