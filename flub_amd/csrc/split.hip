@@ -31,18 +31,34 @@
 
 namespace {
 
+// Work item t -> (i, j, m0): the vector index q is fastest, then the rows.
+// With j-blocking (ljb >= 0) the rows run in blocks of JB = 2^ljb rows along
+// j, i next, then the block: consecutive items sweep an i-plane of JB rows,
+// so the rows i-1 and i+1 a work item reads were fetched at most two such
+// planes earlier (a few hundred KB) and are still in L2 on large grids.
 struct SGeo {
     int nx, ny, nh, nq;   // nh = nz/2 cells per half-row, nq = nh/cpt thread vectors per half-row
     int cpt;              // cells per thread (4, or 8 for 16-bit storage)
-    int lq, lny;          // log2 of nq and ny (-1: not a power of two)
+    int lq, lny, lnx;     // log2 of nq, ny and nx (-1: not a power of two)
+    int ljb;              // log2 of the rows per j-block (-1: no blocking)
     uint32_t half;        // N/2
     __device__ __forceinline__ void decode(uint32_t t, int &i, int &j, int &m0) const
     {
         uint32_t row, q;
         if (lq >= 0) { row = t >> lq; q = t & ((1u << lq) - 1u); }
         else { row = t / (uint32_t)nq; q = t - row * (uint32_t)nq; }
-        if (lny >= 0) { i = (int)(row >> lny); j = (int)(row & ((1u << lny) - 1u)); }
-        else { i = (int)(row / (uint32_t)ny); j = (int)(row - (uint32_t)i * ny); }
+        if (ljb >= 0) {
+            const uint32_t jj = row & ((1u << ljb) - 1u), rest = row >> ljb;   // rest = block * nx + i
+            uint32_t blk, ii;
+            if (lnx >= 0) { blk = rest >> lnx; ii = rest & ((1u << lnx) - 1u); }
+            else { blk = rest / (uint32_t)nx; ii = rest - blk * (uint32_t)nx; }
+            i = (int)ii;
+            j = (int)((blk << ljb) + jj);
+        } else if (lny >= 0) {
+            i = (int)(row >> lny); j = (int)(row & ((1u << lny) - 1u));
+        } else {
+            i = (int)(row / (uint32_t)ny); j = (int)(row - (uint32_t)i * ny);
+        }
         m0 = cpt * (int)q;
     }
     __device__ __forceinline__ uint32_t row(int i, int j) const { return ((uint32_t)i * ny + j) * (uint32_t)nh; }
@@ -462,6 +478,11 @@ SGeo make_sgeo(int nx, int ny, int nz, int cpt = 4)
     g.nx = nx; g.ny = ny; g.nh = nz / 2; g.cpt = cpt; g.nq = g.nh / cpt;
     g.lq = ilog2_or_neg(g.nq);
     g.lny = ilog2_or_neg(ny);
+    g.lnx = ilog2_or_neg(nx);
+    // 32-row blocks: at 256^3 the half-sweep runs 24.0 us instead of 24.9, at
+    // 512^3 261 us instead of 285 (the whole solve 5% faster there)
+    const int jb = 32;
+    g.ljb = (jb < ny && ny % jb == 0) ? ilog2_or_neg(jb) : -1;
     g.half = (uint32_t)nx * ny * (uint32_t)g.nh;
     return g;
 }
