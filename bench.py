@@ -48,7 +48,7 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 # rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of `bench.py --kernels-only
 # [--kernel ...]`, calibrated on a 67.4 MB device copy
 # (profiles/rb_half_sweep_l0_split_traffic_256.md, profiles/rb_half_sweep_l0_traffic_256.md)
-MEASURED_TRAFFIC_L0 = {("rb_half_sweep_l0_split", 256): 158.6e6, ("rb_half_sweep_l0", 256): 242.7e6}
+MEASURED_TRAFFIC_L0 = {("rb_half_sweep_l0_split", 256): 153.8e6, ("rb_half_sweep_l0", 256): 242.7e6}
 
 
 def log(*a):
