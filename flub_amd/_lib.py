@@ -83,6 +83,10 @@ SIGNATURES = {
     "flub_conv3d_bf16": [_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _P],
     "flub_conv3d_wgrad_bf16": [_P, _P, _P, _P, _Z, _I, _I, _I, _I, _I, _I, _P],
     "flub_head_fwd": [_P, _P, _F, _P, _Z, _I, _P],
+    "flub_head_fwd_masked": [_P, _P, _F, _P, _Z, _I, _P, _F, _P],
+    "flub_conv3d_upcat_bf16": [_P, _I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _P],
+    "flub_pool2_bf16": [_P, _P, _I, _I, _I, _I, _I, _P],
+    "flub_pack_input_bf16": [_P, _P, _F, _P, _Z, _P],
 }
 _RESTYPE = {"flub_last_error": ctypes.c_char_p}
 

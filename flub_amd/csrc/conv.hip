@@ -111,6 +111,208 @@ __global__ void __launch_bounds__(256) k_conv3(const __hip_bfloat16 *__restrict_
         }
 }
 
+// Tiled forward convolution.  A workgroup (4 wavefronts) computes an output
+// tile of TZ x TY x 16 voxels and all COUT channels: it loads the input tile
+// with a one-voxel halo, (TZ+2) x (TY+2) x 18 voxels x CIN channels, into LDS
+// once (each input voxel is then read from HBM about (TZ+2)(TY+2)18 /
+// (16 TZ TY) times instead of 27), and every wavefront takes its matrix
+// fragments for all 27 taps from LDS.  Each wavefront owns RB = TZ*TY/4 rows
+// of 16 voxels.  The matrix instruction computes D[cout][voxel] = W x X
+// (weights as the A operand), so a lane ends with 4 consecutive output
+// channels of one voxel: 8-byte stores, contiguous across the wavefront.
+//
+// With CA > 0 the input is the concatenation of the nearest 2x upsampling of
+// Xa (CA channels at half resolution) and Xb (CIN - CA channels): the U-Net
+// decoder's up2 + cat, gathered while the tile is loaded instead of being
+// materialised.  Requires W % 16 == 0, H % TY == 0, D % TZ == 0 (and even
+// sizes with CA > 0).  Tiles go to the XCDs in contiguous ranges (workgroup b
+// runs on XCD b % 8), so the halos a tile shares with its neighbours are
+// mostly read from the same L2.
+template <int CIN, int COUT, int TZ, int TY, int CA>
+__global__ void __launch_bounds__(256) k_conv3t(const __hip_bfloat16 *__restrict__ Xa,
+                                                const __hip_bfloat16 *__restrict__ Xb,
+                                                const __hip_bfloat16 *__restrict__ Wt,
+                                                const float *__restrict__ bias, __hip_bfloat16 *__restrict__ Y,
+                                                float *__restrict__ Y32, int nb, int D, int H, int W, int relu)
+{
+    constexpr int HZ = TZ + 2, HY = TY + 2, HX = 18;
+    constexpr int NV = HZ * HY * HX;
+    constexpr int C8 = CIN / 8;
+    constexpr int VS = CIN;   // LDS values per voxel
+    constexpr int K = 27 * CIN;
+    constexpr int KC = (K + 31) / 32;
+    constexpr int NT = COUT / 16;
+    constexpr int RB = TZ * TY / 4;
+    constexpr int CB = CIN - CA;
+    __shared__ __align__(16) __hip_bfloat16 tile[NV * VS];
+
+    const int txn = W / 16, tyn = H / TY, tzn = D / TZ;
+    const uint32_t ntiles = (uint32_t)nb * tzn * tyn * txn;
+    const uint32_t per = (ntiles + 7u) / 8u;
+    const uint32_t t = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+    if (t >= ntiles) return;
+    uint32_t rest = t;
+    const int tx = (int)(rest % txn); rest /= txn;
+    const int ty = (int)(rest % tyn); rest /= tyn;
+    const int tz = (int)(rest % tzn);
+    const int bi = (int)(rest / tzn);
+    const int x0 = tx * 16, y0 = ty * TY, z0 = tz * TZ;
+
+    // the tile load: every load of a thread issued before the first LDS store
+    constexpr int NLD = (NV * C8 + 255) / 256;
+    uint4 buf[NLD];
+#pragma unroll
+    for (int i = 0; i < NLD; ++i) {
+        const int v = threadIdx.x + i * 256;
+        buf[i] = make_uint4(0u, 0u, 0u, 0u);
+        if (v < NV * C8) {
+            const int vox = v / C8, c = (v - vox * C8) * 8;
+            const int hx = vox % HX, hy = (vox / HX) % HY, hz = vox / (HX * HY);
+            const int z = z0 + hz - 1, y = y0 + hy - 1, x = x0 + hx - 1;
+            if (z >= 0 && z < D && y >= 0 && y < H && x >= 0 && x < W) {
+                if (CA > 0 && c < CA) {
+                    const long src = ((((long)bi * (D >> 1) + (z >> 1)) * (H >> 1) + (y >> 1)) * (W >> 1) + (x >> 1));
+                    buf[i] = *reinterpret_cast<const uint4 *>(Xa + src * CA + c);
+                } else {
+                    const long src = ((((long)bi * D + z) * H + y) * W + x);
+                    buf[i] = *reinterpret_cast<const uint4 *>(Xb + src * CB + (c - CA));
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NLD; ++i) {
+        const int v = threadIdx.x + i * 256;
+        if (v < NV * C8) {
+            const int vox = v / C8, c = (v - vox * C8) * 8;
+            *reinterpret_cast<uint4 *>(tile + vox * VS + c) = buf[i];
+        }
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, r16 = lane & 15;
+    f32x4_t acc[RB][NT];
+#pragma unroll
+    for (int mt = 0; mt < RB; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = (f32x4_t){0.0f, 0.0f, 0.0f, 0.0f};
+
+    // Weight fragments: KB steps of 32 K-values per group, the next group's
+    // global loads issued before the current group's matrix instructions.
+    constexpr int KB = 4;
+    auto wload = [&](int kc, bf16x8_t w[NT]) {
+        const int k = kc * 32 + g * 8;
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) w[nt] = k < K ? load8(Wt + (long)(nt * 16 + r16) * K + k) : zero8();
+    };
+    bf16x8_t wcur[KB][NT], wnxt[KB][NT];
+#pragma unroll
+    for (int j = 0; j < KB; ++j) wload(j, wcur[j]);
+#pragma unroll
+    for (int kg = 0; kg < KC; kg += KB) {
+        if (kg + KB < KC) {
+#pragma unroll
+            for (int j = 0; j < KB; ++j) wload(kg + KB + j, wnxt[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < KB; ++j) {
+            const int kc = kg + j;
+            if (kc >= KC) break;
+            const int k = kc * 32 + g * 8;
+            const bool kvalid = k < K;
+            const int tap = kvalid ? k / CIN : 0;
+            const int ci0 = k - tap * CIN;
+            const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;   // offsets + 1 into the halo tile
+#pragma unroll
+            for (int mt = 0; mt < RB; ++mt) {
+                const int row = wave * RB + mt;
+                const int rz = row / TY, ry = row - rz * TY;
+                const int vox = ((rz + dz) * HY + (ry + dy)) * HX + (r16 + dx);
+                const bf16x8_t a = kvalid ? *reinterpret_cast<const bf16x8_t *>(tile + vox * VS + ci0) : zero8();
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wcur[j][nt], a, acc[mt][nt], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < KB; ++j)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) wcur[j][nt] = wnxt[j][nt];
+    }
+    // D[cout = 4g + i][voxel = r16]
+#pragma unroll
+    for (int mt = 0; mt < RB; ++mt) {
+        const int row = wave * RB + mt;
+        const int rz = row / TY, ry = row - rz * TY;
+        const long m = (((long)bi * D + (z0 + rz)) * H + (y0 + ry)) * W + (x0 + r16);
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const int co = nt * 16 + 4 * g;
+            float v[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                v[i] = acc[mt][nt][i] + (bias ? bias[co + i] : 0.0f);
+                if (relu) v[i] = fmaxf(v[i], 0.0f);
+            }
+            if (Y32) {
+                *reinterpret_cast<float4 *>(Y32 + m * COUT + co) = make_float4(v[0], v[1], v[2], v[3]);
+            } else {
+                uint2 w;
+                __hip_bfloat16 *e = reinterpret_cast<__hip_bfloat16 *>(&w);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) e[i] = __float2bfloat16(v[i]);
+                *reinterpret_cast<uint2 *>(Y + m * COUT + co) = w;
+            }
+        }
+    }
+}
+
+// 2x2x2 average pooling of a channels-last BF16 tensor (FP32 sum of the
+// eight values in (dz, dy, dx) order, times 1/8, rounded to BF16); 8
+// channels per thread.
+__global__ void __launch_bounds__(256) k_pool2(const __hip_bfloat16 *__restrict__ X, __hip_bfloat16 *__restrict__ Y,
+                                               int nb, int D, int H, int W, int C)
+{
+    const int C8 = C / 8, d2 = D / 2, h2 = H / 2, w2 = W / 2;
+    const long n = (long)nb * d2 * h2 * w2 * C8;
+    const long t = (long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    long r = t;
+    const int c = (int)(r % C8) * 8; r /= C8;
+    const int x = (int)(r % w2); r /= w2;
+    const int y = (int)(r % h2); r /= h2;
+    const int z = (int)(r % d2);
+    const int b = (int)(r / d2);
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const long src = ((((long)b * D + 2 * z + (q >> 2)) * H + 2 * y + ((q >> 1) & 1)) * W + 2 * x + (q & 1));
+        const bf16x8_t v = load8(X + src * C + c);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) s[e] += (float)v[e];
+    }
+    bf16x8_t o;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) o[e] = (__bf16)(s[e] * 0.125f);
+    *reinterpret_cast<bf16x8_t *>(Y + t * 8) = o;
+}
+
+// The network input: channel 0 = r / s on FLUID cells (0 elsewhere), channel
+// 1 = the FLUID indicator, channels 2..15 zero (the matrix-core K width).
+__global__ void __launch_bounds__(256) k_pack_input(const float *__restrict__ r, const uint8_t *__restrict__ fluid,
+                                                    float s, __hip_bfloat16 *__restrict__ X, long n)
+{
+    const long v = (long)blockIdx.x * 256 + threadIdx.x;
+    if (v >= n) return;
+    const bool f = fluid[v] != 0;
+    bf16x8_t a = zero8();
+    a[0] = (__bf16)(f ? r[v] / s : 0.0f);
+    a[1] = (__bf16)(f ? 1.0f : 0.0f);
+    *reinterpret_cast<bf16x8_t *>(X + v * 16) = a;
+    *reinterpret_cast<bf16x8_t *>(X + v * 16 + 8) = zero8();
+}
+
 // dW partials: part[slot][co][tap][ci] for the voxel chunk of this
 // (block, wavefront); grid (27 taps, chunks).
 template <int CIN, int COUT>
@@ -186,24 +388,62 @@ __global__ void k_wgrad_reduce(const float *__restrict__ part, float *__restrict
     dw[t] = s * scale;
 }
 
-// 1x1 head with one output channel, FP32: y[v] = b + sum_c x[v][c] w[c].
+// 1x1 head with one output channel, FP32: y[v] = b + sum_c x[v][c] w[c];
+// with a mask, y[v] = mask[v] ? (that) * scale : 0.
 __global__ void k_head_fwd(const __hip_bfloat16 *__restrict__ X, const float *__restrict__ w, float b,
-                           float *__restrict__ y, long nvox, int C)
+                           float *__restrict__ y, long nvox, int C, const uint8_t *__restrict__ mask, float scale)
 {
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nvox) return;
     float s = b;
-    for (int c = 0; c < C; ++c) s += __bfloat162float(X[v * C + c]) * w[c];
+    if (C % 8 == 0) {
+        for (int c = 0; c < C; c += 8) {
+            const bf16x8_t x8 = load8(X + v * C + c);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) s += (float)x8[e] * w[c + e];
+        }
+    } else {
+        for (int c = 0; c < C; ++c) s += __bfloat162float(X[v * C + c]) * w[c];
+    }
+    if (mask) s = mask[v] ? s * scale : 0.0f;
     y[v] = s;
 }
 
 // ------------------------------------------------------------------ host
 namespace {
 
+// Tile shapes of the LDS kernel per channel pair: TZ x TY x 16 output voxels,
+// LDS (TZ+2)(TY+2)18 * CIN * 2 bytes (35-83 KB), TZ*TY/4 rows of 16 voxels
+// per wavefront.
+template <int CIN, int COUT> struct TileCfg {
+    static constexpr int TZ = CIN <= 16 ? 4 : 2;
+    static constexpr int TY = (CIN >= 64 || COUT >= 64) ? 4 : 8;
+};
+
+template <int CIN, int COUT, int CA>
+bool conv_tiled(const void *xa, const void *xb, const void *wt, const float *bias, void *y, float *y32, int nb, int d,
+                int h, int w, int relu, hipStream_t s)
+{
+    constexpr int TZ = TileCfg<CIN, COUT>::TZ, TY = TileCfg<CIN, COUT>::TY;
+    if (w % 16 || h % TY || d % TZ) return false;
+    if (CA > 0 && ((d | h | w) & 1)) return false;
+    const long ntiles = (long)nb * (d / TZ) * (h / TY) * (w / 16);
+    if (ntiles >= (1L << 28)) return false;
+    const unsigned per = (unsigned)((ntiles + 7) / 8);
+    k_conv3t<CIN, COUT, TZ, TY, CA><<<per * 8, 256, 0, s>>>((const __hip_bfloat16 *)xa, (const __hip_bfloat16 *)xb,
+                                                          (const __hip_bfloat16 *)wt, bias, (__hip_bfloat16 *)y, y32,
+                                                          nb, d, h, w, relu);
+    return true;
+}
+
 template <int CIN, int COUT>
 int conv_launch(const void *x, const void *wt, const float *bias, void *y, float *y32, int nb, int d, int h, int w,
                 int relu, hipStream_t s)
 {
+    if (conv_tiled<CIN, COUT, 0>(nullptr, x, wt, bias, y, y32, nb, d, h, w, relu, s)) {
+        FLUB_CHECK_LAUNCH();
+        return FLUB_OK;
+    }
     constexpr int MB = COUT >= 64 ? 2 : 4;
     const long nvox = (long)nb * d * h * w;
     const long per_block = 4L * 16 * MB;
@@ -257,6 +497,47 @@ FLUB_API int flub_conv3d_bf16(const void *x, const void *wt, const float *bias, 
     return FLUB_EINVAL;
 }
 
+FLUB_API int flub_conv3d_upcat_bf16(const void *xa, int ca, const void *xb, const void *wt, const float *bias, void *y,
+                                    float *y32, int nb, int d, int h, int w, int cin, int cout, int relu, void *stream)
+{
+    FLUB_REQUIRE(xa && xb && wt && (y || y32), FLUB_EINVAL, "flub_conv3d_upcat_bf16: null pointer");
+    FLUB_REQUIRE(nb > 0 && d > 0 && h > 0 && w > 0, FLUB_ESHAPE, "flub_conv3d_upcat_bf16: empty volume");
+    hipStream_t s = as_stream(stream);
+    bool done = false;
+    if (cin == 48 && cout == 16 && ca == 32)
+        done = conv_tiled<48, 16, 32>(xa, xb, wt, bias, y, y32, nb, d, h, w, relu, s);
+    else if (cin == 96 && cout == 32 && ca == 64)
+        done = conv_tiled<96, 32, 64>(xa, xb, wt, bias, y, y32, nb, d, h, w, relu, s);
+    else {
+        flub_set_error("flub_conv3d_upcat_bf16: unsupported channels %d (%d upsampled) -> %d", cin, ca, cout);
+        return FLUB_EINVAL;
+    }
+    FLUB_REQUIRE(done, FLUB_ESHAPE, "flub_conv3d_upcat_bf16: volume %dx%dx%d not tileable", d, h, w);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+FLUB_API int flub_pool2_bf16(const void *x, void *y, int nb, int d, int h, int w, int c, void *stream)
+{
+    FLUB_REQUIRE(x && y && c % 8 == 0 && c > 0, FLUB_EINVAL, "flub_pool2_bf16: bad argument");
+    FLUB_REQUIRE(nb > 0 && d > 1 && h > 1 && w > 1 && !((d | h | w) & 1), FLUB_ESHAPE, "flub_pool2_bf16: odd volume");
+    const long n = (long)nb * (d / 2) * (h / 2) * (w / 2) * (c / 8);
+    k_pool2<<<(unsigned)((n + 255) / 256), 256, 0, as_stream(stream)>>>((const __hip_bfloat16 *)x,
+                                                                       (__hip_bfloat16 *)y, nb, d, h, w, c);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+FLUB_API int flub_pack_input_bf16(const float *r, const uint8_t *fluid, float s, void *x, size_t nvox, void *stream)
+{
+    FLUB_REQUIRE(r && fluid && x, FLUB_EINVAL, "flub_pack_input_bf16: null pointer");
+    if (!nvox) return FLUB_OK;
+    k_pack_input<<<(unsigned)((nvox + 255) / 256), 256, 0, as_stream(stream)>>>(r, fluid, s, (__hip_bfloat16 *)x,
+                                                                              (long)nvox);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
 FLUB_API int flub_conv3d_wgrad_bf16(const void *x, const void *dy, float *dw, float *work, size_t work_floats, int nb,
                                     int d, int h, int w, int cin, int cout, void *stream)
 {
@@ -276,7 +557,18 @@ FLUB_API int flub_head_fwd(const void *x, const float *w, float b, float *y, siz
     FLUB_REQUIRE(x && w && y && c > 0, FLUB_EINVAL, "flub_head_fwd: bad argument");
     if (!nvox) return FLUB_OK;
     k_head_fwd<<<(unsigned)((nvox + 255) / 256), 256, 0, as_stream(stream)>>>((const __hip_bfloat16 *)x, w, b, y,
-                                                                            (long)nvox, c);
+                                                                            (long)nvox, c, nullptr, 1.0f);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+FLUB_API int flub_head_fwd_masked(const void *x, const float *w, float b, float *y, size_t nvox, int c,
+                                  const uint8_t *mask, float scale, void *stream)
+{
+    FLUB_REQUIRE(x && w && y && mask && c > 0, FLUB_EINVAL, "flub_head_fwd_masked: bad argument");
+    if (!nvox) return FLUB_OK;
+    k_head_fwd<<<(unsigned)((nvox + 255) / 256), 256, 0, as_stream(stream)>>>((const __hip_bfloat16 *)x, w, b, y,
+                                                                            (long)nvox, c, mask, scale);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
 }

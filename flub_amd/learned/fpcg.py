@@ -21,6 +21,7 @@ r and the solve continues on MGPCG.
 from __future__ import annotations
 
 import ctypes
+import math
 import time
 
 import torch
@@ -47,10 +48,16 @@ class LearnedPCG:
         _lib.call("flub_mgpcg_precondition_into", self.mg.handle, _p(r), _p(z), 1)
 
     def net_precondition(self, r, fluid):
-        s = torch.sqrt((r[fluid].double() ** 2).mean()).clamp_min(1e-300).float()
-        inp = torch.stack([torch.where(fluid, r / s, torch.zeros_like(r)), fluid.float()], dim=-1)[None]
-        z = self.net.infer(inp)[0] * (s * self.net.out_scale)
-        return torch.where(fluid, z, torch.zeros_like(z)).contiguous()
+        """z = s * U-Net([r/s, FLUID]) with s the RMS of r over the FLUID cells (r
+        is 0 elsewhere)."""
+        if getattr(self, "_fluid_ref", None) is not fluid:
+            self._fluid_ref = fluid
+            self._fluid_u8 = fluid.to(torch.uint8).contiguous()
+            self._n_fluid = max(int(fluid.sum()), 1)
+        s = math.sqrt(max(float(torch.dot(r.view(-1), r.view(-1))), 0.0) / self._n_fluid)
+        if not s > 0.0:
+            return torch.zeros_like(r)
+        return self.net.precondition(r, self._fluid_u8, s)
 
     @staticmethod
     def dot(a, b):

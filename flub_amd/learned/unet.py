@@ -79,29 +79,83 @@ class UNet(torch.nn.Module):
         d0 = self._c(self._c(torch.cat([up2(d1), e0], dim=-1), "d0a"), "d0b")
         return (d0.float() @ self.head_w) + self.head_b
 
-    @torch.no_grad()
-    def infer(self, x: torch.Tensor) -> torch.Tensor:
-        """Forward without autograd bookkeeping (same kernels)."""
-        n, d, h, w, c = x.shape
-        x16 = torch.zeros((n, d, h, w, PAD_CH), device=x.device, dtype=torch.bfloat16)
-        x16[..., :IN_CH] = x.to(torch.bfloat16)
-        wt = {k: v.detach().to(torch.bfloat16).contiguous() for k, v in self.w.items()}
+    def _bf16_weights(self):
+        key = tuple(int(v._version) for v in self.w.values())
+        if getattr(self, "_wt_key", None) != key:
+            self._wt = {k: v.detach().to(torch.bfloat16).contiguous() for k, v in self.w.items()}
+            self._wt_key = key
+        return self._wt
+
+    def _trunk(self, x16: torch.Tensor) -> torch.Tensor:
+        """The network up to the head on a 16-channel BF16 input: the LDS-tiled
+        convolutions, BF16 pooling, and the decoder's up2 + cat folded into its
+        first convolution of each level (flub_conv3d_upcat_bf16) where the
+        volume allows, the unfused sequence elsewhere."""
+        n, d, h, w, _ = x16.shape
+        wt = self._bf16_weights()
+        st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
         def c3(t, name):
             return conv3d(t, wt[name], self.b[name].detach(), True)
 
+        def pool(t):
+            nn, dd, hh, ww, cc = t.shape
+            y = torch.empty((nn, dd // 2, hh // 2, ww // 2, cc), device=t.device, dtype=torch.bfloat16)
+            _lib.call("flub_pool2_bf16", ctypes.c_void_p(t.data_ptr()), ctypes.c_void_p(y.data_ptr()), nn, dd, hh, ww,
+                      cc, st)
+            return y
+
+        def upcat(lo, hi, name):
+            nn, dd, hh, ww, ch = hi.shape
+            cin, cout = lo.shape[-1] + ch, wt[name].shape[0]
+            if ww % 16 == 0 and hh % 8 == 0 and dd % 8 == 0:
+                y = torch.empty((nn, dd, hh, ww, cout), device=hi.device, dtype=torch.bfloat16)
+                _lib.call("flub_conv3d_upcat_bf16", ctypes.c_void_p(lo.data_ptr()), lo.shape[-1],
+                          ctypes.c_void_p(hi.data_ptr()), ctypes.c_void_p(wt[name].data_ptr()),
+                          ctypes.c_void_p(self.b[name].detach().data_ptr()), ctypes.c_void_p(y.data_ptr()), None, nn,
+                          dd, hh, ww, cin, cout, 1, st)
+                return y
+            return c3(torch.cat([up2(lo), hi], dim=-1), name)
+
         e0 = c3(c3(x16, "e0a"), "e0b")
-        e1 = c3(c3(pool2(e0), "e1a"), "e1b")
-        e2 = c3(c3(pool2(e1), "e2a"), "e2b")
-        d1 = c3(c3(torch.cat([up2(e2), e1], dim=-1), "d1a"), "d1b")
+        e1 = c3(c3(pool(e0), "e1a"), "e1b")
+        e2 = c3(c3(pool(e1), "e2a"), "e2b")
+        d1 = c3(upcat(e2, e1, "d1a"), "d1b")
         del e1, e2
-        d0 = c3(c3(torch.cat([up2(d1), e0], dim=-1), "d0a"), "d0b")
+        return c3(upcat(d1, e0, "d0a"), "d0b")
+
+    @torch.no_grad()
+    def infer(self, x: torch.Tensor) -> torch.Tensor:
+        """Forward without autograd bookkeeping: x (N, D, H, W, 2) float -> (N, D, H, W)."""
+        n, d, h, w, c = x.shape
+        x16 = torch.zeros((n, d, h, w, PAD_CH), device=x.device, dtype=torch.bfloat16)
+        x16[..., :IN_CH] = x.to(torch.bfloat16)
+        d0 = self._trunk(x16)
         y = torch.empty((n, d, h, w), device=x.device, dtype=torch.float32)
         hw = self.head_w.detach().float().contiguous()
         _lib.call("flub_head_fwd", ctypes.c_void_p(d0.data_ptr()), ctypes.c_void_p(hw.data_ptr()),
                   float(self.head_b.detach()[0]), ctypes.c_void_p(y.data_ptr()), y.numel(), d0.shape[-1],
                   ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
         return y
+
+    @torch.no_grad()
+    def precondition(self, r: torch.Tensor, fluid_u8: torch.Tensor, s: float) -> torch.Tensor:
+        """z = s * out_scale * U-Net([r/s, FLUID]) on FLUID cells, 0 elsewhere, for
+        one (D, H, W) FP32 residual; the input packing and the output scaling
+        and masking are fused into the first and last kernels."""
+        d, h, w = r.shape
+        st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        x16 = torch.empty((1, d, h, w, PAD_CH), device=r.device, dtype=torch.bfloat16)
+        _lib.call("flub_pack_input_bf16", ctypes.c_void_p(r.data_ptr()), ctypes.c_void_p(fluid_u8.data_ptr()),
+                  float(s), ctypes.c_void_p(x16.data_ptr()), r.numel(), st)
+        d0 = self._trunk(x16)
+        del x16
+        z = torch.empty((d, h, w), device=r.device, dtype=torch.float32)
+        hw = self.head_w.detach().float().contiguous()
+        _lib.call("flub_head_fwd_masked", ctypes.c_void_p(d0.data_ptr()), ctypes.c_void_p(hw.data_ptr()),
+                  float(self.head_b.detach()[0]), ctypes.c_void_p(z.data_ptr()), z.numel(), d0.shape[-1],
+                  ctypes.c_void_p(fluid_u8.data_ptr()), float(s * self.out_scale), st)
+        return z
 
     def state_numpy(self):
         out = {f"w_{k}": v.detach().cpu().numpy() for k, v in self.w.items()}

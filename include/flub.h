@@ -231,8 +231,23 @@ int flub_conv3d_bf16(const void *x, const void *wt, const float *bias, void *y, 
  * (work_floats >= 4 * cout * 27 * cin). */
 int flub_conv3d_wgrad_bf16(const void *x, const void *dy, float *dw, float *work, size_t work_floats, int nb, int d,
                            int h, int w, int cin, int cout, void *stream);
+/* The U-Net decoder's first convolution of a level over cat(up2(xa), xb) without
+ * materialising it: channels [0, ca) are xa (nb x d/2 x h/2 x w/2 x ca, nearest 2x
+ * upsampling), channels [ca, cin) are xb (nb x d x h x w x (cin-ca)).  Supported:
+ * (cin, ca, cout) = (48, 32, 16), (96, 64, 32); w % 16 == 0, h, d multiples of 8. */
+int flub_conv3d_upcat_bf16(const void *xa, int ca, const void *xb, const void *wt, const float *bias, void *y,
+                           float *y32, int nb, int d, int h, int w, int cin, int cout, int relu, void *stream);
+/* 2x2x2 average pooling, channels-last BF16 (c a multiple of 8; even sizes). */
+int flub_pool2_bf16(const void *x, void *y, int nb, int d, int h, int w, int c, void *stream);
+/* The network input (16 BF16 channels per voxel): r/s on FLUID voxels, the FLUID
+ * indicator, zeros. */
+int flub_pack_input_bf16(const float *r, const uint8_t *fluid, float s, void *x, size_t nvox, void *stream);
 /* 1x1 output head with one channel: y[v] = b + sum_c x[v][c] w[c] (FP32 out). */
 int flub_head_fwd(const void *x, const float *w, float b, float *y, size_t nvox, int c, void *stream);
+/* The head followed by the preconditioner's output scaling: y[v] = mask[v] ?
+ * (b + sum_c x[v][c] w[c]) * scale : 0. */
+int flub_head_fwd_masked(const void *x, const float *w, float b, float *y, size_t nvox, int c, const uint8_t *mask,
+                         float scale, void *stream);
 
 #ifdef __cplusplus
 }

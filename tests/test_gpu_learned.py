@@ -55,6 +55,78 @@ def test_conv_gradients_match_fp64(cin, cout):
     assert np.allclose(b.grad.cpu().numpy(), gs.reshape(-1, cout).sum(0), rtol=1e-4, atol=1e-3)
 
 
+ALL_PAIRS = PAIRS + [(32, 16), (64, 32), (32, 96), (16, 48)]
+
+
+@pytest.mark.parametrize("cin,cout", ALL_PAIRS)
+@pytest.mark.parametrize("shape", [(1, 4, 8, 16), (2, 8, 8, 32)])
+def test_tiled_conv_matches_fp64(cin, cout, shape):
+    """The LDS-tiled kernel (taken for these tileable volumes, W % 16 == 0),
+    including tiles on every face of the volume, against the FP64 reference."""
+    import torch
+    from flub_amd.learned.conv import conv3d
+    from oracle import unet_ref as R
+    rng = np.random.default_rng(cin * 1000 + cout + shape[3])
+    x = _bf(rng, shape + (cin,))
+    w = _bf(rng, (cout, 27 * cin), 1.0 / np.sqrt(27 * cin))
+    b = torch.from_numpy(rng.normal(size=cout)).float()
+    y = conv3d(x.cuda(), w.cuda(), b.cuda(), relu=True, out_fp32=True).cpu().numpy()
+    ref = R.conv3d(x.float().numpy(), w.float().numpy(), b.numpy(), relu=True)
+    assert np.abs(y - ref).max() < 1e-4 * max(1.0, np.abs(ref).max())
+    yb = conv3d(x.cuda(), w.cuda(), None, relu=False).float().cpu().numpy()
+    refb = R.conv3d(x.float().numpy(), w.float().numpy())
+    assert np.abs(yb - refb).max() < 1e-2 * max(1.0, np.abs(refb).max())   # BF16 output rounding
+
+
+@pytest.mark.parametrize("ca,cb,cout", [(32, 16, 16), (64, 32, 32)])
+def test_upcat_conv_equals_conv_of_concatenation(ca, cb, cout):
+    """conv3d over cat(up2(xa), xb) gathered in the tile load is bitwise the
+    tiled convolution of the materialised concatenation."""
+    import ctypes
+    import torch
+    from flub_amd import _lib
+    from flub_amd.learned.conv import conv3d
+    from flub_amd.learned.unet import up2
+    rng = np.random.default_rng(ca + cb)
+    n, d, h, w = 1, 8, 16, 32
+    xa = _bf(rng, (n, d // 2, h // 2, w // 2, ca)).cuda()
+    xb = _bf(rng, (n, d, h, w, cb)).cuda()
+    wt = _bf(rng, (cout, 27 * (ca + cb)), 1.0 / np.sqrt(27 * (ca + cb))).cuda()
+    b = torch.from_numpy(rng.normal(size=cout)).float().cuda()
+    ref = conv3d(torch.cat([up2(xa), xb], dim=-1).contiguous(), wt, b, relu=True)
+    y = torch.empty_like(ref)
+    _lib.call("flub_conv3d_upcat_bf16", ctypes.c_void_p(xa.data_ptr()), ca, ctypes.c_void_p(xb.data_ptr()),
+              ctypes.c_void_p(wt.data_ptr()), ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(y.data_ptr()), None, n, d,
+              h, w, ca + cb, cout, 1, None)
+    torch.cuda.synchronize()
+    assert torch.equal(y.view(torch.int16), ref.view(torch.int16))
+
+
+def test_pool2_and_fused_precondition():
+    import ctypes
+    import torch
+    from flub_amd import _lib
+    from flub_amd.learned.unet import UNet
+    rng = np.random.default_rng(5)
+    x = _bf(rng, (1, 8, 8, 16, 32)).cuda()
+    y = torch.empty((1, 4, 4, 8, 32), device="cuda", dtype=torch.bfloat16)
+    _lib.call("flub_pool2_bf16", ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()), 1, 8, 8, 16, 32, None)
+    ref = x.double().view(1, 4, 2, 4, 2, 8, 2, 32).mean(dim=(2, 4, 6))
+    rel = ((y.double() - ref).abs() / ref.abs().clamp_min(1e-30)).max()
+    assert float(rel) <= 2.0 ** -8
+    # the fused preconditioner (packing, tiled trunk, masked scaled head)
+    # against the unfused torch packing and head
+    net = UNet(3).cuda()
+    n = 32
+    fluid = torch.from_numpy(rng.random((n, n, n)) < 0.7).cuda()
+    r = torch.from_numpy(rng.normal(size=(n, n, n))).float().cuda() * fluid
+    s = float(torch.sqrt((r.double() ** 2).sum() / fluid.sum()))
+    z = net.precondition(r.contiguous(), fluid.to(torch.uint8).contiguous(), s)
+    inp = torch.stack([torch.where(fluid, r / s, torch.zeros_like(r)), fluid.float()], dim=-1)[None]
+    z_ref = torch.where(fluid, net.infer(inp)[0] * (s * net.out_scale), torch.zeros_like(r))
+    assert float((z - z_ref).abs().max()) <= 1e-6 * float(z_ref.abs().max())
+
+
 def test_unet_trains_and_learned_pcg_converges():
     import torch
     from flub_amd.learned.data import make_pairs
