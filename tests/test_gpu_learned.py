@@ -166,3 +166,55 @@ def test_unet_trains_and_learned_pcg_converges():
     x2, st2 = solver.solve(b, fluid, rel_tol=1e-12, use_net=False)
     assert st2["rTr"] < 1e-12 * st2["rTr0"]
     assert np.linalg.norm((x - x2).cpu().numpy()) < 1e-4 * np.linalg.norm(x2.cpu().numpy())
+
+
+def test_trained_network_gain():
+    """What the trained network gains: on held-out scenes (seeds 3000-3002,
+    outside the training seeds 1000-2999), one application of the trained
+    U-Net followed by the optimal step reduces the residual far more than the
+    same network untrained; the V-cycle's reduction is measured beside it.
+    (Measured at 128^3 and 256^3 the trained network reduces |r| about 2-3x
+    per application against the V-cycle's 3.5x, at 12x a V-cycle's cost:
+    the learned solver hands over to MGPCG after 2-4 iterations.)"""
+    import os
+    import torch
+    from flub_amd.learned.data import divergence, scene_torch
+    from flub_amd.learned.evaluate import load_net
+    from flub_amd.learned.fpcg import LearnedPCG
+    from flub_amd.learned.unet import UNet
+    from flub_amd.multigrid.mgpcg import MGPCG
+    from flub_amd import device as wp
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    trained = load_net(os.path.join(root, "flub_amd", "learned", "weights", "unet_128.npz"))
+    untrained = UNet(7).cuda()
+    untrained.out_scale = trained.out_scale
+    n = 64
+    gains = []
+    for seed in (3000, 3001, 3002):
+        tag, u, v, w = scene_torch(n, seed)
+        b = divergence(tag, u, v, w)
+        fluid = tag == 1
+        mg = MGPCG(3, (n, n, n), 4)
+        mg.pre_and_post_smoothing = 2
+        mg.set_tags(wp.from_numpy(tag.cpu().numpy()))
+
+        def step_ratio(z):
+            """|b - alpha A z| / |b| with the optimal alpha (A z = -ndt(z))."""
+            q = torch.empty_like(b)
+            LearnedPCG(mg, None).apply_A(z.contiguous(), q)
+            alpha = float(torch.dot(z.view(-1).double(), b.view(-1).double()) /
+                          torch.dot(z.view(-1).double(), q.view(-1).double()))
+            r = b.double() - alpha * q.double()
+            return float(r.norm() / b.double().norm())
+
+        rt = step_ratio(LearnedPCG(mg, trained).net_precondition(b, fluid))
+        ru = step_ratio(LearnedPCG(mg, untrained).net_precondition(b, fluid))
+        zv = torch.empty_like(b)
+        LearnedPCG(mg, None).vcycle(b.contiguous(), zv)
+        rv = step_ratio(zv)
+        gains.append((rt, ru, rv))
+        assert rt < 0.8, (seed, rt)            # the trained network reduces the residual
+        assert rt < 0.75 * ru, (seed, rt, ru)  # and far more than the untrained one
+        assert rv < 1.0
+    print("residual ratio after one application (trained, untrained, V-cycle):", gains)
