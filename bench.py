@@ -309,6 +309,19 @@ def droplets_box_main(args):
             out["divergence_fractional"] = {"ms": round(d_ms, 3), "GBs": round(dbytes / (d_ms * 1e-3) / 1e9, 1),
                                             "bytes": dbytes}
             out["kernels"] = profile_kernels(pr)
+            # the step after the projection: velocity extrapolation into the air
+            # (set_unknown + 16 n / 128 passes, computed as two: later passes repeat the second)
+            from flub_amd.const_param import Field
+            from flub_amd.extrapolate import extrapolate
+            phi = Field(f=device.from_numpy(s["phi"]))
+            extrapolate(pr.vel0, pr.vel_owner, phi)
+            e_ms, _ = _timed(lambda: extrapolate(pr.vel0, pr.vel_owner, phi))
+            # set_unknown: phi read + 3 marks written (16 B/cell); each of the two
+            # passes: the mark (4 B) of every face, the velocity written on faces
+            # marked 1, the known neighbours' values read (4 + 4 B/face at most)
+            ebytes = 16 * n ** 3 + 2 * 12 * faces
+            out["extrapolate"] = {"ms": round(e_ms, 3), "GBs": round(ebytes / (e_ms * 1e-3) / 1e9, 1),
+                                  "bytes": ebytes, "passes": 16 * n // 128}
         del pr, solver
     # one network application at this size
     r = torch.randn((1, n, n, n, 2), device="cuda")
@@ -327,6 +340,7 @@ def droplets_box_main(args):
                                              f"subtraction", "grid": n, "best_solver": best},
                       "solvers": {k: out[k] for k in ("fp32", "bf16", "learned+bf16")},
                       "geometry": out["geometry"], "divergence_fractional": out["divergence_fractional"],
+                      "extrapolate": out["extrapolate"],
                       "unet": out["unet"], "kernels": out["kernels"]}), flush=True)
 
 

@@ -50,6 +50,11 @@ SIGNATURES = {
     "flub_face_fractions": [_P] * 6 + [_I, _I, _I, _F, _I, _P, _P, _P, _P, _P],
     "flub_solid_vel": [_P, _P, _I, _I, _I, _I, _F, _P, _P, _P, _P],
     "flub_divergence_fractional_3d": [_P] * 10 + [_I, _I, _I, _P],
+    "flub_set_unknown": [_P, _P, _P, _P, _I, _I, _I, _P],
+    "flub_extrapolate_3d": [_P, _P, _I, _I, _I, _P],
+    "flub_flush_1_3d": [_P, _Z, _P],
+    "flub_extrapolate_by_1": [_P] * 6 + [_I, _I, _I, _P],
+    "flub_extrapolate": [_P] * 7 + [_I, _I, _I, _I, _P],
     "flub_mask_with_tag": [_P, _P, _Z, _P],
     "flub_scale": [_P, _F, _Z, _P],
     "flub_subtract_gradient_tag": [_P, _P, _P, _P, _I, _I, _I, _I, _P],

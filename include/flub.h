@@ -101,6 +101,33 @@ int flub_solid_vel(float *vel, const int32_t *owner, int sx, int sy, int sz, int
                    const float *x, const float *v, const float *omega, void *stream);
 
 /* ---------------------------------------------------------------------------
+ * Velocity extrapolation into the air, the step after the projection
+ * (extrapolate.py:9-47; water_3d.py:70-73).  Marks: 1 = to extrapolate,
+ * 0 = known, -1 = extrapolated in this pass (flushed back to 1).
+ * ------------------------------------------------------------------------- */
+/* utils/util.py:424-437 set_unknown over the nx x ny x nz cells: the face mark
+ * arrays (u, v, w shapes) get phi >= 0 ? 1 : 0 at the cell index; their last
+ * face layer is left as it is. */
+int flub_set_unknown(const float *phi, int32_t *tu, int32_t *tv, int32_t *tw, int nx, int ny, int nz, void *stream);
+/* extrapolate.py:9-31 extrapolate_3d on one face array (sx, sy, sz) in place:
+ * faces marked 1 take the mean of the faces marked 0 in their in-range
+ * 27-neighbourhood and are marked -1 (0.0 and left at 1 when there are none). */
+int flub_extrapolate_3d(float *q, int32_t *unknown, int sx, int sy, int sz, void *stream);
+/* extrapolate.py:33-37 flush_1_3d: -1 -> 1 over n marks. */
+int flub_flush_1_3d(int32_t *tag, size_t n, void *stream);
+/* extrapolate.py:40-47 extrapolate_by_1 on the three face arrays of an
+ * nx x ny x nz grid: each axis' extrapolate + flush as one kernel (marks -1 end
+ * as 1, as after the reference's pair of launches). */
+int flub_extrapolate_by_1(float *u, float *v, float *w, int32_t *tu, int32_t *tv, int32_t *tw, int nx, int ny,
+                          int nz, void *stream);
+/* water_3d.py:70-73 extrapolate: set_unknown from phi, then n_passes times
+ * extrapolate_by_1 -- computed as min(n_passes, 2) passes, which gives the same
+ * fields: after two passes the marks are back in {0, 1, owners >= 2} and every
+ * later pass recomputes the same values. */
+int flub_extrapolate(float *u, float *v, float *w, int32_t *tu, int32_t *tv, int32_t *tw, const float *phi, int nx,
+                     int ny, int nz, int n_passes, void *stream);
+
+/* ---------------------------------------------------------------------------
  * Grid kernels of the projection (launched by example_fluid.py:84-94,120-147).
  * ------------------------------------------------------------------------- */
 /* divergence.py:44-99 divergence_tag: div = (vr-vl)+(vt-vb)+(vn-vf), a face flux

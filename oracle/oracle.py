@@ -102,6 +102,9 @@ def _declare(L):
     L.orc_phi_to_fraction.argtypes = [_P, _P, _I, _I, _I, _I]
     L.orc_solid_vel.argtypes = [_P, _P, _I, _I, _I, _I, _F, _F, _F, _F, _P, _P, _P]
     L.orc_divergence_fractional.argtypes = [_P] * 10 + [_I, _I, _I]
+    L.orc_set_unknown.argtypes = [_P, _P, _P, _P, _I, _I, _I]
+    L.orc_extrapolate_3d.argtypes = [_P, _P, _I, _I, _I]
+    L.orc_flush_1.argtypes = [_P, _Z]
 
 
 def set_threads(n: int) -> int:
@@ -432,3 +435,36 @@ def divergence_fractional(u, v, w, fu, fv, fw, ur, vr, wr):
     lib().orc_divergence_fractional(_ptr(u), _ptr(v), _ptr(w), _ptr(div), _ptr(fu), _ptr(fv), _ptr(fw), _ptr(ur),
                                     _ptr(vr), _ptr(wr), nx, ny, nz)
     return div
+
+
+# ------------------------------------------------------------ extrapolation
+def set_unknown(phi, tu, tv, tw):
+    """utils/util.py:424-437; returns new (tu, tv, tw) (last face layers kept)."""
+    phi = _arr(phi, np.float32)
+    tu, tv, tw = (_arr(t, np.int32).copy() for t in (tu, tv, tw))
+    lib().orc_set_unknown(_ptr(phi), _ptr(tu), _ptr(tv), _ptr(tw), *phi.shape)
+    return tu, tv, tw
+
+
+def extrapolate_3d(q, unknown):
+    """extrapolate.py:9-31 on one face array; returns new (q, unknown)."""
+    q = _arr(q, np.float32).copy()
+    unknown = _arr(unknown, np.int32).copy()
+    lib().orc_extrapolate_3d(_ptr(q), _ptr(unknown), *q.shape)
+    return q, unknown
+
+
+def flush_1(tag):
+    tag = _arr(tag, np.int32).copy()
+    lib().orc_flush_1(_ptr(tag), tag.size)
+    return tag
+
+
+def extrapolate_by_1(vel, tags):
+    """extrapolate.py:40-47 on (u, v, w) and their tags; returns (vel, tags)."""
+    out_v, out_t = [], []
+    for q, t in zip(vel, tags):
+        q, t = extrapolate_3d(q, t)
+        out_v.append(q)
+        out_t.append(t)
+    return tuple(out_v), tuple(flush_1(t) for t in out_t)

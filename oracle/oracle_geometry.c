@@ -8,7 +8,9 @@
  *   utils/util.py:168-228 (fraction_edge / fraction_triangle / fraction_quad),
  *   :231-235 (compute_v_solid_3d);
  *   rigid_coupling.py:51-70 (precompute_solid_vel);
- *   divergence.py:5-41 (divergence_fractional_3d).
+ *   divergence.py:5-41 (divergence_fractional_3d);
+ *   extrapolate.py:9-47 (extrapolate_3d, flush_1_3d, extrapolate_by_1),
+ *   utils/util.py:424-437 (set_unknown).
  *
  * FP32 throughout, one point at a time in the reference's expression order;
  * built with -ffp-contract=off so every operation rounds as written.  Only
@@ -21,7 +23,9 @@
  * per axis against that exact area on every face of a grid cut by a plane;
  * the solid velocity against v + omega x (p - x); the fractional divergence
  * against the plain flux divergence (all fractions 1), the rigid flux
- * divergence (all fractions 0) and u == u_rigid (any fractions).
+ * divergence (all fractions 0) and u == u_rigid (any fractions); the
+ * extrapolation against a vectorised mean over the known 27-neighbourhood,
+ * constant fields (kept exactly), idempotence and unchanged tags.
  *
  * Rigid bodies are structure-of-arrays: x[3n], q[4n] (x, y, z, w),
  * gtype[n] (0 none, 1 quad, 2 circle, 3 rect, 4 "x", 5 mesh), radius[3n].
@@ -253,4 +257,58 @@ void orc_divergence_fractional(const float *u, const float *v, const float *w, f
                                     (1.0f - b) * vr[vl] + (1.0f - n) * wr[wh] - (1.0f - f) * wr[wl];
                 div[GIDX(i, j, k, ny, nz)] = fluid + solid;
             }
+}
+
+/* ---------------------------------------------------------------------------
+ * Velocity extrapolation (extrapolate.py:9-47, utils/util.py:424-437).
+ * ------------------------------------------------------------------------- */
+
+/* utils/util.py:424-437 set_unknown over the nx x ny x nz cells: face arrays
+ * tu/tv/tw (their own shapes) get 1 where phi >= 0, else 0, at the cell index;
+ * the last face layer of each axis is left as it is. */
+void orc_set_unknown(const float *phi, int32_t *tu, int32_t *tv, int32_t *tw, int nx, int ny, int nz)
+{
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const int32_t f = phi[GIDX(i, j, k, ny, nz)] >= 0.0f ? 1 : 0;
+                tu[GIDX(i, j, k, ny, nz)] = f;
+                tv[GIDX(i, j, k, ny + 1, nz)] = f;
+                tw[GIDX(i, j, k, ny, nz + 1)] = f;
+            }
+}
+
+/* extrapolate.py:9-31 extrapolate_3d on one face array (sx, sy, sz), in place
+ * and serially: a face with unknown == 1 takes the mean of its in-range
+ * 27-neighbourhood faces with unknown == 0 (summed ii, jj, kk from -1 to 1,
+ * ii slowest) and is marked -1, or 0.0 when there is none.  Known faces are
+ * never written and -1 / 1 are both "not known", so the serial order gives the
+ * parallel launch's result. */
+void orc_extrapolate_3d(float *q, int32_t *unknown, int sx, int sy, int sz)
+{
+    for (int i = 0; i < sx; ++i)
+        for (int j = 0; j < sy; ++j)
+            for (int k = 0; k < sz; ++k) {
+                const size_t t = GIDX(i, j, k, sy, sz);
+                if (unknown[t] != 1) continue;
+                int n = 0;
+                float s = 0.0f;
+                for (int a = -1; a <= 1; ++a)
+                    for (int b = -1; b <= 1; ++b)
+                        for (int c = -1; c <= 1; ++c) {
+                            const int ii = i + a, jj = j + b, kk = k + c;
+                            if (ii < 0 || ii >= sx || jj < 0 || jj >= sy || kk < 0 || kk >= sz) continue;
+                            const size_t u = GIDX(ii, jj, kk, sy, sz);
+                            if (unknown[u] == 0) { n += 1; s += q[u]; }
+                        }
+                if (n > 0) { q[t] = s / (float)n; unknown[t] = -1; }
+                else q[t] = 0.0f;
+            }
+}
+
+/* extrapolate.py:33-37 flush_1_3d: -1 -> 1. */
+void orc_flush_1(int32_t *tag, size_t n)
+{
+    for (size_t t = 0; t < n; ++t)
+        if (tag[t] == -1) tag[t] = 1;
 }
