@@ -322,6 +322,20 @@ def droplets_box_main(args):
             ebytes = 16 * n ** 3 + 2 * 12 * faces
             out["extrapolate"] = {"ms": round(e_ms, 3), "GBs": round(ebytes / (e_ms * 1e-3) / 1e9, 1),
                                   "bytes": ebytes, "passes": 16 * n // 128}
+            # the step before the projection: MacCormack advection of the velocity
+            # and a scalar (advect_3d), out of place
+            from flub_amd.advect import advect_3d
+            from flub_amd.const_param import make_mac_grid
+            new_vel = make_mac_grid(float, (n, n, n))
+            new_q = Field(f=device.empty((n, n, n), float))
+            advect_3d(pr.vel0, phi, new_vel, new_q, dt=1.0 / 60.0, h0=1.0 / n)
+            a_ms, _ = _timed(lambda: advect_3d(pr.vel0, phi, new_vel, new_q, dt=1.0 / 60.0, h0=1.0 / n))
+            # compulsory bytes: each of the 4 fields read and written once, the
+            # velocity read by the traces (12 B/cell) for each of the 4
+            abytes = 4 * (8 + 12) * n ** 3
+            out["advect"] = {"ms": round(a_ms, 3), "GBs": round(abytes / (a_ms * 1e-3) / 1e9, 1),
+                             "bytes": abytes, "fields": 4}
+            del new_vel, new_q
         del pr, solver
     # one network application at this size
     r = torch.randn((1, n, n, n, 2), device="cuda")
@@ -340,7 +354,7 @@ def droplets_box_main(args):
                                              f"subtraction", "grid": n, "best_solver": best},
                       "solvers": {k: out[k] for k in ("fp32", "bf16", "learned+bf16")},
                       "geometry": out["geometry"], "divergence_fractional": out["divergence_fractional"],
-                      "extrapolate": out["extrapolate"],
+                      "extrapolate": out["extrapolate"], "advect": out["advect"],
                       "unet": out["unet"], "kernels": out["kernels"]}), flush=True)
 
 

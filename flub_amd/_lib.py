@@ -51,6 +51,9 @@ SIGNATURES = {
     "flub_solid_vel": [_P, _P, _I, _I, _I, _I, _F, _P, _P, _P, _P],
     "flub_divergence_fractional_3d": [_P] * 10 + [_I, _I, _I, _P],
     "flub_set_unknown": [_P, _P, _P, _P, _I, _I, _I, _P],
+    "flub_advect_mc": [_P, _P, _P, _I, _I, _I, _P, _P, _I, _I, _I, _F, _F, _F, _F, _F, _P],
+    "flub_advect_phi": [_P, _P, _P, _I, _I, _I, _P, _P, _F, _F, _F, _F, _F, _P],
+    "flub_advect_simple_3d": [_P] * 5 + [_I, _I, _I, _F, _F, _P],
     "flub_extrapolate_3d": [_P, _P, _I, _I, _I, _P],
     "flub_flush_1_3d": [_P, _Z, _P],
     "flub_extrapolate_by_1": [_P] * 6 + [_I, _I, _I, _P],

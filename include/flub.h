@@ -128,6 +128,26 @@ int flub_extrapolate(float *u, float *v, float *w, int32_t *tu, int32_t *tv, int
                      int ny, int nz, int n_passes, void *stream);
 
 /* ---------------------------------------------------------------------------
+ * Advection, the step before the projection (advect.py:11-137): RK3 backtrace
+ * through the MAC velocity (u, v, w of an nx x ny x nz grid) sampled
+ * trilinearly, tricubic sampling of the advected field (utils/util.py:33-159).
+ * Positions are in grid units; h is the cell size.
+ * ------------------------------------------------------------------------- */
+/* advect.py:58-112,133-137 advect_mc of one field q_in (gx x gy x gz, stagger st)
+ * into q_out (not q_in): the reference's three launches (forward, backward,
+ * correction with its min/max limiter) as one kernel, bitwise the same. */
+int flub_advect_mc(const float *u, const float *v, const float *w, int nx, int ny, int nz, const float *q_in,
+                   float *q_out, int gx, int gy, int gz, float stx, float sty, float stz, float dt, float h0,
+                   void *stream);
+/* advect.py:38-56 advect_phi: the tricubic value at the backtraced point when it
+ * lies inside the grid, else h0. */
+int flub_advect_phi(const float *u, const float *v, const float *w, int nx, int ny, int nz, const float *phi,
+                    float *new_phi, float stx, float sty, float stz, float dt, float h0, void *stream);
+/* advect.py:25-35 advect_simple_3d: trilinear value at the backtraced cell centre. */
+int flub_advect_simple_3d(const float *u, const float *v, const float *w, const float *q, float *new_q, int nx,
+                          int ny, int nz, float dt, float h, void *stream);
+
+/* ---------------------------------------------------------------------------
  * Grid kernels of the projection (launched by example_fluid.py:84-94,120-147).
  * ------------------------------------------------------------------------- */
 /* divergence.py:44-99 divergence_tag: div = (vr-vl)+(vt-vb)+(vn-vf), a face flux

@@ -103,6 +103,13 @@ def _declare(L):
     L.orc_solid_vel.argtypes = [_P, _P, _I, _I, _I, _I, _F, _F, _F, _F, _P, _P, _P]
     L.orc_divergence_fractional.argtypes = [_P] * 10 + [_I, _I, _I]
     L.orc_set_unknown.argtypes = [_P, _P, _P, _P, _I, _I, _I]
+    L.orc_sample_velocity.argtypes = [_P, _P, _P, _I, _I, _I, _P, _P, _I]
+    L.orc_backtrace.argtypes = [_P, _P, _P, _I, _I, _I, _P, _I, _F, _F]
+    L.orc_tricubic.argtypes = [_P, _I, _I, _I, _P, _P, _I]
+    L.orc_sample_minmax.argtypes = [_P, _I, _I, _I, _P, _F, _P, _P, _P, _I]
+    L.orc_advect_mc.argtypes = [_P, _P, _P, _I, _I, _I, _P, _P, _P, _I, _I, _I, _P, _F, _F]
+    L.orc_advect_phi.argtypes = [_P, _P, _P, _I, _I, _I, _P, _P, _P, _F, _F]
+    L.orc_advect_simple.argtypes = [_P, _P, _P, _I, _I, _I, _P, _P, _F, _F]
     L.orc_extrapolate_3d.argtypes = [_P, _P, _I, _I, _I]
     L.orc_flush_1.argtypes = [_P, _Z]
 
@@ -468,3 +475,78 @@ def extrapolate_by_1(vel, tags):
         out_v.append(q)
         out_t.append(t)
     return tuple(out_v), tuple(flush_1(t) for t in out_t)
+
+
+# ---------------------------------------------------------------- advection
+def _mac3(u, v, w):
+    u, v, w = (_arr(a, np.float32) for a in (u, v, w))
+    nx, ny, nz = v.shape[0], u.shape[1], u.shape[2]
+    return u, v, w, nx, ny, nz
+
+
+def sample_velocity(u, v, w, pts):
+    """utils/util.py:33-86 at points (n, 3) in grid units; returns (n, 3)."""
+    u, v, w, nx, ny, nz = _mac3(u, v, w)
+    pts = _arr(np.reshape(pts, (-1, 3)), np.float32)
+    out = np.empty_like(pts)
+    lib().orc_sample_velocity(_ptr(u), _ptr(v), _ptr(w), nx, ny, nz, _ptr(pts), _ptr(out), len(pts))
+    return out
+
+
+def backtrace(u, v, w, pts, dt, h):
+    """advect.py:11-22 (RK3) of points (n, 3) in grid units; returns (n, 3)."""
+    u, v, w, nx, ny, nz = _mac3(u, v, w)
+    pts = _arr(np.reshape(pts, (-1, 3)), np.float32).copy()
+    lib().orc_backtrace(_ptr(u), _ptr(v), _ptr(w), nx, ny, nz, _ptr(pts), len(pts), float(dt), float(h))
+    return pts
+
+
+def tricubic(q, str_pts):
+    """utils/util.py:130-159 _tricubic at (s, t, r) points (n, 3), index units."""
+    q = _arr(q, np.float32)
+    pts = _arr(np.reshape(str_pts, (-1, 3)), np.float32)
+    out = np.empty(len(pts), np.float32)
+    lib().orc_tricubic(_ptr(q), *q.shape, _ptr(pts), _ptr(out), len(pts))
+    return out
+
+
+def sample_minmax(q, pts, h, stagger):
+    """advect.py:114-131; returns (min, max) arrays."""
+    q = _arr(q, np.float32)
+    pts = _arr(np.reshape(pts, (-1, 3)), np.float32)
+    st = _arr(stagger, np.float32)
+    mn = np.empty(len(pts), np.float32)
+    mx = np.empty(len(pts), np.float32)
+    lib().orc_sample_minmax(_ptr(q), *q.shape, _ptr(pts), float(h), _ptr(st), _ptr(mn), _ptr(mx), len(pts))
+    return mn, mx
+
+
+def advect_mc(u, v, w, q_in, stagger, dt, h0):
+    """advect.py:58-112,133-137 (the three launches); returns q_out."""
+    u, v, w, nx, ny, nz = _mac3(u, v, w)
+    q_in = _arr(q_in, np.float32)
+    q_out = np.empty_like(q_in)
+    q_tmp = np.empty_like(q_in)
+    st = _arr(stagger, np.float32)
+    lib().orc_advect_mc(_ptr(u), _ptr(v), _ptr(w), nx, ny, nz, _ptr(q_in), _ptr(q_out), _ptr(q_tmp), *q_in.shape,
+                        _ptr(st), float(dt), float(h0))
+    return q_out
+
+
+def advect_phi(u, v, w, phi, dt, h0, stagger=(0.5, 0.5, 0.5)):
+    """advect.py:38-56; returns new phi."""
+    u, v, w, nx, ny, nz = _mac3(u, v, w)
+    phi = _arr(phi, np.float32)
+    out = np.empty_like(phi)
+    st = _arr(stagger, np.float32)
+    lib().orc_advect_phi(_ptr(u), _ptr(v), _ptr(w), nx, ny, nz, _ptr(phi), _ptr(out), _ptr(st), float(dt), float(h0))
+    return out
+
+
+def advect_simple(u, v, w, q, dt, h):
+    """advect.py:25-35; returns new q."""
+    u, v, w, nx, ny, nz = _mac3(u, v, w)
+    q = _arr(q, np.float32)
+    out = np.empty_like(q)
+    lib().orc_advect_simple(_ptr(u), _ptr(v), _ptr(w), nx, ny, nz, _ptr(q), _ptr(out), float(dt), float(h))
+    return out
