@@ -740,7 +740,7 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
         e = hipEventCreateWithFlags(&h->ev, hipEventDisableTiming);
         if (e != hipSuccess) { flub_set_error("hipEventCreate: %s", hipGetErrorString(e)); rc = (int)e; goto fail; }
         h->split = h->fast && split_supported(h->lv[0]) && getenv("FLUB_NO_SPLIT") == nullptr;
-        h->fused_pAp = getenv("FLUB_FUSED_PAP") != nullptr;
+        h->fused_pAp = getenv("FLUB_UNFUSED_PAP") == nullptr;
         if (h->split) {
             ALLOC(h->sp.r, n0 * 4);
             ALLOC(h->sp.z[0], n0 * 4);

@@ -428,10 +428,15 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
     while (it < cap) {
         const bool speculate = it + 1 < cap && r_prev * rate > 16.0 * tol;
         // p = z + beta p, then q = ndt(p) and p.q
-        k_update_p_mx<<<blocks_for(n, 256), 256, 0, h->s>>>(h->p, S.z[zs], h->sc, n);
-        FLUB_CHECK_LAUNCH();
         int nbp = 0;
-        TRYM(split_p_Ap(h->s, L0, h->p, nullptr, h->Ap, nullptr, S.code, h->sc, h->part, &nbp));
+        if (h->fused_pAp) {   // p_new = z + beta p at each stencil point, then ndt(p_new)
+            TRYM(split_p_Ap(h->s, L0, h->p, h->p_alt, h->Ap, S.z[zs], S.code, h->sc, h->part, &nbp));
+            float *t = h->p; h->p = h->p_alt; h->p_alt = t;
+        } else {
+            k_update_p_mx<<<blocks_for(n, 256), 256, 0, h->s>>>(h->p, S.z[zs], h->sc, n);
+            FLUB_CHECK_LAUNCH();
+            TRYM(split_p_Ap(h->s, L0, h->p, nullptr, h->Ap, nullptr, S.code, h->sc, h->part, &nbp));
+        }
         TRYM(mg_finish_alpha(h, nbp, eps));
         k_mx_update<<<nb, RBS, 0, h->s>>>(S.r, S.x, h->Ap, h->p, h->sc, n, h->part);
         FLUB_CHECK_LAUNCH();
@@ -479,6 +484,7 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
     FLUB_CHECK_HIP(hipMemcpyAsync(h->Ap, S.zw, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
     TRYM(split_convert(h->s, L0, h->p, S.zw, 4, false));
     FLUB_CHECK_HIP(hipMemcpyAsync(p_home, S.zw, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
+    if (h->p != p_home) { h->p_alt = h->p; h->p = p_home; }
     if (status == FLUB_OK && it >= cap && max_iters < 0) {
         status = FLUB_ENOTCONV;
         flub_set_error("no convergence within %d iterations", cap);

@@ -173,7 +173,9 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
             float kp, km;
             if (o == 0) { kp = w[e]; km = e > 0 ? w[e - 1] : edge; }
             else { km = w[e]; kp = e < CPT - 1 ? w[e + 1] : edge; }
-            out[e] = gs_update(cd[e], c[e], rv[e], xp[e], xm[e], yp[e], ym[e], kp, km);
+            // neighbours hold +0 on non-FLUID cells except in the prolongating
+            // sweep, which adds the coarse correction to every cell on the fly
+            out[e] = gs_update<MODE != SWEEP_PROLONG>(cd[e], c[e], rv[e], xp[e], xm[e], yp[e], ym[e], kp, km);
         }
         if (OUT == SWEEP_OUT_FINAL) {
             const float inv = 1.0f / s0;   // a power of two: exact
@@ -191,7 +193,20 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
             }
         } else {
             stn<TZO, CPT>(reinterpret_cast<TZO *>(a.zout) + own + base, out);
-            if (MODE != SWEEP_PLAIN) stn<TZO, CPT>(reinterpret_cast<TZO *>(a.zout) + oth + base, w);   // zeros / prolongated
+            if (MODE == SWEEP_PROLONG) {
+                // the other colour's prolongated values, +0 on its non-FLUID cells:
+                // the next sweep writes 0 there and never reads them as values, so
+                // this keeps every later read and the cycle's result unchanged while
+                // the sweeps after it see +0 on all non-FLUID neighbours.  That
+                // cell is this cell's k+ (o = 0) or k- (o = 1) neighbour.
+                float wm[CPT];
+#pragma unroll
+                for (int e = 0; e < CPT; ++e)
+                    wm[e] = keep_if(w[e], 0u - ((cd[e] >> (o == 0 ? 9 : 11)) & 1u));
+                stn<TZO, CPT>(reinterpret_cast<TZO *>(a.zout) + oth + base, wm);
+            } else if (MODE == SWEEP_ZERO) {
+                stn<TZO, CPT>(reinterpret_cast<TZO *>(a.zout) + oth + base, w);   // zeros
+            }
             if (OUT == SWEEP_OUT_DOT) {
                 float ro[CPT];
                 ldn<TR, CPT>(reinterpret_cast<const TR *>(a.r) + oth + base, ro);
@@ -263,7 +278,8 @@ __global__ void __launch_bounds__(256) k_srestrict(const TR *__restrict__ r, con
                         float kp, km;
                         if (o == 0) { kp = w[e]; km = e > 0 ? w[e - 1] : edge; }
                         else { km = w[e]; kp = e < CPT - 1 ? w[e + 1] : edge; }
-                        float nd = nsum_code(cd[e], zz[cc][e], xp[e], xm[e], yp[e], ym[e], kp, km);
+                        // the pre-smoothed z holds +0 on non-FLUID cells
+                        float nd = nsum_code_air0(cd[e], zz[cc][e], xp[e], xm[e], yp[e], ym[e], kp, km);
                         if (!code_fluid(cd[e])) nd = 0.0f;
                         res[aa][b][o][e] = rv[e] - (-nd);
                     }

@@ -156,16 +156,38 @@ __device__ __forceinline__ float nsum_code(uint32_t cd, float xI, float xip, flo
     return s;
 }
 
+// nsum_code for neighbour values that hold +0 on every non-FLUID cell (the
+// state every red-black sweep leaves: the reference's smoother writes 0 there,
+// mgpcg.py:72-86): an AIR neighbour then reads 0 by itself, so only SOLID /
+// outside neighbours are masked -- one mask per neighbour instead of two.
+// Bitwise equal to nsum_code under that condition.
+__device__ __forceinline__ float nsum_code_air0(uint32_t cd, float xI, float xip, float xim, float xjp, float xjm,
+                                                float xkp, float xkm)
+{
+    const uint32_t nz = (cd | (cd >> 1)) & 0x555u;   // bit 2n: neighbour n is AIR or FLUID
+    const float xs[6] = {xip, xim, xjp, xjm, xkp, xkm};
+    float s = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t mp = (uint32_t)((int32_t)(nz << (31 - 4 * a)) >> 31);
+        const uint32_t mm = (uint32_t)((int32_t)(nz << (29 - 4 * a)) >> 31);
+        s = s + (keep_if(xs[2 * a] - xI, mp) + keep_if(xs[2 * a + 1] - xI, mm));
+    }
+    return s;
+}
+
 __device__ __forceinline__ int code_divd(uint32_t cd) { return (int)((cd >> 12) & 7u); }
 __device__ __forceinline__ bool code_fluid(uint32_t cd) { return (cd >> 15) != 0; }
 
 // One red-black Gauss-Seidel update of a cell (mgpcg.py:72-86):
 // z + (r + ndt(z)) / divd on FLUID cells, +0 elsewhere (divd is 0 there; the
 // quotient computed for them is discarded).
+template <bool AIR0 = false>
 __device__ __forceinline__ float gs_update(uint32_t cd, float z, float r, float xip, float xim, float xjp, float xjm,
                                            float xkp, float xkm)
 {
-    const float s = nsum_code(cd, z, xip, xim, xjp, xjm, xkp, xkm);
+    const float s = AIR0 ? nsum_code_air0(cd, z, xip, xim, xjp, xjm, xkp, xkm)
+                         : nsum_code(cd, z, xip, xim, xjp, xjm, xkp, xkm);
     const uint32_t divd = (cd >> 12) & 7u;
     return keep_if(z + (r + s) / (float)divd, 0u - (uint32_t)(divd != 0u));
 }
