@@ -553,19 +553,21 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
         const int s = h->pre_post << l;
         const float *zc = h->lv[l + 1].z;
         if (l == 0 && split && h->prec != FLUB_PREC_FP32) {
-            // FP32 post-smoothing from the FP16/BF16 pre-smoothed z; the last
-            // sweep writes the CG's z (unscaled) and the partials of z.r, z.q
+            // FP32 post-smoothing from the FP16/BF16 pre-smoothed z into the
+            // slot's z buffer, which keeps z times the storage scale; the last
+            // sweep also stores 1/scale and the partials of (z/scale).r, .q
             SplitMixedSweep m{};
+            float *Z = S.z[zslot];
             m.code = S.code; m.rh = S.rh; m.r32 = S.r; m.q32 = h->Ap; m.sc = h->sc;
             for (int it = 0; it < s; ++it)
                 for (int ph = 1; ph >= 0; --ph) {
                     const bool last = it + 1 == s && ph == 0;
                     m.phase = ph;
-                    m.zin = it == 0 && ph == 1 ? S.zh : (const void *)S.zw;
-                    m.zout = S.zw;
+                    m.zin = it == 0 && ph == 1 ? S.zh : (const void *)Z;
+                    m.zout = Z;
                     m.zc = h->lv[1].z;
                     m.kind = it == 0 && ph == 1 ? SMX_POST_PROLONG : (last ? SMX_FINAL : SMX_POST);
-                    if (last) { m.z32 = S.z[zslot]; m.part = h->part; m.nblk = &h->vc_dot_nblk; }
+                    if (last) { m.zinv = h->sc + SC_ZINV0 + zslot; m.part = h->part; m.nblk = &h->vc_dot_nblk; }
                     TRY(split_sweep_mixed(h->s, F, h->prec, m));
                 }
             continue;
@@ -1354,7 +1356,9 @@ FLUB_API int flub_mgpcg_precondition_into(flub_mgpcg *h, const float *r, float *
         TRY(split_convert(h->s, L0, r, h->sp.r, 4, true));
         if (h->prec != FLUB_PREC_FP32) TRY(mg_mixed_rscale_of(h, h->sp.r));
         TRY(mg_precondition_split(h, 0));
-        return split_convert(h->s, L0, h->sp.z[0], z, 4, false);
+        TRY(split_convert(h->s, L0, h->sp.z[0], z, 4, false));
+        // the mixed cycle leaves z times its storage scale
+        return h->prec != FLUB_PREC_FP32 ? scale_by_device(h->s, z, h->sc + SC_ZINV0, L0.n) : FLUB_OK;
     }
     FLUB_CHECK_HIP(hipMemcpyAsync(L0.r, r, (size_t)L0.n * 4, hipMemcpyDeviceToDevice, h->s));
     if (h->prec != FLUB_PREC_FP32) TRY(mg_mixed_rscale(h));

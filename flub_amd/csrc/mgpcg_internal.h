@@ -11,8 +11,10 @@
 #define SOLVE_SAFETY_CAP 20000  // iteration cap applied when max_iters == -1
 
 // Device scalar slots (FP64).
+// SC_ZINV0/1: 1/scale of the mixed-precision V-cycle output held in the split z
+// buffer of that slot (the buffer holds z times the storage scale).
 enum { SC_ALPHA = 0, SC_BETA, SC_SUM, SC_OLDZTR, SC_RTR, SC_MEAN, SC_COUNT, SC_RSCALE, SC_AUX0, SC_AUX1,
-       SC_AUX2, SC_AUX3, SC_NUM = 16 };
+       SC_AUX2, SC_AUX3, SC_ZINV0, SC_ZINV1, SC_NUM = 16 };
 // k_finish operations.
 enum { FIN_SUM = 0, FIN_ALPHA, FIN_BETA, FIN_OLDZTR, FIN_RTR, FIN_MEAN, FIN_OUT };
 

@@ -108,11 +108,13 @@ __global__ void __launch_bounds__(RBS) k_sum_partials(const double *__restrict__
 }
 
 // p = z + beta*p (mgpcg.py:229-232), elementwise (any layout).
+// p = z + beta p; zinv (may be NULL) scales z from its storage scale (a power
+// of two: exact).
 __global__ void k_update_p_mx(float *__restrict__ p, const float *__restrict__ z, const double *__restrict__ sc,
-                              uint32_t n)
+                              const double *__restrict__ zinv, uint32_t n)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) p[t] = fmaf((float)sc[SC_BETA], p[t], z[t]);
+    if (t < n) p[t] = fmaf((float)sc[SC_BETA], p[t], zinv ? z[t] * (float)zinv[0] : z[t]);
 }
 
 // Partials of r.r and max|r| without changing r.
@@ -430,10 +432,11 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
         // p = z + beta p, then q = ndt(p) and p.q
         int nbp = 0;
         if (h->fused_pAp) {   // p_new = z + beta p at each stencil point, then ndt(p_new)
-            TRYM(split_p_Ap(h->s, L0, h->p, h->p_alt, h->Ap, S.z[zs], S.code, h->sc, h->part, &nbp));
+            TRYM(split_p_Ap(h->s, L0, h->p, h->p_alt, h->Ap, S.z[zs], S.code, h->sc, h->part, &nbp,
+                            h->sc + SC_ZINV0 + zs));
             float *t = h->p; h->p = h->p_alt; h->p_alt = t;
         } else {
-            k_update_p_mx<<<blocks_for(n, 256), 256, 0, h->s>>>(h->p, S.z[zs], h->sc, n);
+            k_update_p_mx<<<blocks_for(n, 256), 256, 0, h->s>>>(h->p, S.z[zs], h->sc, h->sc + SC_ZINV0 + zs, n);
             FLUB_CHECK_LAUNCH();
             TRYM(split_p_Ap(h->s, L0, h->p, nullptr, h->Ap, nullptr, S.code, h->sc, h->part, &nbp));
         }
@@ -480,6 +483,7 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
     TRYM(split_convert(h->s, L0, S.x, h->x, 4, false));
     TRYM(split_convert(h->s, L0, S.r, L0.r, 4, false));
     TRYM(split_convert(h->s, L0, S.z[zs], L0.z, 4, false));
+    TRYM(scale_by_device(h->s, L0.z, h->sc + SC_ZINV0 + zs, n));   // z buffers keep z * scale
     TRYM(split_convert(h->s, L0, h->Ap, S.zw, 4, false));
     FLUB_CHECK_HIP(hipMemcpyAsync(h->Ap, S.zw, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
     TRYM(split_convert(h->s, L0, h->p, S.zw, 4, false));

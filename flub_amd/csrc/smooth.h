@@ -110,7 +110,7 @@ struct SplitMixedSweep {
     void *rh;              // T split r
     const uint16_t *code;
     const float *r32, *q32;
-    float *z32;            // FINAL output
+    double *zinv;          // FINAL: receives 1/scale (zout keeps z * scale)
     const double *sc;
     double *part;
     int *nblk;
@@ -124,8 +124,11 @@ int split_restrict(hipStream_t s, const Level &F, const float *r, const float *z
                    const Level &C);
 // pout = z + beta*p; Ap = ndt(pout); partials of pout.Ap (all split).  With
 // pout == nullptr: Ap = ndt(p) and the partials of p.Ap (z unused).
+// With zinv, z holds z * scale and the p update uses z * zinv[0].
 int split_p_Ap(hipStream_t s, const Level &L0, const float *p, float *pout, float *Ap, const float *z,
-               const uint16_t *code, const double *sc, double *part, int *nblk);
+               const uint16_t *code, const double *sc, double *part, int *nblk, const double *zinv = nullptr);
+// x *= f[0] (a device scalar).
+int scale_by_device(hipStream_t s, float *x, const double *f, uint32_t n);
 // r = b - ndt(x) in FP64 (stored FP32), partials of r.r and max|r| (split).
 int split_true_residual(hipStream_t s, const Level &L0, float *r, const float *b, const float *x,
                         const uint16_t *code, double *part, int *nblk);

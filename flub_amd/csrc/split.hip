@@ -82,7 +82,7 @@ struct SSweepArgs {
     const uint16_t *code;  // split stencil codes
     const float *r32;      // SRC32: the FP32 CG residual (split); FINAL: for z.r
     const float *q32;      // FINAL: q = -A p of the CG step (split), for z.q
-    float *z32;            // FINAL: the CG's z = z / scale (split, both colours)
+    double *zinv;          // FINAL: receives 1/scale (the z buffer holds z * scale)
     const double *sc;      // SC_RSCALE: storage scale of r (SRC32, FINAL)
     double *part;          // DOT: partials of z.r; FINAL: z.r then z.q
     SGeo g;
@@ -96,8 +96,9 @@ struct SSweepArgs {
 // writes the other colour's prolongated values (out of place, zin != zout).
 // SRC32 takes r of the updated colour from the FP32 CG residual times the
 // storage scale and stores it (rounded to TR, and continues with the rounded
-// values).  OUT: STORE; DOT accumulates z.r over both colours; FINAL writes
-// z / scale of both colours to z32 and accumulates z.r32 and z.q32.
+// values).  OUT: STORE; DOT accumulates z.r over both colours; FINAL stores
+// like STORE (z stays in storage scale; 1/scale goes to zinv) and accumulates
+// (z / scale).r32 and (z / scale).q32 over both colours.
 template <int MODE, int OUT, typename TZI, typename TZO, typename TR, bool SRC32, int CPT>
 __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
 {
@@ -107,6 +108,7 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
     const int lane = threadIdx.x & 63;
     const uint32_t own = (uint32_t)a.cu * g.half, oth = (uint32_t)(a.cu ^ 1) * g.half;
     const float s0 = (SRC32 || OUT == SWEEP_OUT_FINAL) ? (float)a.sc[SC_RSCALE] : 1.0f;
+    if (OUT == SWEEP_OUT_FINAL && blockIdx.x == 0 && threadIdx.x == 0) a.zinv[0] = (double)(1.0f / s0);
     const TZI *zin = reinterpret_cast<const TZI *>(a.zin);
     double acc1 = 0.0, acc2 = 0.0;
     const Chunk ch = xcd_chunk(total);
@@ -182,8 +184,7 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
             float zo[CPT], zw[CPT], ro[CPT], rw[CPT], qo[CPT], qw[CPT];
 #pragma unroll
             for (int e = 0; e < CPT; ++e) { zo[e] = out[e] * inv; zw[e] = w[e] * inv; }
-            stn<float, CPT>(a.z32 + own + base, zo);
-            stn<float, CPT>(a.z32 + oth + base, zw);
+            stn<TZO, CPT>(reinterpret_cast<TZO *>(a.zout) + own + base, out);
             ldn<float, CPT>(a.r32 + own + base, ro); ldn<float, CPT>(a.r32 + oth + base, rw);
             ldn<float, CPT>(a.q32 + own + base, qo); ldn<float, CPT>(a.q32 + oth + base, qw);
 #pragma unroll
@@ -310,14 +311,16 @@ __global__ void __launch_bounds__(256) k_srestrict(const TR *__restrict__ r, con
 // p update ran before).  Then Ap = ndt(p_new) (mgpcg.py:36-39) and the FP64
 // partial of p_new.Ap, in the split layout; each thread does the quads of
 // both colours at one position.
-template <bool FUSED>
+template <bool FUSED, bool ZSCALED>
 __global__ void __launch_bounds__(256) k_sp_Ap(const float *__restrict__ p, float *__restrict__ pout,
                                                float *__restrict__ Ap, const float *__restrict__ z,
                                                const uint16_t *__restrict__ code, const double *__restrict__ sc,
-                                               SGeo g, double *__restrict__ part)
+                                               const double *__restrict__ zinv, SGeo g, double *__restrict__ part)
 {
     __shared__ double lds[4];
     const float beta = (float)sc[SC_BETA];
+    const float zi = ZSCALED ? (float)zinv[0] : 1.0f;   // a power of two: z * zi is exact
+    auto zv = [&](float v) { return ZSCALED ? v * zi : v; };
     const uint32_t total = (uint32_t)g.nx * g.ny * g.nq;
     const int lane = threadIdx.x & 63;
     double acc = 0.0;
@@ -331,10 +334,10 @@ __global__ void __launch_bounds__(256) k_sp_Ap(const float *__restrict__ p, floa
             float a[4], zz[4];
             f4(p + b, a);
             f4(z + b, zz);
-            v[0] = fmaf(beta, a[0], zz[0]); v[1] = fmaf(beta, a[1], zz[1]);
-            v[2] = fmaf(beta, a[2], zz[2]); v[3] = fmaf(beta, a[3], zz[3]);
+            v[0] = fmaf(beta, a[0], zv(zz[0])); v[1] = fmaf(beta, a[1], zv(zz[1]));
+            v[2] = fmaf(beta, a[2], zv(zz[2])); v[3] = fmaf(beta, a[3], zv(zz[3]));
         };
-        auto p1 = [&](uint32_t b) -> float { return FUSED ? fmaf(beta, p[b], z[b]) : p[b]; };
+        auto p1 = [&](uint32_t b) -> float { return FUSED ? fmaf(beta, p[b], zv(z[b])) : p[b]; };
         const uint32_t base = g.row(i, j) + m0;
         float pn[2][4];
         pq(0, i, j, pn[0]);
@@ -524,7 +527,7 @@ int split_sweep_mixed_t(hipStream_t s, const Level &L0, const SplitMixedSweep &m
 {
     SSweepArgs a{};
     a.zin = m.zin; a.zout = m.zout; a.zc = m.zc; a.r = m.rh; a.code = m.code;
-    a.r32 = m.r32; a.q32 = m.q32; a.z32 = m.z32; a.sc = m.sc; a.part = m.part;
+    a.r32 = m.r32; a.q32 = m.q32; a.zinv = m.zinv; a.sc = m.sc; a.part = m.part;
     a.cu = m.phase ^ 1;
     // the pre-smoothing sweeps (16-bit z and r) move 8 cells per thread, in
     // 16-byte vectors, when the half-rows allow it
@@ -626,13 +629,29 @@ int split_restrict_mixed(hipStream_t s, const Level &F, int prec, const void *rh
 }
 
 int split_p_Ap(hipStream_t s, const Level &L0, const float *p, float *pout, float *Ap, const float *z,
-               const uint16_t *code, const double *sc, double *part, int *nblk)
+               const uint16_t *code, const double *sc, double *part, int *nblk, const double *zinv)
 {
     const SGeo g = make_sgeo(L0.nx, L0.ny, L0.nz);
     const unsigned nb = grid8((uint32_t)L0.nx * L0.ny * g.nq);
     *nblk = (int)nb;
-    if (pout) k_sp_Ap<true><<<nb, 256, 0, s>>>(p, pout, Ap, z, code, sc, g, part);
-    else k_sp_Ap<false><<<nb, 256, 0, s>>>(p, nullptr, Ap, nullptr, code, sc, g, part);
+    if (pout && zinv) k_sp_Ap<true, true><<<nb, 256, 0, s>>>(p, pout, Ap, z, code, sc, zinv, g, part);
+    else if (pout) k_sp_Ap<true, false><<<nb, 256, 0, s>>>(p, pout, Ap, z, code, sc, nullptr, g, part);
+    else k_sp_Ap<false, false><<<nb, 256, 0, s>>>(p, nullptr, Ap, nullptr, code, sc, nullptr, g, part);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+namespace {
+__global__ void k_scale_by(float *__restrict__ x, const double *__restrict__ f, uint32_t n)
+{
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t < n) x[t] = x[t] * (float)f[0];
+}
+}  // namespace
+
+int scale_by_device(hipStream_t s, float *x, const double *f, uint32_t n)
+{
+    k_scale_by<<<blocks_for(n, 256), 256, 0, s>>>(x, f, n);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
 }
