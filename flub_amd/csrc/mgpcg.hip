@@ -287,8 +287,9 @@ __global__ void k_sub_mean(float *__restrict__ q, const uint8_t *__restrict__ ta
     q[t] = tag[t] == 1 ? q[t] - mean : 0.0f;
 }
 
-// r -= alpha*Ap and x += alpha*p (the two are independent of the
+// r -= alpha*Ap and, with DOX, x += alpha*p (the two are independent of the
 // preconditioner between them in mgpcg.py:421-457), partial r.r.
+template <bool DOX>
 __global__ void __launch_bounds__(RBS) k_update_rx_dot(float *__restrict__ r, float *__restrict__ x,
                                                        const float *__restrict__ Ap, const float *__restrict__ p,
                                                        const double *__restrict__ sc, uint32_t n,
@@ -300,7 +301,7 @@ __global__ void __launch_bounds__(RBS) k_update_rx_dot(float *__restrict__ r, fl
     for (uint32_t t = blockIdx.x * RBS + threadIdx.x; t < n; t += gridDim.x * RBS) {
         const float rn = fmaf(-alpha, Ap[t], r[t]);
         r[t] = rn;
-        x[t] = fmaf(alpha, p[t], x[t]);
+        if (DOX) x[t] = fmaf(alpha, p[t], x[t]);
         acc += (double)rn * (double)rn;
     }
     const double s = block_sum<RBS>(acc, lds);
@@ -742,7 +743,6 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
         e = hipEventCreateWithFlags(&h->ev, hipEventDisableTiming);
         if (e != hipSuccess) { flub_set_error("hipEventCreate: %s", hipGetErrorString(e)); rc = (int)e; goto fail; }
         h->split = h->fast && split_supported(h->lv[0]) && getenv("FLUB_NO_SPLIT") == nullptr;
-        h->fused_pAp = getenv("FLUB_UNFUSED_PAP") == nullptr;
         if (h->split) {
             ALLOC(h->sp.r, n0 * 4);
             ALLOC(h->sp.z[0], n0 * 4);
@@ -966,11 +966,13 @@ static int read_scalar(flub_mgpcg *h, int idx, double *out)
     return FLUB_OK;
 }
 
-// mgpcg.py:378-462.  Same control flow; x's update is fused into r's (both
-// use this iteration's alpha and p, which the preconditioner does not touch).
 // mgpcg.py:378-462 for the "phases" smoother on the fused kernels: the z.r
-// partials come out of the V-cycle's last sweep, update_p is folded into the
-// Ap kernel (out of place, alternating p and p_alt), update_x into update_r.
+// partials come out of the V-cycle's last sweep.  Split layout: update_p then
+// Ap, and update_x deferred into the next update_p (x += alpha p with the
+// same alpha and p, before p changes; the last one after the loop), so that
+// update_r reads and writes only r.  Reference layout: update_p folded into
+// the Ap kernel (out of place, alternating p and p_alt), update_x into
+// update_r.
 static int solve_fast(flub_mgpcg *h, int max_iters, double eps, double abs_tol, double rel_tol, int verbose,
                       int *iters, double *rTr_out)
 {
@@ -1003,22 +1005,28 @@ static int solve_fast(flub_mgpcg *h, int max_iters, double eps, double abs_tol, 
     int it = 0;
     double rTr = rTr0, r_prev = rTr0, rate = 1.0;   // rate: last reduction factor of r.r
     int status = FLUB_OK;
+    bool x_pending = false;   // split: x += alpha p of the last iteration not applied yet
     while (it < cap) {
         int nbp = 0;
         // speculate unless this test is predicted to stop the solve (a wasted
         // speculative cycle would delay the end of the solve)
         const bool speculate = split && it + 1 < cap && r_prev * rate > 16.0 * tol;
-        if (split && !h->fused_pAp) {   // p = z + beta*p, then Ap = ndt(p)
-            k_update_p<<<nb256(n), 256, 0, h->s>>>(h->p, Z, h->sc, n);
-            FLUB_CHECK_LAUNCH();
-            TRY(split_p_Ap(h->s, L0, h->p, nullptr, h->Ap, nullptr, h->sp.code, h->sc, h->part, &nbp));
+        if (split) {   // p = z + beta*p (and the deferred x update), then Ap = ndt(p)
+            TRY(x_pending ? launch_update_px<true>(h->s, h->p, X, Z, h->sc, nullptr, n)
+                          : launch_update_px<false>(h->s, h->p, X, Z, h->sc, nullptr, n));
+            x_pending = false;
+            TRY(split_Ap(h->s, L0, h->p, h->Ap, h->sp.code, h->part, &nbp));
         } else {
-            if (split) TRY(split_p_Ap(h->s, L0, h->p, h->p_alt, h->Ap, Z, h->sp.code, h->sc, h->part, &nbp));
-            else TRY(fast_p_Ap(h->s, L0, h->p, h->p_alt, h->Ap, h->sc, h->part, &nbp));
+            TRY(fast_p_Ap(h->s, L0, h->p, h->p_alt, h->Ap, h->sc, h->part, &nbp));
             float *t = h->p; h->p = h->p_alt; h->p_alt = t;
         }
         TRY(launch_finish(h, nbp, FIN_ALPHA, eps));
-        k_update_rx_dot<<<nb, RBS, 0, h->s>>>(R, X, h->Ap, h->p, h->sc, n, h->part);
+        if (split) {   // r only: x += alpha p waits for the next p update (or the end)
+            k_update_rx_dot<false><<<nb, RBS, 0, h->s>>>(R, nullptr, h->Ap, nullptr, h->sc, n, h->part);
+            x_pending = true;
+        } else {
+            k_update_rx_dot<true><<<nb, RBS, 0, h->s>>>(R, X, h->Ap, h->p, h->sc, n, h->part);
+        }
         FLUB_CHECK_LAUNCH();
         TRY(launch_finish(h, nb, FIN_RTR, 0.0));
         if (speculate) {
@@ -1045,6 +1053,7 @@ static int solve_fast(flub_mgpcg *h, int max_iters, double eps, double abs_tol, 
         TRY(launch_finish(h, h->vc_dot_nblk, FIN_BETA, eps));
         it += 1;
     }
+    if (x_pending) TRY((launch_update_px<true, false>(h->s, h->p, X, nullptr, h->sc, nullptr, n)));
     if (split) {   // back to the reference layout (sp.zpre is free scratch now)
         TRY(split_convert(h->s, L0, X, h->x, 4, false));
         TRY(split_convert(h->s, L0, R, L0.r, 4, false));
@@ -1098,7 +1107,7 @@ static int solve_fp32(flub_mgpcg *h, int max_iters, double eps, double abs_tol, 
             TRY(mg_zero_centered(h));
             TRY(mg_dot(h, L0.r, L0.r, FIN_RTR, 0.0));
         } else {
-            k_update_rx_dot<<<nb, RBS, 0, h->s>>>(L0.r, h->x, h->Ap, h->p, h->sc, n, h->part);
+            k_update_rx_dot<true><<<nb, RBS, 0, h->s>>>(L0.r, h->x, h->Ap, h->p, h->sc, n, h->part);
             FLUB_CHECK_LAUNCH();
             TRY(launch_finish(h, nb, FIN_RTR, 0.0));
         }
@@ -1253,7 +1262,7 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
             }
             return compute_Ap_dot(h, FLUB_SMOOTHER_PHASES, FIN_SUM, 0.0);
         case 2:
-            k_update_rx_dot<<<nb, RBS, 0, h->s>>>(L0.r, h->x, h->Ap, h->p, h->sc, L0.n, h->part);
+            k_update_rx_dot<true><<<nb, RBS, 0, h->s>>>(L0.r, h->x, h->Ap, h->p, h->sc, L0.n, h->part);
             FLUB_CHECK_LAUNCH();
             return launch_finish(h, nb, FIN_RTR, 0.0);
         case 3: return h->split ? mg_precondition_split(h, 0) : mg_precondition(h, FLUB_SMOOTHER_PHASES, true);
@@ -1286,20 +1295,17 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
         case 9:
             FLUB_REQUIRE(h->split, FLUB_ESTATE, "no colour-split level 0 for this handle");
             return split_restrict(h->s, L0, h->sp.r, h->sp.zpre, h->sp.code, h->lv[1]);
-        case 10: {
+        case 10: {   // the split solve's p step: p (and the deferred x) update, then Ap and p.Ap
             FLUB_REQUIRE(h->split, FLUB_ESTATE, "no colour-split level 0 for this handle");
             int nbp = 0;
-            TRY(split_p_Ap(h->s, L0, h->p, h->p_alt, h->Ap, h->sp.z[0], h->sp.code, h->sc, h->part, &nbp));
+            TRY(launch_update_px<true>(h->s, h->p, h->sp.x, h->sp.z[0], h->sc, nullptr, L0.n));
+            TRY(split_Ap(h->s, L0, h->p, h->Ap, h->sp.code, h->part, &nbp));
             return launch_finish(h, nbp, FIN_SUM, 0.0);
         }
-        case 11: {   // the unfused pair: p update, then Ap of p (split)
-            FLUB_REQUIRE(h->split, FLUB_ESTATE, "no colour-split level 0 for this handle");
-            int nbp = 0;
-            k_update_p<<<nb256(L0.n), 256, 0, h->s>>>(h->p, h->sp.z[0], h->sc, L0.n);
+        case 11:   // the split solve's r step: r -= alpha Ap and r.r
+            k_update_rx_dot<false><<<nb, RBS, 0, h->s>>>(L0.r, nullptr, h->Ap, nullptr, h->sc, L0.n, h->part);
             FLUB_CHECK_LAUNCH();
-            TRY(split_p_Ap(h->s, L0, h->p, nullptr, h->Ap, nullptr, h->sp.code, h->sc, h->part, &nbp));
-            return launch_finish(h, nbp, FIN_SUM, 0.0);
-        }
+            return launch_finish(h, nb, FIN_RTR, 0.0);
         default: flub_set_error("unknown kernel id %d", kernel); return FLUB_EINVAL;
         }
     };
@@ -1314,8 +1320,8 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
     case 7: b = 4 * N * (2 * v + 1); break;                  // four half-sweeps
     case 8: b = N * (2 * v + 1); break;
     case 9: b = N * (2 * v + 1) + N / 8 * (v + 1); break;
-    case 10: b = N * (4 * 4 + 1); break;
-    case 11: b = N * (4 * 4 + 1); break;
+    case 10: b = N * (7 * 4 + 2); break;                     // p, z, x -> p, x; p, codes -> Ap
+    case 11: b = N * 3 * 4; break;                           // r, Ap -> r
     default: break;
     }
     int rc = launch_once();   // warm-up (captures the graph for id 3)

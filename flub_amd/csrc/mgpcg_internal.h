@@ -46,7 +46,7 @@ struct flub_mgpcg {
     int L, prec;
     Level lv[MAXL];
     float *x, *p, *Ap;
-    float *p_alt;   // second p buffer: the fused p-update + Ap writes out of place
+    float *p_alt;   // second p buffer: the reference-layout fused p-update + Ap writes out of place
     double *sc;     // device scalars
     bool fast;      // quad kernels usable (tagged smoother, levels >= 2, pre_post >= 1)
     int vc_dot_nblk;  // partial count of the z.r fused into the fast V-cycle (0: none)
@@ -65,10 +65,61 @@ struct flub_mgpcg {
     bool graph_ok[5];
     hipEvent_t ev;     // marks the residual norm's copy to the host
     bool split;        // level 0 of the FP32 solve runs in the colour-split layout
-    bool fused_pAp;    // split solve: p update fused into the Ap kernel (else an elementwise pass first)
     SplitBuf sp;
     MixedState *mx;
 };
+
+// Elementwise CG vector updates over float4 vectors (n4 = n / 4), any layout.
+// DOP: p = beta p + z (mgpcg.py:229-232), z times zinv[0] when zinv is set
+// (the z buffer holds z times a power-of-two scale: exact).  DOX: the x
+// update of the previous iteration, x += alpha p (mgpcg.py:207-210), with the
+// p it used, deferred from the r update to this pass, which reads p anyway.
+// It is the same FP32 fmaf on the same values as an update in place at its
+// own iteration, so x is bitwise the same; the solve applies the last one
+// (DOP false) before x is read or returned.
+template <bool DOX, bool DOP = true>
+__global__ void __launch_bounds__(256) k_update_px(float *__restrict__ p, float *__restrict__ x,
+                                                   const float *__restrict__ z, const double *__restrict__ sc,
+                                                   const double *__restrict__ zinv, uint32_t n4)
+{
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= n4) return;
+    float4 pv = reinterpret_cast<const float4 *>(p)[t];
+    if (DOX) {
+        const float alpha = (float)sc[SC_ALPHA];
+        float4 xv = reinterpret_cast<const float4 *>(x)[t];
+        xv.x = fmaf(alpha, pv.x, xv.x);
+        xv.y = fmaf(alpha, pv.y, xv.y);
+        xv.z = fmaf(alpha, pv.z, xv.z);
+        xv.w = fmaf(alpha, pv.w, xv.w);
+        reinterpret_cast<float4 *>(x)[t] = xv;
+    }
+    if (DOP) {
+        const float beta = (float)sc[SC_BETA];
+        float4 zv = reinterpret_cast<const float4 *>(z)[t];
+        if (zinv) {
+            const float zi = (float)zinv[0];
+            zv.x *= zi; zv.y *= zi; zv.z *= zi; zv.w *= zi;
+        }
+        pv.x = fmaf(beta, pv.x, zv.x);
+        pv.y = fmaf(beta, pv.y, zv.y);
+        pv.z = fmaf(beta, pv.z, zv.z);
+        pv.w = fmaf(beta, pv.w, zv.w);
+        reinterpret_cast<float4 *>(p)[t] = pv;
+    }
+}
+
+// k_update_px for n values (n % 4 == 0), on stream s.
+template <bool DOX, bool DOP = true>
+int launch_update_px(hipStream_t s, float *p, float *x, const float *z, const double *sc, const double *zinv,
+                     uint32_t n)
+{
+    FLUB_REQUIRE(n % 4 == 0, FLUB_EINVAL, "vector update: n = %u is not a multiple of 4", n);
+    const uint32_t n4 = n / 4;
+    k_update_px<DOX, DOP><<<blocks_for(n4, 256), 256, 0, s>>>(p, x, z, sc, zinv, n4);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
 
 int mg_dot(flub_mgpcg *h, const float *a, const float *b, int op, double eps);
 int mg_zero_centered(flub_mgpcg *h);

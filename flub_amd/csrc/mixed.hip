@@ -23,7 +23,8 @@
 #include "mgpcg_internal.h"
 #include "smooth.h"
 
-// r -= alpha q; x += alpha p; partials of r.r and max|r|.
+// r -= alpha q; with DOX x += alpha p; partials of r.r and max|r|.
+template <bool DOX>
 __global__ void __launch_bounds__(RBS) k_mx_update(float *__restrict__ r, float *__restrict__ x,
                                                    const float *__restrict__ q, const float *__restrict__ p,
                                                    const double *__restrict__ sc, uint32_t n, double *__restrict__ part)
@@ -35,7 +36,7 @@ __global__ void __launch_bounds__(RBS) k_mx_update(float *__restrict__ r, float 
     for (uint32_t t = blockIdx.x * RBS + threadIdx.x; t < n; t += gridDim.x * RBS) {
         const float rn = fmaf(-alpha, q[t], r[t]);
         r[t] = rn;
-        x[t] = fmaf(alpha, p[t], x[t]);
+        if (DOX) x[t] = fmaf(alpha, p[t], x[t]);
         acc += (double)rn * (double)rn;
         mx = fmaxf(mx, fabsf(rn));
     }
@@ -105,16 +106,6 @@ __global__ void __launch_bounds__(RBS) k_sum_partials(const double *__restrict__
     for (int b = threadIdx.x; b < nblk; b += RBS) acc += part[b];
     const double s = block_sum<RBS>(acc, lds);
     if (threadIdx.x == 0) out[0] = s;
-}
-
-// p = z + beta*p (mgpcg.py:229-232), elementwise (any layout).
-// p = z + beta p; zinv (may be NULL) scales z from its storage scale (a power
-// of two: exact).
-__global__ void k_update_p_mx(float *__restrict__ p, const float *__restrict__ z, const double *__restrict__ sc,
-                              const double *__restrict__ zinv, uint32_t n)
-{
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) p[t] = fmaf((float)sc[SC_BETA], p[t], zinv ? z[t] * (float)zinv[0] : z[t]);
 }
 
 // Partials of r.r and max|r| without changing r.
@@ -419,7 +410,6 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
     double rTr0;
     TRYM(read_sc(h, SC_RTR, &rTr0));
     const double tol = fmax(abs_tol, rTr0 * rel_tol);
-    float *const p_home = h->p;
     int zs = 0;
     TRYM(mg_precondition_split(h, zs));
     k_mx_finish_beta<<<1, RBS, 0, h->s>>>(h->part, h->vc_dot_nblk, h->sc, eps, 1);
@@ -427,22 +417,20 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
     const int cap = max_iters < 0 ? SOLVE_SAFETY_CAP : max_iters;
     int it = 0, status = FLUB_OK;
     double rTr = rTr0, r_prev = rTr0, rate = 1.0;
+    bool x_pending = false;   // x += alpha p of the last iteration not applied yet
     while (it < cap) {
         const bool speculate = it + 1 < cap && r_prev * rate > 16.0 * tol;
-        // p = z + beta p, then q = ndt(p) and p.q
+        // p = z + beta p (and the deferred x update), then q = ndt(p) and p.q
         int nbp = 0;
-        if (h->fused_pAp) {   // p_new = z + beta p at each stencil point, then ndt(p_new)
-            TRYM(split_p_Ap(h->s, L0, h->p, h->p_alt, h->Ap, S.z[zs], S.code, h->sc, h->part, &nbp,
-                            h->sc + SC_ZINV0 + zs));
-            float *t = h->p; h->p = h->p_alt; h->p_alt = t;
-        } else {
-            k_update_p_mx<<<blocks_for(n, 256), 256, 0, h->s>>>(h->p, S.z[zs], h->sc, h->sc + SC_ZINV0 + zs, n);
-            FLUB_CHECK_LAUNCH();
-            TRYM(split_p_Ap(h->s, L0, h->p, nullptr, h->Ap, nullptr, S.code, h->sc, h->part, &nbp));
-        }
+        TRYM(x_pending ? launch_update_px<true>(h->s, h->p, S.x, S.z[zs], h->sc, h->sc + SC_ZINV0 + zs, n)
+                       : launch_update_px<false>(h->s, h->p, S.x, S.z[zs], h->sc, h->sc + SC_ZINV0 + zs, n));
+        x_pending = false;
+        TRYM(split_Ap(h->s, L0, h->p, h->Ap, S.code, h->part, &nbp));
         TRYM(mg_finish_alpha(h, nbp, eps));
-        k_mx_update<<<nb, RBS, 0, h->s>>>(S.r, S.x, h->Ap, h->p, h->sc, n, h->part);
+        // r only: x += alpha p waits for the next p update (or a read of x)
+        k_mx_update<false><<<nb, RBS, 0, h->s>>>(S.r, nullptr, h->Ap, nullptr, h->sc, n, h->part);
         FLUB_CHECK_LAUNCH();
+        x_pending = true;
         k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nb, h->sc, 0);
         FLUB_CHECK_LAUNCH();
         if (speculate) {
@@ -460,6 +448,8 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
         if (rTr < tol) {
             // the reference's test passed on the recursive residual: check the
             // FP64 true residual for drift (it replaces r either way)
+            TRYM((launch_update_px<true, false>(h->s, h->p, S.x, nullptr, h->sc, nullptr, n)));
+            x_pending = false;
             int nbt = 0;
             TRYM(split_true_residual(h->s, L0, S.r, S.b, S.x, S.code, h->part, &nbt));
             k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nbt, h->sc, 0);
@@ -479,6 +469,7 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
         FLUB_CHECK_LAUNCH();
         it += 1;
     }
+    if (x_pending) TRYM((launch_update_px<true, false>(h->s, h->p, S.x, nullptr, h->sc, nullptr, n)));
     // back to the reference layout (S.zw is free scratch now)
     TRYM(split_convert(h->s, L0, S.x, h->x, 4, false));
     TRYM(split_convert(h->s, L0, S.r, L0.r, 4, false));
@@ -487,8 +478,7 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
     TRYM(split_convert(h->s, L0, h->Ap, S.zw, 4, false));
     FLUB_CHECK_HIP(hipMemcpyAsync(h->Ap, S.zw, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
     TRYM(split_convert(h->s, L0, h->p, S.zw, 4, false));
-    FLUB_CHECK_HIP(hipMemcpyAsync(p_home, S.zw, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
-    if (h->p != p_home) { h->p_alt = h->p; h->p = p_home; }
+    FLUB_CHECK_HIP(hipMemcpyAsync(h->p, S.zw, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
     if (status == FLUB_OK && it >= cap && max_iters < 0) {
         status = FLUB_ENOTCONV;
         flub_set_error("no convergence within %d iterations", cap);
@@ -528,7 +518,7 @@ int mg_solve_mixed(flub_mgpcg *h, int max_iters, double eps, double abs_tol, dou
         TRYM(fast_p_Ap(h->s, L0, h->p, h->p_alt, h->Ap, h->sc, h->part, &nbp));
         float *t = h->p; h->p = h->p_alt; h->p_alt = t;
         TRYM(mg_finish_alpha(h, nbp, eps));   // alpha = zr / (p.q + eps)
-        k_mx_update<<<nb, RBS, 0, h->s>>>(L0.r, h->x, h->Ap, h->p, h->sc, n, h->part);
+        k_mx_update<true><<<nb, RBS, 0, h->s>>>(L0.r, h->x, h->Ap, h->p, h->sc, n, h->part);
         FLUB_CHECK_LAUNCH();
         k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nb, h->sc, h->mx->shift);
         FLUB_CHECK_LAUNCH();

@@ -122,11 +122,8 @@ int split_restrict_mixed(hipStream_t s, const Level &F, int prec, const void *rh
 // Restriction of the split level 0 (r, z) into level 1's r.
 int split_restrict(hipStream_t s, const Level &F, const float *r, const float *z, const uint16_t *code,
                    const Level &C);
-// pout = z + beta*p; Ap = ndt(pout); partials of pout.Ap (all split).  With
-// pout == nullptr: Ap = ndt(p) and the partials of p.Ap (z unused).
-// With zinv, z holds z * scale and the p update uses z * zinv[0].
-int split_p_Ap(hipStream_t s, const Level &L0, const float *p, float *pout, float *Ap, const float *z,
-               const uint16_t *code, const double *sc, double *part, int *nblk, const double *zinv = nullptr);
+// Ap = ndt(p) and the partials of p.Ap (both split); *nblk: partial count.
+int split_Ap(hipStream_t s, const Level &L0, const float *p, float *Ap, const uint16_t *code, double *part, int *nblk);
 // x *= f[0] (a device scalar).
 int scale_by_device(hipStream_t s, float *x, const double *f, uint32_t n);
 // r = b - ndt(x) in FP64 (stored FP32), partials of r.r and max|r| (split).

@@ -306,21 +306,15 @@ __global__ void __launch_bounds__(256) k_srestrict(const TR *__restrict__ r, con
     }
 }
 
-// FUSED: p_new = z + beta*p_old at every stencil point (mgpcg.py:229-232),
-// stored to pout (the neighbours still read p_old); otherwise p_new = p (the
-// p update ran before).  Then Ap = ndt(p_new) (mgpcg.py:36-39) and the FP64
-// partial of p_new.Ap, in the split layout; each thread does the quads of
-// both colours at one position.
-template <bool FUSED, bool ZSCALED>
-__global__ void __launch_bounds__(256) k_sp_Ap(const float *__restrict__ p, float *__restrict__ pout,
-                                               float *__restrict__ Ap, const float *__restrict__ z,
-                                               const uint16_t *__restrict__ code, const double *__restrict__ sc,
-                                               const double *__restrict__ zinv, SGeo g, double *__restrict__ part)
+// Ap = ndt(p) (mgpcg.py:36-39) and the FP64 partial of p.Ap, in the split
+// layout; each thread does the quads of both colours at one position.  (The
+// p update runs before, elementwise: fused into this kernel it costs the
+// neighbour reads of z as well, and the kernel is then bound by the L1 fills
+// of the neighbour rows.)
+__global__ void __launch_bounds__(256) k_sp_Ap(const float *__restrict__ p, float *__restrict__ Ap,
+                                               const uint16_t *__restrict__ code, SGeo g, double *__restrict__ part)
 {
     __shared__ double lds[4];
-    const float beta = (float)sc[SC_BETA];
-    const float zi = ZSCALED ? (float)zinv[0] : 1.0f;   // a power of two: z * zi is exact
-    auto zv = [&](float v) { return ZSCALED ? v * zi : v; };
     const uint32_t total = (uint32_t)g.nx * g.ny * g.nq;
     const int lane = threadIdx.x & 63;
     double acc = 0.0;
@@ -328,16 +322,8 @@ __global__ void __launch_bounds__(256) k_sp_Ap(const float *__restrict__ p, floa
     for (uint32_t t = ch.first; t < ch.end; t += ch.stride) {
         int i, j, m0;
         g.decode(t, i, j, m0);
-        auto pq = [&](uint32_t h, int ii, int jj, float v[4]) {
-            const uint32_t b = h + g.row(ii, jj) + m0;
-            if (!FUSED) { f4(p + b, v); return; }
-            float a[4], zz[4];
-            f4(p + b, a);
-            f4(z + b, zz);
-            v[0] = fmaf(beta, a[0], zv(zz[0])); v[1] = fmaf(beta, a[1], zv(zz[1]));
-            v[2] = fmaf(beta, a[2], zv(zz[2])); v[3] = fmaf(beta, a[3], zv(zz[3]));
-        };
-        auto p1 = [&](uint32_t b) -> float { return FUSED ? fmaf(beta, p[b], zv(z[b])) : p[b]; };
+        auto pq = [&](uint32_t h, int ii, int jj, float v[4]) { f4(p + h + g.row(ii, jj) + m0, v); };
+        auto p1 = [&](uint32_t b) -> float { return p[b]; };
         const uint32_t base = g.row(i, j) + m0;
         float pn[2][4];
         pq(0, i, j, pn[0]);
@@ -372,7 +358,6 @@ __global__ void __launch_bounds__(256) k_sp_Ap(const float *__restrict__ p, floa
                 out[e] = code_fluid(cd[e]) ? s : 0.0f;
                 acc += (double)pn[cc][e] * (double)out[e];
             }
-            if (FUSED) st_f4(pout + own + base, pn[cc]);
             st_f4(Ap + own + base, out);
         }
     }
@@ -628,15 +613,12 @@ int split_restrict_mixed(hipStream_t s, const Level &F, int prec, const void *rh
     return FLUB_OK;
 }
 
-int split_p_Ap(hipStream_t s, const Level &L0, const float *p, float *pout, float *Ap, const float *z,
-               const uint16_t *code, const double *sc, double *part, int *nblk, const double *zinv)
+int split_Ap(hipStream_t s, const Level &L0, const float *p, float *Ap, const uint16_t *code, double *part, int *nblk)
 {
     const SGeo g = make_sgeo(L0.nx, L0.ny, L0.nz);
     const unsigned nb = grid8((uint32_t)L0.nx * L0.ny * g.nq);
     *nblk = (int)nb;
-    if (pout && zinv) k_sp_Ap<true, true><<<nb, 256, 0, s>>>(p, pout, Ap, z, code, sc, zinv, g, part);
-    else if (pout) k_sp_Ap<true, false><<<nb, 256, 0, s>>>(p, pout, Ap, z, code, sc, nullptr, g, part);
-    else k_sp_Ap<false, false><<<nb, 256, 0, s>>>(p, nullptr, Ap, nullptr, code, sc, nullptr, g, part);
+    k_sp_Ap<<<nb, 256, 0, s>>>(p, Ap, code, g, part);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
 }
