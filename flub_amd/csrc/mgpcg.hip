@@ -494,7 +494,9 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
     // (blocked.hip); the pre-smoothed z then lands in the buffer that the
     // first post-smoothing launch does not write.  With `split`, level 0 is
     // the colour-split copy (split.hip): r in sp.r, pre-smoothing in place on
-    // sp.zpre, post-smoothing into sp.z[zslot].
+    // sp.zpre, post-smoothing into sp.z[zslot]; levels 1 .. nsplit-1 are
+    // colour-split in their own r, zpre and z (the restriction into them and
+    // the prolongation from them use that layout).
     const void *pre_z[MAXL];
     uint8_t ph[MG_MAX_PHASES];
     const SplitBuf &S = h->sp;
@@ -511,17 +513,19 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
                     m.kind = it == 0 ? (ph == 0 ? SMX_PRE_ZERO_SRC : SMX_PRE_SRC) : SMX_PRE;
                     TRY(split_sweep_mixed(h->s, F, h->prec, m));
                 }
-            TRY(split_restrict_mixed(h->s, F, h->prec, S.rh, S.zh, S.code, h->lv[1]));
+            TRY(split_restrict_mixed(h->s, F, h->prec, S.rh, S.zh, S.code, h->lv[1], h->nsplit > 1));
             continue;
         }
-        if (l == 0 && split) {
-            TRY(split_sweep(h->s, F, nullptr, S.zpre, nullptr, S.r, S.code, 0, SWEEP_ZERO, nullptr, nullptr));
-            TRY(split_sweep(h->s, F, S.zpre, S.zpre, nullptr, S.r, S.code, 1, SWEEP_PLAIN, nullptr, nullptr));
+        if (split && l < h->nsplit) {   // level 0 in sp.*, the next split levels in their own buffers
+            float *R = l == 0 ? S.r : F.r, *ZP = l == 0 ? S.zpre : F.zpre;
+            const uint16_t *CD = S.lcode[l];
+            TRY(split_sweep(h->s, F, nullptr, ZP, nullptr, R, CD, 0, SWEEP_ZERO, nullptr, nullptr));
+            TRY(split_sweep(h->s, F, ZP, ZP, nullptr, R, CD, 1, SWEEP_PLAIN, nullptr, nullptr));
             for (int it = 1; it < s; ++it) {
-                TRY(split_sweep(h->s, F, S.zpre, S.zpre, nullptr, S.r, S.code, 0, SWEEP_PLAIN, nullptr, nullptr));
-                TRY(split_sweep(h->s, F, S.zpre, S.zpre, nullptr, S.r, S.code, 1, SWEEP_PLAIN, nullptr, nullptr));
+                TRY(split_sweep(h->s, F, ZP, ZP, nullptr, R, CD, 0, SWEEP_PLAIN, nullptr, nullptr));
+                TRY(split_sweep(h->s, F, ZP, ZP, nullptr, R, CD, 1, SWEEP_PLAIN, nullptr, nullptr));
             }
-            TRY(split_restrict(h->s, F, S.r, S.zpre, S.code, h->lv[1]));
+            TRY(split_restrict(h->s, F, R, ZP, CD, h->lv[l + 1], l + 1 < h->nsplit));
             continue;
         }
         if (mg_use_tb(h, l, 2 * s, true)) {
@@ -567,19 +571,22 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
                     m.zin = it == 0 && ph == 1 ? S.zh : (const void *)Z;
                     m.zout = Z;
                     m.zc = h->lv[1].z;
+                    m.zc_split = h->nsplit > 1;
                     m.kind = it == 0 && ph == 1 ? SMX_POST_PROLONG : (last ? SMX_FINAL : SMX_POST);
                     if (last) { m.zinv = h->sc + SC_ZINV0 + zslot; m.part = h->part; m.nblk = &h->vc_dot_nblk; }
                     TRY(split_sweep_mixed(h->s, F, h->prec, m));
                 }
             continue;
         }
-        if (l == 0 && split) {
-            float *Z = S.z[zslot];
+        if (split && l < h->nsplit) {
+            float *Z = l == 0 ? S.z[zslot] : F.z, *R = l == 0 ? S.r : F.r, *ZP = l == 0 ? S.zpre : F.zpre;
+            const uint16_t *CD = S.lcode[l];
+            const bool zcs = l + 1 < h->nsplit;
             for (int it = 0; it < s; ++it) {
-                const bool last = it + 1 == s;
-                if (it == 0) TRY(split_sweep(h->s, F, S.zpre, Z, zc, S.r, S.code, 1, SWEEP_PROLONG, nullptr, nullptr));
-                else TRY(split_sweep(h->s, F, Z, Z, nullptr, S.r, S.code, 1, SWEEP_PLAIN, nullptr, nullptr));
-                TRY(split_sweep(h->s, F, Z, Z, nullptr, S.r, S.code, 0, SWEEP_PLAIN, last ? h->part : nullptr,
+                const bool last = l == 0 && it + 1 == s;   // level 0's last sweep also yields z.r
+                if (it == 0) TRY(split_sweep(h->s, F, ZP, Z, zc, R, CD, 1, SWEEP_PROLONG, nullptr, nullptr, zcs));
+                else TRY(split_sweep(h->s, F, Z, Z, nullptr, R, CD, 1, SWEEP_PLAIN, nullptr, nullptr));
+                TRY(split_sweep(h->s, F, Z, Z, nullptr, R, CD, 0, SWEEP_PLAIN, last ? h->part : nullptr,
                                 last ? &h->vc_dot_nblk : nullptr));
             }
             continue;
@@ -750,6 +757,19 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
             ALLOC(h->sp.zpre, n0 * 4);
             ALLOC(h->sp.x, n0 * 4);
             ALLOC(h->sp.code, n0 * 2);
+            h->sp.lcode[0] = h->sp.code;
+            // The next levels are colour-split too while they hold at least
+            // min_cells (default 2^20): there one launch per half-sweep on the
+            // split layout streams faster than the blocked natural-layout
+            // smoother (128^3: 4.5 us per half-sweep against 8.4).  The
+            // bottom level stays in the reference layout.
+            const char *mc = getenv("FLUB_SPLIT_MIN_CELLS");
+            const uint32_t min_cells = mc ? (uint32_t)strtoul(mc, nullptr, 10) : (1u << 20);
+            h->nsplit = 1;
+            while (h->nsplit + 1 < levels && split_supported(h->lv[h->nsplit]) && h->lv[h->nsplit].n >= min_cells) {
+                ALLOC(h->sp.lcode[h->nsplit], (size_t)h->lv[h->nsplit].n * 2);
+                ++h->nsplit;
+            }
             if (precision != FLUB_PREC_FP32) {
                 ALLOC(h->sp.rh, n0 * 2);
                 ALLOC(h->sp.zh, n0 * 2);
@@ -780,6 +800,7 @@ FLUB_API int flub_mgpcg_destroy(flub_mgpcg *h)
     hipFree(h->tb_order);
     hipFree(h->sp.r); hipFree(h->sp.z[0]); hipFree(h->sp.z[1]); hipFree(h->sp.zpre); hipFree(h->sp.x);
     hipFree(h->sp.code); hipFree(h->sp.rh); hipFree(h->sp.zh); hipFree(h->sp.zw); hipFree(h->sp.b);
+    for (int l = 1; l < MAXL; ++l) hipFree(h->sp.lcode[l]);
     if (h->ev) hipEventDestroy(h->ev);
     if (h->hsc) hipHostFree(h->hsc);
     if (h->cap) hipStreamDestroy(h->cap);
@@ -817,7 +838,7 @@ FLUB_API int flub_mgpcg_set_tags(flub_mgpcg *h, const int32_t *tag)
         FLUB_CHECK_LAUNCH();
     }
     for (int l = 0; l < h->L; ++l) TRY(make_codes(h->s, h->lv[l]));
-    if (h->split) TRY(split_convert(h->s, L0, L0.code, h->sp.code, 2, true));
+    for (int l = 0; l < h->nsplit; ++l) TRY(split_convert(h->s, h->lv[l], h->lv[l].code, h->sp.lcode[l], 2, true));
     h->tags_set = true;
     return FLUB_OK;
 }

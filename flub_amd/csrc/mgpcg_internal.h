@@ -40,6 +40,9 @@ struct SplitBuf {
     void *rh, *zh;
     float *zw;
     float *b;       // mixed precision: the right-hand side (split), for the true residual
+    // split stencil codes of the split levels: lcode[0] = code; levels
+    // 1 .. nsplit-1 keep their r, zpre and z in the level's own buffers
+    uint16_t *lcode[MAXL];
 };
 
 struct flub_mgpcg {
@@ -65,6 +68,7 @@ struct flub_mgpcg {
     bool graph_ok[5];
     hipEvent_t ev;     // marks the residual norm's copy to the host
     bool split;        // level 0 of the FP32 solve runs in the colour-split layout
+    int nsplit;        // levels 0 .. nsplit-1 of the split V-cycle are colour-split (0: none)
     SplitBuf sp;
     MixedState *mx;
 };

@@ -88,14 +88,16 @@ static inline int mg_bottom_phases(uint8_t *ph, int nb)
     return 4 * nb;
 }
 
-// Colour-split level 0 (split.hip).
+// Colour-split levels (split.hip): level 0 inside the fused solves, and the
+// next levels of the split V-cycle while they are large enough to stream.
 bool split_supported(const Level &L0);
-// Reference layout <-> split layout of a level-0 array of 4- or 2-byte values.
+// Reference layout <-> split layout of a level's array of 4- or 2-byte values.
 int split_convert(hipStream_t s, const Level &L0, const void *src, void *dst, int elem_bytes, bool to_split);
-// One half-sweep (mode SWEEP_ZERO / PLAIN / PROLONG from zin + zc[parent]);
-// with part, also the partials of z.r (count in *nblk).
+// One half-sweep of a split level (mode SWEEP_ZERO / PLAIN / PROLONG from
+// zin + zc[parent], zc colour-split too when zc_split); with part, also the
+// partials of z.r (count in *nblk).
 int split_sweep(hipStream_t s, const Level &L0, const float *zin, float *zout, const float *zc, const float *r,
-                const uint16_t *code, int phase, int mode, double *part, int *nblk);
+                const uint16_t *code, int phase, int mode, double *part, int *nblk, bool zc_split = false);
 // Mixed-precision level-0 sweeps on the split layout (T = FP16 or BF16 by
 // prec): pre-smoothing stores z and r as T (the first two sweeps take r from
 // the FP32 CG residual times the storage scale), post-smoothing keeps z in
@@ -107,6 +109,7 @@ struct SplitMixedSweep {
     const void *zin;       // T (pre, post-prolong) or FP32 (post)
     void *zout;            // T (pre) or FP32 (post)
     const float *zc;       // level-1 z (post-prolong)
+    int zc_split;          // level 1 is colour-split too
     void *rh;              // T split r
     const uint16_t *code;
     const float *r32, *q32;
@@ -116,12 +119,14 @@ struct SplitMixedSweep {
     int *nblk;
 };
 int split_sweep_mixed(hipStream_t s, const Level &L0, int prec, const SplitMixedSweep &m);
-// Restriction of the split level 0 (r, z stored as T) into level 1's FP32 r.
+// Restriction of the split level 0 (r, z stored as T) into level 1's FP32 r,
+// colour-split when csplit.
 int split_restrict_mixed(hipStream_t s, const Level &F, int prec, const void *rh, const void *zh,
-                         const uint16_t *code, const Level &C);
-// Restriction of the split level 0 (r, z) into level 1's r.
+                         const uint16_t *code, const Level &C, bool csplit = false);
+// Restriction of a split level (r, z) into the next level's r, colour-split
+// when csplit.
 int split_restrict(hipStream_t s, const Level &F, const float *r, const float *z, const uint16_t *code,
-                   const Level &C);
+                   const Level &C, bool csplit = false);
 // Ap = ndt(p) and the partials of p.Ap (both split); *nblk: partial count.
 int split_Ap(hipStream_t s, const Level &L0, const float *p, float *Ap, const uint16_t *code, double *part, int *nblk);
 // x *= f[0] (a device scalar).

@@ -87,7 +87,36 @@ struct SSweepArgs {
     double *part;          // DOT: partials of z.r; FINAL: z.r then z.q
     SGeo g;
     int cu;                // colour updated (the half-sweep phase ^ 1)
+    int zc_split;          // PROLONG: the coarse z is colour-split too
 };
+
+// z of the coarse level at the parents of the CPT fine cells m0.. of row
+// (ii, jj), the coarse cells (ii/2, jj/2, m0 + e).  The coarse level is in the
+// reference layout or (csplit) colour-split like the fine one: cell (I, J, K)
+// in half (I + J + K) & 1 at (I * nyc + J) * nzc/2 + K/2, so with m0 even the
+// even e come from half (I + J) & 1 and the odd e from the other.
+template <int CPT>
+__device__ __forceinline__ void coarse_zq(const float *zc, bool csplit, const SGeo &g, int ii, int jj, int m0,
+                                          float p[CPT])
+{
+    const int I = ii >> 1, J = jj >> 1, nyc = g.ny >> 1;
+    if (!csplit) { ldn<float, CPT>(zc + cidx(I, J, m0, nyc, g.nh), p); return; }
+    const uint32_t nhc = (uint32_t)g.nh >> 1, halfc = (uint32_t)(g.nx >> 1) * (uint32_t)nyc * nhc;
+    const uint32_t b = ((uint32_t)I * nyc + J) * nhc + (uint32_t)(m0 >> 1);
+    const uint32_t c0 = (uint32_t)((I + J) & 1);
+    float ev[CPT / 2], od[CPT / 2];
+    ldn<float, CPT / 2>(zc + c0 * halfc + b, ev);
+    ldn<float, CPT / 2>(zc + (c0 ^ 1u) * halfc + b, od);
+#pragma unroll
+    for (int e = 0; e < CPT / 2; ++e) { p[2 * e] = ev[e]; p[2 * e + 1] = od[e]; }
+}
+__device__ __forceinline__ float coarse_z1(const float *zc, bool csplit, const SGeo &g, int ii, int jj, int m)
+{
+    const int I = ii >> 1, J = jj >> 1, nyc = g.ny >> 1;
+    if (!csplit) return zc[cidx(I, J, m, nyc, g.nh)];
+    const uint32_t nhc = (uint32_t)g.nh >> 1, halfc = (uint32_t)(g.nx >> 1) * (uint32_t)nyc * nhc;
+    return zc[(uint32_t)((I + J + m) & 1) * halfc + ((uint32_t)I * nyc + J) * nhc + (uint32_t)(m >> 1)];
+}
 
 // One red-black half-sweep of level 0 in the split layout, with z read as
 // TZI, written as TZO and r stored as TR (float, __half or __hip_bfloat16;
@@ -125,7 +154,7 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
             ldn<TZI, CPT>(zin + h + g.row(ii, jj) + m0, v);
             if (MODE == SWEEP_PROLONG) {
                 float p[CPT];
-                ldn<float, CPT>(a.zc + cidx(ii >> 1, jj >> 1, m0, g.ny >> 1, g.nh), p);
+                coarse_zq<CPT>(a.zc, a.zc_split != 0, g, ii, jj, m0, p);
 #pragma unroll
                 for (int e = 0; e < CPT; ++e) v[e] = v[e] + p[e];
             }
@@ -133,7 +162,7 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
         auto z1 = [&](uint32_t h, int ii, int jj, int m) -> float {
             if (MODE == SWEEP_ZERO) return 0.0f;
             float v = ldv<TZI>(zin[h + g.row(ii, jj) + m]);
-            if (MODE == SWEEP_PROLONG) v = v + a.zc[cidx(ii >> 1, jj >> 1, m, g.ny >> 1, g.nh)];
+            if (MODE == SWEEP_PROLONG) v = v + coarse_z1(a.zc, a.zc_split != 0, g, ii, jj, m);
             return v;
         };
         // neighbour rows clamped to the domain (a wall neighbour has stencil
@@ -234,7 +263,7 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
 template <typename TR, typename TZ, int CPT>
 __global__ void __launch_bounds__(256) k_srestrict(const TR *__restrict__ r, const TZ *__restrict__ z,
                                                    const uint16_t *__restrict__ code, float *__restrict__ rc,
-                                                   const uint8_t *__restrict__ tagc, SGeo g, SGeo cg)
+                                                   const uint8_t *__restrict__ tagc, SGeo g, SGeo cg, int csplit)
 {
     // cg: the coarse grid in thread vectors of CPT cells (nx, ny coarse; nh = coarse nz = fine nz/2)
     const uint32_t total = (uint32_t)cg.nx * cg.ny * cg.nq;
@@ -302,7 +331,18 @@ __global__ void __launch_bounds__(256) k_srestrict(const TR *__restrict__ r, con
                     for (int kk = 0; kk < 2; ++kk) s += res[aa][b][kk][e];
             out[e] = ((tg[e >> 2] >> (8 * (e & 3))) & 0xffu) == 1u ? s : 0.0f;
         }
-        stn<float, CPT>(rc + cb, out);
+        if (csplit) {   // colour-split coarse r: even e to half (I + J) & 1, odd e to the other (K0 even)
+            const uint32_t nhc = (uint32_t)cg.nh >> 1, halfc = (uint32_t)cg.nx * (uint32_t)cg.ny * nhc;
+            const uint32_t b = ((uint32_t)I * cg.ny + J) * nhc + (uint32_t)(K0 >> 1);
+            const uint32_t c0 = (uint32_t)((I + J) & 1);
+            float ev[CPT / 2], od[CPT / 2];
+#pragma unroll
+            for (int e = 0; e < CPT / 2; ++e) { ev[e] = out[2 * e]; od[e] = out[2 * e + 1]; }
+            stn<float, CPT / 2>(rc + c0 * halfc + b, ev);
+            stn<float, CPT / 2>(rc + (c0 ^ 1u) * halfc + b, od);
+        } else {
+            stn<float, CPT>(rc + cb, out);
+        }
     }
 }
 
@@ -514,6 +554,7 @@ int split_sweep_mixed_t(hipStream_t s, const Level &L0, const SplitMixedSweep &m
     a.zin = m.zin; a.zout = m.zout; a.zc = m.zc; a.r = m.rh; a.code = m.code;
     a.r32 = m.r32; a.q32 = m.q32; a.zinv = m.zinv; a.sc = m.sc; a.part = m.part;
     a.cu = m.phase ^ 1;
+    a.zc_split = m.zc_split;
     // the pre-smoothing sweeps (16-bit z and r) move 8 cells per thread, in
     // 16-byte vectors, when the half-rows allow it
     const bool pre = m.kind == SMX_PRE_ZERO_SRC || m.kind == SMX_PRE_SRC || m.kind == SMX_PRE;
@@ -562,10 +603,11 @@ int split_convert(hipStream_t s, const Level &L0, const void *src, void *dst, in
 }
 
 int split_sweep(hipStream_t s, const Level &L0, const float *zin, float *zout, const float *zc, const float *r,
-                const uint16_t *code, int phase, int mode, double *part, int *nblk)
+                const uint16_t *code, int phase, int mode, double *part, int *nblk, bool zc_split)
 {
     SSweepArgs a{};
     a.zin = zin; a.zout = zout; a.zc = zc; a.r = const_cast<float *>(r); a.code = code; a.part = part;
+    a.zc_split = zc_split;
     a.g = make_sgeo(L0.nx, L0.ny, L0.nz);
     a.cu = phase ^ 1;
     const unsigned nb = grid8((uint32_t)L0.nx * L0.ny * a.g.nq);
@@ -585,18 +627,18 @@ int split_sweep_mixed(hipStream_t s, const Level &L0, int prec, const SplitMixed
 }
 
 int split_restrict(hipStream_t s, const Level &F, const float *r, const float *z, const uint16_t *code,
-                   const Level &C)
+                   const Level &C, bool csplit)
 {
     const SGeo g = make_sgeo(F.nx, F.ny, F.nz);
     SGeo cg = make_sgeo(C.nx, C.ny, 2 * C.nz);   // nh = C.nz
     const unsigned nb = grid8((uint32_t)C.nx * C.ny * cg.nq);
-    k_srestrict<float, float, 4><<<nb, 256, 0, s>>>(r, z, code, C.r, C.tag, g, cg);
+    k_srestrict<float, float, 4><<<nb, 256, 0, s>>>(r, z, code, C.r, C.tag, g, cg, (int)csplit);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
 }
 
 int split_restrict_mixed(hipStream_t s, const Level &F, int prec, const void *rh, const void *zh,
-                         const uint16_t *code, const Level &C)
+                         const uint16_t *code, const Level &C, bool csplit)
 {
     // 4 coarse cells per thread: at 8 (16-byte loads) the kernel needs 256
     // VGPRs and runs 53 us instead of 40 us at 256^3
@@ -605,10 +647,11 @@ int split_restrict_mixed(hipStream_t s, const Level &F, int prec, const void *rh
     const unsigned nb = grid8((uint32_t)C.nx * C.ny * cg.nq);
     using BF = __hip_bfloat16;
     if (prec == FLUB_PREC_BF16)
-        k_srestrict<BF, BF, 4><<<nb, 256, 0, s>>>((const BF *)rh, (const BF *)zh, code, C.r, C.tag, g, cg);
+        k_srestrict<BF, BF, 4><<<nb, 256, 0, s>>>((const BF *)rh, (const BF *)zh, code, C.r, C.tag, g, cg,
+                                                  (int)csplit);
     else
         k_srestrict<__half, __half, 4><<<nb, 256, 0, s>>>((const __half *)rh, (const __half *)zh, code, C.r, C.tag,
-                                                          g, cg);
+                                                          g, cg, (int)csplit);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
 }
