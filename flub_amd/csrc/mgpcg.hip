@@ -495,8 +495,10 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
     // first post-smoothing launch does not write.  With `split`, level 0 is
     // the colour-split copy (split.hip): r in sp.r, pre-smoothing in place on
     // sp.zpre, post-smoothing into sp.z[zslot]; levels 1 .. nsplit-1 are
-    // colour-split in their own r, zpre and z (the restriction into them and
-    // the prolongation from them use that layout).
+    // colour-split in their own r, zpre and z (the restriction writes their r
+    // split); the last post-sweep of such a level writes its whole z in the
+    // reference layout into its zpre, which the finer level's prolongating
+    // sweep reads.
     const void *pre_z[MAXL];
     uint8_t ph[MG_MAX_PHASES];
     const SplitBuf &S = h->sp;
@@ -520,7 +522,7 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
             float *R = l == 0 ? S.r : F.r, *ZP = l == 0 ? S.zpre : F.zpre;
             const uint16_t *CD = S.lcode[l];
             TRY(split_sweep(h->s, F, nullptr, ZP, nullptr, R, CD, 0, SWEEP_ZERO, nullptr, nullptr));
-            TRY(split_sweep(h->s, F, ZP, ZP, nullptr, R, CD, 1, SWEEP_PLAIN, nullptr, nullptr));
+            TRY(split_sweep(h->s, F, ZP, ZP, nullptr, R, CD, 1, SWEEP_PLAIN_OWN0, nullptr, nullptr));
             for (int it = 1; it < s; ++it) {
                 TRY(split_sweep(h->s, F, ZP, ZP, nullptr, R, CD, 0, SWEEP_PLAIN, nullptr, nullptr));
                 TRY(split_sweep(h->s, F, ZP, ZP, nullptr, R, CD, 1, SWEEP_PLAIN, nullptr, nullptr));
@@ -556,7 +558,8 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
     for (int l = h->L - 2; l >= 0; --l) {
         Level &F = h->lv[l];
         const int s = h->pre_post << l;
-        const float *zc = h->lv[l + 1].z;
+        // the coarse z in the reference layout: a split level leaves it in its zpre
+        const float *zc = split && l + 1 < h->nsplit ? h->lv[l + 1].zpre : h->lv[l + 1].z;
         if (l == 0 && split && h->prec != FLUB_PREC_FP32) {
             // FP32 post-smoothing from the FP16/BF16 pre-smoothed z into the
             // slot's z buffer, which keeps z times the storage scale; the last
@@ -570,8 +573,7 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
                     m.phase = ph;
                     m.zin = it == 0 && ph == 1 ? S.zh : (const void *)Z;
                     m.zout = Z;
-                    m.zc = h->lv[1].z;
-                    m.zc_split = h->nsplit > 1;
+                    m.zc = h->nsplit > 1 ? h->lv[1].zpre : h->lv[1].z;   // reference layout
                     m.kind = it == 0 && ph == 1 ? SMX_POST_PROLONG : (last ? SMX_FINAL : SMX_POST);
                     if (last) { m.zinv = h->sc + SC_ZINV0 + zslot; m.part = h->part; m.nblk = &h->vc_dot_nblk; }
                     TRY(split_sweep_mixed(h->s, F, h->prec, m));
@@ -581,13 +583,15 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
         if (split && l < h->nsplit) {
             float *Z = l == 0 ? S.z[zslot] : F.z, *R = l == 0 ? S.r : F.r, *ZP = l == 0 ? S.zpre : F.zpre;
             const uint16_t *CD = S.lcode[l];
-            const bool zcs = l + 1 < h->nsplit;
             for (int it = 0; it < s; ++it) {
-                const bool last = l == 0 && it + 1 == s;   // level 0's last sweep also yields z.r
-                if (it == 0) TRY(split_sweep(h->s, F, ZP, Z, zc, R, CD, 1, SWEEP_PROLONG, nullptr, nullptr, zcs));
+                const bool last = it + 1 == s;
+                if (it == 0) TRY(split_sweep(h->s, F, ZP, Z, zc, R, CD, 1, SWEEP_PROLONG, nullptr, nullptr));
                 else TRY(split_sweep(h->s, F, Z, Z, nullptr, R, CD, 1, SWEEP_PLAIN, nullptr, nullptr));
-                TRY(split_sweep(h->s, F, Z, Z, nullptr, R, CD, 0, SWEEP_PLAIN, last ? h->part : nullptr,
-                                last ? &h->vc_dot_nblk : nullptr));
+                if (last && l > 0)   // the whole z, reference layout, into zpre (read by the finer level)
+                    TRY(split_sweep(h->s, F, Z, ZP, nullptr, R, CD, 0, SWEEP_PLAIN, nullptr, nullptr, true));
+                else   // level 0's last sweep also yields z.r
+                    TRY(split_sweep(h->s, F, Z, Z, nullptr, R, CD, 0, SWEEP_PLAIN, last ? h->part : nullptr,
+                                    last ? &h->vc_dot_nblk : nullptr));
             }
             continue;
         }

@@ -3,8 +3,13 @@
 #pragma once
 #include "mgpcg_internal.h"
 
-enum { SWEEP_PLAIN = 0, SWEEP_ZERO = 1, SWEEP_PROLONG = 2 };
-enum { SWEEP_OUT_STORE = 0, SWEEP_OUT_DOT = 1, SWEEP_OUT_FINAL = 2 };
+// SWEEP_PLAIN_OWN0 (split sweeps only): PLAIN with the updated colour's old
+// z taken as +0 (the sweep after a SWEEP_ZERO one, which then need not store
+// the other colour's zeros).
+enum { SWEEP_PLAIN = 0, SWEEP_ZERO = 1, SWEEP_PROLONG = 2, SWEEP_PLAIN_OWN0 = 3 };
+// SWEEP_OUT_NATURAL (split sweeps only): the whole z after the half-sweep, both
+// colours, in the reference layout into zout (a buffer other than zin).
+enum { SWEEP_OUT_STORE = 0, SWEEP_OUT_DOT = 1, SWEEP_OUT_FINAL = 2, SWEEP_OUT_NATURAL = 3 };
 
 // One half-sweep of a level in the storage type of `prec`.
 struct SweepInfo {
@@ -93,11 +98,12 @@ static inline int mg_bottom_phases(uint8_t *ph, int nb)
 bool split_supported(const Level &L0);
 // Reference layout <-> split layout of a level's array of 4- or 2-byte values.
 int split_convert(hipStream_t s, const Level &L0, const void *src, void *dst, int elem_bytes, bool to_split);
-// One half-sweep of a split level (mode SWEEP_ZERO / PLAIN / PROLONG from
-// zin + zc[parent], zc colour-split too when zc_split); with part, also the
-// partials of z.r (count in *nblk).
+// One half-sweep of a split level (mode SWEEP_ZERO / PLAIN / PLAIN_OWN0 /
+// PROLONG from zin + zc[parent], zc in the reference layout); with part, also
+// the partials of z.r (count in *nblk); with natural_out, zout receives the
+// whole z in the reference layout (SWEEP_OUT_NATURAL).
 int split_sweep(hipStream_t s, const Level &L0, const float *zin, float *zout, const float *zc, const float *r,
-                const uint16_t *code, int phase, int mode, double *part, int *nblk, bool zc_split = false);
+                const uint16_t *code, int phase, int mode, double *part, int *nblk, bool natural_out = false);
 // Mixed-precision level-0 sweeps on the split layout (T = FP16 or BF16 by
 // prec): pre-smoothing stores z and r as T (the first two sweeps take r from
 // the FP32 CG residual times the storage scale), post-smoothing keeps z in
@@ -108,8 +114,7 @@ struct SplitMixedSweep {
     int kind, phase;
     const void *zin;       // T (pre, post-prolong) or FP32 (post)
     void *zout;            // T (pre) or FP32 (post)
-    const float *zc;       // level-1 z (post-prolong)
-    int zc_split;          // level 1 is colour-split too
+    const float *zc;       // level-1 z in the reference layout (post-prolong)
     void *rh;              // T split r
     const uint16_t *code;
     const float *r32, *q32;

@@ -87,47 +87,20 @@ struct SSweepArgs {
     double *part;          // DOT: partials of z.r; FINAL: z.r then z.q
     SGeo g;
     int cu;                // colour updated (the half-sweep phase ^ 1)
-    int zc_split;          // PROLONG: the coarse z is colour-split too
 };
 
-// z of the coarse level at the parents of the CPT fine cells m0.. of row
-// (ii, jj), the coarse cells (ii/2, jj/2, m0 + e).  The coarse level is in the
-// reference layout or (csplit) colour-split like the fine one: cell (I, J, K)
-// in half (I + J + K) & 1 at (I * nyc + J) * nzc/2 + K/2, so with m0 even the
-// even e come from half (I + J) & 1 and the odd e from the other.
-template <int CPT>
-__device__ __forceinline__ void coarse_zq(const float *zc, bool csplit, const SGeo &g, int ii, int jj, int m0,
-                                          float p[CPT])
-{
-    const int I = ii >> 1, J = jj >> 1, nyc = g.ny >> 1;
-    if (!csplit) { ldn<float, CPT>(zc + cidx(I, J, m0, nyc, g.nh), p); return; }
-    const uint32_t nhc = (uint32_t)g.nh >> 1, halfc = (uint32_t)(g.nx >> 1) * (uint32_t)nyc * nhc;
-    const uint32_t b = ((uint32_t)I * nyc + J) * nhc + (uint32_t)(m0 >> 1);
-    const uint32_t c0 = (uint32_t)((I + J) & 1);
-    float ev[CPT / 2], od[CPT / 2];
-    ldn<float, CPT / 2>(zc + c0 * halfc + b, ev);
-    ldn<float, CPT / 2>(zc + (c0 ^ 1u) * halfc + b, od);
-#pragma unroll
-    for (int e = 0; e < CPT / 2; ++e) { p[2 * e] = ev[e]; p[2 * e + 1] = od[e]; }
-}
-__device__ __forceinline__ float coarse_z1(const float *zc, bool csplit, const SGeo &g, int ii, int jj, int m)
-{
-    const int I = ii >> 1, J = jj >> 1, nyc = g.ny >> 1;
-    if (!csplit) return zc[cidx(I, J, m, nyc, g.nh)];
-    const uint32_t nhc = (uint32_t)g.nh >> 1, halfc = (uint32_t)(g.nx >> 1) * (uint32_t)nyc * nhc;
-    return zc[(uint32_t)((I + J + m) & 1) * halfc + ((uint32_t)I * nyc + J) * nhc + (uint32_t)(m >> 1)];
-}
-
-// One red-black half-sweep of level 0 in the split layout, with z read as
+// One red-black half-sweep of a level in the split layout, with z read as
 // TZI, written as TZO and r stored as TR (float, __half or __hip_bfloat16;
-// arithmetic in FP32).  ZERO starts from z = 0 and also writes the other
-// colour's zeros; PROLONG reads z + z_coarse[parent] everywhere and also
-// writes the other colour's prolongated values (out of place, zin != zout).
-// SRC32 takes r of the updated colour from the FP32 CG residual times the
-// storage scale and stores it (rounded to TR, and continues with the rounded
-// values).  OUT: STORE; DOT accumulates z.r over both colours; FINAL stores
-// like STORE (z stays in storage scale; 1/scale goes to zinv) and accumulates
-// (z / scale).r32 and (z / scale).q32 over both colours.
+// arithmetic in FP32).  ZERO starts from z = 0 and writes only the updated
+// colour; the next sweep (PLAIN_OWN0) takes its own colour's old z as +0
+// instead of reading zeros.  PROLONG reads z + z_coarse[parent] everywhere and
+// also writes the other colour's prolongated values (out of place, zin !=
+// zout).  SRC32 takes r of the updated colour from the FP32 CG residual times
+// the storage scale and stores it (rounded to TR, and continues with the
+// rounded values).  OUT: STORE; DOT accumulates z.r over both colours; FINAL
+// stores like STORE (z stays in storage scale; 1/scale goes to zinv) and
+// accumulates (z / scale).r32 and (z / scale).q32 over both colours; NATURAL
+// stores both colours in the reference layout.
 template <int MODE, int OUT, typename TZI, typename TZO, typename TR, bool SRC32, int CPT>
 __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
 {
@@ -154,7 +127,7 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
             ldn<TZI, CPT>(zin + h + g.row(ii, jj) + m0, v);
             if (MODE == SWEEP_PROLONG) {
                 float p[CPT];
-                coarse_zq<CPT>(a.zc, a.zc_split != 0, g, ii, jj, m0, p);
+                ldn<float, CPT>(a.zc + cidx(ii >> 1, jj >> 1, m0, g.ny >> 1, g.nh), p);
 #pragma unroll
                 for (int e = 0; e < CPT; ++e) v[e] = v[e] + p[e];
             }
@@ -162,13 +135,18 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
         auto z1 = [&](uint32_t h, int ii, int jj, int m) -> float {
             if (MODE == SWEEP_ZERO) return 0.0f;
             float v = ldv<TZI>(zin[h + g.row(ii, jj) + m]);
-            if (MODE == SWEEP_PROLONG) v = v + coarse_z1(a.zc, a.zc_split != 0, g, ii, jj, m);
+            if (MODE == SWEEP_PROLONG) v = v + a.zc[cidx(ii >> 1, jj >> 1, m, g.ny >> 1, g.nh)];
             return v;
         };
         // neighbour rows clamped to the domain (a wall neighbour has stencil
         // class 0 and is never used): no branches, so all loads issue at once
         float c[CPT], w[CPT], xp[CPT], xm[CPT], yp[CPT], ym[CPT];
-        zq(own, i, j, c);
+        if (MODE == SWEEP_PLAIN_OWN0) {
+#pragma unroll
+            for (int e = 0; e < CPT; ++e) c[e] = 0.0f;
+        } else {
+            zq(own, i, j, c);
+        }
         zq(oth, i, j, w);
         zq(oth, min(i + 1, g.nx - 1), j, xp);
         zq(oth, max(i - 1, 0), j, xm);
@@ -221,6 +199,13 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
                 acc1 += (double)zo[e] * (double)ro[e] + (double)zw[e] * (double)rw[e];
                 acc2 += (double)zo[e] * (double)qo[e] + (double)zw[e] * (double)qw[e];
             }
+        } else if (OUT == SWEEP_OUT_NATURAL) {
+            // cells k = 2 m0 .. 2 m0 + 2 CPT - 1 of row (i, j): this colour at
+            // k = 2m + o, the other one (unchanged, +0 on non-FLUID cells) at 2m + 1 - o
+            float nat[2 * CPT];
+#pragma unroll
+            for (int e = 0; e < CPT; ++e) { nat[2 * e + o] = out[e]; nat[2 * e + 1 - o] = w[e]; }
+            stn<float, 2 * CPT>(reinterpret_cast<float *>(a.zout) + cidx(i, j, 2 * m0, g.ny, 2 * g.nh), nat);
         } else {
             stn<TZO, CPT>(reinterpret_cast<TZO *>(a.zout) + own + base, out);
             if (MODE == SWEEP_PROLONG) {
@@ -234,8 +219,6 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
                 for (int e = 0; e < CPT; ++e)
                     wm[e] = keep_if(w[e], 0u - ((cd[e] >> (o == 0 ? 9 : 11)) & 1u));
                 stn<TZO, CPT>(reinterpret_cast<TZO *>(a.zout) + oth + base, wm);
-            } else if (MODE == SWEEP_ZERO) {
-                stn<TZO, CPT>(reinterpret_cast<TZO *>(a.zout) + oth + base, w);   // zeros
             }
             if (OUT == SWEEP_OUT_DOT) {
                 float ro[CPT];
@@ -245,7 +228,7 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
             }
         }
     }
-    if (OUT != SWEEP_OUT_STORE) {
+    if (OUT == SWEEP_OUT_DOT || OUT == SWEEP_OUT_FINAL) {   // the only modes with partials (part != nullptr)
         const double s1 = block_sum<256>(acc1, lds);
         if (OUT == SWEEP_OUT_FINAL) {
             const double s2 = block_sum<256>(acc2, lds);
@@ -554,7 +537,6 @@ int split_sweep_mixed_t(hipStream_t s, const Level &L0, const SplitMixedSweep &m
     a.zin = m.zin; a.zout = m.zout; a.zc = m.zc; a.r = m.rh; a.code = m.code;
     a.r32 = m.r32; a.q32 = m.q32; a.zinv = m.zinv; a.sc = m.sc; a.part = m.part;
     a.cu = m.phase ^ 1;
-    a.zc_split = m.zc_split;
     // the pre-smoothing sweeps (16-bit z and r) move 8 cells per thread, in
     // 16-byte vectors, when the half-rows allow it
     const bool pre = m.kind == SMX_PRE_ZERO_SRC || m.kind == SMX_PRE_SRC || m.kind == SMX_PRE;
@@ -567,9 +549,9 @@ int split_sweep_mixed_t(hipStream_t s, const Level &L0, const SplitMixedSweep &m
         if (cpt == 8) sweep_launch<SWEEP_ZERO, SWEEP_OUT_STORE, T, T, T, true, 8>(s, nb, a);
         else sweep_launch<SWEEP_ZERO, SWEEP_OUT_STORE, T, T, T, true>(s, nb, a);
         break;
-    case SMX_PRE_SRC:
-        if (cpt == 8) sweep_launch<SWEEP_PLAIN, SWEEP_OUT_STORE, T, T, T, true, 8>(s, nb, a);
-        else sweep_launch<SWEEP_PLAIN, SWEEP_OUT_STORE, T, T, T, true>(s, nb, a);
+    case SMX_PRE_SRC:   // follows the ZERO sweep: this colour starts at 0
+        if (cpt == 8) sweep_launch<SWEEP_PLAIN_OWN0, SWEEP_OUT_STORE, T, T, T, true, 8>(s, nb, a);
+        else sweep_launch<SWEEP_PLAIN_OWN0, SWEEP_OUT_STORE, T, T, T, true>(s, nb, a);
         break;
     case SMX_PRE:
         if (cpt == 8) sweep_launch<SWEEP_PLAIN, SWEEP_OUT_STORE, T, T, T, false, 8>(s, nb, a);
@@ -603,17 +585,18 @@ int split_convert(hipStream_t s, const Level &L0, const void *src, void *dst, in
 }
 
 int split_sweep(hipStream_t s, const Level &L0, const float *zin, float *zout, const float *zc, const float *r,
-                const uint16_t *code, int phase, int mode, double *part, int *nblk, bool zc_split)
+                const uint16_t *code, int phase, int mode, double *part, int *nblk, bool natural_out)
 {
     SSweepArgs a{};
     a.zin = zin; a.zout = zout; a.zc = zc; a.r = const_cast<float *>(r); a.code = code; a.part = part;
-    a.zc_split = zc_split;
     a.g = make_sgeo(L0.nx, L0.ny, L0.nz);
     a.cu = phase ^ 1;
     const unsigned nb = grid8((uint32_t)L0.nx * L0.ny * a.g.nq);
     if (nblk) *nblk = (int)nb;
     if (mode == SWEEP_ZERO) sweep_launch<SWEEP_ZERO, SWEEP_OUT_STORE, float, float, float, false>(s, nb, a);
     else if (mode == SWEEP_PROLONG) sweep_launch<SWEEP_PROLONG, SWEEP_OUT_STORE, float, float, float, false>(s, nb, a);
+    else if (mode == SWEEP_PLAIN_OWN0) sweep_launch<SWEEP_PLAIN_OWN0, SWEEP_OUT_STORE, float, float, float, false>(s, nb, a);
+    else if (natural_out) sweep_launch<SWEEP_PLAIN, SWEEP_OUT_NATURAL, float, float, float, false>(s, nb, a);
     else if (part) sweep_launch<SWEEP_PLAIN, SWEEP_OUT_DOT, float, float, float, false>(s, nb, a);
     else sweep_launch<SWEEP_PLAIN, SWEEP_OUT_STORE, float, float, float, false>(s, nb, a);
     FLUB_CHECK_LAUNCH();
