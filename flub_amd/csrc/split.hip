@@ -255,26 +255,46 @@ __global__ void __launch_bounds__(256) k_srestrict(const TR *__restrict__ r, con
     for (uint32_t t = ch.first; t < ch.end; t += ch.stride) {
         int I, J, K0;
         cg.decode(t, I, J, K0);
-        float res[2][2][2][CPT];   // [a][b][kk][e]: child (2I+a, 2J+b, 2(K0+e)+kk)
+        const int i0 = 2 * I, j0 = 2 * J;
+        // both colours of fine row (ii, jj) at m = K0 ..
+        auto ld2 = [&](int ii, int jj, float v[2][CPT]) {
+            const uint32_t b = g.row(ii, jj) + K0;
+            ldn<TZ, CPT>(z + b, v[0]);
+            ldn<TZ, CPT>(z + g.half + b, v[1]);
+        };
+        // Each row is loaded once: the rows i-1, i, i+1 of the two child j
+        // rotate through P, C, N as the child i advances; the y-halo rows
+        // (i, j0-1) and (i, j0+2) are loaded per child i.  Rows clamped to
+        // the domain (a wall neighbour has stencil class 0 and is never used).
+        float P[2][2][CPT], C[2][2][CPT], N[2][2][CPT];   // [b][colour][e]
 #pragma unroll
-        for (int aa = 0; aa < 2; ++aa)
+        for (int b = 0; b < 2; ++b) {
+            ld2(max(i0 - 1, 0), j0 + b, P[b]);
+            ld2(i0, j0 + b, C[b]);
+        }
+        float s[CPT];
+#pragma unroll
+        for (int e = 0; e < CPT; ++e) s[e] = 0.0f;
+#pragma unroll
+        for (int aa = 0; aa < 2; ++aa) {
+            const int i = i0 + aa;
+#pragma unroll
+            for (int b = 0; b < 2; ++b) ld2(min(i + 1, g.nx - 1), j0 + b, N[b]);
+            float YM[2][CPT], YP[2][CPT];
+            ld2(i, max(j0 - 1, 0), YM);
+            ld2(i, min(j0 + 2, g.ny - 1), YP);
 #pragma unroll
             for (int b = 0; b < 2; ++b) {
-                const int i = 2 * I + aa, j = 2 * J + b;
+                const int j = j0 + b;
                 const uint32_t base = g.row(i, j) + K0;
-                float zz[2][CPT];
-                ldn<TZ, CPT>(z + base, zz[0]);
-                ldn<TZ, CPT>(z + g.half + base, zz[1]);
+                float res[2][CPT];   // [kk][e]: child (i, j, 2(K0+e)+kk)
 #pragma unroll
                 for (int cc = 0; cc < 2; ++cc) {
                     const uint32_t own = (uint32_t)cc * g.half, oth = (uint32_t)(cc ^ 1) * g.half;
                     const int o = (cc - i - j) & 1;   // this colour's child is kk = o
-                    const float *w = zz[cc ^ 1];
-                    float xp[CPT], xm[CPT], yp[CPT], ym[CPT];   // rows clamped to the domain (walls: class 0)
-                    ldn<TZ, CPT>(z + oth + g.row(min(i + 1, g.nx - 1), j) + K0, xp);
-                    ldn<TZ, CPT>(z + oth + g.row(max(i - 1, 0), j) + K0, xm);
-                    ldn<TZ, CPT>(z + oth + g.row(i, min(j + 1, g.ny - 1)) + K0, yp);
-                    ldn<TZ, CPT>(z + oth + g.row(i, max(j - 1, 0)) + K0, ym);
+                    const float *w = C[b][cc ^ 1];
+                    const float *yp = b == 0 ? C[1][cc ^ 1] : YP[cc ^ 1];   // row j+1
+                    const float *ym = b == 0 ? YM[cc ^ 1] : C[0][cc ^ 1];   // row j-1
                     const float left = __shfl_up(w[CPT - 1], 1, 64), right = __shfl_down(w[0], 1, 64);
                     float edge = 0.0f;
                     if (o == 0) {
@@ -292,28 +312,30 @@ __global__ void __launch_bounds__(256) k_srestrict(const TR *__restrict__ r, con
                         if (o == 0) { kp = w[e]; km = e > 0 ? w[e - 1] : edge; }
                         else { km = w[e]; kp = e < CPT - 1 ? w[e + 1] : edge; }
                         // the pre-smoothed z holds +0 on non-FLUID cells
-                        float nd = nsum_code_air0(cd[e], zz[cc][e], xp[e], xm[e], yp[e], ym[e], kp, km);
+                        float nd = nsum_code_air0(cd[e], C[b][cc][e], N[b][cc ^ 1][e], P[b][cc ^ 1][e], yp[e], ym[e],
+                                                  kp, km);
                         if (!code_fluid(cd[e])) nd = 0.0f;
-                        res[aa][b][o][e] = rv[e] - (-nd);
+                        res[o][e] = rv[e] - (-nd);
                     }
                 }
+                // the children in the serial order (a, b, kk)
+#pragma unroll
+                for (int e = 0; e < CPT; ++e) s[e] = (s[e] + res[0][e]) + res[1][e];
             }
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int e = 0; e < CPT; ++e) { P[b][c][e] = C[b][c][e]; C[b][c][e] = N[b][c][e]; }
+        }
         const uint32_t cb = cidx(I, J, K0, cg.ny, cg.nh);
         uint32_t tg[CPT / 4];   // the coarse tags, 4 to a word
 #pragma unroll
         for (int q = 0; q < CPT / 4; ++q) tg[q] = reinterpret_cast<const uint32_t *>(tagc + cb)[q];
         float out[CPT];
 #pragma unroll
-        for (int e = 0; e < CPT; ++e) {
-            float s = 0.0f;
-#pragma unroll
-            for (int aa = 0; aa < 2; ++aa)
-#pragma unroll
-                for (int b = 0; b < 2; ++b)
-#pragma unroll
-                    for (int kk = 0; kk < 2; ++kk) s += res[aa][b][kk][e];
-            out[e] = ((tg[e >> 2] >> (8 * (e & 3))) & 0xffu) == 1u ? s : 0.0f;
-        }
+        for (int e = 0; e < CPT; ++e) out[e] = ((tg[e >> 2] >> (8 * (e & 3))) & 0xffu) == 1u ? s[e] : 0.0f;
         if (csplit) {   // colour-split coarse r: even e to half (I + J) & 1, odd e to the other (K0 even)
             const uint32_t nhc = (uint32_t)cg.nh >> 1, halfc = (uint32_t)cg.nx * (uint32_t)cg.ny * nhc;
             const uint32_t b = ((uint32_t)I * cg.ny + J) * nhc + (uint32_t)(K0 >> 1);
