@@ -80,7 +80,7 @@ struct SSweepArgs {
     const float *zc;       // PROLONG: z of level 1 (reference layout, FP32)
     void *r;               // TR split r of the V-cycle (SRC32: written)
     const uint16_t *code;  // split stencil codes
-    const float *r32;      // SRC32: the FP32 CG residual (split); FINAL: for z.r
+    const float *r32;      // SRC32: the FP32 CG residual (split)
     const float *q32;      // FINAL: q = -A p of the CG step (split), for z.q
     double *zinv;          // FINAL: receives 1/scale (the z buffer holds z * scale)
     const double *sc;      // SC_RSCALE: storage scale of r (SRC32, FINAL)
@@ -99,8 +99,8 @@ struct SSweepArgs {
 // the storage scale and stores it (rounded to TR, and continues with the
 // rounded values).  OUT: STORE; DOT accumulates z.r over both colours; FINAL
 // stores like STORE (z stays in storage scale; 1/scale goes to zinv) and
-// accumulates (z / scale).r32 and (z / scale).q32 over both colours; NATURAL
-// stores both colours in the reference layout.
+// accumulates (z / scale).(r / scale) (r as stored, TR) and (z / scale).q32
+// over both colours; NATURAL stores both colours in the reference layout.
 template <int MODE, int OUT, typename TZI, typename TZO, typename TR, bool SRC32, int CPT>
 __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
 {
@@ -192,7 +192,12 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
 #pragma unroll
             for (int e = 0; e < CPT; ++e) { zo[e] = out[e] * inv; zw[e] = w[e] * inv; }
             stn<TZO, CPT>(reinterpret_cast<TZO *>(a.zout) + own + base, out);
-            ldn<float, CPT>(a.r32 + own + base, ro); ldn<float, CPT>(a.r32 + oth + base, rw);
+            // r from the sweep's stored r (r times the storage scale, rounded to
+            // TR; inv is a power of two): the dot's rounding errors average out
+            // over the level, and the other colour costs sizeof(TR) bytes a cell
+            ldn<TR, CPT>(reinterpret_cast<const TR *>(a.r) + oth + base, rw);
+#pragma unroll
+            for (int e = 0; e < CPT; ++e) { ro[e] = rv[e] * inv; rw[e] = rw[e] * inv; }
             ldn<float, CPT>(a.q32 + own + base, qo); ldn<float, CPT>(a.q32 + oth + base, qw);
 #pragma unroll
             for (int e = 0; e < CPT; ++e) {
