@@ -18,29 +18,37 @@
 
 namespace {
 
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// A sampled array with its clamped-index byte offsets: sample (u, v, w) reads
+// q at oi(u) + oj(v) + ok(w) bytes (utils/util.py:20-28 sample_3d clamps each
+// index to the array).  The offsets of each distinct index are formed once per
+// sample and the loads add them, so a load costs two 32-bit adds from the
+// array's (uniform) base; the launchers keep every array below 4 GB.
 struct Arr {
     const float *q;
     int gx, gy, gz;
+    __device__ __forceinline__ uint32_t oi(int u) const
+    {
+        return (uint32_t)clampi(u, 0, gx - 1) * ((uint32_t)gy * (uint32_t)gz * 4u);
+    }
+    __device__ __forceinline__ uint32_t oj(int v) const { return (uint32_t)clampi(v, 0, gy - 1) * ((uint32_t)gz * 4u); }
+    __device__ __forceinline__ uint32_t ok(int w) const { return (uint32_t)clampi(w, 0, gz - 1) * 4u; }
+    __device__ __forceinline__ float at(uint32_t off) const
+    {
+        return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(q) + off);
+    }
 };
-
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
-
-// utils/util.py:20-28 sample_3d
-__device__ __forceinline__ float smp(const Arr &a, int u, int v, int w)
-{
-    return a.q[((uint32_t)clampi(u, 0, a.gx - 1) * (uint32_t)a.gy + (uint32_t)clampi(v, 0, a.gy - 1)) *
-                   (uint32_t)a.gz +
-               (uint32_t)clampi(w, 0, a.gz - 1)];
-}
 
 // wp.lerp
 __device__ __forceinline__ float lerp_(float a, float b, float t) { return a * (1.0f - t) + b * t; }
 
 __device__ __forceinline__ float tri8(const Arr &q, int i, int j, int k, float fi, float fj, float fk)
 {
-    const float a = smp(q, i, j, k), b = smp(q, i + 1, j, k), c = smp(q, i, j + 1, k), d = smp(q, i + 1, j + 1, k);
-    const float e = smp(q, i, j, k + 1), f = smp(q, i + 1, j, k + 1), g = smp(q, i, j + 1, k + 1),
-                h = smp(q, i + 1, j + 1, k + 1);
+    const uint32_t i0 = q.oi(i), i1 = q.oi(i + 1), j0 = q.oj(j), j1 = q.oj(j + 1), k0 = q.ok(k), k1 = q.ok(k + 1);
+    const uint32_t r00 = i0 + j0, r10 = i1 + j0, r01 = i0 + j1, r11 = i1 + j1;
+    const float a = q.at(r00 + k0), b = q.at(r10 + k0), c = q.at(r01 + k0), d = q.at(r11 + k0);
+    const float e = q.at(r00 + k1), f = q.at(r10 + k1), g = q.at(r01 + k1), h = q.at(r11 + k1);
     return lerp_(lerp_(lerp_(a, b, fi), lerp_(c, d, fi), fj), lerp_(lerp_(e, f, fi), lerp_(g, h, fi), fj), fk);
 }
 
@@ -89,14 +97,18 @@ __device__ float tricubic(const Arr &q, float s, float t, float r)
 {
     const int iu = (int)floorf(s), iv = (int)floorf(t), iw = (int)floorf(r);
     const float fu = s - (float)iu, fv = t - (float)iv, fw = r - (float)iw;
+    uint32_t oi[4], oj[4], ok[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) { oi[a] = q.oi(iu + a - 1); oj[a] = q.oj(iv + a - 1); ok[a] = q.ok(iw + a - 1); }
     float zs[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         float ys[4];
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
-            ys[b] = cubic(smp(q, iu - 1, iv + b - 1, iw + c - 1), smp(q, iu, iv + b - 1, iw + c - 1),
-                          smp(q, iu + 1, iv + b - 1, iw + c - 1), smp(q, iu + 2, iv + b - 1, iw + c - 1), fu);
+        for (int b = 0; b < 4; ++b) {
+            const uint32_t jk = oj[b] + ok[c];
+            ys[b] = cubic(q.at(oi[0] + jk), q.at(oi[1] + jk), q.at(oi[2] + jk), q.at(oi[3] + jk), fu);
+        }
         zs[c] = cubic(ys[0], ys[1], ys[2], ys[3], fv);
     }
     return cubic(zs[0], zs[1], zs[2], zs[3], fw);
@@ -139,9 +151,9 @@ __global__ void __launch_bounds__(256) k_advect_mc(const float *__restrict__ u, 
     // advect.py:114-131 sample_minmax(q_in, p, h0, stagger)
     const float rs = p1[0] / h0 - st[0], rt = p1[1] / h0 - st[1], rr = p1[2] / h0 - st[2];
     const int ci = (int)floorf(rs), cj = (int)floorf(rt), ck = (int)floorf(rr);
-    const float a = smp(Q, ci, cj, ck), b = smp(Q, ci + 1, cj, ck), c = smp(Q, ci, cj + 1, ck),
-                d = smp(Q, ci + 1, cj + 1, ck), e = smp(Q, ci, cj, ck + 1), f = smp(Q, ci + 1, cj, ck + 1),
-                g = smp(Q, ci, cj + 1, ck + 1), hh = smp(Q, ci + 1, cj + 1, ck + 1);
+    const uint32_t i0 = Q.oi(ci), i1 = Q.oi(ci + 1), j0 = Q.oj(cj), j1 = Q.oj(cj + 1), k0 = Q.ok(ck), k1 = Q.ok(ck + 1);
+    const float a = Q.at(i0 + j0 + k0), b = Q.at(i1 + j0 + k0), c = Q.at(i0 + j1 + k0), d = Q.at(i1 + j1 + k0),
+                e = Q.at(i0 + j0 + k1), f = Q.at(i1 + j0 + k1), g = Q.at(i0 + j1 + k1), hh = Q.at(i1 + j1 + k1);
     const float mn = fminf(fminf(fminf(fminf(a, b), c), d), fminf(fminf(fminf(e, f), g), hh));
     const float mx = fmaxf(fmaxf(fmaxf(fmaxf(a, b), c), d), fmaxf(fmaxf(fmaxf(e, f), g), hh));
     float out = q_sl;
@@ -193,7 +205,8 @@ __global__ void __launch_bounds__(256) k_advect_simple(const float *__restrict__
     new_q[t] = tri8(Arr{q, nx, ny, nz}, iu, iv, iw, s - (float)iu, tt - (float)iv, r - (float)iw);
 }
 
-bool dims_ok(int a, int b, int c) { return a > 0 && b > 0 && c > 0 && (uint64_t)a * b * c < (1ull << 32) - 256; }
+// positive sizes, and the array's bytes addressable by 32-bit offsets
+bool dims_ok(int a, int b, int c) { return a > 0 && b > 0 && c > 0 && (uint64_t)a * b * c * 4u < (1ull << 32); }
 
 }  // namespace
 
