@@ -310,16 +310,17 @@ def droplets_box_main(args):
                                             "bytes": dbytes}
             out["kernels"] = profile_kernels(pr)
             # the step after the projection: velocity extrapolation into the air
-            # (set_unknown + 16 n / 128 passes, computed as two: later passes repeat the second)
+            # (set_unknown + 16 n / 128 passes, computed as one launch per face array:
+            # later passes repeat the second, which only adds the faces the first flushed)
             from flub_amd.const_param import Field
             from flub_amd.extrapolate import extrapolate
             phi = Field(f=device.from_numpy(s["phi"]))
             extrapolate(pr.vel0, pr.vel_owner, phi)
             e_ms, _ = _timed(lambda: extrapolate(pr.vel0, pr.vel_owner, phi))
-            # set_unknown: phi read + 3 marks written (16 B/cell); each of the two
-            # passes: the mark (4 B) of every face, the velocity written on faces
-            # marked 1, the known neighbours' values read (4 + 4 B/face at most)
-            ebytes = 16 * n ** 3 + 2 * 12 * faces
+            # set_unknown: phi read + 3 marks written (16 B/cell); the fill: the mark
+            # (4 B) of every face, the velocity written on faces to fill, the known
+            # neighbours' values read (4 + 4 B/face at most)
+            ebytes = 16 * n ** 3 + 12 * faces
             out["extrapolate"] = {"ms": round(e_ms, 3), "GBs": round(ebytes / (e_ms * 1e-3) / 1e9, 1),
                                   "bytes": ebytes, "passes": 16 * n // 128}
             # the step before the projection: MacCormack advection of the velocity

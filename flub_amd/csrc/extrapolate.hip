@@ -42,12 +42,18 @@ __global__ void __launch_bounds__(256) k_set_unknown(const float *__restrict__ p
     tw[((uint32_t)i * ny + j) * (nz + 1) + k] = f;
 }
 
-// One extrapolate_3d launch on a face array; KEEP also applies flush_1_3d (the
-// extrapolate + flush pair in one pass: extrapolated faces keep mark 1 and
-// faces marked -1 get 1), else it marks extrapolated faces -1.  Faces marked
-// -1 are never read by other threads (only marks 0 are), so flushing them in
-// the same launch changes nothing the extrapolation sees.
-template <bool KEEP>
+// One extrapolate_3d launch on a face array (MODE 0), also applying
+// flush_1_3d (MODE 1: the extrapolate + flush pair in one pass: extrapolated
+// faces keep mark 1 and faces marked -1 get 1; else extrapolated faces are
+// marked -1), or (MODE 2) two or more extrapolate + flush passes in one: the
+// faces marked 1 or -1 are all filled and marked 1.  MODE 2 equals the passes
+// because only faces marked 0 are read and none is ever written: the second
+// pass recomputes the first pass's faces from the same values and adds the
+// faces the first pass flushed from -1, and later passes repeat the second.
+// Faces marked -1 are never read by other threads (only marks 0 are), so
+// changing their marks in the same launch changes nothing the extrapolation
+// sees.
+template <int MODE>
 __global__ void __launch_bounds__(256) k_extrapolate(float *__restrict__ q, int32_t *__restrict__ unknown, int sx,
                                                      int sy, int sz)
 {
@@ -55,8 +61,8 @@ __global__ void __launch_bounds__(256) k_extrapolate(float *__restrict__ q, int3
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (t >= n) return;
     const int32_t mark = unknown[t];
-    if (KEEP && mark == -1) unknown[t] = 1;
-    if (mark != 1) return;
+    if (MODE >= 1 && mark == -1) unknown[t] = 1;
+    if (!(mark == 1 || (MODE == 2 && mark == -1))) return;
     int i, j, k;
     decode3(t, sy, sz, i, j, k);
     int cnt = 0;
@@ -80,7 +86,7 @@ __global__ void __launch_bounds__(256) k_extrapolate(float *__restrict__ q, int3
     }
     if (cnt > 0) {
         q[t] = s / (float)cnt;
-        if (!KEEP) unknown[t] = -1;
+        if (MODE == 0) unknown[t] = -1;
     } else {
         q[t] = 0.0f;
     }
@@ -110,7 +116,7 @@ FLUB_API int flub_extrapolate_3d(float *q, int32_t *unknown, int sx, int sy, int
 {
     FLUB_REQUIRE(q && unknown && dims_ok(sx, sy, sz), FLUB_EINVAL, "flub_extrapolate_3d: bad argument");
     const uint32_t n = (uint32_t)sx * sy * sz;
-    k_extrapolate<false><<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(q, unknown, sx, sy, sz);
+    k_extrapolate<0><<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(q, unknown, sx, sy, sz);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
 }
@@ -124,32 +130,41 @@ FLUB_API int flub_flush_1_3d(int32_t *tag, size_t n, void *stream)
     return FLUB_OK;
 }
 
-FLUB_API int flub_extrapolate_by_1(float *u, float *v, float *w, int32_t *tu, int32_t *tv, int32_t *tw, int nx, int ny,
-                                   int nz, void *stream)
+namespace {
+// extrapolate_by_1 (MODE 1), or the result of two or more of them (MODE 2)
+int extrapolate_faces(float *u, float *v, float *w, int32_t *tu, int32_t *tv, int32_t *tw, int nx, int ny, int nz,
+                      bool several, void *stream)
 {
-    FLUB_REQUIRE(u && v && w && tu && tv && tw && dims_ok(nx + 1, ny + 1, nz + 1), FLUB_EINVAL,
-                 "flub_extrapolate_by_1: bad argument");
     hipStream_t s = as_stream(stream);
     float *q[3] = {u, v, w};
     int32_t *t[3] = {tu, tv, tw};
     for (int d = 0; d < 3; ++d) {
         const int sx = nx + (d == 0), sy = ny + (d == 1), sz = nz + (d == 2);
         const uint32_t n = (uint32_t)sx * sy * sz;
-        k_extrapolate<true><<<blocks_for(n, 256), 256, 0, s>>>(q[d], t[d], sx, sy, sz);
+        if (several) k_extrapolate<2><<<blocks_for(n, 256), 256, 0, s>>>(q[d], t[d], sx, sy, sz);
+        else k_extrapolate<1><<<blocks_for(n, 256), 256, 0, s>>>(q[d], t[d], sx, sy, sz);
         FLUB_CHECK_LAUNCH();
     }
     return FLUB_OK;
+}
+}  // namespace
+
+FLUB_API int flub_extrapolate_by_1(float *u, float *v, float *w, int32_t *tu, int32_t *tv, int32_t *tw, int nx, int ny,
+                                   int nz, void *stream)
+{
+    FLUB_REQUIRE(u && v && w && tu && tv && tw && dims_ok(nx + 1, ny + 1, nz + 1), FLUB_EINVAL,
+                 "flub_extrapolate_by_1: bad argument");
+    return extrapolate_faces(u, v, w, tu, tv, tw, nx, ny, nz, false, stream);
 }
 
 FLUB_API int flub_extrapolate(float *u, float *v, float *w, int32_t *tu, int32_t *tv, int32_t *tw, const float *phi,
                               int nx, int ny, int nz, int n_passes, void *stream)
 {
     FLUB_REQUIRE(phi && n_passes >= 0, FLUB_EINVAL, "flub_extrapolate: bad argument");
+    FLUB_REQUIRE(u && v && w && tu && tv && tw, FLUB_EINVAL, "flub_extrapolate: bad argument");
     int rc = flub_set_unknown(phi, tu, tv, tw, nx, ny, nz, stream);
-    if (rc != FLUB_OK) return rc;
-    // passes after the second recompute the same values (see the top of this
-    // file), so two passes give the result of any n_passes >= 2
-    for (int p = 0; p < (n_passes < 2 ? n_passes : 2) && rc == FLUB_OK; ++p)
-        rc = flub_extrapolate_by_1(u, v, w, tu, tv, tw, nx, ny, nz, stream);
-    return rc;
+    if (rc != FLUB_OK || n_passes == 0) return rc;
+    // any n_passes >= 2 gives the result of two, which one launch per face
+    // array computes (see k_extrapolate)
+    return extrapolate_faces(u, v, w, tu, tv, tw, nx, ny, nz, n_passes >= 2, stream);
 }

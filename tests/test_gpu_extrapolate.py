@@ -77,3 +77,21 @@ def test_driver_equals_every_pass(oracle):
         assert np.array_equal(got.f.numpy().view(np.uint32), want.view(np.uint32))
     for got, want in zip((T.u, T.v, T.w), ref_t):
         assert np.array_equal(got.f.numpy(), want)
+
+
+@pytest.mark.parametrize("passes", [0, 1, 2, 3])
+def test_driver_pass_counts(passes, oracle):
+    """Zero and one pass, and two or more computed as one launch per axis,
+    against the oracle running every pass."""
+    n = 32
+    phi, vel, owners = _inputs(n, 5)
+    ref_v, ref_t = vel, oracle.set_unknown(phi, *owners)
+    for _ in range(passes):
+        ref_v, ref_t = oracle.extrapolate_by_1(ref_v, ref_t)
+    V = _mac(vel, MACGrid, Field)
+    T = _mac(owners, MACGridInt, IntField)
+    extrapolate(V, T, Field(f=wp.from_numpy(phi)), n_cells=passes)
+    for got, want in zip((V.u, V.v, V.w), ref_v):
+        assert np.array_equal(got.f.numpy().view(np.uint32), np.asarray(want).view(np.uint32))
+    for got, want in zip((T.u, T.v, T.w), ref_t):
+        assert np.array_equal(got.f.numpy(), want)
