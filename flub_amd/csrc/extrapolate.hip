@@ -53,43 +53,94 @@ __global__ void __launch_bounds__(256) k_set_unknown(const float *__restrict__ p
 // Faces marked -1 are never read by other threads (only marks 0 are), so
 // changing their marks in the same launch changes nothing the extrapolation
 // sees.
+//
+// A workgroup fills a tile of TI x TJ x TK faces: it first stages the tile
+// and a one-face halo in LDS (a class byte per face -- 0 known, 1 marked 1,
+// 3 marked -1, 2 anything else -- and the value of the known ones), then
+// each face sums its known neighbours from LDS in the reference's order.
+constexpr int ETI = 8, ETJ = 8, ETK = 62;   // halo rows of 64 faces: one wave per row
+constexpr int EHI = ETI + 2, EHJ = ETJ + 2, EHK = ETK + 2, EH = EHI * EHJ * EHK;
+static_assert(EHK == 64, "staging maps one wave to a halo row");
 template <int MODE>
 __global__ void __launch_bounds__(256) k_extrapolate(float *__restrict__ q, int32_t *__restrict__ unknown, int sx,
-                                                     int sy, int sz)
+                                                     int sy, int sz, int tiles_j, int tiles_k)
 {
-    const uint32_t n = (uint32_t)sx * sy * sz;
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    if (t >= n) return;
-    const int32_t mark = unknown[t];
-    if (MODE >= 1 && mark == -1) unknown[t] = 1;
-    if (!(mark == 1 || (MODE == 2 && mark == -1))) return;
-    int i, j, k;
-    decode3(t, sy, sz, i, j, k);
-    int cnt = 0;
-    float s = 0.0f;
-    for (int a = -1; a <= 1; ++a) {
-        const int ii = i + a;
-        if (ii < 0 || ii >= sx) continue;
-        for (int b = -1; b <= 1; ++b) {
-            const int jj = j + b;
-            if (jj < 0 || jj >= sy) continue;
-            const uint32_t row = ((uint32_t)ii * sy + jj) * (uint32_t)sz;
-            for (int c = -1; c <= 1; ++c) {
-                const int kk = k + c;
-                if (kk < 0 || kk >= sz) continue;
-                if (unknown[row + kk] == 0) {
-                    cnt += 1;
-                    s += q[row + kk];
-                }
-            }
+    __shared__ float qs[EH];
+    __shared__ uint8_t cls[EH];
+    const int tk = blockIdx.x % tiles_k, rest = blockIdx.x / tiles_k;
+    const int tj = rest % tiles_j, ti = rest / tiles_j;
+    const int i0 = ti * ETI - 1, j0 = tj * ETJ - 1, k0 = tk * ETK - 1;   // halo origin
+    // staging: every load of the thread is issued before any is used
+    constexpr int PER = (EH + 255) / 256;
+    int32_t mk[PER];
+    float qv[PER];
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        const int idx = threadIdx.x + 256 * r;
+        const int row = idx >> 6, lk = idx & 63;   // EHK == 64
+        const int li = row / EHJ, lj = row - li * EHJ;
+        const int i = i0 + li, j = j0 + lj, k = k0 + lk;
+        mk[r] = 2;
+        qv[r] = 0.0f;
+        if (idx < EH && i >= 0 && i < sx && j >= 0 && j < sy && k >= 0 && k < sz) {
+            const uint32_t g = ((uint32_t)i * sy + j) * (uint32_t)sz + k;
+            mk[r] = unknown[g];
+            qv[r] = q[g];
         }
     }
-    if (cnt > 0) {
-        q[t] = s / (float)cnt;
-        if (MODE == 0) unknown[t] = -1;
-    } else {
-        q[t] = 0.0f;
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        const int idx = threadIdx.x + 256 * r;
+        if (idx < EH) {
+            const int32_t m = mk[r];
+            cls[idx] = m == 0 ? 0 : (m == 1 ? 1 : (m == -1 ? 3 : 2));
+            qs[idx] = m == 0 ? qv[r] : 0.0f;   // +0 on every face that is not known
+        }
     }
+    __syncthreads();
+    const int lk = (threadIdx.x & 63) + 1;
+    if (lk > ETK) return;   // no barrier follows
+    for (int f = threadIdx.x >> 6; f < ETI * ETJ; f += 4) {
+        const int li = f / ETJ + 1, lj = f % ETJ + 1;
+        const int i = i0 + li, j = j0 + lj, k = k0 + lk;
+        if (i >= sx || j >= sy || k >= sz) continue;
+        const int c = cls[(li * EHJ + lj) * EHK + lk];
+        const bool fill = c == 1 || (MODE == 2 && c == 3);
+        const uint32_t g = ((uint32_t)i * sy + j) * (uint32_t)sz + k;
+        if (MODE >= 1 && c == 3) unknown[g] = 1;
+        if (!fill) continue;
+        // The reference sums the known neighbours inside the array in this
+        // order.  Every other neighbour (unknown, or the halo beyond the
+        // array: class 2) is staged as +0 and adding it changes nothing: the
+        // running sum starts at +0 and is never -0 (an exact zero sum rounds
+        // to +0), so s + (+0) == s bitwise.
+        int cnt = 0;
+        float s = 0.0f;
+#pragma unroll
+        for (int a = -1; a <= 1; ++a)
+#pragma unroll
+            for (int b = -1; b <= 1; ++b) {
+                const int row = ((li + a) * EHJ + (lj + b)) * EHK + lk;
+#pragma unroll
+                for (int cc = -1; cc <= 1; ++cc) {
+                    cnt += cls[row + cc] == 0 ? 1 : 0;
+                    s = s + qs[row + cc];
+                }
+            }
+        if (cnt > 0) {
+            q[g] = s / (float)cnt;
+            if (MODE == 0) unknown[g] = -1;
+        } else {
+            q[g] = 0.0f;
+        }
+    }
+}
+
+template <int MODE>
+void launch_extrapolate(float *q, int32_t *unknown, int sx, int sy, int sz, hipStream_t s)
+{
+    const int ti = (sx + ETI - 1) / ETI, tj = (sy + ETJ - 1) / ETJ, tk = (sz + ETK - 1) / ETK;
+    k_extrapolate<MODE><<<(unsigned)(ti * tj * tk), 256, 0, s>>>(q, unknown, sx, sy, sz, tj, tk);
 }
 
 __global__ void __launch_bounds__(256) k_flush_1(int32_t *__restrict__ tag, size_t n)
@@ -115,8 +166,7 @@ FLUB_API int flub_set_unknown(const float *phi, int32_t *tu, int32_t *tv, int32_
 FLUB_API int flub_extrapolate_3d(float *q, int32_t *unknown, int sx, int sy, int sz, void *stream)
 {
     FLUB_REQUIRE(q && unknown && dims_ok(sx, sy, sz), FLUB_EINVAL, "flub_extrapolate_3d: bad argument");
-    const uint32_t n = (uint32_t)sx * sy * sz;
-    k_extrapolate<0><<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(q, unknown, sx, sy, sz);
+    launch_extrapolate<0>(q, unknown, sx, sy, sz, as_stream(stream));
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
 }
@@ -140,9 +190,8 @@ int extrapolate_faces(float *u, float *v, float *w, int32_t *tu, int32_t *tv, in
     int32_t *t[3] = {tu, tv, tw};
     for (int d = 0; d < 3; ++d) {
         const int sx = nx + (d == 0), sy = ny + (d == 1), sz = nz + (d == 2);
-        const uint32_t n = (uint32_t)sx * sy * sz;
-        if (several) k_extrapolate<2><<<blocks_for(n, 256), 256, 0, s>>>(q[d], t[d], sx, sy, sz);
-        else k_extrapolate<1><<<blocks_for(n, 256), 256, 0, s>>>(q[d], t[d], sx, sy, sz);
+        if (several) launch_extrapolate<2>(q[d], t[d], sx, sy, sz, s);
+        else launch_extrapolate<1>(q[d], t[d], sx, sy, sz, s);
         FLUB_CHECK_LAUNCH();
     }
     return FLUB_OK;
