@@ -1081,12 +1081,14 @@ static int solve_fast(flub_mgpcg *h, int max_iters, double eps, double abs_tol, 
     if (x_pending) TRY((launch_update_px<true, false>(h->s, h->p, X, nullptr, h->sc, nullptr, n)));
     if (split) {   // back to the reference layout (sp.zpre is free scratch now)
         TRY(split_convert(h->s, L0, X, h->x, 4, false));
-        TRY(split_convert(h->s, L0, R, L0.r, 4, false));
-        TRY(split_convert(h->s, L0, Z, L0.z, 4, false));
-        TRY(split_convert(h->s, L0, h->Ap, h->sp.zpre, 4, false));
-        FLUB_CHECK_HIP(hipMemcpyAsync(h->Ap, h->sp.zpre, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
-        TRY(split_convert(h->s, L0, h->p, h->sp.zpre, 4, false));
-        FLUB_CHECK_HIP(hipMemcpyAsync(p_home, h->sp.zpre, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
+        if (!h->x_only) {
+            TRY(split_convert(h->s, L0, R, L0.r, 4, false));
+            TRY(split_convert(h->s, L0, Z, L0.z, 4, false));
+            TRY(split_convert(h->s, L0, h->Ap, h->sp.zpre, 4, false));
+            FLUB_CHECK_HIP(hipMemcpyAsync(h->Ap, h->sp.zpre, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
+            TRY(split_convert(h->s, L0, h->p, h->sp.zpre, 4, false));
+            FLUB_CHECK_HIP(hipMemcpyAsync(p_home, h->sp.zpre, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
+        }
         if (h->p != p_home) { h->p_alt = h->p; h->p = p_home; }
     } else if (h->p != p_home) {   // leave p in the buffer the handle exposes
         FLUB_CHECK_HIP(hipMemcpyAsync(p_home, h->p, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
@@ -1251,7 +1253,11 @@ FLUB_API int flub_project(flub_mgpcg *h, float *u, float *v, float *w, const flo
     TRY(flub_divergence_fused_u8(u, v, w, d, ur, vr, wr, L0.tag, nx, ny, nz, s_div, h->s));
     TRY(flub_mgpcg_init(h, d, -h_cell));
     TRY(flub_mgpcg_mask_interior(h));
+    // only the pressure x is read after the solve: r, z, p, Ap stay where the
+    // solver keeps them (the next init resets them)
+    h->x_only = true;
     int st = flub_mgpcg_solve(h, max_iters, 1e-12, abs_tol, rel_tol, smoother, 0, iters, rTr);
+    h->x_only = false;
     if (st != FLUB_OK && st != FLUB_ENOTCONV) return st;
     TRY(flub_subtract_gradient_x_u8(u, ur, h->x, L0.tag, nx, ny, nz, 1, h_cell, h->s));
     TRY(flub_subtract_gradient_x_u8(v, vr, h->x, L0.tag, nx, ny, nz, 2, h_cell, h->s));

@@ -69,6 +69,8 @@ struct flub_mgpcg {
     hipEvent_t ev;     // marks the residual norm's copy to the host
     bool split;        // level 0 of the FP32 solve runs in the colour-split layout
     int nsplit;        // levels 0 .. nsplit-1 of the split V-cycle are colour-split (0: none)
+    bool x_only;       // the solve's caller reads only x afterwards (the fused projection):
+                       // r, z, p and Ap are left in the solver's internal layout
     SplitBuf sp;
     MixedState *mx;
 };

@@ -472,13 +472,15 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
     if (x_pending) TRYM((launch_update_px<true, false>(h->s, h->p, S.x, nullptr, h->sc, nullptr, n)));
     // back to the reference layout (S.zw is free scratch now)
     TRYM(split_convert(h->s, L0, S.x, h->x, 4, false));
-    TRYM(split_convert(h->s, L0, S.r, L0.r, 4, false));
-    TRYM(split_convert(h->s, L0, S.z[zs], L0.z, 4, false));
-    TRYM(scale_by_device(h->s, L0.z, h->sc + SC_ZINV0 + zs, n));   // z buffers keep z * scale
-    TRYM(split_convert(h->s, L0, h->Ap, S.zw, 4, false));
-    FLUB_CHECK_HIP(hipMemcpyAsync(h->Ap, S.zw, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
-    TRYM(split_convert(h->s, L0, h->p, S.zw, 4, false));
-    FLUB_CHECK_HIP(hipMemcpyAsync(h->p, S.zw, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
+    if (!h->x_only) {
+        TRYM(split_convert(h->s, L0, S.r, L0.r, 4, false));
+        TRYM(split_convert(h->s, L0, S.z[zs], L0.z, 4, false));
+        TRYM(scale_by_device(h->s, L0.z, h->sc + SC_ZINV0 + zs, n));   // z buffers keep z * scale
+        TRYM(split_convert(h->s, L0, h->Ap, S.zw, 4, false));
+        FLUB_CHECK_HIP(hipMemcpyAsync(h->Ap, S.zw, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
+        TRYM(split_convert(h->s, L0, h->p, S.zw, 4, false));
+        FLUB_CHECK_HIP(hipMemcpyAsync(h->p, S.zw, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
+    }
     if (status == FLUB_OK && it >= cap && max_iters < 0) {
         status = FLUB_ENOTCONV;
         flub_set_error("no convergence within %d iterations", cap);
