@@ -75,6 +75,89 @@ __global__ void k_divergence(const float *__restrict__ u, const float *__restric
     div[t] = F;
 }
 
+// 4 consecutive values (16-byte or 4-byte aligned vector loads and stores)
+__device__ __forceinline__ void ldf4(const float *p, float v[4])
+{
+    const float4 a = *reinterpret_cast<const float4 *>(p);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+__device__ __forceinline__ void stf4(float *p, const float v[4])
+{
+    *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ void ldt4(const uint8_t *p, int v[4])
+{
+    const uchar4 a = *reinterpret_cast<const uchar4 *>(p);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+__device__ __forceinline__ void ldt4(const int32_t *p, int v[4])
+{
+    const int4 a = *reinterpret_cast<const int4 *>(p);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+
+// k_divergence for 4 consecutive cells along k per thread (nz % 4 == 0,
+// 16-byte aligned arrays): u, v, the tag rows and div move as vectors, w
+// (rows of nz + 1) and the rigid values as scalars; the per-cell arithmetic
+// is k_divergence's.
+template <typename TagT, bool FUSE>
+__global__ void __launch_bounds__(256) k_divergence4(const float *__restrict__ u, const float *__restrict__ v,
+                                                     const float *__restrict__ w, float *__restrict__ div,
+                                                     const float *__restrict__ ur, const float *__restrict__ vr,
+                                                     const float *__restrict__ wr, const TagT *__restrict__ tag,
+                                                     int nx, int ny, int nz, float s)
+{
+    const uint32_t nq = (uint32_t)nz >> 2;
+    const uint32_t tq = blockIdx.x * 256u + threadIdx.x;
+    if (tq >= (uint32_t)nx * ny * nq) return;
+    const uint32_t row = tq / nq, q = tq - row * nq;
+    const int i = (int)(row / (uint32_t)ny), j = (int)(row - (uint32_t)i * ny);
+    const int k0 = 4 * (int)q;
+    const uint32_t plane = (uint32_t)ny * nz;
+    const uint32_t t = row * (uint32_t)nz + k0;
+    const uint32_t iv0 = ((uint32_t)i * (ny + 1) + j) * nz + k0;
+    const uint32_t iw0 = ((uint32_t)i * ny + j) * (nz + 1) + k0;
+    float U0[4], U1[4], V0[4], V1[4], W[5];
+    ldf4(u + t, U0);
+    ldf4(u + t + plane, U1);
+    ldf4(v + iv0, V0);
+    ldf4(v + iv0 + nz, V1);
+#pragma unroll
+    for (int e = 0; e < 5; ++e) W[e] = w[iw0 + e];
+    int T[4], Txm[4] = {0, 0, 0, 0}, Txp[4] = {0, 0, 0, 0}, Tym[4] = {0, 0, 0, 0}, Typ[4] = {0, 0, 0, 0};
+    ldt4(tag + t, T);
+    if (i > 0) ldt4(tag + t - plane, Txm);
+    if (i < nx - 1) ldt4(tag + t + plane, Txp);
+    if (j > 0) ldt4(tag + t - nz, Tym);
+    if (j < ny - 1) ldt4(tag + t + nz, Typ);
+    const int tk0 = k0 > 0 ? (int)tag[t - 1] : 0, tk4 = k0 + 4 < nz ? (int)tag[t + 4] : 0;
+    float out[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int k = k0 + e;
+        float vl = U0[e], vrr = U1[e], vb = V0[e], vt = V1[e], vf = W[e], vn = W[e + 1];
+        if (i == 0) vl = 0.0f;
+        else if (Txm[e] == 2) vl = ur[t + e];
+        if (i == nx - 1) vrr = 0.0f;
+        else if (Txp[e] == 2) vrr = ur[t + e + plane];
+        if (j == 0) vb = 0.0f;
+        else if (Tym[e] == 2) vb = vr[iv0 + e];
+        if (j == ny - 1) vt = 0.0f;
+        else if (Typ[e] == 2) vt = vr[iv0 + e + nz];
+        const int tm = e > 0 ? T[e - 1] : tk0, tp = e < 3 ? T[e + 1] : tk4;
+        if (k == 0) vf = 0.0f;
+        else if (tm == 2) vf = wr[iw0 + e];
+        if (k == nz - 1) vn = 0.0f;
+        else if (tp == 2) vn = wr[iw0 + e + 1];
+        float F = vrr - vl + vt - vb + vn - vf;
+        if (FUSE) F = (T[e] == 1 ? F : 0.0f) * s;
+        out[e] = F;
+    }
+    stf4(div + t, out);
+}
+
+static bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
+
 static bool grid_ok(int nx, int ny, int nz)
 {
     if (nx < 1 || ny < 1 || nz < 1) return false;
@@ -90,8 +173,12 @@ static int divergence_impl(const float *u, const float *v, const float *w, float
     FLUB_REQUIRE(u && v && w && div && ur && vr && wr && tag, FLUB_EINVAL, "divergence: null pointer");
     FLUB_REQUIRE(grid_ok(nx, ny, nz), FLUB_ESHAPE, "divergence: bad grid %dx%dx%d", nx, ny, nz);
     const size_t n = (size_t)nx * ny * nz;
-    k_divergence<TagT, FUSE><<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(
-        u, v, w, div, ur, vr, wr, tag, nx, ny, nz, s);
+    if (nz % 4 == 0 && aligned16(u) && aligned16(v) && aligned16(div) && aligned16(tag))
+        k_divergence4<TagT, FUSE><<<blocks_for(n / 4, 256), 256, 0, as_stream(stream)>>>(u, v, w, div, ur, vr, wr, tag,
+                                                                                     nx, ny, nz, s);
+    else
+        k_divergence<TagT, FUSE><<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(u, v, w, div, ur, vr, wr, tag,
+                                                                                  nx, ny, nz, s);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
 }
@@ -175,6 +262,89 @@ __global__ void k_subtract_gradient(float *__restrict__ uf, const float *__restr
     uf[t] = out;
 }
 
+// k_subtract_gradient for the u and v faces (rows of nz faces), 4 consecutive
+// faces along k per thread (nz % 4 == 0, 16-byte aligned arrays); the rigid
+// velocities are read only by vectors that have a SOLID side.
+template <typename TagT, bool PS>
+__global__ void __launch_bounds__(256) k_subtract_gradient4(float *__restrict__ uf, const float *__restrict__ urig,
+                                                            const float *__restrict__ p,
+                                                            const TagT *__restrict__ tag, int nx, int ny, int nz,
+                                                            int d, float h, float inv_h)
+{
+    const int fx = nx + (d == 1), fy = ny + (d == 2);
+    const uint32_t nq = (uint32_t)nz >> 2;
+    const uint32_t tq = blockIdx.x * 256u + threadIdx.x;
+    if (tq >= (uint32_t)fx * fy * nq) return;
+    const uint32_t row = tq / nq, q = tq - row * nq;
+    const int i = (int)(row / (uint32_t)fy), j = (int)(row - (uint32_t)i * fy);
+    const int k0 = 4 * (int)q;
+    const uint32_t t = row * (uint32_t)nz + k0;
+    const int ii = d == 1 ? i : j, n_d = d == 1 ? nx : ny;
+    float out[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (0 < ii && ii < n_d) {
+        const uint32_t I = cidx(i, j, k0, ny, nz);
+        const uint32_t off = d == 1 ? (uint32_t)ny * nz : (uint32_t)nz;
+        float PL[4], PR[4], UF[4], UR[4];
+        int TL[4], TR[4];
+        ldf4(p + I - off, PL);
+        ldf4(p + I, PR);
+        ldt4(tag + I - off, TL);
+        ldt4(tag + I, TR);
+        ldf4(uf + t, UF);
+        bool solid = false;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) solid = solid || TL[e] == 2 || TR[e] == 2;
+        if (solid) ldf4(urig + t, UR);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            float pl = PL[e], pr = PR[e];
+            if (PS) { pl = (pl * h) * inv_h; pr = (pr * h) * inv_h; }
+            if (TL[e] == 2 || TR[e] == 2) out[e] = UR[e];
+            else if (TL[e] == 1 || TR[e] == 1) out[e] = UF[e] - (pr - pl);
+        }
+    }
+    stf4(uf + t, out);
+}
+
+// k_subtract_gradient for the w faces (rows of nz + 1 faces): face k lies
+// between cells k - 1 and k, so the 4 faces k0 .. k0 + 3 of a thread read the
+// cells k0 .. k0 + 3 as one vector and cell k0 - 1 as a scalar; the face
+// values (unaligned rows) are scalars.  The thread of the last vector also
+// writes the boundary face k = nz.
+template <typename TagT, bool PS>
+__global__ void __launch_bounds__(256) k_subtract_gradient4w(float *__restrict__ uf, const float *__restrict__ urig,
+                                                             const float *__restrict__ p,
+                                                             const TagT *__restrict__ tag, int nx, int ny, int nz,
+                                                             float h, float inv_h)
+{
+    const uint32_t nq = (uint32_t)nz >> 2;
+    const uint32_t tq = blockIdx.x * 256u + threadIdx.x;
+    if (tq >= (uint32_t)nx * ny * nq) return;
+    const uint32_t row = tq / nq, q = tq - row * nq;
+    const int k0 = 4 * (int)q;
+    const uint32_t I = row * (uint32_t)nz + k0, t = row * (uint32_t)(nz + 1) + k0;
+    float PR[4];
+    int TR[4];
+    ldf4(p + I, PR);
+    ldt4(tag + I, TR);
+    const float p_m = k0 > 0 ? p[I - 1] : 0.0f;
+    const int t_m = k0 > 0 ? (int)tag[I - 1] : 0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int k = k0 + e;
+        float out = 0.0f;
+        if (k > 0) {
+            float pl = e > 0 ? PR[e - 1] : p_m, pr = PR[e];
+            if (PS) { pl = (pl * h) * inv_h; pr = (pr * h) * inv_h; }
+            const int tl = e > 0 ? TR[e - 1] : t_m, tr = TR[e];
+            if (tl == 2 || tr == 2) out = urig[t + e];
+            else if (tl == 1 || tr == 1) out = uf[t + e] - (pr - pl);
+        }
+        uf[t + e] = out;
+    }
+    if (q == nq - 1) uf[t + 4] = 0.0f;   // k = nz
+}
+
 template <typename TagT, bool PS>
 static int subgrad_impl(float *uf, const float *urig, const float *p, const TagT *tag, int nx, int ny,
                         int nz, int d, float h, void *stream)
@@ -184,8 +354,15 @@ static int subgrad_impl(float *uf, const float *urig, const float *p, const TagT
     FLUB_REQUIRE(grid_ok(nx, ny, nz), FLUB_ESHAPE, "subtract_gradient_tag: bad grid");
     const size_t nf = (size_t)(nx + (d == 1)) * (ny + (d == 2)) * (nz + (d == 3));
     const float inv_h = (float)(1.0 / (double)h);
-    k_subtract_gradient<TagT, PS><<<blocks_for(nf, 256), 256, 0, as_stream(stream)>>>(
-        uf, urig, p, tag, nx, ny, nz, d, h, inv_h);
+    if (d != 3 && nz % 4 == 0 && aligned16(uf) && aligned16(urig) && aligned16(p) && aligned16(tag))
+        k_subtract_gradient4<TagT, PS><<<blocks_for(nf / 4, 256), 256, 0, as_stream(stream)>>>(uf, urig, p, tag, nx,
+                                                                                             ny, nz, d, h, inv_h);
+    else if (d == 3 && nz % 4 == 0 && aligned16(p) && aligned16(tag))
+        k_subtract_gradient4w<TagT, PS><<<blocks_for((size_t)nx * ny * (nz / 4), 256), 256, 0, as_stream(stream)>>>(
+            uf, urig, p, tag, nx, ny, nz, h, inv_h);
+    else
+        k_subtract_gradient<TagT, PS><<<blocks_for(nf, 256), 256, 0, as_stream(stream)>>>(uf, urig, p, tag, nx, ny,
+                                                                                        nz, d, h, inv_h);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
 }

@@ -109,6 +109,28 @@ def test_subtract_gradient_bitexact(d, oracle):
     assert bits_equal(got, ref)
 
 
+@pytest.mark.parametrize("shape", [(12, 10, 14), (16, 12, 20)])
+def test_divergence_and_gradient_random_bitexact(shape, oracle):
+    """Random tags (a fifth SOLID) and velocities on grids whose rows take the
+    4-wide kernels (nz % 4 == 0) and the scalar ones (nz = 14)."""
+    nx, ny, nz = shape
+    rng = np.random.default_rng(sum(shape))
+    tag = random_tags(shape, 3, (0.35, 0.45, 0.2))
+    sh = {"u": (nx + 1, ny, nz), "v": (nx, ny + 1, nz), "w": (nx, ny, nz + 1)}
+    s = {c: rng.normal(0, 1, sh[c[0]]).astype(np.float32) for c in ("u", "v", "w", "ur", "vr", "wr")}
+    vel, rig = _mac(s)
+    tg = IntField(f=wp.from_numpy(tag))
+    div = Field(f=wp.zeros(shape))
+    launch(divergence_tag, dim=shape, inputs=[vel, div, rig, tg])
+    assert bits_equal(div.f.numpy(), oracle.divergence_tag(s["u"], s["v"], s["w"], s["ur"], s["vr"], s["wr"], tag))
+    p = rng.normal(0, 1, shape).astype(np.float32)
+    pf = Field(f=wp.from_numpy(p))
+    for d, c in ((1, "u"), (2, "v"), (3, "w")):
+        launch(subtract_gradient_tag, dim=sh[c], inputs=[vel, rig, pf, tg, d])
+        got = {1: vel.u, 2: vel.v, 3: vel.w}[d].f.numpy()
+        assert bits_equal(got, oracle.subtract_gradient_tag(s[c], s[c + "r"], p, tag, d)), c
+
+
 def _loaded_mgpcg(tag, levels, pre_post=2, bottom=6):
     mg = MGPCG(3, tag.shape, levels)
     mg.pre_and_post_smoothing = pre_post
