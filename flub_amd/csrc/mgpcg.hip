@@ -22,6 +22,14 @@ __global__ void k_pack_tags(const int32_t *__restrict__ in, uint8_t *__restrict_
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) out[t] = (uint8_t)in[t];
 }
+// 4 tags per thread (n % 4 == 0, 16-byte aligned input)
+__global__ void k_pack_tags4(const int32_t *__restrict__ in, uint8_t *__restrict__ out, uint32_t n4)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n4) return;
+    const int4 a = reinterpret_cast<const int4 *>(in)[t];
+    reinterpret_cast<uchar4 *>(out)[t] = make_uchar4((uint8_t)a.x, (uint8_t)a.y, (uint8_t)a.z, (uint8_t)a.w);
+}
 
 // mgpcg.py:162-174 restrict_tag, literally: t = 2; a 0 child -> 0; else a 1
 // child -> 1 (equal to min over the children for tags in {0,1,2}).
@@ -831,7 +839,10 @@ FLUB_API int flub_mgpcg_set_tags(flub_mgpcg *h, const int32_t *tag)
     FLUB_REQUIRE(h, FLUB_EINVAL, "null handle");
     Level &L0 = h->lv[0];
     if (tag) {
-        k_pack_tags<<<nb256(L0.n), 256, 0, h->s>>>(tag, L0.tag, L0.n);
+        if (L0.n % 4 == 0 && ((uintptr_t)tag & 15u) == 0)
+            k_pack_tags4<<<nb256(L0.n / 4), 256, 0, h->s>>>(tag, L0.tag, L0.n / 4);
+        else
+            k_pack_tags<<<nb256(L0.n), 256, 0, h->s>>>(tag, L0.tag, L0.n);
         FLUB_CHECK_LAUNCH();
     } else {
         FLUB_REQUIRE(h->tags_set, FLUB_ESTATE, "set_tags(None) before any tags were given");
