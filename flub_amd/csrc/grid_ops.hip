@@ -100,12 +100,19 @@ __device__ __forceinline__ void ldt4(const int32_t *p, int v[4])
 // 16-byte aligned arrays): u, v, the tag rows and div move as vectors, w
 // (rows of nz + 1) and the rigid values as scalars; the per-cell arithmetic
 // is k_divergence's.
-template <typename TagT, bool FUSE>
+//
+// RSPLIT (the fused projection; FUSE): instead of, or (div != nullptr) as
+// well as, div it writes the solver's right-hand side r = (FLUID ? d * k : +0)
+// with d the fused value (the scale copy and mask_interior of the solve's
+// init), in the colour-split layout: cell (i, j, k) in half (i + j + k) & 1
+// at (i * ny + j) * nz/2 + k/2.
+template <typename TagT, bool FUSE, bool RSPLIT = false>
 __global__ void __launch_bounds__(256) k_divergence4(const float *__restrict__ u, const float *__restrict__ v,
                                                      const float *__restrict__ w, float *__restrict__ div,
                                                      const float *__restrict__ ur, const float *__restrict__ vr,
                                                      const float *__restrict__ wr, const TagT *__restrict__ tag,
-                                                     int nx, int ny, int nz, float s)
+                                                     int nx, int ny, int nz, float s, float *__restrict__ rsplit = nullptr,
+                                                     float k_r = 0.0f)
 {
     const uint32_t nq = (uint32_t)nz >> 2;
     const uint32_t tq = blockIdx.x * 256u + threadIdx.x;
@@ -153,7 +160,17 @@ __global__ void __launch_bounds__(256) k_divergence4(const float *__restrict__ u
         if (FUSE) F = (T[e] == 1 ? F : 0.0f) * s;
         out[e] = F;
     }
-    stf4(div + t, out);
+    if (!RSPLIT || div) stf4(div + t, out);
+    if (RSPLIT) {
+        // k0 is even: cells k0, k0 + 2 go to half (i + j) & 1, k0 + 1, k0 + 3 to the other
+        float rv[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) rv[e] = T[e] == 1 ? out[e] * k_r : 0.0f;
+        const uint32_t c0 = (uint32_t)((i + j) & 1), half = (uint32_t)nx * ny * (uint32_t)(nz >> 1);
+        const uint32_t b = row * (uint32_t)(nz >> 1) + (uint32_t)(k0 >> 1);
+        *reinterpret_cast<float2 *>(rsplit + c0 * half + b) = make_float2(rv[0], rv[2]);
+        *reinterpret_cast<float2 *>(rsplit + (c0 ^ 1u) * half + b) = make_float2(rv[1], rv[3]);
+    }
 }
 
 static bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
@@ -196,6 +213,25 @@ int flub_divergence_fused_u8(const float *u, const float *v, const float *w, flo
                              int nx, int ny, int nz, float s, void *stream)
 {
     return divergence_impl<uint8_t, true>(u, v, w, div, ur, vr, wr, tag, nx, ny, nz, s, stream);
+}
+
+bool divergence_split_ok(const float *u, const float *v, const float *div, const uint8_t *tag, int nz)
+{
+    return nz % 8 == 0 && aligned16(u) && aligned16(v) && (!div || aligned16(div)) && aligned16(tag);
+}
+
+int divergence_fused_split(const float *u, const float *v, const float *w, float *div, const float *ur,
+                           const float *vr, const float *wr, const uint8_t *tag, int nx, int ny, int nz, float s,
+                           float *rsplit, float k_r, void *stream)
+{
+    FLUB_REQUIRE(u && v && w && ur && vr && wr && tag && rsplit, FLUB_EINVAL, "divergence: null pointer");
+    FLUB_REQUIRE(grid_ok(nx, ny, nz) && divergence_split_ok(u, v, div, tag, nz), FLUB_ESHAPE,
+                 "divergence: the split right-hand side needs nz %% 8 == 0 and aligned arrays");
+    const size_t n = (size_t)nx * ny * nz;
+    k_divergence4<uint8_t, true, true><<<blocks_for(n / 4, 256), 256, 0, as_stream(stream)>>>(
+        u, v, w, div, ur, vr, wr, tag, nx, ny, nz, s, rsplit, k_r);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
 }
 
 __global__ void k_mask_with_tag(float *__restrict__ div, const int32_t *__restrict__ tag, size_t n)

@@ -1027,8 +1027,10 @@ static int solve_fast(flub_mgpcg *h, int max_iters, double eps, double abs_tol, 
     int zs = 0;
     float *R = L0.r, *X = h->x, *Z = L0.z;
     if (split) {
-        TRY(split_convert(h->s, L0, L0.r, h->sp.r, 4, true));
-        TRY(split_convert(h->s, L0, h->x, h->sp.x, 4, true));
+        if (!h->presplit) {
+            TRY(split_convert(h->s, L0, L0.r, h->sp.r, 4, true));
+            TRY(split_convert(h->s, L0, h->x, h->sp.x, 4, true));
+        }
         R = h->sp.r; X = h->sp.x; Z = h->sp.z[zs];
     }
     TRY(mg_dot(h, R, R, FIN_RTR, 0.0));
@@ -1259,16 +1261,32 @@ FLUB_API int flub_project(flub_mgpcg *h, float *u, float *v, float *w, const flo
     Level &L0 = h->lv[0];
     const int nx = L0.nx, ny = L0.ny, nz = L0.nz;
     TRY(flub_mgpcg_set_tags(h, tag));
-    float *d = div ? div : h->Ap;   // Ap is rewritten by init
     const float s_div = (float)(-1.0 / (double)h_cell);
-    TRY(flub_divergence_fused_u8(u, v, w, d, ur, vr, wr, L0.tag, nx, ny, nz, s_div, h->s));
-    TRY(flub_mgpcg_init(h, d, -h_cell));
-    TRY(flub_mgpcg_mask_interior(h));
+    // The colour-split solve takes r from the divergence kernel directly, in
+    // its layout: r = (FLUID ? d * (-h) : 0), the values init + mask_interior
+    // give; x starts at 0 in the split layout, p and the scalars as init sets
+    // them (z and Ap are written before they are read).
+    const bool presplit = h->split && smoother == FLUB_SMOOTHER_PHASES && use_fast(h, smoother) &&
+                          divergence_split_ok(u, v, div, L0.tag, nz);
+    if (presplit) {
+        TRY(divergence_fused_split(u, v, w, div, ur, vr, wr, L0.tag, nx, ny, nz, s_div, h->sp.r, -h_cell, h->s));
+        k_fill_f32<<<nb256(L0.n), 256, 0, h->s>>>(h->sp.x, 0.0f, L0.n);
+        k_fill_f32<<<nb256(L0.n), 256, 0, h->s>>>(h->p, 0.0f, L0.n);
+        FLUB_CHECK_LAUNCH();
+        FLUB_CHECK_HIP(hipMemsetAsync(h->sc, 0, SC_NUM * sizeof(double), h->s));
+    } else {
+        float *d = div ? div : h->Ap;   // Ap is rewritten by init
+        TRY(flub_divergence_fused_u8(u, v, w, d, ur, vr, wr, L0.tag, nx, ny, nz, s_div, h->s));
+        TRY(flub_mgpcg_init(h, d, -h_cell));
+        TRY(flub_mgpcg_mask_interior(h));
+    }
     // only the pressure x is read after the solve: r, z, p, Ap stay where the
     // solver keeps them (the next init resets them)
     h->x_only = true;
+    h->presplit = presplit;
     int st = flub_mgpcg_solve(h, max_iters, 1e-12, abs_tol, rel_tol, smoother, 0, iters, rTr);
     h->x_only = false;
+    h->presplit = false;
     if (st != FLUB_OK && st != FLUB_ENOTCONV) return st;
     TRY(flub_subtract_gradient_x_u8(u, ur, h->x, L0.tag, nx, ny, nz, 1, h_cell, h->s));
     TRY(flub_subtract_gradient_x_u8(v, vr, h->x, L0.tag, nx, ny, nz, 2, h_cell, h->s));

@@ -71,6 +71,7 @@ struct flub_mgpcg {
     int nsplit;        // levels 0 .. nsplit-1 of the split V-cycle are colour-split (0: none)
     bool x_only;       // the solve's caller reads only x afterwards (the fused projection):
                        // r, z, p and Ap are left in the solver's internal layout
+    bool presplit;     // the fused projection wrote r (and zero x) in the split layout already
     SplitBuf sp;
     MixedState *mx;
 };

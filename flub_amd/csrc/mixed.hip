@@ -403,9 +403,9 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
     const int max_replacements = 3;
     int replacements = 0;
     SplitBuf &S = h->sp;
-    TRYM(split_convert(h->s, L0, L0.r, S.r, 4, true));
+    if (!h->presplit) TRYM(split_convert(h->s, L0, L0.r, S.r, 4, true));
     FLUB_CHECK_HIP(hipMemcpyAsync(S.b, S.r, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
-    TRYM(split_convert(h->s, L0, h->x, S.x, 4, true));
+    if (!h->presplit) TRYM(split_convert(h->s, L0, h->x, S.x, 4, true));
     TRYM(mg_mixed_rscale_of(h, S.r));
     double rTr0;
     TRYM(read_sc(h, SC_RTR, &rTr0));
