@@ -1034,11 +1034,15 @@ static int solve_fast(flub_mgpcg *h, int max_iters, double eps, double abs_tol, 
         R = h->sp.r; X = h->sp.x; Z = h->sp.z[zs];
     }
     TRY(mg_dot(h, R, R, FIN_RTR, 0.0));
-    double rTr0;
-    TRY(read_scalar(h, SC_RTR, &rTr0));
-    const double tol = fmax(abs_tol, rTr0 * rel_tol);
+    // the tolerance is first needed after iteration 1: the first V-cycle runs
+    // while the initial r.r travels to the host
+    FLUB_CHECK_HIP(hipMemcpyAsync(h->hsc + SC_RTR, h->sc + SC_RTR, sizeof(double), hipMemcpyDeviceToHost, h->s));
+    FLUB_CHECK_HIP(hipEventRecord(h->ev, h->s));
     TRY(split ? mg_precondition_split(h, zs) : mg_precondition(h, FLUB_SMOOTHER_PHASES, true));
     TRY(launch_finish(h, h->vc_dot_nblk, FIN_OLDZTR, 0.0));
+    FLUB_CHECK_HIP(hipEventSynchronize(h->ev));
+    const double rTr0 = h->hsc[SC_RTR];
+    const double tol = fmax(abs_tol, rTr0 * rel_tol);
     const int cap = max_iters < 0 ? SOLVE_SAFETY_CAP : max_iters;
     int it = 0;
     double rTr = rTr0, r_prev = rTr0, rate = 1.0;   // rate: last reduction factor of r.r

@@ -407,13 +407,17 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
     FLUB_CHECK_HIP(hipMemcpyAsync(S.b, S.r, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
     if (!h->presplit) TRYM(split_convert(h->s, L0, h->x, S.x, 4, true));
     TRYM(mg_mixed_rscale_of(h, S.r));
-    double rTr0;
-    TRYM(read_sc(h, SC_RTR, &rTr0));
-    const double tol = fmax(abs_tol, rTr0 * rel_tol);
+    // the tolerance is first needed after iteration 1: the first V-cycle runs
+    // while the initial r.r travels to the host
+    FLUB_CHECK_HIP(hipMemcpyAsync(h->hsc + SC_RTR, h->sc + SC_RTR, sizeof(double), hipMemcpyDeviceToHost, h->s));
+    FLUB_CHECK_HIP(hipEventRecord(h->ev, h->s));
     int zs = 0;
     TRYM(mg_precondition_split(h, zs));
     k_mx_finish_beta<<<1, RBS, 0, h->s>>>(h->part, h->vc_dot_nblk, h->sc, eps, 1);
     FLUB_CHECK_LAUNCH();
+    FLUB_CHECK_HIP(hipEventSynchronize(h->ev));
+    const double rTr0 = h->hsc[SC_RTR];
+    const double tol = fmax(abs_tol, rTr0 * rel_tol);
     const int cap = max_iters < 0 ? SOLVE_SAFETY_CAP : max_iters;
     int it = 0, status = FLUB_OK;
     double rTr = rTr0, r_prev = rTr0, rate = 1.0;
