@@ -132,6 +132,21 @@ __device__ __forceinline__ Chunk xcd_chunk(uint32_t total)
     return {begin, end, begin + lb * 256u + threadIdx.x, nb8 * 256u};
 }
 
+// One group of a single-block pass over nblk partials, thread t summing
+// part[t], part[t + BS], ... in that order: v[q] = part[g + q * BS + t] (0
+// past the end, which leaves a sum or a max of non-negative values unchanged).
+// The Q loads are issued together: the partials come from other XCDs' L2, so a
+// loop that waits for each one in turn is latency-bound.
+template <int BS, int Q>
+__device__ __forceinline__ void load_partials(const double *__restrict__ part, int g, int nblk, double (&v)[Q])
+{
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int b = g + q * BS + (int)threadIdx.x;
+        v[q] = b < nblk ? part[b] : 0.0;
+    }
+}
+
 // Deterministic block reduction of a double (fixed shuffle/LDS tree).
 template <int BS>
 __device__ __forceinline__ double block_sum(double v, double *lds)

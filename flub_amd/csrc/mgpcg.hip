@@ -342,9 +342,12 @@ __global__ void __launch_bounds__(RBS) k_finish(const double *__restrict__ part,
 {
     __shared__ double lds[RBS / 64];
     double acc = 0.0, acc2 = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += RBS) {
-        acc += part[b];
-        if (op == FIN_MEAN) acc2 += part[nblk + b];
+    for (int g = 0; g < nblk; g += 8 * RBS) {
+        double v[8], w[8] = {};
+        load_partials<RBS>(part, g, nblk, v);
+        if (op == FIN_MEAN) load_partials<RBS>(part + nblk, g, nblk, w);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) { acc += v[q]; acc2 += w[q]; }
     }
     const double s = block_sum<RBS>(acc, lds);
     const double s2 = op == FIN_MEAN ? block_sum<RBS>(acc2, lds) : 0.0;
@@ -654,7 +657,7 @@ int mg_precondition(flub_mgpcg *h, int smoother, bool use_graph)
 // in slot 3 + zslot.
 int mg_precondition_split(flub_mgpcg *h, int zslot)
 {
-    if (h->no_graph) return vcycle_fast(h, true, zslot);
+    if (h->no_graph || !h->split_graph) return vcycle_fast(h, true, zslot);
     const int gs = 3 + zslot;
     if (!h->graph_ok[gs]) {
         hipGraph_t g;
@@ -704,6 +707,7 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
     h->pre_post = 6;
     h->bottom = 6;
     h->no_graph = getenv("FLUB_NO_GRAPH") != nullptr;
+    h->split_graph = getenv("FLUB_SPLIT_GRAPH") != nullptr;
     h->no_tb = getenv("FLUB_NO_TB") != nullptr;
     h->no_tb_l0 = getenv("FLUB_NO_TB_L0") != nullptr;
     h->tb_g8 = getenv("FLUB_TB_G8") != nullptr;

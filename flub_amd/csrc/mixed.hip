@@ -103,7 +103,12 @@ __global__ void __launch_bounds__(RBS) k_sum_partials(const double *__restrict__
 {
     __shared__ double lds[RBS / 64];
     double acc = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += RBS) acc += part[b];
+    for (int g = 0; g < nblk; g += 8 * RBS) {
+        double v[8];
+        load_partials<RBS>(part, g, nblk, v);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc += v[q];
+    }
     const double s = block_sum<RBS>(acc, lds);
     if (threadIdx.x == 0) out[0] = s;
 }
@@ -140,7 +145,13 @@ __global__ void __launch_bounds__(RBS) k_mx_finish_r(const double *__restrict__ 
     __shared__ double lds[RBS / 64];
     __shared__ double lm[RBS / 64];
     double acc = 0.0, mx = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += RBS) { acc += part[b]; mx = fmax(mx, part[nblk + b]); }
+    for (int g = 0; g < nblk; g += 8 * RBS) {
+        double v[8], w[8];
+        load_partials<RBS>(part, g, nblk, v);
+        load_partials<RBS>(part + nblk, g, nblk, w);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) { acc += v[q]; mx = fmax(mx, w[q]); }
+    }
     const double s = block_sum<RBS>(acc, lds);
     for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
     if ((threadIdx.x & 63) == 0) lm[threadIdx.x >> 6] = mx;
@@ -161,7 +172,13 @@ __global__ void __launch_bounds__(RBS) k_mx_finish_beta(const double *__restrict
 {
     __shared__ double lds[RBS / 64];
     double a1 = 0.0, a2 = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += RBS) { a1 += part[b]; a2 += part[nblk + b]; }
+    for (int g = 0; g < nblk; g += 8 * RBS) {
+        double v[8], w[8];
+        load_partials<RBS>(part, g, nblk, v);
+        load_partials<RBS>(part + nblk, g, nblk, w);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) { a1 += v[q]; a2 += w[q]; }
+    }
     const double zr = block_sum<RBS>(a1, lds);
     const double zq = block_sum<RBS>(a2, lds);
     if (threadIdx.x == 0) {

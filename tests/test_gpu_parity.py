@@ -203,26 +203,50 @@ def test_split_vcycle_bitexact(shape, levels, oracle):
     assert bits_equal(zd.numpy(), orc.get("z", 0))
 
 
+@pytest.mark.parametrize("graph", [False, True])
 @pytest.mark.parametrize("shape,levels", [((64, 64, 64), 4), ((64, 32, 48), 3), ((32, 48, 64), 4)])
-def test_split_coarse_vcycle_bitexact(shape, levels, oracle, monkeypatch):
+def test_split_coarse_vcycle_bitexact(shape, levels, graph, oracle, monkeypatch):
     """The split V-cycle with every level but the bottom colour-split (the
     layout grids of 2^20 cells and more use on their coarse levels), forced
-    down to small grids: bitwise the reference V-cycle."""
+    down to small grids: bitwise the reference V-cycle, launched kernel by
+    kernel and replayed from its captured graph."""
     import ctypes
     from flub_amd import _lib
     monkeypatch.setenv("FLUB_SPLIT_MIN_CELLS", "1")
+    if graph:
+        monkeypatch.setenv("FLUB_SPLIT_GRAPH", "1")
     tag = random_tags(shape, 5, (0.1, 0.85, 0.05))
     r = np.random.default_rng(13).normal(0, 1, shape).astype(np.float32)
     r[tag != 1] = 0.0
     mg = _loaded_mgpcg(tag, levels)
     rd, zd = wp.from_numpy(r), wp.zeros(shape)
-    for _ in range(2):   # the second call replays the captured graph
+    for _ in range(2):   # with the graph, the second call replays it
         _lib.call("flub_mgpcg_precondition_into", mg.handle, ctypes.c_void_p(rd.ptr), ctypes.c_void_p(zd.ptr), 1)
     orc = oracle.MGPCG(shape, levels, pre_and_post_smoothing=2, bottom_smoothing=6)
     orc.set_tags(tag)
     orc.put("r", r, 0)
     orc.apply_preconditioner("phases")
     assert bits_equal(zd.numpy(), orc.get("z", 0))
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_split_graph_solve_bitwise(precision, monkeypatch):
+    """The fused solves launch the split V-cycle kernel by kernel by default;
+    replaying the captured cycle instead gives bitwise the same pressure in
+    the same iterations."""
+    s = scenes.dam_break_spheres(64, 1)
+    res = []
+    for graph in (False, True):
+        if graph:
+            monkeypatch.setenv("FLUB_SPLIT_GRAPH", "1")
+        pr = Projection((64, 64, 64), n_mg_levels=4, pre_and_post_smoothing=2, bottom_smoothing=6,
+                        precision=precision)
+        pr.load(s)
+        it = pr.project(max_iters=-1, rel_tol=1e-12)
+        res.append((it, pr.p0.f.numpy()))
+    (i0, p0), (i1, p1) = res
+    assert i0 == i1
+    assert bits_equal(p0, p1)
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
