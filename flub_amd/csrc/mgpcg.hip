@@ -337,8 +337,10 @@ __global__ void k_update_p(float *__restrict__ p, const float *__restrict__ z, c
 }
 
 // Sum the partials (fixed order) and apply the scalar update `op`.
+// With host set, the sum also goes to host[0] (pinned host memory mapped for
+// the device): the host reads it after an event instead of a copy.
 __global__ void __launch_bounds__(RBS) k_finish(const double *__restrict__ part, int nblk, double *__restrict__ sc,
-                                                int op, double eps)
+                                                int op, double eps, double *__restrict__ host = nullptr)
 {
     __shared__ double lds[RBS / 64];
     double acc = 0.0, acc2 = 0.0;
@@ -352,6 +354,7 @@ __global__ void __launch_bounds__(RBS) k_finish(const double *__restrict__ part,
     const double s = block_sum<RBS>(acc, lds);
     const double s2 = op == FIN_MEAN ? block_sum<RBS>(acc2, lds) : 0.0;
     if (threadIdx.x != 0) return;
+    if (host) host[0] = s;
     switch (op) {
     case FIN_SUM: sc[SC_SUM] = s; break;
     case FIN_ALPHA: sc[SC_SUM] = s; sc[SC_ALPHA] = sc[SC_OLDZTR] / (s + eps); break;
@@ -385,6 +388,23 @@ static int launch_finish(flub_mgpcg *h, int nblk, int op, double eps, double *ds
 {
     k_finish<<<1, RBS, 0, h->s>>>(h->part, nblk, dst ? dst : h->sc, op, eps);
     FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+// sc[SC_RTR] = sum of the r.r partials, on its way to hsc[SC_RTR] for the
+// host (h->ev marks its arrival).
+static int finish_rtr_to_host(flub_mgpcg *h, int nblk)
+{
+    k_finish<<<1, RBS, 0, h->s>>>(h->part, nblk, h->sc, FIN_RTR, 0.0, h->hsc_dev ? h->hsc_dev + SC_RTR : nullptr);
+    FLUB_CHECK_LAUNCH();
+    return mg_rtr_sent(h);
+}
+
+int mg_rtr_sent(flub_mgpcg *h)
+{
+    if (!h->hsc_dev)
+        FLUB_CHECK_HIP(hipMemcpyAsync(h->hsc + SC_RTR, h->sc + SC_RTR, sizeof(double), hipMemcpyDeviceToHost, h->s));
+    FLUB_CHECK_HIP(hipEventRecord(h->ev, h->s));
     return FLUB_OK;
 }
 
@@ -751,8 +771,20 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
         h->fast = levels >= 2 && (nz % 4) == 0;
         ALLOC(h->sc, SC_NUM * sizeof(double));
         ALLOC(h->part, 2 * MAXPART * sizeof(double));
-        hipError_t e = hipHostMalloc((void **)&h->hsc, SC_NUM * sizeof(double), hipHostMallocDefault);
+        // The host mirror is mapped for the device and coherent, so that the
+        // residual norm's finish kernel can store into it directly; without
+        // the mapping the solves copy the value instead.
+        hipError_t e = hipHostMalloc((void **)&h->hsc, SC_NUM * sizeof(double),
+                                     hipHostMallocMapped | hipHostMallocCoherent);
+        if (e == hipSuccess) {
+            if (hipHostGetDevicePointer((void **)&h->hsc_dev, h->hsc, 0) != hipSuccess) h->hsc_dev = nullptr;
+        } else {
+            (void)hipGetLastError();
+            h->hsc_dev = nullptr;
+            e = hipHostMalloc((void **)&h->hsc, SC_NUM * sizeof(double), hipHostMallocDefault);
+        }
         if (e != hipSuccess) { flub_set_error("hipHostMalloc: %s", hipGetErrorString(e)); rc = (int)e; goto fail; }
+        if (getenv("FLUB_RTR_COPY")) h->hsc_dev = nullptr;
         e = hipMemsetAsync(h->sc, 0, SC_NUM * sizeof(double), h->s);
         if (e == hipSuccess) e = hipMemsetAsync(h->x, 0, n0 * 4, h->s);
         if (e == hipSuccess) e = hipStreamSynchronize(h->s);
@@ -1037,11 +1069,11 @@ static int solve_fast(flub_mgpcg *h, int max_iters, double eps, double abs_tol, 
         }
         R = h->sp.r; X = h->sp.x; Z = h->sp.z[zs];
     }
-    TRY(mg_dot(h, R, R, FIN_RTR, 0.0));
+    k_dot<<<nb, RBS, 0, h->s>>>(R, R, n, h->part);
+    FLUB_CHECK_LAUNCH();
     // the tolerance is first needed after iteration 1: the first V-cycle runs
     // while the initial r.r travels to the host
-    FLUB_CHECK_HIP(hipMemcpyAsync(h->hsc + SC_RTR, h->sc + SC_RTR, sizeof(double), hipMemcpyDeviceToHost, h->s));
-    FLUB_CHECK_HIP(hipEventRecord(h->ev, h->s));
+    TRY(finish_rtr_to_host(h, nb));
     TRY(split ? mg_precondition_split(h, zs) : mg_precondition(h, FLUB_SMOOTHER_PHASES, true));
     TRY(launch_finish(h, h->vc_dot_nblk, FIN_OLDZTR, 0.0));
     FLUB_CHECK_HIP(hipEventSynchronize(h->ev));
@@ -1074,16 +1106,10 @@ static int solve_fast(flub_mgpcg *h, int max_iters, double eps, double abs_tol, 
             k_update_rx_dot<true><<<nb, RBS, 0, h->s>>>(R, X, h->Ap, h->p, h->sc, n, h->part);
         }
         FLUB_CHECK_LAUNCH();
-        TRY(launch_finish(h, nb, FIN_RTR, 0.0));
-        if (speculate) {
-            FLUB_CHECK_HIP(hipMemcpyAsync(h->hsc + SC_RTR, h->sc + SC_RTR, sizeof(double), hipMemcpyDeviceToHost, h->s));
-            FLUB_CHECK_HIP(hipEventRecord(h->ev, h->s));
-            TRY(mg_precondition_split(h, zs ^ 1));
-            FLUB_CHECK_HIP(hipEventSynchronize(h->ev));
-            rTr = h->hsc[SC_RTR];
-        } else {
-            TRY(read_scalar(h, SC_RTR, &rTr));
-        }
+        TRY(finish_rtr_to_host(h, nb));
+        if (speculate) TRY(mg_precondition_split(h, zs ^ 1));
+        FLUB_CHECK_HIP(hipEventSynchronize(h->ev));
+        rTr = h->hsc[SC_RTR];
         if (verbose) printf("after iter %d, |residual|_2=%.9g\n", it + 1, sqrt(rTr));
         if (!isfinite(rTr)) { status = FLUB_ENONFINITE; flub_set_error("non-finite residual at iteration %d", it + 1); break; }
         if (rTr < tol) break;

@@ -55,6 +55,7 @@ struct flub_mgpcg {
     int vc_dot_nblk;  // partial count of the z.r fused into the fast V-cycle (0: none)
     double *part;   // reduction partials (2*MAXPART)
     double *hsc;    // pinned host mirror of sc
+    double *hsc_dev;  // device address of hsc (nullptr: not mapped, values are copied)
     int pre_post, bottom;
     hipStream_t s;     // work stream (caller's; may be the null stream)
     hipStream_t cap;   // private stream used only to capture graphs
@@ -141,12 +142,19 @@ int mg_precondition(flub_mgpcg *h, int smoother, bool use_graph);
 // handle's mixed precision), replayed from a captured graph.
 int mg_precondition_split(flub_mgpcg *h, int zslot);
 
+// After the kernel that wrote sc[SC_RTR] (and hsc[SC_RTR] through hsc_dev
+// when mapped): queue the copy to hsc when not mapped, then record h->ev;
+// hsc[SC_RTR] is valid once h->ev has completed.
+int mg_rtr_sent(flub_mgpcg *h);
+
 // alpha = old_zTr / (sum of the partials + eps) (mgpcg.py:212-216)
 int mg_finish_alpha(flub_mgpcg *h, int nblk, double eps);
 
 // mixed.hip
 int mg_mixed_rscale(flub_mgpcg *h);
-int mg_mixed_rscale_of(flub_mgpcg *h, const float *r);
+// Storage scale and r.r of r (split V-cycle); to_host: r.r also goes to
+// hsc[SC_RTR] (see mg_rtr_sent).
+int mg_mixed_rscale_of(flub_mgpcg *h, const float *r, bool to_host = false);
 int mg_mixed_setup(flub_mgpcg *h);
 void mg_mixed_teardown(flub_mgpcg *h);
 int mg_vcycle_mixed(flub_mgpcg *h, int smoother);

@@ -138,9 +138,10 @@ __global__ void __launch_bounds__(RBS) k_mx_rnorm(const float *__restrict__ r, u
 }
 
 // Reduce (sum, max) partials: sc[RTR] = sum, sc[RSCALE] = 2^-(e+shift) with
-// 2^(e-1) < max <= 2^e (2^-shift when max == 0).
+// 2^(e-1) < max <= 2^e (2^-shift when max == 0).  With host set, the sum also
+// goes to host[0] (pinned host memory mapped for the device).
 __global__ void __launch_bounds__(RBS) k_mx_finish_r(const double *__restrict__ part, int nblk, double *__restrict__ sc,
-                                                     int shift)
+                                                     int shift, double *__restrict__ host = nullptr)
 {
     __shared__ double lds[RBS / 64];
     __shared__ double lm[RBS / 64];
@@ -160,6 +161,7 @@ __global__ void __launch_bounds__(RBS) k_mx_finish_r(const double *__restrict__ 
         double m = 0.0;
         for (int w = 0; w < RBS / 64; ++w) m = fmax(m, lm[w]);
         sc[SC_RTR] = s;
+        if (host) host[0] = s;
         sc[SC_RSCALE] = ldexp((m > 0.0 && isfinite(m)) ? ldexp(1.0, -ilogb(m) - 1) : 1.0, -shift);
     }
 }
@@ -243,15 +245,16 @@ int mg_mixed_rscale(flub_mgpcg *h)
 
 // The same for the split V-cycle (its reduced-precision z stays O(max|r|):
 // no extra headroom shift).
-int mg_mixed_rscale_of(flub_mgpcg *h, const float *r)
+int mg_mixed_rscale_of(flub_mgpcg *h, const float *r, bool to_host)
 {
     const uint32_t n = h->lv[0].n;
     const int nb = red_nb(n);
     k_mx_rnorm<<<nb, RBS, 0, h->s>>>(r, n, h->part);
     FLUB_CHECK_LAUNCH();
-    k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nb, h->sc, 0);
+    k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nb, h->sc, 0,
+                                       to_host && h->hsc_dev ? h->hsc_dev + SC_RTR : nullptr);
     FLUB_CHECK_LAUNCH();
-    return FLUB_OK;
+    return to_host ? mg_rtr_sent(h) : FLUB_OK;
 }
 
 #define TRYM(x)               \
@@ -423,11 +426,9 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
     if (!h->presplit) TRYM(split_convert(h->s, L0, L0.r, S.r, 4, true));
     FLUB_CHECK_HIP(hipMemcpyAsync(S.b, S.r, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
     if (!h->presplit) TRYM(split_convert(h->s, L0, h->x, S.x, 4, true));
-    TRYM(mg_mixed_rscale_of(h, S.r));
     // the tolerance is first needed after iteration 1: the first V-cycle runs
     // while the initial r.r travels to the host
-    FLUB_CHECK_HIP(hipMemcpyAsync(h->hsc + SC_RTR, h->sc + SC_RTR, sizeof(double), hipMemcpyDeviceToHost, h->s));
-    FLUB_CHECK_HIP(hipEventRecord(h->ev, h->s));
+    TRYM(mg_mixed_rscale_of(h, S.r, true));
     int zs = 0;
     TRYM(mg_precondition_split(h, zs));
     k_mx_finish_beta<<<1, RBS, 0, h->s>>>(h->part, h->vc_dot_nblk, h->sc, eps, 1);
@@ -452,17 +453,12 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
         k_mx_update<false><<<nb, RBS, 0, h->s>>>(S.r, nullptr, h->Ap, nullptr, h->sc, n, h->part);
         FLUB_CHECK_LAUNCH();
         x_pending = true;
-        k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nb, h->sc, 0);
+        k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nb, h->sc, 0, h->hsc_dev ? h->hsc_dev + SC_RTR : nullptr);
         FLUB_CHECK_LAUNCH();
-        if (speculate) {
-            FLUB_CHECK_HIP(hipMemcpyAsync(h->hsc + SC_RTR, h->sc + SC_RTR, sizeof(double), hipMemcpyDeviceToHost, h->s));
-            FLUB_CHECK_HIP(hipEventRecord(h->ev, h->s));
-            TRYM(mg_precondition_split(h, zs ^ 1));
-            FLUB_CHECK_HIP(hipEventSynchronize(h->ev));
-            rTr = h->hsc[SC_RTR];
-        } else {
-            TRYM(read_sc(h, SC_RTR, &rTr));
-        }
+        TRYM(mg_rtr_sent(h));
+        if (speculate) TRYM(mg_precondition_split(h, zs ^ 1));
+        FLUB_CHECK_HIP(hipEventSynchronize(h->ev));
+        rTr = h->hsc[SC_RTR];
         if (verbose) printf("after iter %d, |residual|_2=%.9g\n", it + 1, sqrt(rTr));
         if (!isfinite(rTr)) { status = FLUB_ENONFINITE; flub_set_error("non-finite residual at iteration %d", it + 1); break; }
         bool redo = false;   // the speculative cycle used an r that was then replaced

@@ -229,23 +229,26 @@ def test_split_coarse_vcycle_bitexact(shape, levels, graph, oracle, monkeypatch)
     assert bits_equal(zd.numpy(), orc.get("z", 0))
 
 
+@pytest.mark.parametrize("variant", ["FLUB_SPLIT_GRAPH", "FLUB_RTR_COPY"])
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
-def test_split_graph_solve_bitwise(precision, monkeypatch):
-    """The fused solves launch the split V-cycle kernel by kernel by default;
-    replaying the captured cycle instead gives bitwise the same pressure in
-    the same iterations."""
+def test_split_launch_variants_bitwise(precision, variant, monkeypatch):
+    """How the fused solves queue their work changes no value: replaying the
+    captured split V-cycle instead of launching it kernel by kernel, or
+    copying the residual norm to the host instead of storing it there from
+    the finish kernel, gives bitwise the same pressure in the same
+    iterations."""
     s = scenes.dam_break_spheres(64, 1)
     res = []
-    for graph in (False, True):
-        if graph:
-            monkeypatch.setenv("FLUB_SPLIT_GRAPH", "1")
+    for on in (False, True):
+        if on:
+            monkeypatch.setenv(variant, "1")
         pr = Projection((64, 64, 64), n_mg_levels=4, pre_and_post_smoothing=2, bottom_smoothing=6,
                         precision=precision)
         pr.load(s)
         it = pr.project(max_iters=-1, rel_tol=1e-12)
-        res.append((it, pr.p0.f.numpy()))
-    (i0, p0), (i1, p1) = res
-    assert i0 == i1
+        res.append((it, pr.p0.f.numpy(), pr.rTr))
+    (i0, p0, r0), (i1, p1, r1) = res
+    assert i0 == i1 and r0 == r1
     assert bits_equal(p0, p1)
 
 
