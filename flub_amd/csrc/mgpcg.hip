@@ -552,6 +552,16 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
         if (split && l < h->nsplit) {   // level 0 in sp.*, the next split levels in their own buffers
             float *R = l == 0 ? S.r : F.r, *ZP = l == 0 ? S.zpre : F.zpre;
             const uint16_t *CD = S.lcode[l];
+            if (l > 0 && S.zt[l] && s >= 1) {   // the sweeps in pairs, alternating with zt, the last pair into ZP
+                const float *in = nullptr;
+                for (int it = 0; it < s; ++it) {
+                    float *out = ((s - 1 - it) & 1) ? S.zt[l] : ZP;
+                    TRY(split_sweep_pair(h->s, F, in, out, nullptr, R, CD, 0, it == 0 ? SWEEP_ZERO : SWEEP_PLAIN));
+                    in = out;
+                }
+                TRY(split_restrict(h->s, F, R, ZP, CD, h->lv[l + 1], l + 1 < h->nsplit));
+                continue;
+            }
             TRY(split_sweep(h->s, F, nullptr, ZP, nullptr, R, CD, 0, SWEEP_ZERO, nullptr, nullptr));
             TRY(split_sweep(h->s, F, ZP, ZP, nullptr, R, CD, 1, SWEEP_PLAIN_OWN0, nullptr, nullptr));
             for (int it = 1; it < s; ++it) {
@@ -614,6 +624,19 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
         if (split && l < h->nsplit) {
             float *Z = l == 0 ? S.z[zslot] : F.z, *R = l == 0 ? S.r : F.r, *ZP = l == 0 ? S.zpre : F.zpre;
             const uint16_t *CD = S.lcode[l];
+            if (l > 0 && S.zt[l] && s >= 2) {
+                // the sweeps in pairs from ZP through zt and z (alternating),
+                // the last pair writing the whole z, reference layout, into ZP
+                const float *in = ZP;
+                for (int it = 0; it < s; ++it) {
+                    const bool last = it + 1 == s;
+                    float *out = last ? ZP : ((it & 1) ? Z : S.zt[l]);
+                    TRY(split_sweep_pair(h->s, F, in, out, it == 0 ? zc : nullptr, R, CD, 1,
+                                         it == 0 ? SWEEP_PROLONG : SWEEP_PLAIN, last));
+                    in = out;
+                }
+                continue;
+            }
             for (int it = 0; it < s; ++it) {
                 const bool last = it + 1 == s;
                 if (it == 0) TRY(split_sweep(h->s, F, ZP, Z, zc, R, CD, 1, SWEEP_PROLONG, nullptr, nullptr));
@@ -816,6 +839,8 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
             h->nsplit = 1;
             while (h->nsplit + 1 < levels && split_supported(h->lv[h->nsplit]) && h->lv[h->nsplit].n >= min_cells) {
                 ALLOC(h->sp.lcode[h->nsplit], (size_t)h->lv[h->nsplit].n * 2);
+                if (split_pair_supported(h->lv[h->nsplit]) && getenv("FLUB_NO_PAIR") == nullptr)
+                    ALLOC(h->sp.zt[h->nsplit], (size_t)h->lv[h->nsplit].n * 4);
                 ++h->nsplit;
             }
             if (precision != FLUB_PREC_FP32) {
@@ -848,7 +873,7 @@ FLUB_API int flub_mgpcg_destroy(flub_mgpcg *h)
     hipFree(h->tb_order);
     hipFree(h->sp.r); hipFree(h->sp.z[0]); hipFree(h->sp.z[1]); hipFree(h->sp.zpre); hipFree(h->sp.x);
     hipFree(h->sp.code); hipFree(h->sp.rh); hipFree(h->sp.zh); hipFree(h->sp.zw); hipFree(h->sp.b);
-    for (int l = 1; l < MAXL; ++l) hipFree(h->sp.lcode[l]);
+    for (int l = 1; l < MAXL; ++l) { hipFree(h->sp.lcode[l]); hipFree(h->sp.zt[l]); }
     if (h->ev) hipEventDestroy(h->ev);
     if (h->hsc) hipHostFree(h->hsc);
     if (h->cap) hipStreamDestroy(h->cap);

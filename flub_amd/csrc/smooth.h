@@ -104,6 +104,15 @@ int split_convert(hipStream_t s, const Level &L0, const void *src, void *dst, in
 // whole z in the reference layout (SWEEP_OUT_NATURAL).
 int split_sweep(hipStream_t s, const Level &L0, const float *zin, float *zout, const float *zc, const float *r,
                 const uint16_t *code, int phase, int mode, double *part, int *nblk, bool natural_out = false);
+// Two half-sweeps of a split level in one launch, phase `phase` then the
+// other: mode SWEEP_ZERO (then SWEEP_PLAIN_OWN0), SWEEP_PLAIN or
+// SWEEP_PROLONG (from zin + zc[parent]) for the first, SWEEP_PLAIN for the
+// second; from zin into zout (a different buffer; both colours, or with
+// natural_out the whole z in the reference layout).  Bitwise the two
+// split_sweep launches.
+bool split_pair_supported(const Level &L);
+int split_sweep_pair(hipStream_t s, const Level &L, const float *zin, float *zout, const float *zc, const float *r,
+                     const uint16_t *code, int phase, int mode, bool natural_out = false);
 // Mixed-precision level-0 sweeps on the split layout (T = FP16 or BF16 by
 // prec): pre-smoothing stores z and r as T (the first two sweeps take r from
 // the FP32 CG residual times the storage scale), post-smoothing keeps z in

@@ -243,6 +243,135 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
     }
 }
 
+// Two half-sweeps of a split level in one launch (FP32): colour cuA (the
+// first sweep's, in mode MODE1: SWEEP_ZERO, SWEEP_PLAIN or SWEEP_PROLONG), then
+// colour cuA ^ 1 (SWEEP_PLAIN_OWN0 after a ZERO one, else SWEEP_PLAIN), read
+// from zin and written to zout (zout != zin: both colours; NAT: the whole z in
+// the reference layout, like SWEEP_OUT_NATURAL).  A workgroup owns a tile of
+// P2_TI x P2_TJ rows (i, j) over the full k range.  The first half-sweep runs
+// on the tile and a one-row ring around it (clamped to the domain) into LDS;
+// the second reads its neighbours of the first colour from there.  Each cell
+// update is gs_update on the same operands as in the two k_ssweep launches,
+// so zout is bitwise what they leave.  Used on the coarse split levels, where
+// a half-sweep is short and launches, not bytes, set the time.
+#define P2_TI 8
+#define P2_TJ 4
+#define P2_NHMAX 128
+template <int MODE1, bool NAT, int NT>
+__global__ void __launch_bounds__(NT) k_ssweep2(const float *__restrict__ zin, float *__restrict__ zout,
+                                                 const float *__restrict__ zc, const float *__restrict__ r,
+                                                 const uint16_t *__restrict__ code, SGeo g, int cuA)
+{
+    constexpr int HJ = P2_TJ + 2, HR = (P2_TI + 2) * HJ;
+    __shared__ float4 A4[HR * P2_NHMAX / 4];   // first colour after its half-sweep, tile + ring rows
+    float *const A = reinterpret_cast<float *>(A4);
+    const int tpr = g.ny / P2_TJ, ntiles = (g.nx / P2_TI) * tpr;
+    int b = (int)blockIdx.x;
+    if ((ntiles & 7) == 0) b = (b & 7) * (ntiles >> 3) + (b >> 3);   // an XCD's blocks take adjacent tiles
+    const int i0 = (b / tpr) * P2_TI, j0 = (b % tpr) * P2_TJ;
+    const int cuB = cuA ^ 1, nh = g.nh, nq = g.nq, lq = g.lq;
+    const uint32_t hA = (uint32_t)cuA * g.half, hB = (uint32_t)cuB * g.half;
+    auto zq = [&](uint32_t h, int ii, int jj, int m0, float v[4]) {
+        if (MODE1 == SWEEP_ZERO) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[e] = 0.0f;
+            return;
+        }
+        ldn<float, 4>(zin + h + g.row(ii, jj) + m0, v);
+        if (MODE1 == SWEEP_PROLONG) {
+            float pc[4];
+            ldn<float, 4>(zc + cidx(ii >> 1, jj >> 1, m0, g.ny >> 1, nh), pc);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[e] = v[e] + pc[e];
+        }
+    };
+    // first half-sweep: colour cuA on the tile and its ring
+    for (int u = (int)threadIdx.x; u < HR * nq; u += NT) {
+        const int hr = u >> lq, m0 = 4 * (u & (nq - 1));
+        const int hi = hr / HJ, hj = hr - hi * HJ;
+        const int i = i0 - 1 + hi, j = j0 - 1 + hj;
+        if (i < 0 || i >= g.nx || j < 0 || j >= g.ny) continue;
+        const int o = (cuA - i - j) & 1;
+        float c[4], w[4], xp[4], xm[4], yp[4], ym[4];
+        zq(hA, i, j, m0, c);
+        zq(hB, i, j, m0, w);
+        zq(hB, min(i + 1, g.nx - 1), j, m0, xp);
+        zq(hB, max(i - 1, 0), j, m0, xm);
+        zq(hB, i, min(j + 1, g.ny - 1), m0, yp);
+        zq(hB, i, max(j - 1, 0), m0, ym);
+        float edge = 0.0f;   // outside the domain: stencil class 0, unused
+        if (MODE1 != SWEEP_ZERO) {
+            const int me = o == 0 ? m0 - 1 : m0 + 4;
+            if (me >= 0 && me < nh) {
+                edge = zin[hB + g.row(i, j) + me];
+                if (MODE1 == SWEEP_PROLONG) edge = edge + zc[cidx(i >> 1, j >> 1, me, g.ny >> 1, nh)];
+            }
+        }
+        const uint32_t base = g.row(i, j) + m0;
+        uint32_t cd[4];
+        codesn<4>(code, hA + base, cd);
+        float rv[4], out[4];
+        ldn<float, 4>(r + hA + base, rv);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            float kp, km;
+            if (o == 0) { kp = w[e]; km = e > 0 ? w[e - 1] : edge; }
+            else { km = w[e]; kp = e < 3 ? w[e + 1] : edge; }
+            out[e] = gs_update<MODE1 != SWEEP_PROLONG>(cd[e], c[e], rv[e], xp[e], xm[e], yp[e], ym[e], kp, km);
+        }
+        A4[(hr * nh + m0) >> 2] = make_float4(out[0], out[1], out[2], out[3]);
+    }
+    __syncthreads();
+    // second half-sweep: colour cuB on the tile, its cuA neighbours from LDS
+    for (int u = (int)threadIdx.x; u < P2_TI * P2_TJ * nq; u += NT) {
+        const int tr = u >> lq, m0 = 4 * (u & (nq - 1));
+        const int ti = tr / P2_TJ, tj = tr - ti * P2_TJ;
+        const int i = i0 + ti, j = j0 + tj;
+        const int o = (cuB - i - j) & 1;
+        auto arow = [&](int ii, int jj) -> const float * { return A + ((ii - i0 + 1) * HJ + (jj - j0 + 1)) * nh; };
+        auto a4 = [&](const float *rowp, float v[4]) {
+            const float4 t = *reinterpret_cast<const float4 *>(rowp + m0);
+            v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+        };
+        const float *Ac = arow(i, j);
+        float c[4], w[4], xp[4], xm[4], yp[4], ym[4];
+        a4(Ac, w);
+        a4(arow(min(i + 1, g.nx - 1), j), xp);
+        a4(arow(max(i - 1, 0), j), xm);
+        a4(arow(i, min(j + 1, g.ny - 1)), yp);
+        a4(arow(i, max(j - 1, 0)), ym);
+        const int me = o == 0 ? m0 - 1 : m0 + 4;
+        const float edge = (me >= 0 && me < nh) ? Ac[me] : 0.0f;
+        if (MODE1 == SWEEP_ZERO) {   // SWEEP_PLAIN_OWN0
+#pragma unroll
+            for (int e = 0; e < 4; ++e) c[e] = 0.0f;
+        } else {
+            zq(hB, i, j, m0, c);
+        }
+        const uint32_t base = g.row(i, j) + m0;
+        uint32_t cd[4];
+        codesn<4>(code, hB + base, cd);
+        float rv[4], out[4];
+        ldn<float, 4>(r + hB + base, rv);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            float kp, km;
+            if (o == 0) { kp = w[e]; km = e > 0 ? w[e - 1] : edge; }
+            else { km = w[e]; kp = e < 3 ? w[e + 1] : edge; }
+            out[e] = gs_update<true>(cd[e], c[e], rv[e], xp[e], xm[e], yp[e], ym[e], kp, km);
+        }
+        if (NAT) {
+            float nat[8];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) { nat[2 * e + o] = out[e]; nat[2 * e + 1 - o] = w[e]; }
+            stn<float, 8>(zout + cidx(i, j, 2 * m0, g.ny, 2 * nh), nat);
+        } else {
+            stn<float, 4>(zout + hB + base, out);
+            stn<float, 4>(zout + hA + base, w);
+        }
+    }
+}
+
 // Gather restriction (mgpcg.py:94-101) from split level 0 into level 1
 // (reference layout), four coarse cells per thread: the children of coarse
 // (I, J, K) are, per child row (2I+a, 2J+b), the cells m = K of both halves.
@@ -626,6 +755,42 @@ int split_sweep(hipStream_t s, const Level &L0, const float *zin, float *zout, c
     else if (natural_out) sweep_launch<SWEEP_PLAIN, SWEEP_OUT_NATURAL, float, float, float, false>(s, nb, a);
     else if (part) sweep_launch<SWEEP_PLAIN, SWEEP_OUT_DOT, float, float, float, false>(s, nb, a);
     else sweep_launch<SWEEP_PLAIN, SWEEP_OUT_STORE, float, float, float, false>(s, nb, a);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+template <int NT>
+static void pair_launch(hipStream_t s, unsigned nb, int mode, bool nat, const float *zin, float *zout, const float *zc,
+                 const float *r, const uint16_t *code, const SGeo &g, int cu)
+{
+    if (mode == SWEEP_ZERO && !nat) k_ssweep2<SWEEP_ZERO, false, NT><<<nb, NT, 0, s>>>(zin, zout, zc, r, code, g, cu);
+    else if (mode == SWEEP_PROLONG && !nat) k_ssweep2<SWEEP_PROLONG, false, NT><<<nb, NT, 0, s>>>(zin, zout, zc, r, code, g, cu);
+    else if (mode == SWEEP_PLAIN && !nat) k_ssweep2<SWEEP_PLAIN, false, NT><<<nb, NT, 0, s>>>(zin, zout, zc, r, code, g, cu);
+    else if (mode == SWEEP_PLAIN) k_ssweep2<SWEEP_PLAIN, true, NT><<<nb, NT, 0, s>>>(zin, zout, zc, r, code, g, cu);
+    else if (mode == SWEEP_PROLONG) k_ssweep2<SWEEP_PROLONG, true, NT><<<nb, NT, 0, s>>>(zin, zout, zc, r, code, g, cu);
+    else k_ssweep2<SWEEP_ZERO, true, NT><<<nb, NT, 0, s>>>(zin, zout, zc, r, code, g, cu);
+}
+
+bool split_pair_supported(const Level &L)
+{
+    const SGeo g = make_sgeo(L.nx, L.ny, L.nz);
+    return split_supported(L) && L.nx % P2_TI == 0 && L.ny % P2_TJ == 0 && g.lq >= 0 && g.nh <= P2_NHMAX;
+}
+
+int split_sweep_pair(hipStream_t s, const Level &L, const float *zin, float *zout, const float *zc, const float *r,
+                     const uint16_t *code, int phase, int mode, bool natural_out)
+{
+    FLUB_REQUIRE(split_pair_supported(L), FLUB_ESHAPE, "sweep pair: level %dx%dx%d not supported", L.nx, L.ny, L.nz);
+    FLUB_REQUIRE(zin != zout || mode == SWEEP_ZERO, FLUB_EINVAL, "sweep pair: zin and zout must differ");
+    FLUB_REQUIRE(mode == SWEEP_ZERO || mode == SWEEP_PLAIN || mode == SWEEP_PROLONG, FLUB_EINVAL,
+                 "sweep pair: bad mode %d", mode);
+    const SGeo g = make_sgeo(L.nx, L.ny, L.nz);
+    const unsigned nb = (unsigned)((L.nx / P2_TI) * (L.ny / P2_TJ));
+    const int cu = phase ^ 1;
+    static const int nt = getenv("FLUB_PAIR_NT") ? atoi(getenv("FLUB_PAIR_NT")) : 1024;
+    if (nt == 256) pair_launch<256>(s, nb, mode, natural_out, zin, zout, zc, r, code, g, cu);
+    else if (nt == 512) pair_launch<512>(s, nb, mode, natural_out, zin, zout, zc, r, code, g, cu);
+    else pair_launch<1024>(s, nb, mode, natural_out, zin, zout, zc, r, code, g, cu);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
 }

@@ -203,18 +203,20 @@ def test_split_vcycle_bitexact(shape, levels, oracle):
     assert bits_equal(zd.numpy(), orc.get("z", 0))
 
 
-@pytest.mark.parametrize("graph", [False, True])
+@pytest.mark.parametrize("variant", [None, "FLUB_SPLIT_GRAPH", "FLUB_NO_PAIR"])
 @pytest.mark.parametrize("shape,levels", [((64, 64, 64), 4), ((64, 32, 48), 3), ((32, 48, 64), 4)])
-def test_split_coarse_vcycle_bitexact(shape, levels, graph, oracle, monkeypatch):
+def test_split_coarse_vcycle_bitexact(shape, levels, variant, oracle, monkeypatch):
     """The split V-cycle with every level but the bottom colour-split (the
     layout grids of 2^20 cells and more use on their coarse levels), forced
     down to small grids: bitwise the reference V-cycle, launched kernel by
-    kernel and replayed from its captured graph."""
+    kernel with the coarse half-sweeps in pairs (levels whose shape allows it;
+    (64, 32, 48) has none), replayed from its captured graph, and with one
+    launch per half-sweep."""
     import ctypes
     from flub_amd import _lib
     monkeypatch.setenv("FLUB_SPLIT_MIN_CELLS", "1")
-    if graph:
-        monkeypatch.setenv("FLUB_SPLIT_GRAPH", "1")
+    if variant:
+        monkeypatch.setenv(variant, "1")
     tag = random_tags(shape, 5, (0.1, 0.85, 0.05))
     r = np.random.default_rng(13).normal(0, 1, shape).astype(np.float32)
     r[tag != 1] = 0.0
@@ -229,15 +231,17 @@ def test_split_coarse_vcycle_bitexact(shape, levels, graph, oracle, monkeypatch)
     assert bits_equal(zd.numpy(), orc.get("z", 0))
 
 
-@pytest.mark.parametrize("variant", ["FLUB_SPLIT_GRAPH", "FLUB_RTR_COPY"])
+@pytest.mark.parametrize("variant", ["FLUB_SPLIT_GRAPH", "FLUB_RTR_COPY", "FLUB_NO_PAIR"])
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
 def test_split_launch_variants_bitwise(precision, variant, monkeypatch):
     """How the fused solves queue their work changes no value: replaying the
-    captured split V-cycle instead of launching it kernel by kernel, or
-    copying the residual norm to the host instead of storing it there from
-    the finish kernel, gives bitwise the same pressure in the same
-    iterations."""
+    captured split V-cycle instead of launching it kernel by kernel, copying
+    the residual norm to the host instead of storing it there from the finish
+    kernel, or one launch per coarse half-sweep instead of pairs, gives
+    bitwise the same pressure in the same iterations (coarse levels split as
+    on large grids)."""
     s = scenes.dam_break_spheres(64, 1)
+    monkeypatch.setenv("FLUB_SPLIT_MIN_CELLS", "1")
     res = []
     for on in (False, True):
         if on:
