@@ -43,8 +43,8 @@ struct SplitBuf {
     // split stencil codes of the split levels: lcode[0] = code; levels
     // 1 .. nsplit-1 keep their r, zpre and z in the level's own buffers
     uint16_t *lcode[MAXL];
-    // levels 1 .. nsplit-1 that sweep in pairs (split_sweep_pair): a third z
-    // buffer of the level (nullptr: one launch per half-sweep)
+    // split levels that sweep in pairs (split_sweep_pair): a third z buffer
+    // of the level (nullptr: one launch per half-sweep); level 0 only in FP32
     float *zt[MAXL];
 };
 
