@@ -552,7 +552,20 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
         if (split && l < h->nsplit) {   // level 0 in sp.*, the next split levels in their own buffers
             float *R = l == 0 ? S.r : F.r, *ZP = l == 0 ? S.zpre : F.zpre;
             const uint16_t *CD = S.lcode[l];
-            if (S.zt[l] && s >= 1) {   // the sweeps in pairs, alternating with zt, the last pair into ZP
+            if (l == 0 && h->pair_l0) {
+                // the first two half-sweeps in one launch (the second colour
+                // reads the first from LDS instead of HBM), the others one
+                // launch each: on level 0 a pair of plain sweeps streams
+                // slower than two single launches
+                TRY(split_sweep_pair(h->s, F, nullptr, ZP, nullptr, R, CD, 0, SWEEP_ZERO));
+                for (int it = 1; it < s; ++it) {
+                    TRY(split_sweep(h->s, F, ZP, ZP, nullptr, R, CD, 0, SWEEP_PLAIN, nullptr, nullptr));
+                    TRY(split_sweep(h->s, F, ZP, ZP, nullptr, R, CD, 1, SWEEP_PLAIN, nullptr, nullptr));
+                }
+                TRY(split_restrict(h->s, F, R, ZP, CD, h->lv[l + 1], l + 1 < h->nsplit));
+                continue;
+            }
+            if (l > 0 && S.zt[l] && s >= 1) {   // the sweeps in pairs, alternating with zt, the last pair into ZP
                 const float *in = nullptr;
                 for (int it = 0; it < s; ++it) {
                     float *out = ((s - 1 - it) & 1) ? S.zt[l] : ZP;
@@ -637,19 +650,16 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
                 }
                 continue;
             }
-            if (l == 0 && S.zt[0] && s >= 2) {
-                // pairs from ZP through zt and Z, the last one into Z; the last
-                // two half-sweeps one launch each, in place, the final one with
-                // the partials of z.r in the single sweep's order
-                const float *in = ZP;
-                for (int it = 0; it + 1 < s; ++it) {
-                    float *out = ((s - 2 - it) & 1) ? S.zt[0] : Z;
-                    TRY(split_sweep_pair(h->s, F, in, out, it == 0 ? zc : nullptr, R, CD, 1,
-                                         it == 0 ? SWEEP_PROLONG : SWEEP_PLAIN));
-                    in = out;
+            if (l == 0 && h->pair_l0 && s >= 2) {
+                // the prolongating pair from ZP into Z, then one launch per
+                // half-sweep in place, the final one with the partials of z.r
+                TRY(split_sweep_pair(h->s, F, ZP, Z, zc, R, CD, 1, SWEEP_PROLONG));
+                for (int it = 1; it < s; ++it) {
+                    const bool last = it + 1 == s;
+                    TRY(split_sweep(h->s, F, Z, Z, nullptr, R, CD, 1, SWEEP_PLAIN, nullptr, nullptr));
+                    TRY(split_sweep(h->s, F, Z, Z, nullptr, R, CD, 0, SWEEP_PLAIN, last ? h->part : nullptr,
+                                    last ? &h->vc_dot_nblk : nullptr));
                 }
-                TRY(split_sweep(h->s, F, Z, Z, nullptr, R, CD, 1, SWEEP_PLAIN, nullptr, nullptr));
-                TRY(split_sweep(h->s, F, Z, Z, nullptr, R, CD, 0, SWEEP_PLAIN, h->part, &h->vc_dot_nblk));
                 continue;
             }
             for (int it = 0; it < s; ++it) {
@@ -844,9 +854,10 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
             ALLOC(h->sp.x, n0 * 4);
             ALLOC(h->sp.code, n0 * 2);
             h->sp.lcode[0] = h->sp.code;
-            if (precision == FLUB_PREC_FP32 && split_pair_supported(h->lv[0]) && getenv("FLUB_NO_PAIR") == nullptr &&
-                getenv("FLUB_NO_PAIR_L0") == nullptr)
-                ALLOC(h->sp.zt[0], n0 * 4);
+            // (the 16-bit sweeps of the mixed V-cycle stream better one launch
+            // each: as pairs they measured slower)
+            h->pair_l0 = precision == FLUB_PREC_FP32 && split_pair_supported(h->lv[0]) &&
+                         getenv("FLUB_NO_PAIR") == nullptr && getenv("FLUB_NO_PAIR_L0") == nullptr;
             // The next levels are colour-split too while they hold at least
             // min_cells (default 2^20): there one launch per half-sweep on the
             // split layout streams faster than the blocked natural-layout
@@ -891,8 +902,7 @@ FLUB_API int flub_mgpcg_destroy(flub_mgpcg *h)
     hipFree(h->tb_order);
     hipFree(h->sp.r); hipFree(h->sp.z[0]); hipFree(h->sp.z[1]); hipFree(h->sp.zpre); hipFree(h->sp.x);
     hipFree(h->sp.code); hipFree(h->sp.rh); hipFree(h->sp.zh); hipFree(h->sp.zw); hipFree(h->sp.b);
-    for (int l = 1; l < MAXL; ++l) hipFree(h->sp.lcode[l]);
-    for (int l = 0; l < MAXL; ++l) hipFree(h->sp.zt[l]);
+    for (int l = 1; l < MAXL; ++l) { hipFree(h->sp.lcode[l]); hipFree(h->sp.zt[l]); }
     if (h->ev) hipEventDestroy(h->ev);
     if (h->hsc) hipHostFree(h->hsc);
     if (h->cap) hipStreamDestroy(h->cap);

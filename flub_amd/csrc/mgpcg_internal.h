@@ -43,8 +43,8 @@ struct SplitBuf {
     // split stencil codes of the split levels: lcode[0] = code; levels
     // 1 .. nsplit-1 keep their r, zpre and z in the level's own buffers
     uint16_t *lcode[MAXL];
-    // split levels that sweep in pairs (split_sweep_pair): a third z buffer
-    // of the level (nullptr: one launch per half-sweep); level 0 only in FP32
+    // levels 1 .. nsplit-1 that sweep in pairs (split_sweep_pair): a third z
+    // buffer of the level (nullptr: one launch per half-sweep)
     float *zt[MAXL];
 };
 
@@ -82,6 +82,7 @@ struct flub_mgpcg {
     bool x_only;       // the solve's caller reads only x afterwards (the fused projection):
                        // r, z, p and Ap are left in the solver's internal layout
     bool presplit;     // the fused projection wrote r (and zero x) in the split layout already
+    bool pair_l0;      // FP32 split level 0: the first pre- and post-smoothing half-sweeps in pairs
     SplitBuf sp;
     MixedState *mx;
 };

@@ -253,7 +253,9 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
 // the second reads its neighbours of the first colour from there.  Each cell
 // update is gs_update on the same operands as in the two k_ssweep launches,
 // so zout is bitwise what they leave.  Used on the coarse split levels, where
-// a half-sweep is short and launches, not bytes, set the time.
+// a half-sweep is short and launches, not bytes, set the time, and for the
+// first pre- and post-smoothing pairs of the FP32 level 0 (the second sweep of
+// each reads the first one's colour from LDS instead of HBM).
 #define P2_TI 8
 #define P2_TJ 4
 #define P2_NHMAX 128

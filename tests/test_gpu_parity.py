@@ -232,7 +232,7 @@ def test_split_coarse_vcycle_bitexact(shape, levels, variant, oracle, monkeypatc
 
 
 @pytest.mark.parametrize("variant", ["FLUB_SPLIT_GRAPH", "FLUB_RTR_COPY", "FLUB_NO_PAIR"])
-@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+@pytest.mark.parametrize("precision", ["fp32", "fp16", "bf16"])
 def test_split_launch_variants_bitwise(precision, variant, monkeypatch):
     """How the fused solves queue their work changes no value: replaying the
     captured split V-cycle instead of launching it kernel by kernel, copying
