@@ -539,14 +539,17 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
         if (l == 0 && split && h->prec != FLUB_PREC_FP32) {
             // FP16/BF16 pre-smoothing (z and r), the first two sweeps converting r
             SplitMixedSweep m{};
-            m.code = S.code; m.rh = S.rh; m.r32 = S.r; m.sc = h->sc; m.zin = S.zh; m.zout = S.zh;
+            // the 16-bit r of this V-cycle in its slot's half of sp.rh: the final
+            // post-sweep of the next cycle reads it as r_old
+            void *RH = static_cast<uint16_t *>(S.rh) + (size_t)zslot * F.n;
+            m.code = S.code; m.rh = RH; m.r32 = S.r; m.sc = h->sc; m.zin = S.zh; m.zout = S.zh;
             for (int it = 0; it < s; ++it)
                 for (int ph = 0; ph < 2; ++ph) {
                     m.phase = ph;
                     m.kind = it == 0 ? (ph == 0 ? SMX_PRE_ZERO_SRC : SMX_PRE_SRC) : SMX_PRE;
                     TRY(split_sweep_mixed(h->s, F, h->prec, m));
                 }
-            TRY(split_restrict_mixed(h->s, F, h->prec, S.rh, S.zh, S.code, h->lv[1], h->nsplit > 1));
+            TRY(split_restrict_mixed(h->s, F, h->prec, RH, S.zh, S.code, h->lv[1], h->nsplit > 1));
             continue;
         }
         if (split && l < h->nsplit) {   // level 0 in sp.*, the next split levels in their own buffers
@@ -620,7 +623,9 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
             // sweep also stores 1/scale and the partials of (z/scale).r, .q
             SplitMixedSweep m{};
             float *Z = S.z[zslot];
-            m.code = S.code; m.rh = S.rh; m.r32 = S.r; m.q32 = h->Ap; m.sc = h->sc;
+            m.code = S.code; m.rh = static_cast<uint16_t *>(S.rh) + (size_t)zslot * F.n; m.r32 = S.r; m.sc = h->sc;
+            m.rold = static_cast<const uint16_t *>(S.rh) + (size_t)(zslot ^ 1) * F.n;
+            m.zinv_old = h->sc + SC_ZINV0 + (zslot ^ 1);
             for (int it = 0; it < s; ++it)
                 for (int ph = 1; ph >= 0; --ph) {
                     const bool last = it + 1 == s && ph == 0;
@@ -873,10 +878,15 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
                 ++h->nsplit;
             }
             if (precision != FLUB_PREC_FP32) {
-                ALLOC(h->sp.rh, n0 * 2);
+                ALLOC(h->sp.rh, n0 * 2 * 2);   // one half per z slot
                 ALLOC(h->sp.zh, n0 * 2);
                 ALLOC(h->sp.zw, n0 * 4);
                 ALLOC(h->sp.b, n0 * 4);
+                // r_old of a solve's first final post-sweep is read but not used
+                // (its beta is 0): zeros keep those partials finite
+                hipError_t e_ = hipMemsetAsync(h->sp.rh, 0, n0 * 2 * 2, h->s);
+                if (e_ == hipSuccess) e_ = hipStreamSynchronize(h->s);
+                if (e_ != hipSuccess) { flub_set_error("init: %s", hipGetErrorString(e_)); rc = (int)e_; goto fail; }
             }
         }
     }

@@ -166,11 +166,14 @@ __global__ void __launch_bounds__(RBS) k_mx_finish_r(const double *__restrict__ 
     }
 }
 
-// Polak-Ribiere update from the V-cycle's partials: zr = z.r, zq = z.q;
-// beta = -alpha * zq / (zr_old + eps) (q is -A p of the step, alpha the
-// reference's sign), zr_old = zr.  First call (first=1): only zr_old = zr.
+// Polak-Ribiere update from the V-cycle's partials: zr = z.r and, second,
+// zq = z.q: beta = -alpha * zq / (zr_old + eps) (q is -A p of the step, alpha
+// the reference's sign); with rform the second partials are z.r_old (r of the
+// previous iteration): beta = (zr - z.r_old) / (zr_old + eps), the same
+// z.(r_new - r_old) / zr_old.  zr_old = zr.  First call (first=1): only
+// zr_old = zr.
 __global__ void __launch_bounds__(RBS) k_mx_finish_beta(const double *__restrict__ part, int nblk,
-                                                        double *__restrict__ sc, double eps, int first)
+                                                        double *__restrict__ sc, double eps, int first, int rform)
 {
     __shared__ double lds[RBS / 64];
     double a1 = 0.0, a2 = 0.0;
@@ -184,7 +187,7 @@ __global__ void __launch_bounds__(RBS) k_mx_finish_beta(const double *__restrict
     const double zr = block_sum<RBS>(a1, lds);
     const double zq = block_sum<RBS>(a2, lds);
     if (threadIdx.x == 0) {
-        sc[SC_BETA] = first ? 0.0 : -sc[SC_ALPHA] * zq / (sc[SC_OLDZTR] + eps);
+        sc[SC_BETA] = first ? 0.0 : (rform ? (zr - zq) : -sc[SC_ALPHA] * zq) / (sc[SC_OLDZTR] + eps);
         sc[SC_OLDZTR] = zr;
         sc[SC_AUX0] = zr;
     }
@@ -431,7 +434,7 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
     TRYM(mg_mixed_rscale_of(h, S.r, true));
     int zs = 0;
     TRYM(mg_precondition_split(h, zs));
-    k_mx_finish_beta<<<1, RBS, 0, h->s>>>(h->part, h->vc_dot_nblk, h->sc, eps, 1);
+    k_mx_finish_beta<<<1, RBS, 0, h->s>>>(h->part, h->vc_dot_nblk, h->sc, eps, 1, 1);
     FLUB_CHECK_LAUNCH();
     FLUB_CHECK_HIP(hipEventSynchronize(h->ev));
     const double rTr0 = h->hsc[SC_RTR];
@@ -482,7 +485,7 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
         r_prev = rTr;
         if (!speculate || redo) TRYM(mg_precondition_split(h, zs ^ 1));
         zs ^= 1;
-        k_mx_finish_beta<<<1, RBS, 0, h->s>>>(h->part, h->vc_dot_nblk, h->sc, eps, 0);
+        k_mx_finish_beta<<<1, RBS, 0, h->s>>>(h->part, h->vc_dot_nblk, h->sc, eps, 0, 1);
         FLUB_CHECK_LAUNCH();
         it += 1;
     }
@@ -527,7 +530,7 @@ int mg_solve_mixed(flub_mgpcg *h, int max_iters, double eps, double abs_tol, dou
     const double tol = fmax(abs_tol, rTr0 * rel_tol);
     float *const p_home = h->p;
     TRYM(mg_vcycle_mixed(h, smoother));
-    k_mx_finish_beta<<<1, RBS, 0, h->s>>>(h->part, h->vc_dot_nblk, h->sc, eps, 1);
+    k_mx_finish_beta<<<1, RBS, 0, h->s>>>(h->part, h->vc_dot_nblk, h->sc, eps, 1, 0);
     FLUB_CHECK_LAUNCH();
     const int cap = max_iters < 0 ? SOLVE_SAFETY_CAP : max_iters;
     int it = 0, status = FLUB_OK;
@@ -555,7 +558,7 @@ int mg_solve_mixed(flub_mgpcg *h, int max_iters, double eps, double abs_tol, dou
             ++replacements;
         }
         TRYM(mg_vcycle_mixed(h, smoother));
-        k_mx_finish_beta<<<1, RBS, 0, h->s>>>(h->part, h->vc_dot_nblk, h->sc, eps, 0);
+        k_mx_finish_beta<<<1, RBS, 0, h->s>>>(h->part, h->vc_dot_nblk, h->sc, eps, 0, 0);
         FLUB_CHECK_LAUNCH();
         it += 1;
     }

@@ -117,7 +117,7 @@ int split_sweep_pair(hipStream_t s, const Level &L, const float *zin, float *zou
 // prec): pre-smoothing stores z and r as T (the first two sweeps take r from
 // the FP32 CG residual times the storage scale), post-smoothing keeps z in
 // FP32; the last one writes the CG's z = z / scale and the partials of z.r and
-// z.q.
+// z.r_old (r_old: the r the previous V-cycle stored).
 enum { SMX_PRE_ZERO_SRC = 0, SMX_PRE_SRC, SMX_PRE, SMX_POST_PROLONG, SMX_POST, SMX_FINAL };
 struct SplitMixedSweep {
     int kind, phase;
@@ -126,7 +126,9 @@ struct SplitMixedSweep {
     const float *zc;       // level-1 z in the reference layout (post-prolong)
     void *rh;              // T split r
     const uint16_t *code;
-    const float *r32, *q32;
+    const float *r32;
+    const void *rold;      // FINAL: T split r of the previous V-cycle (its r times its scale)
+    const double *zinv_old;  // FINAL: 1/scale of rold
     double *zinv;          // FINAL: receives 1/scale (zout keeps z * scale)
     const double *sc;
     double *part;

@@ -81,10 +81,11 @@ struct SSweepArgs {
     void *r;               // TR split r of the V-cycle (SRC32: written)
     const uint16_t *code;  // split stencil codes
     const float *r32;      // SRC32: the FP32 CG residual (split)
-    const float *q32;      // FINAL: q = -A p of the CG step (split), for z.q
+    const void *rold;      // FINAL: TR split r of the previous V-cycle (r_old times its scale)
+    const double *zinv_old;  // FINAL: 1/scale of rold
     double *zinv;          // FINAL: receives 1/scale (the z buffer holds z * scale)
     const double *sc;      // SC_RSCALE: storage scale of r (SRC32, FINAL)
-    double *part;          // DOT: partials of z.r; FINAL: z.r then z.q
+    double *part;          // DOT: partials of z.r; FINAL: z.r then z.r_old
     SGeo g;
     int cu;                // colour updated (the half-sweep phase ^ 1)
 };
@@ -99,7 +100,7 @@ struct SSweepArgs {
 // the storage scale and stores it (rounded to TR, and continues with the
 // rounded values).  OUT: STORE; DOT accumulates z.r over both colours; FINAL
 // stores like STORE (z stays in storage scale; 1/scale goes to zinv) and
-// accumulates (z / scale).(r / scale) (r as stored, TR) and (z / scale).q32
+// accumulates (z / scale).(r / scale) (r as stored, TR) and (z / scale).r_old
 // over both colours; NATURAL stores both colours in the reference layout.
 template <int MODE, int OUT, typename TZI, typename TZO, typename TR, bool SRC32, int CPT>
 __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
@@ -198,7 +199,12 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
             ldn<TR, CPT>(reinterpret_cast<const TR *>(a.r) + oth + base, rw);
 #pragma unroll
             for (int e = 0; e < CPT; ++e) { ro[e] = rv[e] * inv; rw[e] = rw[e] * inv; }
-            ldn<float, CPT>(a.q32 + own + base, qo); ldn<float, CPT>(a.q32 + oth + base, qw);
+            // r_old as the previous V-cycle stored it, over its own scale
+            const float iold = (float)a.zinv_old[0];
+            ldn<TR, CPT>(reinterpret_cast<const TR *>(a.rold) + own + base, qo);
+            ldn<TR, CPT>(reinterpret_cast<const TR *>(a.rold) + oth + base, qw);
+#pragma unroll
+            for (int e = 0; e < CPT; ++e) { qo[e] = qo[e] * iold; qw[e] = qw[e] * iold; }
 #pragma unroll
             for (int e = 0; e < CPT; ++e) {
                 acc1 += (double)zo[e] * (double)ro[e] + (double)zw[e] * (double)rw[e];
@@ -693,7 +699,7 @@ int split_sweep_mixed_t(hipStream_t s, const Level &L0, const SplitMixedSweep &m
 {
     SSweepArgs a{};
     a.zin = m.zin; a.zout = m.zout; a.zc = m.zc; a.r = m.rh; a.code = m.code;
-    a.r32 = m.r32; a.q32 = m.q32; a.zinv = m.zinv; a.sc = m.sc; a.part = m.part;
+    a.r32 = m.r32; a.rold = m.rold; a.zinv_old = m.zinv_old; a.zinv = m.zinv; a.sc = m.sc; a.part = m.part;
     a.cu = m.phase ^ 1;
     // the pre-smoothing sweeps (16-bit z and r) move 8 cells per thread, in
     // 16-byte vectors, when the half-rows allow it
