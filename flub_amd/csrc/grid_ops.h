@@ -8,12 +8,15 @@ int flub_divergence_fused_u8(const float *u, const float *v, const float *w, flo
                              const float *ur, const float *vr, const float *wr, const uint8_t *tag,
                              int nx, int ny, int nz, float s, void *stream);
 // The fused divergence that also writes the solver's right-hand side
-// r = (FLUID ? d * k_r : 0) in the colour-split layout (div may be NULL);
-// divergence_split_ok tells whether the shapes and pointers allow it.
+// r = (FLUID ? d * k_r : 0) in the colour-split layout (div may be NULL), and
+// when set the same values into rcopy and zeros into zero0 and zero1 (split
+// arrays of the grid's size); divergence_split_ok tells whether the shapes and
+// pointers allow it.
 bool divergence_split_ok(const float *u, const float *v, const float *div, const uint8_t *tag, int nz);
 int divergence_fused_split(const float *u, const float *v, const float *w, float *div, const float *ur,
                            const float *vr, const float *wr, const uint8_t *tag, int nx, int ny, int nz, float s,
-                           float *rsplit, float k_r, void *stream);
+                           float *rsplit, float k_r, void *stream, float *rcopy = nullptr, float *zero0 = nullptr,
+                           float *zero1 = nullptr);
 // subtract_gradient_tag reading the solver's x and applying the reference's
 // pressure scale chain (x*h)*(1/h) on the fly.
 int flub_subtract_gradient_x_u8(float *uf, const float *u_rigid, const float *x, const uint8_t *tag,

@@ -105,14 +105,18 @@ __device__ __forceinline__ void ldt4(const int32_t *p, int v[4])
 // well as, div it writes the solver's right-hand side r = (FLUID ? d * k : +0)
 // with d the fused value (the scale copy and mask_interior of the solve's
 // init), in the colour-split layout: cell (i, j, k) in half (i + j + k) & 1
-// at (i * ny + j) * nz/2 + k/2.
+// at (i * ny + j) * nz/2 + k/2; the same values into rcopy and +0 into zero0
+// and zero1 at the same positions when they are set (the mixed solve's copy of
+// the right-hand side; the solver's x and p).
 template <typename TagT, bool FUSE, bool RSPLIT = false>
 __global__ void __launch_bounds__(256) k_divergence4(const float *__restrict__ u, const float *__restrict__ v,
                                                      const float *__restrict__ w, float *__restrict__ div,
                                                      const float *__restrict__ ur, const float *__restrict__ vr,
                                                      const float *__restrict__ wr, const TagT *__restrict__ tag,
                                                      int nx, int ny, int nz, float s, float *__restrict__ rsplit = nullptr,
-                                                     float k_r = 0.0f)
+                                                     float k_r = 0.0f, float *__restrict__ rcopy = nullptr,
+                                                     float *__restrict__ zero0 = nullptr,
+                                                     float *__restrict__ zero1 = nullptr)
 {
     const uint32_t nq = (uint32_t)nz >> 2;
     const uint32_t tq = blockIdx.x * 256u + threadIdx.x;
@@ -170,6 +174,19 @@ __global__ void __launch_bounds__(256) k_divergence4(const float *__restrict__ u
         const uint32_t b = row * (uint32_t)(nz >> 1) + (uint32_t)(k0 >> 1);
         *reinterpret_cast<float2 *>(rsplit + c0 * half + b) = make_float2(rv[0], rv[2]);
         *reinterpret_cast<float2 *>(rsplit + (c0 ^ 1u) * half + b) = make_float2(rv[1], rv[3]);
+        if (rcopy) {
+            *reinterpret_cast<float2 *>(rcopy + c0 * half + b) = make_float2(rv[0], rv[2]);
+            *reinterpret_cast<float2 *>(rcopy + (c0 ^ 1u) * half + b) = make_float2(rv[1], rv[3]);
+        }
+        const float2 z2 = make_float2(0.0f, 0.0f);
+        if (zero0) {
+            *reinterpret_cast<float2 *>(zero0 + c0 * half + b) = z2;
+            *reinterpret_cast<float2 *>(zero0 + (c0 ^ 1u) * half + b) = z2;
+        }
+        if (zero1) {
+            *reinterpret_cast<float2 *>(zero1 + c0 * half + b) = z2;
+            *reinterpret_cast<float2 *>(zero1 + (c0 ^ 1u) * half + b) = z2;
+        }
     }
 }
 
@@ -222,14 +239,14 @@ bool divergence_split_ok(const float *u, const float *v, const float *div, const
 
 int divergence_fused_split(const float *u, const float *v, const float *w, float *div, const float *ur,
                            const float *vr, const float *wr, const uint8_t *tag, int nx, int ny, int nz, float s,
-                           float *rsplit, float k_r, void *stream)
+                           float *rsplit, float k_r, void *stream, float *rcopy, float *zero0, float *zero1)
 {
     FLUB_REQUIRE(u && v && w && ur && vr && wr && tag && rsplit, FLUB_EINVAL, "divergence: null pointer");
     FLUB_REQUIRE(grid_ok(nx, ny, nz) && divergence_split_ok(u, v, div, tag, nz), FLUB_ESHAPE,
                  "divergence: the split right-hand side needs nz %% 8 == 0 and aligned arrays");
     const size_t n = (size_t)nx * ny * nz;
     k_divergence4<uint8_t, true, true><<<blocks_for(n / 4, 256), 256, 0, as_stream(stream)>>>(
-        u, v, w, div, ur, vr, wr, tag, nx, ny, nz, s, rsplit, k_r);
+        u, v, w, div, ur, vr, wr, tag, nx, ny, nz, s, rsplit, k_r, rcopy, zero0, zero1);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
 }

@@ -1362,15 +1362,15 @@ FLUB_API int flub_project(flub_mgpcg *h, float *u, float *v, float *w, const flo
     const float s_div = (float)(-1.0 / (double)h_cell);
     // The colour-split solve takes r from the divergence kernel directly, in
     // its layout: r = (FLUID ? d * (-h) : 0), the values init + mask_interior
-    // give; x starts at 0 in the split layout, p and the scalars as init sets
-    // them (z and Ap are written before they are read).
+    // give; the same kernel zeroes x (split layout) and p as init does, and for
+    // a mixed-precision handle also writes the copy of r its true residual
+    // uses; the scalars are zeroed as init does (z and Ap are written before
+    // they are read).
     const bool presplit = h->split && smoother == FLUB_SMOOTHER_PHASES && use_fast(h, smoother) &&
                           divergence_split_ok(u, v, div, L0.tag, nz);
     if (presplit) {
-        TRY(divergence_fused_split(u, v, w, div, ur, vr, wr, L0.tag, nx, ny, nz, s_div, h->sp.r, -h_cell, h->s));
-        k_fill_f32<<<nb256(L0.n), 256, 0, h->s>>>(h->sp.x, 0.0f, L0.n);
-        k_fill_f32<<<nb256(L0.n), 256, 0, h->s>>>(h->p, 0.0f, L0.n);
-        FLUB_CHECK_LAUNCH();
+        TRY(divergence_fused_split(u, v, w, div, ur, vr, wr, L0.tag, nx, ny, nz, s_div, h->sp.r, -h_cell, h->s,
+                                   h->prec != FLUB_PREC_FP32 ? h->sp.b : nullptr, h->sp.x, h->p));
         FLUB_CHECK_HIP(hipMemsetAsync(h->sc, 0, SC_NUM * sizeof(double), h->s));
     } else {
         float *d = div ? div : h->Ap;   // Ap is rewritten by init

@@ -426,9 +426,11 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
     const int max_replacements = 3;
     int replacements = 0;
     SplitBuf &S = h->sp;
-    if (!h->presplit) TRYM(split_convert(h->s, L0, L0.r, S.r, 4, true));
-    FLUB_CHECK_HIP(hipMemcpyAsync(S.b, S.r, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
-    if (!h->presplit) TRYM(split_convert(h->s, L0, h->x, S.x, 4, true));
+    if (!h->presplit) {   // (the fused projection's divergence kernel wrote S.r, S.b and S.x)
+        TRYM(split_convert(h->s, L0, L0.r, S.r, 4, true));
+        FLUB_CHECK_HIP(hipMemcpyAsync(S.b, S.r, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
+        TRYM(split_convert(h->s, L0, h->x, S.x, 4, true));
+    }
     // the tolerance is first needed after iteration 1: the first V-cycle runs
     // while the initial r.r travels to the host
     TRYM(mg_mixed_rscale_of(h, S.r, true));
@@ -472,10 +474,11 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
             x_pending = false;
             int nbt = 0;
             TRYM(split_true_residual(h->s, L0, S.r, S.b, S.x, S.code, h->part, &nbt));
-            k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nbt, h->sc, 0);
+            k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nbt, h->sc, 0, h->hsc_dev ? h->hsc_dev + SC_RTR : nullptr);
             FLUB_CHECK_LAUNCH();
-            double rTrue;
-            TRYM(read_sc(h, SC_RTR, &rTrue));
+            TRYM(mg_rtr_sent(h));
+            FLUB_CHECK_HIP(hipEventSynchronize(h->ev));
+            const double rTrue = h->hsc[SC_RTR];
             if (verbose) printf("  true |residual|_2=%.9g\n", sqrt(rTrue));
             if (rTrue < drift * tol || replacements >= max_replacements) { rTr = rTrue; break; }
             ++replacements;
