@@ -543,10 +543,13 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
             // post-sweep of the next cycle reads it as r_old
             void *RH = static_cast<uint16_t *>(S.rh) + (size_t)zslot * F.n;
             m.code = S.code; m.rh = RH; m.r32 = S.r; m.sc = h->sc; m.zin = S.zh; m.zout = S.zh;
+            const bool have_rh = h->rh_ready == zslot;   // the r update stored this cycle's T r
             for (int it = 0; it < s; ++it)
                 for (int ph = 0; ph < 2; ++ph) {
                     m.phase = ph;
-                    m.kind = it == 0 ? (ph == 0 ? SMX_PRE_ZERO_SRC : SMX_PRE_SRC) : SMX_PRE;
+                    m.kind = it > 0 ? SMX_PRE
+                                    : ph == 0 ? (have_rh ? SMX_PRE_ZERO_RH : SMX_PRE_ZERO_SRC)
+                                              : (have_rh ? SMX_PRE_OWN0_RH : SMX_PRE_SRC);
                     TRY(split_sweep_mixed(h->s, F, h->prec, m));
                 }
             TRY(split_restrict_mixed(h->s, F, h->prec, RH, S.zh, S.code, h->lv[1], h->nsplit > 1));
@@ -777,6 +780,7 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
     FLUB_REQUIRE(h, FLUB_EINVAL, "out of host memory");
     h->L = levels;
     h->prec = precision;
+    h->rh_ready = -1;
     h->pre_post = 6;
     h->bottom = 6;
     h->no_graph = getenv("FLUB_NO_GRAPH") != nullptr;
@@ -863,6 +867,7 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
             // each: as pairs they measured slower)
             h->pair_l0 = precision == FLUB_PREC_FP32 && split_pair_supported(h->lv[0]) &&
                          getenv("FLUB_NO_PAIR") == nullptr && getenv("FLUB_NO_PAIR_L0") == nullptr;
+            h->mx_unit = precision == FLUB_PREC_BF16 && getenv("FLUB_MX_SRC32") == nullptr;
             // The next levels are colour-split too while they hold at least
             // min_cells (default 2^20): there one launch per half-sweep on the
             // split layout streams faster than the blocked natural-layout

@@ -83,6 +83,8 @@ struct flub_mgpcg {
                        // r, z, p and Ap are left in the solver's internal layout
     bool presplit;     // the fused projection wrote r (and zero x) in the split layout already
     bool pair_l0;      // FP32 split level 0: the first pre- and post-smoothing half-sweeps in pairs
+    bool mx_unit;      // BF16 split solve: storage scale 1, the r update stores the next cycle's BF16 r
+    int rh_ready;      // z slot whose BF16 r the r update stored for the next V-cycle (-1: none)
     SplitBuf sp;
     MixedState *mx;
 };

@@ -23,11 +23,13 @@
 #include "mgpcg_internal.h"
 #include "smooth.h"
 
-// r -= alpha q; with DOX x += alpha p; partials of r.r and max|r|.
+// r -= alpha q; with DOX x += alpha p; partials of r.r and max|r|; with rh
+// also the BF16 r (storage scale 1) for the next V-cycle.
 template <bool DOX>
 __global__ void __launch_bounds__(RBS) k_mx_update(float *__restrict__ r, float *__restrict__ x,
                                                    const float *__restrict__ q, const float *__restrict__ p,
-                                                   const double *__restrict__ sc, uint32_t n, double *__restrict__ part)
+                                                   const double *__restrict__ sc, uint32_t n, double *__restrict__ part,
+                                                   __hip_bfloat16 *__restrict__ rh = nullptr)
 {
     __shared__ double lds[RBS / 64];
     const float alpha = (float)sc[SC_ALPHA];
@@ -36,6 +38,7 @@ __global__ void __launch_bounds__(RBS) k_mx_update(float *__restrict__ r, float 
     for (uint32_t t = blockIdx.x * RBS + threadIdx.x; t < n; t += gridDim.x * RBS) {
         const float rn = fmaf(-alpha, q[t], r[t]);
         r[t] = rn;
+        if (rh) rh[t] = __float2bfloat16(rn);   // the next V-cycle's BF16 r (storage scale 1)
         if (DOX) x[t] = fmaf(alpha, p[t], x[t]);
         acc += (double)rn * (double)rn;
         mx = fmaxf(mx, fabsf(rn));
@@ -138,8 +141,10 @@ __global__ void __launch_bounds__(RBS) k_mx_rnorm(const float *__restrict__ r, u
 }
 
 // Reduce (sum, max) partials: sc[RTR] = sum, sc[RSCALE] = 2^-(e+shift) with
-// 2^(e-1) < max <= 2^e (2^-shift when max == 0).  With host set, the sum also
-// goes to host[0] (pinned host memory mapped for the device).
+// 2^(e-1) < max <= 2^e (2^-shift when max == 0); shift < 0: sc[RSCALE] = 1
+// (BF16 storage, whose exponent range is FP32's: a power-of-two scale changes
+// no rounding there).  With host set, the sum also goes to host[0] (pinned
+// host memory mapped for the device).
 __global__ void __launch_bounds__(RBS) k_mx_finish_r(const double *__restrict__ part, int nblk, double *__restrict__ sc,
                                                      int shift, double *__restrict__ host = nullptr)
 {
@@ -162,7 +167,7 @@ __global__ void __launch_bounds__(RBS) k_mx_finish_r(const double *__restrict__ 
         for (int w = 0; w < RBS / 64; ++w) m = fmax(m, lm[w]);
         sc[SC_RTR] = s;
         if (host) host[0] = s;
-        sc[SC_RSCALE] = ldexp((m > 0.0 && isfinite(m)) ? ldexp(1.0, -ilogb(m) - 1) : 1.0, -shift);
+        sc[SC_RSCALE] = shift < 0 ? 1.0 : ldexp((m > 0.0 && isfinite(m)) ? ldexp(1.0, -ilogb(m) - 1) : 1.0, -shift);
     }
 }
 
@@ -254,7 +259,7 @@ int mg_mixed_rscale_of(flub_mgpcg *h, const float *r, bool to_host)
     const int nb = red_nb(n);
     k_mx_rnorm<<<nb, RBS, 0, h->s>>>(r, n, h->part);
     FLUB_CHECK_LAUNCH();
-    k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nb, h->sc, 0,
+    k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nb, h->sc, h->mx_unit ? -1 : 0,
                                        to_host && h->hsc_dev ? h->hsc_dev + SC_RTR : nullptr);
     FLUB_CHECK_LAUNCH();
     return to_host ? mg_rtr_sent(h) : FLUB_OK;
@@ -426,6 +431,7 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
     const int max_replacements = 3;
     int replacements = 0;
     SplitBuf &S = h->sp;
+    h->rh_ready = -1;
     if (!h->presplit) {   // (the fused projection's divergence kernel wrote S.r, S.b and S.x)
         TRYM(split_convert(h->s, L0, L0.r, S.r, 4, true));
         FLUB_CHECK_HIP(hipMemcpyAsync(S.b, S.r, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
@@ -455,10 +461,16 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
         TRYM(split_Ap(h->s, L0, h->p, h->Ap, S.code, h->part, &nbp));
         TRYM(mg_finish_alpha(h, nbp, eps));
         // r only: x += alpha p waits for the next p update (or a read of x)
-        k_mx_update<false><<<nb, RBS, 0, h->s>>>(S.r, nullptr, h->Ap, nullptr, h->sc, n, h->part);
+        // BF16: the update also stores the next V-cycle's r (storage scale 1)
+        // into that cycle's slot, so its first sweeps read 2 bytes of r a cell
+        __hip_bfloat16 *rh_next =
+            h->mx_unit ? static_cast<__hip_bfloat16 *>(S.rh) + (size_t)(zs ^ 1) * n : nullptr;
+        k_mx_update<false><<<nb, RBS, 0, h->s>>>(S.r, nullptr, h->Ap, nullptr, h->sc, n, h->part, rh_next);
         FLUB_CHECK_LAUNCH();
         x_pending = true;
-        k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nb, h->sc, 0, h->hsc_dev ? h->hsc_dev + SC_RTR : nullptr);
+        h->rh_ready = h->mx_unit ? (zs ^ 1) : -1;
+        k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nb, h->sc, h->mx_unit ? -1 : 0,
+                                           h->hsc_dev ? h->hsc_dev + SC_RTR : nullptr);
         FLUB_CHECK_LAUNCH();
         TRYM(mg_rtr_sent(h));
         if (speculate) TRYM(mg_precondition_split(h, zs ^ 1));
@@ -474,7 +486,9 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
             x_pending = false;
             int nbt = 0;
             TRYM(split_true_residual(h->s, L0, S.r, S.b, S.x, S.code, h->part, &nbt));
-            k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nbt, h->sc, 0, h->hsc_dev ? h->hsc_dev + SC_RTR : nullptr);
+            k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nbt, h->sc, h->mx_unit ? -1 : 0,
+                                               h->hsc_dev ? h->hsc_dev + SC_RTR : nullptr);
+            h->rh_ready = -1;   // r replaced: the next cycle converts it again
             FLUB_CHECK_LAUNCH();
             TRYM(mg_rtr_sent(h));
             FLUB_CHECK_HIP(hipEventSynchronize(h->ev));
@@ -504,6 +518,7 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
         TRYM(split_convert(h->s, L0, h->p, S.zw, 4, false));
         FLUB_CHECK_HIP(hipMemcpyAsync(h->p, S.zw, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
     }
+    h->rh_ready = -1;
     if (status == FLUB_OK && it >= cap && max_iters < 0) {
         status = FLUB_ENOTCONV;
         flub_set_error("no convergence within %d iterations", cap);

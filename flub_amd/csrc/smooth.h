@@ -117,8 +117,11 @@ int split_sweep_pair(hipStream_t s, const Level &L, const float *zin, float *zou
 // prec): pre-smoothing stores z and r as T (the first two sweeps take r from
 // the FP32 CG residual times the storage scale), post-smoothing keeps z in
 // FP32; the last one writes the CG's z = z / scale and the partials of z.r and
-// z.r_old (r_old: the r the previous V-cycle stored).
-enum { SMX_PRE_ZERO_SRC = 0, SMX_PRE_SRC, SMX_PRE, SMX_POST_PROLONG, SMX_POST, SMX_FINAL };
+// z.r_old (r_old: the r the previous V-cycle stored).  SMX_PRE_ZERO_RH and
+// SMX_PRE_OWN0_RH are the first two sweeps with the T r already stored (BF16
+// at storage scale 1, written by the r update).
+enum { SMX_PRE_ZERO_SRC = 0, SMX_PRE_SRC, SMX_PRE, SMX_POST_PROLONG, SMX_POST, SMX_FINAL, SMX_PRE_ZERO_RH,
+       SMX_PRE_OWN0_RH };
 struct SplitMixedSweep {
     int kind, phase;
     const void *zin;       // T (pre, post-prolong) or FP32 (post)

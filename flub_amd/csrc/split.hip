@@ -703,7 +703,8 @@ int split_sweep_mixed_t(hipStream_t s, const Level &L0, const SplitMixedSweep &m
     a.cu = m.phase ^ 1;
     // the pre-smoothing sweeps (16-bit z and r) move 8 cells per thread, in
     // 16-byte vectors, when the half-rows allow it
-    const bool pre = m.kind == SMX_PRE_ZERO_SRC || m.kind == SMX_PRE_SRC || m.kind == SMX_PRE;
+    const bool pre = m.kind == SMX_PRE_ZERO_SRC || m.kind == SMX_PRE_SRC || m.kind == SMX_PRE ||
+                     m.kind == SMX_PRE_ZERO_RH || m.kind == SMX_PRE_OWN0_RH;
     const int cpt = pre && (L0.nz / 2) % 8 == 0 ? 8 : 4;
     a.g = make_sgeo(L0.nx, L0.ny, L0.nz, cpt);
     const unsigned nb = grid8((uint32_t)L0.nx * L0.ny * a.g.nq);
@@ -716,6 +717,14 @@ int split_sweep_mixed_t(hipStream_t s, const Level &L0, const SplitMixedSweep &m
     case SMX_PRE_SRC:   // follows the ZERO sweep: this colour starts at 0
         if (cpt == 8) sweep_launch<SWEEP_PLAIN_OWN0, SWEEP_OUT_STORE, T, T, T, true, 8>(s, nb, a);
         else sweep_launch<SWEEP_PLAIN_OWN0, SWEEP_OUT_STORE, T, T, T, true>(s, nb, a);
+        break;
+    case SMX_PRE_ZERO_RH:
+        if (cpt == 8) sweep_launch<SWEEP_ZERO, SWEEP_OUT_STORE, T, T, T, false, 8>(s, nb, a);
+        else sweep_launch<SWEEP_ZERO, SWEEP_OUT_STORE, T, T, T, false>(s, nb, a);
+        break;
+    case SMX_PRE_OWN0_RH:
+        if (cpt == 8) sweep_launch<SWEEP_PLAIN_OWN0, SWEEP_OUT_STORE, T, T, T, false, 8>(s, nb, a);
+        else sweep_launch<SWEEP_PLAIN_OWN0, SWEEP_OUT_STORE, T, T, T, false>(s, nb, a);
         break;
     case SMX_PRE:
         if (cpt == 8) sweep_launch<SWEEP_PLAIN, SWEEP_OUT_STORE, T, T, T, false, 8>(s, nb, a);
