@@ -400,6 +400,34 @@ static int finish_rtr_to_host(flub_mgpcg *h, int nblk)
     return mg_rtr_sent(h);
 }
 
+double mg_host_partial_sum(const double *part, int nblk)
+{
+    double acc[RBS];
+    for (int t = 0; t < RBS; ++t) {
+        double a = 0.0;
+        for (int b = t; b < nblk; b += RBS) a += part[b];
+        acc[t] = a;
+    }
+    double s = 0.0;
+    for (int w = 0; w < RBS / 64; ++w) {
+        double v[64], nv[64];
+        for (int l = 0; l < 64; ++l) v[l] = acc[64 * w + l];
+        for (int off = 32; off > 0; off >>= 1) {   // __shfl_down: lanes past the end keep their value
+            for (int l = 0; l < 64; ++l) nv[l] = v[l] + (l + off < 64 ? v[l + off] : v[l]);
+            for (int l = 0; l < 64; ++l) v[l] = nv[l];
+        }
+        s += v[0];
+    }
+    return s;
+}
+
+int mg_rtr_to_device(flub_mgpcg *h, double rTr)
+{
+    h->hsc[SC_RTR] = rTr;
+    FLUB_CHECK_HIP(hipMemcpyAsync(h->sc + SC_RTR, h->hsc + SC_RTR, sizeof(double), hipMemcpyHostToDevice, h->s));
+    return FLUB_OK;
+}
+
 int mg_rtr_sent(flub_mgpcg *h)
 {
     if (!h->hsc_dev)
@@ -842,6 +870,15 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
         }
         if (e != hipSuccess) { flub_set_error("hipHostMalloc: %s", hipGetErrorString(e)); rc = (int)e; goto fail; }
         if (getenv("FLUB_RTR_COPY")) h->hsc_dev = nullptr;
+        if (h->hsc_dev) {   // the partials' host buffer, mapped the same way (else: finish kernels)
+            if (hipHostMalloc((void **)&h->hpart, 2 * MAXBLK * sizeof(double),
+                              hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess) {
+                if (hipHostGetDevicePointer((void **)&h->hpart_dev, h->hpart, 0) != hipSuccess) h->hpart_dev = nullptr;
+            } else {
+                (void)hipGetLastError();
+                h->hpart = nullptr;
+            }
+        }
         e = hipMemsetAsync(h->sc, 0, SC_NUM * sizeof(double), h->s);
         if (e == hipSuccess) e = hipMemsetAsync(h->x, 0, n0 * 4, h->s);
         if (e == hipSuccess) e = hipStreamSynchronize(h->s);
@@ -920,6 +957,7 @@ FLUB_API int flub_mgpcg_destroy(flub_mgpcg *h)
     for (int l = 1; l < MAXL; ++l) { hipFree(h->sp.lcode[l]); hipFree(h->sp.zt[l]); }
     if (h->ev) hipEventDestroy(h->ev);
     if (h->hsc) hipHostFree(h->hsc);
+    if (h->hpart) hipHostFree(h->hpart);
     if (h->cap) hipStreamDestroy(h->cap);
     free(h);
     return FLUB_OK;
@@ -1138,15 +1176,19 @@ static int solve_fast(flub_mgpcg *h, int max_iters, double eps, double abs_tol, 
         }
         R = h->sp.r; X = h->sp.x; Z = h->sp.z[zs];
     }
-    k_dot<<<nb, RBS, 0, h->s>>>(R, R, n, h->part);
+    // split: the r.r partials go to the mapped host buffer and the host sums
+    // them (no finish kernel on the stream)
+    const bool hsum = split && h->hpart_dev;
+    k_dot<<<nb, RBS, 0, h->s>>>(R, R, n, hsum ? h->hpart_dev : h->part);
     FLUB_CHECK_LAUNCH();
     // the tolerance is first needed after iteration 1: the first V-cycle runs
     // while the initial r.r travels to the host
-    TRY(finish_rtr_to_host(h, nb));
+    if (hsum) FLUB_CHECK_HIP(hipEventRecord(h->ev, h->s));
+    else TRY(finish_rtr_to_host(h, nb));
     TRY(split ? mg_precondition_split(h, zs) : mg_precondition(h, FLUB_SMOOTHER_PHASES, true));
     TRY(launch_finish(h, h->vc_dot_nblk, FIN_OLDZTR, 0.0));
     FLUB_CHECK_HIP(hipEventSynchronize(h->ev));
-    const double rTr0 = h->hsc[SC_RTR];
+    const double rTr0 = hsum ? mg_host_partial_sum(h->hpart, nb) : h->hsc[SC_RTR];
     const double tol = fmax(abs_tol, rTr0 * rel_tol);
     const int cap = max_iters < 0 ? SOLVE_SAFETY_CAP : max_iters;
     int it = 0;
@@ -1169,16 +1211,18 @@ static int solve_fast(flub_mgpcg *h, int max_iters, double eps, double abs_tol, 
         }
         TRY(launch_finish(h, nbp, FIN_ALPHA, eps));
         if (split) {   // r only: x += alpha p waits for the next p update (or the end)
-            k_update_rx_dot<false><<<nb, RBS, 0, h->s>>>(R, nullptr, h->Ap, nullptr, h->sc, n, h->part);
+            k_update_rx_dot<false><<<nb, RBS, 0, h->s>>>(R, nullptr, h->Ap, nullptr, h->sc, n,
+                                                          hsum ? h->hpart_dev : h->part);
             x_pending = true;
         } else {
             k_update_rx_dot<true><<<nb, RBS, 0, h->s>>>(R, X, h->Ap, h->p, h->sc, n, h->part);
         }
         FLUB_CHECK_LAUNCH();
-        TRY(finish_rtr_to_host(h, nb));
+        if (hsum) FLUB_CHECK_HIP(hipEventRecord(h->ev, h->s));
+        else TRY(finish_rtr_to_host(h, nb));
         if (speculate) TRY(mg_precondition_split(h, zs ^ 1));
         FLUB_CHECK_HIP(hipEventSynchronize(h->ev));
-        rTr = h->hsc[SC_RTR];
+        rTr = hsum ? mg_host_partial_sum(h->hpart, nb) : h->hsc[SC_RTR];
         if (verbose) printf("after iter %d, |residual|_2=%.9g\n", it + 1, sqrt(rTr));
         if (!isfinite(rTr)) { status = FLUB_ENONFINITE; flub_set_error("non-finite residual at iteration %d", it + 1); break; }
         if (rTr < tol) break;
@@ -1195,6 +1239,7 @@ static int solve_fast(flub_mgpcg *h, int max_iters, double eps, double abs_tol, 
         it += 1;
     }
     if (x_pending) TRY((launch_update_px<true, false>(h->s, h->p, X, nullptr, h->sc, nullptr, n)));
+    if (hsum) TRY(mg_rtr_to_device(h, rTr));
     if (split) {   // back to the reference layout (sp.zpre is free scratch now)
         TRY(split_convert(h->s, L0, X, h->x, 4, false));
         if (!h->x_only) {

@@ -59,6 +59,10 @@ struct flub_mgpcg {
     double *part;   // reduction partials (2*MAXPART)
     double *hsc;    // pinned host mirror of sc
     double *hsc_dev;  // device address of hsc (nullptr: not mapped, values are copied)
+    // pinned host buffer of 2 * MAXBLK reduction partials, mapped for the
+    // device: the r update stores its r.r partials there and the host sums
+    // them in the finish kernel's order (nullptr: a finish kernel sums them)
+    double *hpart, *hpart_dev;
     int pre_post, bottom;
     hipStream_t s;     // work stream (caller's; may be the null stream)
     hipStream_t cap;   // private stream used only to capture graphs
@@ -152,6 +156,12 @@ int mg_precondition_split(flub_mgpcg *h, int zslot);
 // when mapped): queue the copy to hsc when not mapped, then record h->ev;
 // hsc[SC_RTR] is valid once h->ev has completed.
 int mg_rtr_sent(flub_mgpcg *h);
+// The sum of nblk partials in exactly the order of the finish kernels
+// (per-thread strided sums over RBS threads, the wavefront shuffle tree, then
+// the wavefronts in order): bitwise their result, computed on the host.
+double mg_host_partial_sum(const double *part, int nblk);
+// After a solve whose r.r stayed on the host: store it into sc[SC_RTR].
+int mg_rtr_to_device(flub_mgpcg *h, double rTr);
 
 // alpha = old_zTr / (sum of the partials + eps) (mgpcg.py:212-216)
 int mg_finish_alpha(flub_mgpcg *h, int nblk, double eps);

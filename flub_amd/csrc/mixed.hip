@@ -465,17 +465,25 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
         // into that cycle's slot, so its first sweeps read 2 bytes of r a cell
         __hip_bfloat16 *rh_next =
             h->mx_unit ? static_cast<__hip_bfloat16 *>(S.rh) + (size_t)(zs ^ 1) * n : nullptr;
-        k_mx_update<false><<<nb, RBS, 0, h->s>>>(S.r, nullptr, h->Ap, nullptr, h->sc, n, h->part, rh_next);
+        // BF16 (scale 1: no max|r| needed on the device): the r.r partials go
+        // to the mapped host buffer and the host sums them
+        const bool hsum = h->mx_unit && h->hpart_dev;
+        k_mx_update<false><<<nb, RBS, 0, h->s>>>(S.r, nullptr, h->Ap, nullptr, h->sc, n,
+                                                 hsum ? h->hpart_dev : h->part, rh_next);
         FLUB_CHECK_LAUNCH();
         x_pending = true;
         h->rh_ready = h->mx_unit ? (zs ^ 1) : -1;
-        k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nb, h->sc, h->mx_unit ? -1 : 0,
-                                           h->hsc_dev ? h->hsc_dev + SC_RTR : nullptr);
-        FLUB_CHECK_LAUNCH();
-        TRYM(mg_rtr_sent(h));
+        if (hsum) {
+            FLUB_CHECK_HIP(hipEventRecord(h->ev, h->s));
+        } else {
+            k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nb, h->sc, h->mx_unit ? -1 : 0,
+                                               h->hsc_dev ? h->hsc_dev + SC_RTR : nullptr);
+            FLUB_CHECK_LAUNCH();
+            TRYM(mg_rtr_sent(h));
+        }
         if (speculate) TRYM(mg_precondition_split(h, zs ^ 1));
         FLUB_CHECK_HIP(hipEventSynchronize(h->ev));
-        rTr = h->hsc[SC_RTR];
+        rTr = hsum ? mg_host_partial_sum(h->hpart, nb) : h->hsc[SC_RTR];
         if (verbose) printf("after iter %d, |residual|_2=%.9g\n", it + 1, sqrt(rTr));
         if (!isfinite(rTr)) { status = FLUB_ENONFINITE; flub_set_error("non-finite residual at iteration %d", it + 1); break; }
         bool redo = false;   // the speculative cycle used an r that was then replaced
@@ -507,6 +515,7 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
         it += 1;
     }
     if (x_pending) TRYM((launch_update_px<true, false>(h->s, h->p, S.x, nullptr, h->sc, nullptr, n)));
+    if (h->mx_unit && h->hpart_dev) TRYM(mg_rtr_to_device(h, rTr));
     // back to the reference layout (S.zw is free scratch now)
     TRYM(split_convert(h->s, L0, S.x, h->x, 4, false));
     if (!h->x_only) {
