@@ -258,6 +258,44 @@ def test_split_launch_variants_bitwise(precision, variant, monkeypatch):
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_split_solve_iteration_cap(precision, monkeypatch):
+    """A fused solve stopped by max_iters returns that count; its pressure and
+    the r.r it reports (summed on the host from the r update's partials) are
+    bitwise those of the path with a finish kernel and a copy."""
+    s = scenes.dam_break_spheres(64, 1)
+    monkeypatch.setenv("FLUB_SPLIT_MIN_CELLS", "1")
+    res = []
+    for copy in (False, True):
+        if copy:
+            monkeypatch.setenv("FLUB_RTR_COPY", "1")
+        pr = Projection((64, 64, 64), n_mg_levels=4, pre_and_post_smoothing=2, bottom_smoothing=6,
+                        precision=precision)
+        pr.load(s)
+        it = pr.project(max_iters=3, rel_tol=1e-12)
+        res.append((it, pr.p0.f.numpy(), pr.rTr))
+    (i0, p0, r0), (i1, p1, r1) = res
+    assert i0 == i1 == 3
+    assert r0 == r1 and np.isfinite(r0) and r0 > 0.0
+    assert bits_equal(p0, p1)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_split_solve_nonfinite(precision):
+    """A non-finite right-hand side stops the fused solve with the non-finite
+    status instead of iterating on NaN."""
+    from flub_amd import _lib
+    s = scenes.dam_break_spheres(64, 1)
+    pr = Projection((64, 64, 64), n_mg_levels=4, pre_and_post_smoothing=2, bottom_smoothing=6,
+                    precision=precision)
+    pr.load(s)
+    u = np.full_like(np.asarray(s["u"], np.float32), np.nan)
+    pr.set_velocity(u, s["v"], s["w"])
+    with pytest.raises(_lib.FlubError) as e:
+        pr.project(max_iters=-1, rel_tol=1e-12)
+    assert e.value.code == _lib.ENONFINITE
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
 def test_split_coarse_solve_bitwise(precision, monkeypatch):
     """Colour-split coarse levels change where values are stored, not the
     arithmetic: the solve gives bitwise the same pressure in the same
