@@ -20,6 +20,7 @@
 // colour); LDS holds only z.  ZERO starts from z = 0 without reading it;
 // PROLONG starts from z + pfac * z_coarse[parent] (mgpcg.py:113-116).
 #include "common.h"
+#include "div_small.h"
 #include "mgpcg_internal.h"
 #include "smooth.h"
 #include "stencil.h"
@@ -44,20 +45,6 @@ struct TbArgs {
     uint32_t phases;      // bit s: phase of step s (updates (i+j+k)&1 == phase^1)
     int cnt[TB_GMAX];     // cnt[d]: pairs within distance d of the core
 };
-
-// x / d for d in 1..6 (y = RN(1/d)): q0 = x*y, one FMA residual step.  For
-// every finite x with |x| >= 2^-125 this is the correctly rounded quotient
-// (checked exhaustively over all FP32 x for each d; for d = 6 it fails only
-// where the quotient is subnormal); copysign restores -0 for x = -0.  Tiny
-// nonzero, infinite and NaN x take the IEEE division.
-__device__ __forceinline__ float div_small(float x, float df, float y)
-{
-    const float q0 = x * y;
-    float q = copysignf(fmaf(fmaf(-q0, df, x), y, q0), x);
-    const float ax = fabsf(x);
-    if ((ax < 0x1p-125f && ax != 0.0f) || !(ax <= 0x1.fffffep127f)) q = x / df;
-    return q;
-}
 
 // The red-black update of a cell (mgpcg.py:72-86) with the LDS invariant of
 // this kernel: non-FLUID cells hold +0 there (zeroed at load; every update of

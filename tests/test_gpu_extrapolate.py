@@ -95,3 +95,52 @@ def test_driver_pass_counts(passes, oracle):
         assert np.array_equal(got.f.numpy().view(np.uint32), np.asarray(want).view(np.uint32))
     for got, want in zip((T.u, T.v, T.w), ref_t):
         assert np.array_equal(got.f.numpy(), want)
+
+
+def _inputs_shape(shape, seed):
+    """Non-cubic grids: a level set of random spheres (in cell units) on an
+    nx x ny x nz grid, random face velocities and owner marks."""
+    nx, ny, nz = shape
+    rng = np.random.default_rng(seed)
+    i, j, k = np.meshgrid(np.arange(nx) + 0.5, np.arange(ny) + 0.5, np.arange(nz) + 0.5, indexing="ij")
+    phi = np.full(shape, np.inf, np.float32)
+    for _ in range(6):
+        c = rng.uniform(0, 1, 3) * np.array(shape)
+        r = rng.uniform(3, 9)
+        phi = np.minimum(phi, (np.sqrt((i - c[0]) ** 2 + (j - c[1]) ** 2 + (k - c[2]) ** 2) - r).astype(np.float32))
+    vel = (rng.normal(0, 1, (nx + 1, ny, nz)).astype(np.float32), rng.normal(0, 1, (nx, ny + 1, nz)).astype(np.float32),
+           rng.normal(0, 1, (nx, ny, nz + 1)).astype(np.float32))
+    owners = tuple(rng.choice([-1, 0, 1], size=a.shape, p=[0.8, 0.1, 0.1]).astype(np.int32) for a in vel)
+    return phi, vel, owners
+
+
+@pytest.mark.parametrize("shape,seed", [((40, 24, 56), 6), ((72, 40, 56), 7), ((17, 9, 70), 8)])
+def test_extrapolate_by_1_non_cubic_bitexact(shape, seed, oracle):
+    """The LDS-tiled fill (8 x 8 x 62 face tiles) on grids whose extents are not
+    multiples of the tile, and not equal to each other."""
+    phi, vel, owners = _inputs_shape(shape, seed)
+    tags = oracle.set_unknown(phi, *owners)
+    ref_v, ref_t = oracle.extrapolate_by_1(vel, tags)
+    V = _mac(vel, MACGrid, Field)
+    T = _mac(owners, MACGridInt, IntField)
+    launch(set_unknown, dim=phi.shape, inputs=[Field(f=wp.from_numpy(phi)), T])
+    extrapolate_by_1(V, T)
+    for got, want in zip((V.u, V.v, V.w), ref_v):
+        assert np.array_equal(got.f.numpy().view(np.uint32), want.view(np.uint32))
+    for got, want in zip((T.u, T.v, T.w), ref_t):
+        assert np.array_equal(got.f.numpy(), want)
+
+
+@pytest.mark.parametrize("shape,passes", [((40, 24, 56), 3), ((72, 40, 56), 8)])
+def test_driver_non_cubic_equals_every_pass(shape, passes, oracle):
+    phi, vel, owners = _inputs_shape(shape, 9)
+    ref_v, ref_t = vel, oracle.set_unknown(phi, *owners)
+    for _ in range(passes):
+        ref_v, ref_t = oracle.extrapolate_by_1(ref_v, ref_t)
+    V = _mac(vel, MACGrid, Field)
+    T = _mac(owners, MACGridInt, IntField)
+    extrapolate(V, T, Field(f=wp.from_numpy(phi)), n_cells=passes)
+    for got, want in zip((V.u, V.v, V.w), ref_v):
+        assert np.array_equal(got.f.numpy().view(np.uint32), np.asarray(want).view(np.uint32))
+    for got, want in zip((T.u, T.v, T.w), ref_t):
+        assert np.array_equal(got.f.numpy(), want)

@@ -69,3 +69,16 @@ def test_bench_refuses_several_gpus():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stdout
+
+
+def test_div_small_is_ieee_division(tmp_path):
+    """The blocked smoother's reciprocal division (csrc/div_small.h) equals
+    x / d for d = 1..6 on every x near its FMA bound and on a sample of the
+    rest (tests/div_small_check.cpp, built for the host CPU)."""
+    exe = tmp_path / "div_small_check"
+    src = os.path.join(ROOT, "tests", "div_small_check.cpp")
+    subprocess.run(["g++", "-O2", "-fopenmp", "-std=c++17", "-ffp-contract=off", "-o", str(exe), src, "-lm"],
+                   check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, env={**os.environ, "OMP_NUM_THREADS": "8"})
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "mismatches 0" in r.stdout
