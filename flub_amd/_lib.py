@@ -86,6 +86,8 @@ SIGNATURES = {
     "flub_mgpcg_profile": [_P, _I, _I, _PF, _PD],
     "flub_mgpcg_apply_A": [_P, _P, _P],
     "flub_mgpcg_residual_norm": [_P, _P, _P, _PD],
+    "flub_mgpcg_true_residual_f64": [_P, _P, _P, _P, _PD],
+    "flub_mgpcg_solution_f64": [_P, _P],
     "flub_mgpcg_precondition_into": [_P, _P, _P, _I],
     "flub_project": [_P] * 8 + [_F, _I, _D, _D, _I, _P, _P, _PI, _PD],
     "flub_conv3d_bf16": [_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _P],

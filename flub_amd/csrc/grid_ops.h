@@ -21,3 +21,8 @@ int divergence_fused_split(const float *u, const float *v, const float *w, float
 // pressure scale chain (x*h)*(1/h) on the fly.
 int flub_subtract_gradient_x_u8(float *uf, const float *u_rigid, const float *x, const uint8_t *tag,
                                 int nx, int ny, int nz, int d, float h, void *stream);
+// The same face update from an FP64 x (the fused projection's FP64 solution,
+// reference layout; p / h of the scale chain is x itself), computed in FP64
+// and rounded once.
+int flub_subtract_gradient_x64_u8(float *uf, const float *u_rigid, const double *x, const uint8_t *tag,
+                                  int nx, int ny, int nz, int d, void *stream);

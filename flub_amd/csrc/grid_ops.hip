@@ -81,6 +81,11 @@ __device__ __forceinline__ void ldf4(const float *p, float v[4])
     const float4 a = *reinterpret_cast<const float4 *>(p);
     v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
 }
+__device__ __forceinline__ void ldf4(const double *p, double v[4])
+{
+    const double2 a = reinterpret_cast<const double2 *>(p)[0], b = reinterpret_cast<const double2 *>(p)[1];
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+}
 __device__ __forceinline__ void stf4(float *p, const float v[4])
 {
     *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
@@ -284,10 +289,12 @@ FLUB_API int flub_scale(float *q, float s, size_t n, void *stream)
 // subtract_gradient.py:38-68 over the face grid of axis d.  PS != 0 reads the
 // pressure as (x * h) * (1/h) -- the value the reference's scale chain
 // (water_3d.py:67-68, example_fluid.py:134) hands to the kernel -- so the
-// fused projection can read x directly.
-template <typename TagT, bool PS>
+// fused projection can read x directly.  PT: the pressure's type; with FP64
+// (the fused projection's FP64 solution, PS false) the face update
+// u - (p_r - p_l) is computed in FP64 and rounded once.
+template <typename TagT, bool PS, typename PT = float>
 __global__ void k_subtract_gradient(float *__restrict__ uf, const float *__restrict__ urig,
-                                    const float *__restrict__ p, const TagT *__restrict__ tag, int nx,
+                                    const PT *__restrict__ p, const TagT *__restrict__ tag, int nx,
                                     int ny, int nz, int d, float h, float inv_h)
 {
     const int fx = nx + (d == 1), fy = ny + (d == 2), fz = nz + (d == 3);
@@ -303,11 +310,11 @@ __global__ void k_subtract_gradient(float *__restrict__ uf, const float *__restr
     if (0 < ii && ii < n_d) {
         const uint32_t I = cidx(i, j, k, ny, nz);
         const uint32_t off = d == 1 ? (uint32_t)ny * nz : (d == 2 ? (uint32_t)nz : 1u);
-        float pl = p[I - off], pr = p[I];
+        PT pl = p[I - off], pr = p[I];
         if (PS) { pl = (pl * h) * inv_h; pr = (pr * h) * inv_h; }
         const int tl = tag[I - off], tr = tag[I];
         if (tl == 2 || tr == 2) out = urig[t];
-        else if (tl == 1 || tr == 1) out = uf[t] - (pr - pl);
+        else if (tl == 1 || tr == 1) out = (float)((PT)uf[t] - (pr - pl));
         else out = 0.0f;
     } else {
         out = 0.0f;
@@ -318,9 +325,9 @@ __global__ void k_subtract_gradient(float *__restrict__ uf, const float *__restr
 // k_subtract_gradient for the u and v faces (rows of nz faces), 4 consecutive
 // faces along k per thread (nz % 4 == 0, 16-byte aligned arrays); the rigid
 // velocities are read only by vectors that have a SOLID side.
-template <typename TagT, bool PS>
+template <typename TagT, bool PS, typename PT = float>
 __global__ void __launch_bounds__(256) k_subtract_gradient4(float *__restrict__ uf, const float *__restrict__ urig,
-                                                            const float *__restrict__ p,
+                                                            const PT *__restrict__ p,
                                                             const TagT *__restrict__ tag, int nx, int ny, int nz,
                                                             int d, float h, float inv_h)
 {
@@ -337,7 +344,8 @@ __global__ void __launch_bounds__(256) k_subtract_gradient4(float *__restrict__ 
     if (0 < ii && ii < n_d) {
         const uint32_t I = cidx(i, j, k0, ny, nz);
         const uint32_t off = d == 1 ? (uint32_t)ny * nz : (uint32_t)nz;
-        float PL[4], PR[4], UF[4], UR[4];
+        PT PL[4], PR[4];
+        float UF[4], UR[4];
         int TL[4], TR[4];
         ldf4(p + I - off, PL);
         ldf4(p + I, PR);
@@ -350,10 +358,10 @@ __global__ void __launch_bounds__(256) k_subtract_gradient4(float *__restrict__ 
         if (solid) ldf4(urig + t, UR);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            float pl = PL[e], pr = PR[e];
+            PT pl = PL[e], pr = PR[e];
             if (PS) { pl = (pl * h) * inv_h; pr = (pr * h) * inv_h; }
             if (TL[e] == 2 || TR[e] == 2) out[e] = UR[e];
-            else if (TL[e] == 1 || TR[e] == 1) out[e] = UF[e] - (pr - pl);
+            else if (TL[e] == 1 || TR[e] == 1) out[e] = (float)((PT)UF[e] - (pr - pl));
         }
     }
     stf4(uf + t, out);
@@ -364,9 +372,9 @@ __global__ void __launch_bounds__(256) k_subtract_gradient4(float *__restrict__ 
 // cells k0 .. k0 + 3 as one vector and cell k0 - 1 as a scalar; the face
 // values (unaligned rows) are scalars.  The thread of the last vector also
 // writes the boundary face k = nz.
-template <typename TagT, bool PS>
+template <typename TagT, bool PS, typename PT = float>
 __global__ void __launch_bounds__(256) k_subtract_gradient4w(float *__restrict__ uf, const float *__restrict__ urig,
-                                                             const float *__restrict__ p,
+                                                             const PT *__restrict__ p,
                                                              const TagT *__restrict__ tag, int nx, int ny, int nz,
                                                              float h, float inv_h)
 {
@@ -376,30 +384,30 @@ __global__ void __launch_bounds__(256) k_subtract_gradient4w(float *__restrict__
     const uint32_t row = tq / nq, q = tq - row * nq;
     const int k0 = 4 * (int)q;
     const uint32_t I = row * (uint32_t)nz + k0, t = row * (uint32_t)(nz + 1) + k0;
-    float PR[4];
+    PT PR[4];
     int TR[4];
     ldf4(p + I, PR);
     ldt4(tag + I, TR);
-    const float p_m = k0 > 0 ? p[I - 1] : 0.0f;
+    const PT p_m = k0 > 0 ? p[I - 1] : (PT)0;
     const int t_m = k0 > 0 ? (int)tag[I - 1] : 0;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const int k = k0 + e;
         float out = 0.0f;
         if (k > 0) {
-            float pl = e > 0 ? PR[e - 1] : p_m, pr = PR[e];
+            PT pl = e > 0 ? PR[e - 1] : p_m, pr = PR[e];
             if (PS) { pl = (pl * h) * inv_h; pr = (pr * h) * inv_h; }
             const int tl = e > 0 ? TR[e - 1] : t_m, tr = TR[e];
             if (tl == 2 || tr == 2) out = urig[t + e];
-            else if (tl == 1 || tr == 1) out = uf[t + e] - (pr - pl);
+            else if (tl == 1 || tr == 1) out = (float)((PT)uf[t + e] - (pr - pl));
         }
         uf[t + e] = out;
     }
     if (q == nq - 1) uf[t + 4] = 0.0f;   // k = nz
 }
 
-template <typename TagT, bool PS>
-static int subgrad_impl(float *uf, const float *urig, const float *p, const TagT *tag, int nx, int ny,
+template <typename TagT, bool PS, typename PT = float>
+static int subgrad_impl(float *uf, const float *urig, const PT *p, const TagT *tag, int nx, int ny,
                         int nz, int d, float h, void *stream)
 {
     FLUB_REQUIRE(uf && urig && p && tag, FLUB_EINVAL, "subtract_gradient_tag: null pointer");
@@ -408,14 +416,15 @@ static int subgrad_impl(float *uf, const float *urig, const float *p, const TagT
     const size_t nf = (size_t)(nx + (d == 1)) * (ny + (d == 2)) * (nz + (d == 3));
     const float inv_h = (float)(1.0 / (double)h);
     if (d != 3 && nz % 4 == 0 && aligned16(uf) && aligned16(urig) && aligned16(p) && aligned16(tag))
-        k_subtract_gradient4<TagT, PS><<<blocks_for(nf / 4, 256), 256, 0, as_stream(stream)>>>(uf, urig, p, tag, nx,
-                                                                                             ny, nz, d, h, inv_h);
+        k_subtract_gradient4<TagT, PS, PT><<<blocks_for(nf / 4, 256), 256, 0, as_stream(stream)>>>(uf, urig, p, tag,
+                                                                                                 nx, ny, nz, d, h,
+                                                                                                 inv_h);
     else if (d == 3 && nz % 4 == 0 && aligned16(p) && aligned16(tag))
-        k_subtract_gradient4w<TagT, PS><<<blocks_for((size_t)nx * ny * (nz / 4), 256), 256, 0, as_stream(stream)>>>(
-            uf, urig, p, tag, nx, ny, nz, h, inv_h);
+        k_subtract_gradient4w<TagT, PS, PT><<<blocks_for((size_t)nx * ny * (nz / 4), 256), 256, 0,
+                                              as_stream(stream)>>>(uf, urig, p, tag, nx, ny, nz, h, inv_h);
     else
-        k_subtract_gradient<TagT, PS><<<blocks_for(nf, 256), 256, 0, as_stream(stream)>>>(uf, urig, p, tag, nx, ny,
-                                                                                        nz, d, h, inv_h);
+        k_subtract_gradient<TagT, PS, PT><<<blocks_for(nf, 256), 256, 0, as_stream(stream)>>>(uf, urig, p, tag, nx,
+                                                                                            ny, nz, d, h, inv_h);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
 }
@@ -430,4 +439,10 @@ int flub_subtract_gradient_x_u8(float *uf, const float *u_rigid, const float *x,
                                 int nx, int ny, int nz, int d, float h, void *stream)
 {
     return subgrad_impl<uint8_t, true>(uf, u_rigid, x, tag, nx, ny, nz, d, h, stream);
+}
+
+int flub_subtract_gradient_x64_u8(float *uf, const float *u_rigid, const double *x, const uint8_t *tag,
+                                  int nx, int ny, int nz, int d, void *stream)
+{
+    return subgrad_impl<uint8_t, false, double>(uf, u_rigid, x, tag, nx, ny, nz, d, 1.0f, stream);
 }

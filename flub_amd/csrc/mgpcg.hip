@@ -897,7 +897,9 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
             ALLOC(h->sp.z[0], n0 * 4);
             ALLOC(h->sp.z[1], n0 * 4);
             ALLOC(h->sp.zpre, n0 * 4);
-            ALLOC(h->sp.x, n0 * 4);
+            ALLOC(h->sp.x, n0 * 8);
+            ALLOC(h->x64n, n0 * 8);
+            ALLOC(h->sp.b, n0 * 4);
             ALLOC(h->sp.code, n0 * 2);
             h->sp.lcode[0] = h->sp.code;
             // (the 16-bit sweeps of the mixed V-cycle stream better one launch
@@ -923,7 +925,6 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
                 ALLOC(h->sp.rh, n0 * 2 * 2);   // one half per z slot
                 ALLOC(h->sp.zh, n0 * 2);
                 ALLOC(h->sp.zw, n0 * 4);
-                ALLOC(h->sp.b, n0 * 4);
                 // r_old of a solve's first final post-sweep is read but not used
                 // (its beta is 0): zeros keep those partials finite
                 hipError_t e_ = hipMemsetAsync(h->sp.rh, 0, n0 * 2 * 2, h->s);
@@ -953,6 +954,7 @@ FLUB_API int flub_mgpcg_destroy(flub_mgpcg *h)
     hipFree(h->x); hipFree(h->p); hipFree(h->p_alt); hipFree(h->Ap); hipFree(h->sc); hipFree(h->part);
     hipFree(h->tb_order);
     hipFree(h->sp.r); hipFree(h->sp.z[0]); hipFree(h->sp.z[1]); hipFree(h->sp.zpre); hipFree(h->sp.x);
+    hipFree(h->x64n);
     hipFree(h->sp.code); hipFree(h->sp.rh); hipFree(h->sp.zh); hipFree(h->sp.zw); hipFree(h->sp.b);
     for (int l = 1; l < MAXL; ++l) { hipFree(h->sp.lcode[l]); hipFree(h->sp.zt[l]); }
     if (h->ev) hipEventDestroy(h->ev);
@@ -1014,6 +1016,7 @@ FLUB_API int flub_mgpcg_init(flub_mgpcg *h, const float *r, float k)
     k_fill_f32<<<nb256(n), 256, 0, h->s>>>(h->x, 0.0f, n);
     FLUB_CHECK_LAUNCH();
     FLUB_CHECK_HIP(hipMemsetAsync(h->sc, 0, SC_NUM * sizeof(double), h->s));
+    h->x64_valid = false;
     return FLUB_OK;
 }
 
@@ -1090,6 +1093,7 @@ FLUB_API int flub_mgpcg_update_r(flub_mgpcg *h)
 FLUB_API int flub_mgpcg_update_x(flub_mgpcg *h)
 {
     FLUB_REQUIRE(h, FLUB_EINVAL, "null handle");
+    h->x64_valid = false;
     k_update_x<<<nb256(h->lv[0].n), 256, 0, h->s>>>(h->x, h->p, h->sc, h->lv[0].n);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
@@ -1145,47 +1149,105 @@ static int read_scalar(flub_mgpcg *h, int idx, double *out)
     return FLUB_OK;
 }
 
-// mgpcg.py:378-462 for the "phases" smoother on the fused kernels: the z.r
-// partials come out of the V-cycle's last sweep.  Split layout: update_p then
-// Ap, and update_x deferred into the next update_p (x += alpha p with the
-// same alpha and p, before p changes; the last one after the loop), so that
-// update_r reads and writes only r.  Reference layout: update_p folded into
-// the Ap kernel (out of place, alternating p and p_alt), update_x into
-// update_r.
-static int solve_fast(flub_mgpcg *h, int max_iters, double eps, double abs_tol, double rel_tol, int verbose,
-                      int *iters, double *rTr_out)
+// The FP64 true residual of a split solve's FP64 x: r = b - ndt(x) (FP64
+// stencil, stored FP32) replaces the solve's r; returns ||r||^2.  The split
+// solves stop when it is below the tolerance of the reference's test (the
+// recursive residual passed it first).
+int mg_split_true_residual(flub_mgpcg *h, float *R, double *rTrue)
 {
     Level &L0 = h->lv[0];
+    const bool hsum = h->hpart_dev != nullptr;
+    int nbt = 0;
+    TRY(split_true_residual(h->s, L0, R, h->sp.b, h->sp.x, h->sp.code, hsum ? h->hpart_dev : h->part, &nbt));
+    if (hsum) FLUB_CHECK_HIP(hipEventRecord(h->ev, h->s));
+    else TRY(finish_rtr_to_host(h, nbt));
+    FLUB_CHECK_HIP(hipEventSynchronize(h->ev));
+    *rTrue = hsum ? mg_host_partial_sum(h->hpart, nbt) : h->hsc[SC_RTR];
+    return FLUB_OK;
+}
+
+int mg_split_enter(flub_mgpcg *h, bool &x_set)
+{
+    Level &L0 = h->lv[0];
+    SplitBuf &S = h->sp;
+    if (h->presplit) {   // the fused projection's divergence kernel wrote S.r and S.b; x = 0
+        x_set = false;
+        return FLUB_OK;
+    }
+    // r and x in the split layout; b = r - ndt(x), the right-hand side the
+    // caller's r and x stand for (r itself when x = 0, as after init)
+    TRY(split_convert(h->s, L0, L0.r, S.r, 4, true));
+    TRY(split_x_from_natural(h->s, L0, h->x, S.x));
+    int nbt = 0;
+    TRY(split_true_residual(h->s, L0, S.b, S.r, S.x, S.code, h->part, &nbt));
+    x_set = true;
+    return FLUB_OK;
+}
+
+int mg_split_leave(flub_mgpcg *h, bool x_set)
+{
+    Level &L0 = h->lv[0];
+    if (!x_set) FLUB_CHECK_HIP(hipMemsetAsync(h->sp.x, 0, (size_t)L0.n * sizeof(double), h->s));
+    TRY(split_x_to_natural(h->s, L0, h->sp.x, h->x, h->want_x64n ? h->x64n : nullptr));
+    h->x64_valid = true;
+    return FLUB_OK;
+}
+
+namespace {
+__global__ void k_widen(const float *__restrict__ x, double *__restrict__ y, uint32_t n)
+{
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t < n) y[t] = (double)x[t];
+}
+}  // namespace
+
+FLUB_API int flub_mgpcg_solution_f64(flub_mgpcg *h, double *x)
+{
+    FLUB_REQUIRE(h && x, FLUB_EINVAL, "flub_mgpcg_solution_f64: null argument");
+    Level &L0 = h->lv[0];
+    if (h->x64_valid) return split_x_to_natural(h->s, L0, h->sp.x, nullptr, x);
+    k_widen<<<nb256(L0.n), 256, 0, h->s>>>(h->x, x, L0.n);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+// mgpcg.py:378-462 for the "phases" smoother on the colour-split level 0:
+// r, x, z, p and Ap of level 0 are colour-split for the whole solve and
+// converted back at the end.  update_p then Ap, and update_x deferred into
+// the next update_p (x += alpha p with the same alpha and p, before p
+// changes; the last one after the loop), so that update_r reads and writes
+// only r.  x accumulates in FP64 (SplitBuf::x).  The V-cycle is queued
+// before the host reads the residual norm, into the z buffer the iteration
+// does not use: the GPU keeps working while the host tests for convergence.
+// If the test stops the solve, that speculative z is not used (x, r, z and
+// the scalars are those of the reference sequence; the coarse levels hold the
+// speculative cycle's values).  When the reference's test passes on the
+// recursive residual, the FP64 true residual b - ndt(x) replaces r; the solve
+// stops if it passes the same test, else it goes on (at most
+// max_replacements times).  *rTr_out: that FP64 true residual.
+static int solve_split(flub_mgpcg *h, int max_iters, double eps, double abs_tol, double rel_tol, int verbose,
+                       int *iters, double *rTr_out)
+{
+    Level &L0 = h->lv[0];
+    SplitBuf &S = h->sp;
     const uint32_t n = L0.n;
     const int nb = red_blocks(n);
-    float *const p_home = h->p;
-    // With the split layout, r, x, z, p and Ap of level 0 are colour-split
-    // for the whole solve and converted back at the end.  The V-cycle is then
-    // queued before the host reads the residual norm, into the z buffer the
-    // iteration does not use: the GPU keeps working while the host tests for
-    // convergence.  If the test stops the solve, that speculative z is not
-    // used (x, r, z and the scalars are those of the reference sequence; the
-    // coarse levels hold the speculative cycle's values).
-    const bool split = h->split;
+    const int max_replacements = 3;
+    int replacements = 0;
     int zs = 0;
-    float *R = L0.r, *X = h->x, *Z = L0.z;
-    if (split) {
-        if (!h->presplit) {
-            TRY(split_convert(h->s, L0, L0.r, h->sp.r, 4, true));
-            TRY(split_convert(h->s, L0, h->x, h->sp.x, 4, true));
-        }
-        R = h->sp.r; X = h->sp.x; Z = h->sp.z[zs];
-    }
-    // split: the r.r partials go to the mapped host buffer and the host sums
-    // them (no finish kernel on the stream)
-    const bool hsum = split && h->hpart_dev;
+    bool x_set = true;
+    TRY(mg_split_enter(h, x_set));
+    float *const R = S.r;
+    // the r.r partials go to the mapped host buffer and the host sums them
+    // (no finish kernel on the stream)
+    const bool hsum = h->hpart_dev != nullptr;
     k_dot<<<nb, RBS, 0, h->s>>>(R, R, n, hsum ? h->hpart_dev : h->part);
     FLUB_CHECK_LAUNCH();
     // the tolerance is first needed after iteration 1: the first V-cycle runs
     // while the initial r.r travels to the host
     if (hsum) FLUB_CHECK_HIP(hipEventRecord(h->ev, h->s));
     else TRY(finish_rtr_to_host(h, nb));
-    TRY(split ? mg_precondition_split(h, zs) : mg_precondition(h, FLUB_SMOOTHER_PHASES, true));
+    TRY(mg_precondition_split(h, zs));
     TRY(launch_finish(h, h->vc_dot_nblk, FIN_OLDZTR, 0.0));
     FLUB_CHECK_HIP(hipEventSynchronize(h->ev));
     const double rTr0 = hsum ? mg_host_partial_sum(h->hpart, nb) : h->hsc[SC_RTR];
@@ -1194,30 +1256,23 @@ static int solve_fast(flub_mgpcg *h, int max_iters, double eps, double abs_tol, 
     int it = 0;
     double rTr = rTr0, r_prev = rTr0, rate = 1.0;   // rate: last reduction factor of r.r
     int status = FLUB_OK;
-    bool x_pending = false;   // split: x += alpha p of the last iteration not applied yet
+    bool x_pending = false;   // x += alpha p of the last iteration not applied yet
     while (it < cap) {
         int nbp = 0;
         // speculate unless this test is predicted to stop the solve (a wasted
         // speculative cycle would delay the end of the solve)
-        const bool speculate = split && it + 1 < cap && r_prev * rate > 16.0 * tol;
-        if (split) {   // p = z + beta*p (and the deferred x update), then Ap = ndt(p)
-            TRY(x_pending ? launch_update_px<true>(h->s, h->p, X, Z, h->sc, nullptr, n)
-                          : launch_update_px<false>(h->s, h->p, X, Z, h->sc, nullptr, n));
-            x_pending = false;
-            TRY(split_Ap(h->s, L0, h->p, h->Ap, h->sp.code, h->part, &nbp));
-        } else {
-            TRY(fast_p_Ap(h->s, L0, h->p, h->p_alt, h->Ap, h->sc, h->part, &nbp));
-            float *t = h->p; h->p = h->p_alt; h->p_alt = t;
-        }
+        const bool speculate = it + 1 < cap && r_prev * rate > 16.0 * tol;
+        // p = z + beta*p (and the deferred x update), then Ap = ndt(p)
+        if (x_pending) TRY(update_px_pending(h->s, h->p, S.x, S.z[zs], h->sc, nullptr, n, x_set, true));
+        else TRY(launch_update_px<XM_NONE>(h->s, h->p, S.x, S.z[zs], h->sc, nullptr, n));
+        x_pending = false;
+        TRY(split_Ap(h->s, L0, h->p, h->Ap, S.code, h->part, &nbp));
         TRY(launch_finish(h, nbp, FIN_ALPHA, eps));
-        if (split) {   // r only: x += alpha p waits for the next p update (or the end)
-            k_update_rx_dot<false><<<nb, RBS, 0, h->s>>>(R, nullptr, h->Ap, nullptr, h->sc, n,
-                                                          hsum ? h->hpart_dev : h->part);
-            x_pending = true;
-        } else {
-            k_update_rx_dot<true><<<nb, RBS, 0, h->s>>>(R, X, h->Ap, h->p, h->sc, n, h->part);
-        }
+        // r only: x += alpha p waits for the next p update (or the end)
+        k_update_rx_dot<false><<<nb, RBS, 0, h->s>>>(R, nullptr, h->Ap, nullptr, h->sc, n,
+                                                      hsum ? h->hpart_dev : h->part);
         FLUB_CHECK_LAUNCH();
+        x_pending = true;
         if (hsum) FLUB_CHECK_HIP(hipEventRecord(h->ev, h->s));
         else TRY(finish_rtr_to_host(h, nb));
         if (speculate) TRY(mg_precondition_split(h, zs ^ 1));
@@ -1225,33 +1280,91 @@ static int solve_fast(flub_mgpcg *h, int max_iters, double eps, double abs_tol, 
         rTr = hsum ? mg_host_partial_sum(h->hpart, nb) : h->hsc[SC_RTR];
         if (verbose) printf("after iter %d, |residual|_2=%.9g\n", it + 1, sqrt(rTr));
         if (!isfinite(rTr)) { status = FLUB_ENONFINITE; flub_set_error("non-finite residual at iteration %d", it + 1); break; }
-        if (rTr < tol) break;
+        bool redo = false;   // the speculative cycle used an r that was then replaced
+        if (rTr < tol) {
+            TRY(update_px_pending(h->s, h->p, S.x, nullptr, h->sc, nullptr, n, x_set, false));
+            x_pending = false;
+            double rTrue;
+            TRY(mg_split_true_residual(h, R, &rTrue));
+            if (verbose) printf("  true |residual|_2=%.9g\n", sqrt(rTrue));
+            rTr = rTrue;
+            if (!isfinite(rTrue)) { status = FLUB_ENONFINITE; flub_set_error("non-finite true residual"); break; }
+            if (rTrue < tol || replacements >= max_replacements) break;
+            ++replacements;
+            redo = true;
+        }
         rate = r_prev > 0.0 ? fmin(rTr / r_prev, 1.0) : 1.0;
         r_prev = rTr;
-        if (split) {
-            if (!speculate) TRY(mg_precondition_split(h, zs ^ 1));
-            zs ^= 1;
-            Z = h->sp.z[zs];
-        } else {
-            TRY(mg_precondition(h, FLUB_SMOOTHER_PHASES, true));
-        }
+        if (!speculate || redo) TRY(mg_precondition_split(h, zs ^ 1));
+        zs ^= 1;
         TRY(launch_finish(h, h->vc_dot_nblk, FIN_BETA, eps));
         it += 1;
     }
-    if (x_pending) TRY((launch_update_px<true, false>(h->s, h->p, X, nullptr, h->sc, nullptr, n)));
+    if (x_pending) TRY(update_px_pending(h->s, h->p, S.x, nullptr, h->sc, nullptr, n, x_set, false));
     if (hsum) TRY(mg_rtr_to_device(h, rTr));
-    if (split) {   // back to the reference layout (sp.zpre is free scratch now)
-        TRY(split_convert(h->s, L0, X, h->x, 4, false));
-        if (!h->x_only) {
-            TRY(split_convert(h->s, L0, R, L0.r, 4, false));
-            TRY(split_convert(h->s, L0, Z, L0.z, 4, false));
-            TRY(split_convert(h->s, L0, h->Ap, h->sp.zpre, 4, false));
-            FLUB_CHECK_HIP(hipMemcpyAsync(h->Ap, h->sp.zpre, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
-            TRY(split_convert(h->s, L0, h->p, h->sp.zpre, 4, false));
-            FLUB_CHECK_HIP(hipMemcpyAsync(p_home, h->sp.zpre, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
-        }
-        if (h->p != p_home) { h->p_alt = h->p; h->p = p_home; }
-    } else if (h->p != p_home) {   // leave p in the buffer the handle exposes
+    // back to the reference layout (sp.zpre is free scratch now)
+    TRY(mg_split_leave(h, x_set));
+    if (!h->x_only) {
+        TRY(split_convert(h->s, L0, R, L0.r, 4, false));
+        TRY(split_convert(h->s, L0, S.z[zs], L0.z, 4, false));
+        TRY(split_convert(h->s, L0, h->Ap, S.zpre, 4, false));
+        FLUB_CHECK_HIP(hipMemcpyAsync(h->Ap, S.zpre, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
+        TRY(split_convert(h->s, L0, h->p, S.zpre, 4, false));
+        FLUB_CHECK_HIP(hipMemcpyAsync(h->p, S.zpre, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
+    }
+    if (status == FLUB_OK && it >= cap && max_iters < 0) {
+        status = FLUB_ENOTCONV;
+        flub_set_error("no convergence within %d iterations", cap);
+    }
+    if (verbose) printf("finished with  %d  iterations\n", it);
+    if (iters) *iters = it;
+    if (rTr_out) *rTr_out = rTr;
+    return status;
+}
+
+// mgpcg.py:378-462 for the "phases" smoother on the fused kernels in the
+// reference layout (grids the colour-split layout does not support): the z.r
+// partials come out of the V-cycle's last sweep, update_p is folded into the
+// Ap kernel (out of place, alternating p and p_alt) and update_x into
+// update_r, all in FP32 as in the reference.
+static int solve_fast(flub_mgpcg *h, int max_iters, double eps, double abs_tol, double rel_tol, int verbose,
+                      int *iters, double *rTr_out)
+{
+    Level &L0 = h->lv[0];
+    const uint32_t n = L0.n;
+    const int nb = red_blocks(n);
+    float *const p_home = h->p;
+    float *R = L0.r, *X = h->x;
+    k_dot<<<nb, RBS, 0, h->s>>>(R, R, n, h->part);
+    FLUB_CHECK_LAUNCH();
+    TRY(finish_rtr_to_host(h, nb));
+    TRY(mg_precondition(h, FLUB_SMOOTHER_PHASES, true));
+    TRY(launch_finish(h, h->vc_dot_nblk, FIN_OLDZTR, 0.0));
+    FLUB_CHECK_HIP(hipEventSynchronize(h->ev));
+    const double rTr0 = h->hsc[SC_RTR];
+    const double tol = fmax(abs_tol, rTr0 * rel_tol);
+    const int cap = max_iters < 0 ? SOLVE_SAFETY_CAP : max_iters;
+    int it = 0;
+    double rTr = rTr0;
+    int status = FLUB_OK;
+    while (it < cap) {
+        int nbp = 0;
+        TRY(fast_p_Ap(h->s, L0, h->p, h->p_alt, h->Ap, h->sc, h->part, &nbp));
+        float *t = h->p; h->p = h->p_alt; h->p_alt = t;
+        TRY(launch_finish(h, nbp, FIN_ALPHA, eps));
+        k_update_rx_dot<true><<<nb, RBS, 0, h->s>>>(R, X, h->Ap, h->p, h->sc, n, h->part);
+        FLUB_CHECK_LAUNCH();
+        TRY(finish_rtr_to_host(h, nb));
+        FLUB_CHECK_HIP(hipEventSynchronize(h->ev));
+        rTr = h->hsc[SC_RTR];
+        if (verbose) printf("after iter %d, |residual|_2=%.9g\n", it + 1, sqrt(rTr));
+        if (!isfinite(rTr)) { status = FLUB_ENONFINITE; flub_set_error("non-finite residual at iteration %d", it + 1); break; }
+        if (rTr < tol) break;
+        TRY(mg_precondition(h, FLUB_SMOOTHER_PHASES, true));
+        TRY(launch_finish(h, h->vc_dot_nblk, FIN_BETA, eps));
+        it += 1;
+    }
+    if (h->p != p_home) {   // leave p in the buffer the handle exposes
         FLUB_CHECK_HIP(hipMemcpyAsync(p_home, h->p, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
         h->p_alt = h->p;
         h->p = p_home;
@@ -1324,10 +1437,12 @@ FLUB_API int flub_mgpcg_solve(flub_mgpcg *h, int max_iters, double eps, double a
 {
     FLUB_REQUIRE(h && smoother_ok(smoother), FLUB_EINVAL, "bad handle or smoother");
     FLUB_REQUIRE(h->tags_set, FLUB_ESTATE, "solve before set_tags");
+    h->x64_valid = false;   // a split solve sets it when it hands x back
     if (h->prec != FLUB_PREC_FP32)
         return mg_solve_mixed(h, max_iters, eps, abs_tol, rel_tol, smoother, verbose, iters, rTr);
     if (smoother == FLUB_SMOOTHER_PHASES && use_fast(h, smoother))
-        return solve_fast(h, max_iters, eps, abs_tol, rel_tol, verbose, iters, rTr);
+        return h->split ? solve_split(h, max_iters, eps, abs_tol, rel_tol, verbose, iters, rTr)
+                        : solve_fast(h, max_iters, eps, abs_tol, rel_tol, verbose, iters, rTr);
     return solve_fp32(h, max_iters, eps, abs_tol, rel_tol, smoother, verbose, iters, rTr);
 }
 
@@ -1412,15 +1527,15 @@ FLUB_API int flub_project(flub_mgpcg *h, float *u, float *v, float *w, const flo
     const float s_div = (float)(-1.0 / (double)h_cell);
     // The colour-split solve takes r from the divergence kernel directly, in
     // its layout: r = (FLUID ? d * (-h) : 0), the values init + mask_interior
-    // give; the same kernel zeroes x (split layout) and p as init does, and for
-    // a mixed-precision handle also writes the copy of r its true residual
-    // uses; the scalars are zeroed as init does (z and Ap are written before
-    // they are read).
+    // give, and the copy of it that its FP64 true residual uses as b; the same
+    // kernel zeroes p as init does (x starts at zero without being written:
+    // the solve's first x update stores alpha p); the scalars are zeroed as
+    // init does (z and Ap are written before they are read).
     const bool presplit = h->split && smoother == FLUB_SMOOTHER_PHASES && use_fast(h, smoother) &&
                           divergence_split_ok(u, v, div, L0.tag, nz);
     if (presplit) {
         TRY(divergence_fused_split(u, v, w, div, ur, vr, wr, L0.tag, nx, ny, nz, s_div, h->sp.r, -h_cell, h->s,
-                                   h->prec != FLUB_PREC_FP32 ? h->sp.b : nullptr, h->sp.x, h->p));
+                                   h->sp.b, nullptr, h->p));
         FLUB_CHECK_HIP(hipMemsetAsync(h->sc, 0, SC_NUM * sizeof(double), h->s));
     } else {
         float *d = div ? div : h->Ap;   // Ap is rewritten by init
@@ -1432,13 +1547,24 @@ FLUB_API int flub_project(flub_mgpcg *h, float *u, float *v, float *w, const flo
     // solver keeps them (the next init resets them)
     h->x_only = true;
     h->presplit = presplit;
+    h->want_x64n = h->x64n != nullptr;
     int st = flub_mgpcg_solve(h, max_iters, 1e-12, abs_tol, rel_tol, smoother, 0, iters, rTr);
     h->x_only = false;
     h->presplit = false;
+    h->want_x64n = false;
     if (st != FLUB_OK && st != FLUB_ENOTCONV) return st;
-    TRY(flub_subtract_gradient_x_u8(u, ur, h->x, L0.tag, nx, ny, nz, 1, h_cell, h->s));
-    TRY(flub_subtract_gradient_x_u8(v, vr, h->x, L0.tag, nx, ny, nz, 2, h_cell, h->s));
-    TRY(flub_subtract_gradient_x_u8(w, wr, h->x, L0.tag, nx, ny, nz, 3, h_cell, h->s));
+    if (h->x64_valid) {
+        // a split solve: the gradient of its FP64 x (the scale chain p = x h,
+        // p / h is x itself), each face update rounded once; the projected
+        // field's divergence is then the solve's residual
+        TRY(flub_subtract_gradient_x64_u8(u, ur, h->x64n, L0.tag, nx, ny, nz, 1, h->s));
+        TRY(flub_subtract_gradient_x64_u8(v, vr, h->x64n, L0.tag, nx, ny, nz, 2, h->s));
+        TRY(flub_subtract_gradient_x64_u8(w, wr, h->x64n, L0.tag, nx, ny, nz, 3, h->s));
+    } else {
+        TRY(flub_subtract_gradient_x_u8(u, ur, h->x, L0.tag, nx, ny, nz, 1, h_cell, h->s));
+        TRY(flub_subtract_gradient_x_u8(v, vr, h->x, L0.tag, nx, ny, nz, 2, h_cell, h->s));
+        TRY(flub_subtract_gradient_x_u8(w, wr, h->x, L0.tag, nx, ny, nz, 3, h_cell, h->s));
+    }
     const float inv_h = (float)(1.0 / (double)h_cell);
     k_store_pressure<<<nb256(L0.n), 256, 0, h->s>>>(p, h->x, h_cell, inv_h, L0.n);
     FLUB_CHECK_LAUNCH();
@@ -1450,6 +1576,7 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
 {
     FLUB_REQUIRE(h && avg_ms && reps > 0, FLUB_EINVAL, "flub_mgpcg_profile: bad argument");
     FLUB_REQUIRE(h->tags_set, FLUB_ESTATE, "profile before set_tags");
+    h->x64_valid = false;
     Level &L0 = h->lv[0];
     const double N = (double)L0.n, v = h->prec == FLUB_PREC_FP32 ? 4.0 : 2.0;
     double b = 0.0;
@@ -1506,7 +1633,7 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
         case 10: {   // the split solve's p step: p (and the deferred x) update, then Ap and p.Ap
             FLUB_REQUIRE(h->split, FLUB_ESTATE, "no colour-split level 0 for this handle");
             int nbp = 0;
-            TRY(launch_update_px<true>(h->s, h->p, h->sp.x, h->sp.z[0], h->sc, nullptr, L0.n));
+            TRY(launch_update_px<XM_ADD>(h->s, h->p, h->sp.x, h->sp.z[0], h->sc, nullptr, L0.n));
             TRY(split_Ap(h->s, L0, h->p, h->Ap, h->sp.code, h->part, &nbp));
             return launch_finish(h, nbp, FIN_SUM, 0.0);
         }
@@ -1528,7 +1655,7 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
     case 7: b = 4 * N * (2 * v + 1); break;                  // four half-sweeps
     case 8: b = N * (2 * v + 1); break;
     case 9: b = N * (2 * v + 1) + N / 8 * (v + 1); break;
-    case 10: b = N * (7 * 4 + 2); break;                     // p, z, x -> p, x; p, codes -> Ap
+    case 10: b = N * (9 * 4 + 2); break;                     // p, z, x (FP64) -> p, x; p, codes -> Ap
     case 11: b = N * 3 * 4; break;                           // r, Ap -> r
     default: break;
     }
@@ -1567,6 +1694,9 @@ FLUB_API int flub_mgpcg_precondition_into(flub_mgpcg *h, const float *r, float *
     FLUB_REQUIRE(h->tags_set, FLUB_ESTATE, "precondition before set_tags");
     Level &L0 = h->lv[0];
     if (h->split && smoother == FLUB_SMOOTHER_PHASES) {
+        // the cycle converts the caller's r: no BF16 r stored by an earlier
+        // solve's r update (one that returned early) may stand in for it
+        h->rh_ready = -1;
         TRY(split_convert(h->s, L0, r, h->sp.r, 4, true));
         if (h->prec != FLUB_PREC_FP32) TRY(mg_mixed_rscale_of(h, h->sp.r));
         TRY(mg_precondition_split(h, 0));

@@ -32,14 +32,20 @@ struct MixedState;
 
 // Level 0 in the colour-split layout (split.hip), used inside the FP32 solve.
 struct SplitBuf {
-    float *r, *zpre, *x;
+    float *r, *zpre;
+    // The solution x in FP64 (split layout).  Accumulating the FP32 updates
+    // alpha p in FP32, as the reference does, rounds x at every iteration:
+    // at 256^3 those roundings leave the FP64 true residual 11x above the
+    // recursive one when the 1e-6 test passes.  In FP64 the two agree, and
+    // the FP32 x handed back is x rounded once.
+    double *x;
     float *z[2];    // two z buffers: the solve preconditions speculatively into the spare one
     uint16_t *code;
     // mixed precision: r and pre-smoothed z of the V-cycle in FP16/BF16, the
     // post-smoothing's FP32 working z
     void *rh, *zh;
     float *zw;
-    float *b;       // mixed precision: the right-hand side (split), for the true residual
+    float *b;       // the right-hand side (split), for the FP64 true residual
     // split stencil codes of the split levels: lcode[0] = code; levels
     // 1 .. nsplit-1 keep their r, zpre and z in the level's own buffers
     uint16_t *lcode[MAXL];
@@ -89,34 +95,46 @@ struct flub_mgpcg {
     bool pair_l0;      // FP32 split level 0: the first pre- and post-smoothing half-sweeps in pairs
     bool mx_unit;      // BF16 split solve: storage scale 1, the r update stores the next cycle's BF16 r
     int rh_ready;      // z slot whose BF16 r the r update stored for the next V-cycle (-1: none)
+    bool x64_valid;    // sp.x holds the FP64 x of the last solve (h->x is it rounded to FP32)
+    double *x64n;      // the fused projection: the FP64 x in the reference layout, for the gradient
+    bool want_x64n;    // the next split solve also writes x64n
     SplitBuf sp;
     MixedState *mx;
 };
 
 // Elementwise CG vector updates over float4 vectors (n4 = n / 4), any layout.
 // DOP: p = beta p + z (mgpcg.py:229-232), z times zinv[0] when zinv is set
-// (the z buffer holds z times a power-of-two scale: exact).  DOX: the x
-// update of the previous iteration, x += alpha p (mgpcg.py:207-210), with the
-// p it used, deferred from the r update to this pass, which reads p anyway.
-// It is the same FP32 fmaf on the same values as an update in place at its
-// own iteration, so x is bitwise the same; the solve applies the last one
-// (DOP false) before x is read or returned.
-template <bool DOX, bool DOP = true>
-__global__ void __launch_bounds__(256) k_update_px(float *__restrict__ p, float *__restrict__ x,
+// (the z buffer holds z times a power-of-two scale: exact).  XM: the x update
+// of the previous iteration (mgpcg.py:207-210), x += alpha p with the p it
+// used, deferred from the r update to this pass, which reads p anyway
+// (XM_ADD), or x = alpha p when x is known to be zero (XM_SET: the first
+// update of a solve from x = 0, which then never reads or clears x); XM_NONE:
+// no x update.  x is FP64: alpha is the FP32 value the r update uses, and
+// (double)alpha * (double)p is exact, so x + alpha p has one rounding in
+// FP64.  The solve applies the last update (DOP false) before x is read or
+// returned.
+enum { XM_NONE = 0, XM_ADD = 1, XM_SET = 2 };
+template <int XM, bool DOP = true>
+__global__ void __launch_bounds__(256) k_update_px(float *__restrict__ p, double *__restrict__ x,
                                                    const float *__restrict__ z, const double *__restrict__ sc,
                                                    const double *__restrict__ zinv, uint32_t n4)
 {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (t >= n4) return;
     float4 pv = reinterpret_cast<const float4 *>(p)[t];
-    if (DOX) {
-        const float alpha = (float)sc[SC_ALPHA];
-        float4 xv = reinterpret_cast<const float4 *>(x)[t];
-        xv.x = fmaf(alpha, pv.x, xv.x);
-        xv.y = fmaf(alpha, pv.y, xv.y);
-        xv.z = fmaf(alpha, pv.z, xv.z);
-        xv.w = fmaf(alpha, pv.w, xv.w);
-        reinterpret_cast<float4 *>(x)[t] = xv;
+    if (XM != XM_NONE) {
+        const double alpha = (double)(float)sc[SC_ALPHA];
+        const float pp[4] = {pv.x, pv.y, pv.z, pv.w};
+        double xv[4] = {0.0, 0.0, 0.0, 0.0};
+        if (XM == XM_ADD) {
+            const double2 a = reinterpret_cast<const double2 *>(x)[2 * t];
+            const double2 b = reinterpret_cast<const double2 *>(x)[2 * t + 1];
+            xv[0] = a.x; xv[1] = a.y; xv[2] = b.x; xv[3] = b.y;
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) xv[e] = fma(alpha, (double)pp[e], xv[e]);
+        reinterpret_cast<double2 *>(x)[2 * t] = make_double2(xv[0], xv[1]);
+        reinterpret_cast<double2 *>(x)[2 * t + 1] = make_double2(xv[2], xv[3]);
     }
     if (DOP) {
         const float beta = (float)sc[SC_BETA];
@@ -134,15 +152,29 @@ __global__ void __launch_bounds__(256) k_update_px(float *__restrict__ p, float 
 }
 
 // k_update_px for n values (n % 4 == 0), on stream s.
-template <bool DOX, bool DOP = true>
-int launch_update_px(hipStream_t s, float *p, float *x, const float *z, const double *sc, const double *zinv,
+template <int XM, bool DOP = true>
+int launch_update_px(hipStream_t s, float *p, double *x, const float *z, const double *sc, const double *zinv,
                      uint32_t n)
 {
     FLUB_REQUIRE(n % 4 == 0, FLUB_EINVAL, "vector update: n = %u is not a multiple of 4", n);
     const uint32_t n4 = n / 4;
-    k_update_px<DOX, DOP><<<blocks_for(n4, 256), 256, 0, s>>>(p, x, z, sc, zinv, n4);
+    k_update_px<XM, DOP><<<blocks_for(n4, 256), 256, 0, s>>>(p, x, z, sc, zinv, n4);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
+}
+
+// The pending x update of a split solve: XM_ADD, or XM_SET while x has not
+// been written yet (x_set false: a solve that starts from x = 0).
+inline int update_px_pending(hipStream_t s, float *p, double *x, const float *z, const double *sc,
+                             const double *zinv, uint32_t n, bool &x_set, bool dop)
+{
+    int rc;
+    if (dop) rc = x_set ? launch_update_px<XM_ADD, true>(s, p, x, z, sc, zinv, n)
+                        : launch_update_px<XM_SET, true>(s, p, x, z, sc, zinv, n);
+    else rc = x_set ? launch_update_px<XM_ADD, false>(s, p, x, z, sc, zinv, n)
+                    : launch_update_px<XM_SET, false>(s, p, x, z, sc, zinv, n);
+    if (rc == FLUB_OK) x_set = true;
+    return rc;
 }
 
 int mg_dot(flub_mgpcg *h, const float *a, const float *b, int op, double eps);
@@ -162,6 +194,16 @@ int mg_rtr_sent(flub_mgpcg *h);
 double mg_host_partial_sum(const double *part, int nblk);
 // After a solve whose r.r stayed on the host: store it into sc[SC_RTR].
 int mg_rtr_to_device(flub_mgpcg *h, double rTr);
+
+// The split solves' FP64 solution (SplitBuf::x).  mg_split_enter: r and x
+// into the split layout and b = r - ndt(x) for the true residual, unless the
+// fused projection wrote r and b already (x_set false: x is zero and not
+// written yet).  mg_split_leave: x rounded to FP32 into the reference-layout
+// h->x.  mg_split_true_residual: R = b - ndt(x) (FP64 stencil), *rTrue its
+// squared norm (synchronizes).
+int mg_split_enter(flub_mgpcg *h, bool &x_set);
+int mg_split_leave(flub_mgpcg *h, bool x_set);
+int mg_split_true_residual(flub_mgpcg *h, float *R, double *rTrue);
 
 // alpha = old_zTr / (sum of the partials + eps) (mgpcg.py:212-216)
 int mg_finish_alpha(flub_mgpcg *h, int nblk, double eps);

@@ -56,10 +56,12 @@ __global__ void __launch_bounds__(RBS) k_mx_update(float *__restrict__ r, float 
     }
 }
 
-// r = b - ndt(x) evaluated in FP64 (true residual), stored in FP32; partials
-// of |r|^2 (FP64, from the FP64 values) and max|r|.
+// r = b - ndt(x) evaluated in FP64 (true residual) from the FP32 or FP64 x,
+// stored in FP32 (r may be null); partials of |r|^2 (FP64, from the FP64
+// values) and max|r|.
+template <typename XT>
 __global__ void __launch_bounds__(RBS) k_true_residual(float *__restrict__ r, const float *__restrict__ b,
-                                                       const float *__restrict__ x, const uint8_t *__restrict__ tag,
+                                                       const XT *__restrict__ x, const uint8_t *__restrict__ tag,
                                                        int nx, int ny, int nz, double *__restrict__ part)
 {
     __shared__ double lds[RBS / 64];
@@ -84,7 +86,7 @@ __global__ void __launch_bounds__(RBS) k_true_residual(float *__restrict__ r, co
             }
             rv = (double)b[t] - s;
         }
-        r[t] = (float)rv;
+        if (r) r[t] = (float)rv;
         acc += rv * rv;
         mx = fmax(mx, fabs(rv));
     }
@@ -400,7 +402,7 @@ static int read_sc(flub_mgpcg *h, int idx, double *out)
 static int true_residual(flub_mgpcg *h, int nb)
 {
     Level &L0 = h->lv[0];
-    k_true_residual<<<nb, RBS, 0, h->s>>>(L0.r, h->mx->b, h->x, L0.tag, L0.nx, L0.ny, L0.nz, h->part);
+    k_true_residual<float><<<nb, RBS, 0, h->s>>>(L0.r, h->mx->b, h->x, L0.tag, L0.nx, L0.ny, L0.nz, h->part);
     FLUB_CHECK_LAUNCH();
     k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nb, h->sc, h->mx->shift);
     FLUB_CHECK_LAUNCH();
@@ -408,35 +410,29 @@ static int true_residual(flub_mgpcg *h, int nb)
 }
 
 // Flexible PCG (Polak-Ribiere) in FP32 with FP64 reductions around the
-// reduced-precision V-cycle.  Same stopping rule as mgpcg.py:393,444
-// (squared norms of the recursive residual).  When it passes, the true
-// residual b - ndt(x) is recomputed in FP64 and replaces r; the solve stops
-// unless it exceeds the tolerance by more than the drift allowance (then up
-// to max_replacements more rounds).  *rTr reports that FP64 true residual.
-// The same flexible PCG with level 0 in the colour-split layout (split.hip):
-// the V-cycle stores r and the pre-smoothed z of level 0 in FP16/BF16 and
-// keeps the post-smoothing (where z carries the coarse correction) and the
-// coarse levels in FP32.  As in the FP32 solve, the next V-cycle is queued
-// before the host tests the residual, into the spare z buffer, unless this
-// test is predicted to stop the solve.
+// reduced-precision V-cycle, with level 0 in the colour-split layout
+// (split.hip): the V-cycle stores r and the pre-smoothed z of level 0 in
+// FP16/BF16 and keeps the post-smoothing (where z carries the coarse
+// correction) and the coarse levels in FP32; x accumulates in FP64.  Same
+// stopping rule as mgpcg.py:393,444 (squared norms of the recursive
+// residual); when it passes, the true residual b - ndt(x) is recomputed in
+// FP64 and replaces r, and the solve stops if it passes the same test (else
+// up to max_replacements more rounds): the rule of the FP32 split solve.
+// *rTr reports that FP64 true residual.  As in the FP32 solve, the next
+// V-cycle is queued before the host tests the residual, into the spare z
+// buffer, unless this test is predicted to stop the solve.
 static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double abs_tol, double rel_tol, int verbose,
                              int *iters, double *rTr_out)
 {
     Level &L0 = h->lv[0];
     const uint32_t n = L0.n;
     const int nb = red_nb(n);
-    // The check catches drift of the recursive residual beyond what the FP32
-    // reference algorithm itself shows (about 10x in norm at 256^3).
-    const double drift = 400.0;   // squared norms: 20x in norm
     const int max_replacements = 3;
     int replacements = 0;
     SplitBuf &S = h->sp;
     h->rh_ready = -1;
-    if (!h->presplit) {   // (the fused projection's divergence kernel wrote S.r, S.b and S.x)
-        TRYM(split_convert(h->s, L0, L0.r, S.r, 4, true));
-        FLUB_CHECK_HIP(hipMemcpyAsync(S.b, S.r, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
-        TRYM(split_convert(h->s, L0, h->x, S.x, 4, true));
-    }
+    bool x_set = true;
+    TRYM(mg_split_enter(h, x_set));
     // the tolerance is first needed after iteration 1: the first V-cycle runs
     // while the initial r.r travels to the host
     TRYM(mg_mixed_rscale_of(h, S.r, true));
@@ -455,8 +451,8 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
         const bool speculate = it + 1 < cap && r_prev * rate > 16.0 * tol;
         // p = z + beta p (and the deferred x update), then q = ndt(p) and p.q
         int nbp = 0;
-        TRYM(x_pending ? launch_update_px<true>(h->s, h->p, S.x, S.z[zs], h->sc, h->sc + SC_ZINV0 + zs, n)
-                       : launch_update_px<false>(h->s, h->p, S.x, S.z[zs], h->sc, h->sc + SC_ZINV0 + zs, n));
+        if (x_pending) TRYM(update_px_pending(h->s, h->p, S.x, S.z[zs], h->sc, h->sc + SC_ZINV0 + zs, n, x_set, true));
+        else TRYM(launch_update_px<XM_NONE>(h->s, h->p, S.x, S.z[zs], h->sc, h->sc + SC_ZINV0 + zs, n));
         x_pending = false;
         TRYM(split_Ap(h->s, L0, h->p, h->Ap, S.code, h->part, &nbp));
         TRYM(mg_finish_alpha(h, nbp, eps));
@@ -488,9 +484,9 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
         if (!isfinite(rTr)) { status = FLUB_ENONFINITE; flub_set_error("non-finite residual at iteration %d", it + 1); break; }
         bool redo = false;   // the speculative cycle used an r that was then replaced
         if (rTr < tol) {
-            // the reference's test passed on the recursive residual: check the
-            // FP64 true residual for drift (it replaces r either way)
-            TRYM((launch_update_px<true, false>(h->s, h->p, S.x, nullptr, h->sc, nullptr, n)));
+            // the reference's test passed on the recursive residual: the FP64
+            // true residual replaces r and decides
+            TRYM(update_px_pending(h->s, h->p, S.x, nullptr, h->sc, nullptr, n, x_set, false));
             x_pending = false;
             int nbt = 0;
             TRYM(split_true_residual(h->s, L0, S.r, S.b, S.x, S.code, h->part, &nbt));
@@ -502,7 +498,9 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
             FLUB_CHECK_HIP(hipEventSynchronize(h->ev));
             const double rTrue = h->hsc[SC_RTR];
             if (verbose) printf("  true |residual|_2=%.9g\n", sqrt(rTrue));
-            if (rTrue < drift * tol || replacements >= max_replacements) { rTr = rTrue; break; }
+            rTr = rTrue;
+            if (!isfinite(rTrue)) { status = FLUB_ENONFINITE; flub_set_error("non-finite true residual"); break; }
+            if (rTrue < tol || replacements >= max_replacements) break;
             ++replacements;
             redo = true;
         }
@@ -514,10 +512,10 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
         FLUB_CHECK_LAUNCH();
         it += 1;
     }
-    if (x_pending) TRYM((launch_update_px<true, false>(h->s, h->p, S.x, nullptr, h->sc, nullptr, n)));
+    if (x_pending) TRYM(update_px_pending(h->s, h->p, S.x, nullptr, h->sc, nullptr, n, x_set, false));
     if (h->mx_unit && h->hpart_dev) TRYM(mg_rtr_to_device(h, rTr));
     // back to the reference layout (S.zw is free scratch now)
-    TRYM(split_convert(h->s, L0, S.x, h->x, 4, false));
+    TRYM(mg_split_leave(h, x_set));
     if (!h->x_only) {
         TRYM(split_convert(h->s, L0, S.r, L0.r, 4, false));
         TRYM(split_convert(h->s, L0, S.z[zs], L0.z, 4, false));
@@ -611,7 +609,24 @@ FLUB_API int flub_mgpcg_residual_norm(flub_mgpcg *h, const float *x, const float
     Level &L0 = h->lv[0];
     const int nb = red_nb(L0.n);
     double *out = h->part + 2 * MAXPART - 1;   // beyond the 2*nb partials
-    k_true_residual<<<nb, RBS, 0, h->s>>>(h->p_alt, b, x, L0.tag, L0.nx, L0.ny, L0.nz, h->part);
+    k_true_residual<float><<<nb, RBS, 0, h->s>>>(h->p_alt, b, x, L0.tag, L0.nx, L0.ny, L0.nz, h->part);
+    FLUB_CHECK_LAUNCH();
+    k_sum_partials<<<1, RBS, 0, h->s>>>(h->part, nb, out);
+    FLUB_CHECK_LAUNCH();
+    FLUB_CHECK_HIP(hipMemcpyAsync(h->hsc + SC_AUX3, out, sizeof(double), hipMemcpyDeviceToHost, h->s));
+    FLUB_CHECK_HIP(hipStreamSynchronize(h->s));
+    *rr = h->hsc[SC_AUX3];
+    return FLUB_OK;
+}
+
+FLUB_API int flub_mgpcg_true_residual_f64(flub_mgpcg *h, const double *x, const float *b, float *r, double *rr)
+{
+    FLUB_REQUIRE(h && x && b && rr, FLUB_EINVAL, "flub_mgpcg_true_residual_f64: null argument");
+    FLUB_REQUIRE(h->tags_set, FLUB_ESTATE, "true_residual_f64 before set_tags");
+    Level &L0 = h->lv[0];
+    const int nb = red_nb(L0.n);
+    double *out = h->part + 2 * MAXPART - 1;   // beyond the 2*nb partials
+    k_true_residual<double><<<nb, RBS, 0, h->s>>>(r, b, x, L0.tag, L0.nx, L0.ny, L0.nz, h->part);
     FLUB_CHECK_LAUNCH();
     k_sum_partials<<<1, RBS, 0, h->s>>>(h->part, nb, out);
     FLUB_CHECK_LAUNCH();

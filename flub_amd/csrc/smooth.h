@@ -150,6 +150,12 @@ int split_restrict(hipStream_t s, const Level &F, const float *r, const float *z
 int split_Ap(hipStream_t s, const Level &L0, const float *p, float *Ap, const uint16_t *code, double *part, int *nblk);
 // x *= f[0] (a device scalar).
 int scale_by_device(hipStream_t s, float *x, const double *f, uint32_t n);
-// r = b - ndt(x) in FP64 (stored FP32), partials of r.r and max|r| (split).
-int split_true_residual(hipStream_t s, const Level &L0, float *r, const float *b, const float *x,
+// r = b - ndt(x) in FP64 from the FP64 x (stored FP32), partials of r.r and
+// max|r| (split; r, b and x are distinct buffers).
+int split_true_residual(hipStream_t s, const Level &L0, float *r, const float *b, const double *x,
                         const uint16_t *code, double *part, int *nblk);
+// The FP64 solution between the reference layout and the split layout:
+// x (FP32, reference layout) -> x64 (split); x64 (split) -> x (FP32, rounded
+// once) and/or x64_natural (FP64, reference layout), either may be null.
+int split_x_from_natural(hipStream_t s, const Level &L0, const float *x, double *x64);
+int split_x_to_natural(hipStream_t s, const Level &L0, const double *x64, float *x, double *x64_natural);

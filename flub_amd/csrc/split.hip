@@ -73,6 +73,17 @@ __device__ __forceinline__ void st_f4(float *p, const float v[4])
 {
     *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
 }
+// four consecutive values (16-byte aligned FP32 or 32-byte aligned FP64), widened to FP64
+__device__ __forceinline__ void d4(const float *p, double v[4])
+{
+    const float4 a = *reinterpret_cast<const float4 *>(p);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+__device__ __forceinline__ void d4(const double *p, double v[4])
+{
+    const double2 a = reinterpret_cast<const double2 *>(p)[0], b = reinterpret_cast<const double2 *>(p)[1];
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+}
 
 struct SSweepArgs {
     const void *zin;       // TZI split z (PLAIN, PROLONG)
@@ -553,11 +564,12 @@ __global__ void __launch_bounds__(256) k_sp_Ap(const float *__restrict__ p, floa
 }
 
 // True residual r = b - ndt(x) with the stencil evaluated in FP64 (from the
-// FP32 x), stored as FP32 r; 0 off FLUID cells.  Partials: sum of r^2 (FP64
-// values) in part[block], max |r| in part[nblk + block].  Split layout, both
-// colours at each position.
+// FP32 or FP64 x), stored as FP32 r; 0 off FLUID cells.  Partials: sum of r^2
+// (FP64 values) in part[block], max |r| in part[nblk + block].  Split layout,
+// both colours at each position.
+template <typename XT>
 __global__ void __launch_bounds__(256) k_strue_residual(float *__restrict__ r, const float *__restrict__ b,
-                                                        const float *__restrict__ x,
+                                                        const XT *__restrict__ x,
                                                         const uint16_t *__restrict__ code, SGeo g,
                                                         double *__restrict__ part)
 {
@@ -571,26 +583,26 @@ __global__ void __launch_bounds__(256) k_strue_residual(float *__restrict__ r, c
         int i, j, m0;
         g.decode(t, i, j, m0);
         const uint32_t base = g.row(i, j) + m0;
-        float xs[2][4];
-        f4(x + base, xs[0]);
-        f4(x + g.half + base, xs[1]);
-        const float l0 = __shfl_up(xs[0][3], 1, 64), r0 = __shfl_down(xs[0][0], 1, 64);
-        const float l1 = __shfl_up(xs[1][3], 1, 64), r1 = __shfl_down(xs[1][0], 1, 64);
+        double xs[2][4];
+        d4(x + base, xs[0]);
+        d4(x + g.half + base, xs[1]);
+        const double l0 = __shfl_up(xs[0][3], 1, 64), r0 = __shfl_down(xs[0][0], 1, 64);
+        const double l1 = __shfl_up(xs[1][3], 1, 64), r1 = __shfl_down(xs[1][0], 1, 64);
 #pragma unroll
         for (int cc = 0; cc < 2; ++cc) {
             const uint32_t own = (uint32_t)cc * g.half, oth = (uint32_t)(cc ^ 1) * g.half;
             const int o = (cc - i - j) & 1;
-            const float *w = xs[cc ^ 1];
-            float xp[4], xm[4], yp[4], ym[4];   // rows clamped to the domain (walls: class 0)
-            f4(x + oth + g.row(min(i + 1, g.nx - 1), j) + m0, xp);
-            f4(x + oth + g.row(max(i - 1, 0), j) + m0, xm);
-            f4(x + oth + g.row(i, min(j + 1, g.ny - 1)) + m0, yp);
-            f4(x + oth + g.row(i, max(j - 1, 0)) + m0, ym);
-            float edge = 0.0f;
+            const double *w = xs[cc ^ 1];
+            double xp[4], xm[4], yp[4], ym[4];   // rows clamped to the domain (walls: class 0)
+            d4(x + oth + g.row(min(i + 1, g.nx - 1), j) + m0, xp);
+            d4(x + oth + g.row(max(i - 1, 0), j) + m0, xm);
+            d4(x + oth + g.row(i, min(j + 1, g.ny - 1)) + m0, yp);
+            d4(x + oth + g.row(i, max(j - 1, 0)) + m0, ym);
+            double edge = 0.0;
             if (o == 0) {
-                if (m0 > 0) edge = lane > 0 ? (cc ? l0 : l1) : x[oth + base - 1];
+                if (m0 > 0) edge = lane > 0 ? (cc ? l0 : l1) : (double)x[oth + base - 1];
             } else {
-                if (m0 + 4 < g.nh) edge = (lane < 63 && t + 1 < ch.end) ? (cc ? r0 : r1) : x[oth + base + 4];
+                if (m0 + 4 < g.nh) edge = (lane < 63 && t + 1 < ch.end) ? (cc ? r0 : r1) : (double)x[oth + base + 4];
             }
             uint32_t cd[4];
             codes4(code, own + base, cd);
@@ -599,16 +611,16 @@ __global__ void __launch_bounds__(256) k_strue_residual(float *__restrict__ r, c
             float out[4];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                float kp, km;
+                double kp, km;
                 if (o == 0) { kp = w[e]; km = e > 0 ? w[e - 1] : edge; }
                 else { km = w[e]; kp = e < 3 ? w[e + 1] : edge; }
                 const double xI = xs[cc][e];
-                const float nbv[6] = {xp[e], xm[e], yp[e], ym[e], kp, km};
+                const double nbv[6] = {xp[e], xm[e], yp[e], ym[e], kp, km};
                 double sacc = 0.0;
 #pragma unroll
                 for (int d = 0; d < 6; ++d) {
                     const uint32_t cls = (cd[e] >> (2 * d)) & 3u;
-                    if (cls != 0u) sacc += (cls == 2u ? (double)nbv[d] : 0.0) - xI;
+                    if (cls != 0u) sacc += (cls == 2u ? nbv[d] : 0.0) - xI;
                 }
                 const double rv = code_fluid(cd[e]) ? (double)bv[e] - sacc : 0.0;
                 out[e] = (float)rv;
@@ -631,8 +643,8 @@ __global__ void __launch_bounds__(256) k_strue_residual(float *__restrict__ r, c
 // Reference layout <-> split layout, one reference quad (4 cells along k)
 // per thread: cells k0, k0+2 go to one half, k0+1, k0+3 to the other, both
 // at m = k0/2, k0/2 + 1.
-template <typename T, bool TO_SPLIT>
-__global__ void __launch_bounds__(256) k_split_conv(const T *__restrict__ src, T *__restrict__ dst, SGeo g)
+template <typename TS, typename TD, bool TO_SPLIT>
+__global__ void __launch_bounds__(256) k_split_conv(const TS *__restrict__ src, TD *__restrict__ dst, SGeo g)
 {
     const uint32_t nzq = (uint32_t)g.nh / 2;   // reference quads per row
     const uint32_t total = (uint32_t)g.nx * g.ny * nzq;
@@ -645,13 +657,35 @@ __global__ void __launch_bounds__(256) k_split_conv(const T *__restrict__ src, T
         const uint32_t s0 = c0 * g.half + row * (uint32_t)g.nh + (uint32_t)(k0 >> 1);
         const uint32_t s1 = (c0 ^ 1u) * g.half + row * (uint32_t)g.nh + (uint32_t)(k0 >> 1);
         if (TO_SPLIT) {
-            const T v0 = src[nat], v1 = src[nat + 1], v2 = src[nat + 2], v3 = src[nat + 3];
-            dst[s0] = v0; dst[s0 + 1] = v2;
-            dst[s1] = v1; dst[s1 + 1] = v3;
+            const TS v0 = src[nat], v1 = src[nat + 1], v2 = src[nat + 2], v3 = src[nat + 3];
+            dst[s0] = (TD)v0; dst[s0 + 1] = (TD)v2;
+            dst[s1] = (TD)v1; dst[s1 + 1] = (TD)v3;
         } else {
-            const T v0 = src[s0], v2 = src[s0 + 1], v1 = src[s1], v3 = src[s1 + 1];
-            dst[nat] = v0; dst[nat + 1] = v1; dst[nat + 2] = v2; dst[nat + 3] = v3;
+            const TS v0 = src[s0], v2 = src[s0 + 1], v1 = src[s1], v3 = src[s1 + 1];
+            dst[nat] = (TD)v0; dst[nat + 1] = (TD)v1; dst[nat + 2] = (TD)v2; dst[nat + 3] = (TD)v3;
         }
+    }
+}
+
+// The split FP64 solution to the reference layout, rounded to FP32 into x32
+// and as it is into x64 (k_split_conv's mapping, both outputs in one pass).
+__global__ void __launch_bounds__(256) k_split_x_out(const double *__restrict__ src, float *__restrict__ x32,
+                                                     double *__restrict__ x64, SGeo g)
+{
+    const uint32_t nzq = (uint32_t)g.nh / 2;
+    const uint32_t total = (uint32_t)g.nx * g.ny * nzq;
+    for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < total; t += gridDim.x * 256u) {
+        const uint32_t row = t / nzq, kq = t - row * nzq;
+        const int i = (int)(row / (uint32_t)g.ny), j = (int)(row - (uint32_t)i * g.ny);
+        const int k0 = 4 * (int)kq;
+        const uint32_t nat = row * (uint32_t)(2 * g.nh) + (uint32_t)k0;
+        const uint32_t c0 = (uint32_t)((i + j + k0) & 1);
+        const uint32_t s0 = c0 * g.half + row * (uint32_t)g.nh + (uint32_t)(k0 >> 1);
+        const uint32_t s1 = (c0 ^ 1u) * g.half + row * (uint32_t)g.nh + (uint32_t)(k0 >> 1);
+        const double v0 = src[s0], v2 = src[s0 + 1], v1 = src[s1], v3 = src[s1 + 1];
+        *reinterpret_cast<float4 *>(x32 + nat) = make_float4((float)v0, (float)v1, (float)v2, (float)v3);
+        reinterpret_cast<double2 *>(x64 + nat)[0] = make_double2(v0, v1);
+        reinterpret_cast<double2 *>(x64 + nat)[1] = make_double2(v2, v3);
     }
 }
 
@@ -747,12 +781,32 @@ int split_convert(hipStream_t s, const Level &L0, const void *src, void *dst, in
     const SGeo g = make_sgeo(L0.nx, L0.ny, L0.nz);
     const unsigned nb = blocks_for(L0.n / 4, 256) < 4096 ? blocks_for(L0.n / 4, 256) : 4096;
     if (elem_bytes == 4) {
-        if (to_split) k_split_conv<float, true><<<nb, 256, 0, s>>>((const float *)src, (float *)dst, g);
-        else k_split_conv<float, false><<<nb, 256, 0, s>>>((const float *)src, (float *)dst, g);
+        if (to_split) k_split_conv<float, float, true><<<nb, 256, 0, s>>>((const float *)src, (float *)dst, g);
+        else k_split_conv<float, float, false><<<nb, 256, 0, s>>>((const float *)src, (float *)dst, g);
     } else {
-        if (to_split) k_split_conv<uint16_t, true><<<nb, 256, 0, s>>>((const uint16_t *)src, (uint16_t *)dst, g);
-        else k_split_conv<uint16_t, false><<<nb, 256, 0, s>>>((const uint16_t *)src, (uint16_t *)dst, g);
+        if (to_split) k_split_conv<uint16_t, uint16_t, true><<<nb, 256, 0, s>>>((const uint16_t *)src, (uint16_t *)dst, g);
+        else k_split_conv<uint16_t, uint16_t, false><<<nb, 256, 0, s>>>((const uint16_t *)src, (uint16_t *)dst, g);
     }
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+int split_x_from_natural(hipStream_t s, const Level &L0, const float *x, double *x64)
+{
+    const SGeo g = make_sgeo(L0.nx, L0.ny, L0.nz);
+    const unsigned nb = blocks_for(L0.n / 4, 256) < 4096 ? blocks_for(L0.n / 4, 256) : 4096;
+    k_split_conv<float, double, true><<<nb, 256, 0, s>>>(x, x64, g);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+int split_x_to_natural(hipStream_t s, const Level &L0, const double *x64, float *x, double *x64_natural)
+{
+    const SGeo g = make_sgeo(L0.nx, L0.ny, L0.nz);
+    const unsigned nb = blocks_for(L0.n / 4, 256) < 4096 ? blocks_for(L0.n / 4, 256) : 4096;
+    if (x && x64_natural) k_split_x_out<<<nb, 256, 0, s>>>(x64, x, x64_natural, g);
+    else if (x) k_split_conv<double, float, false><<<nb, 256, 0, s>>>(x64, x, g);
+    else if (x64_natural) k_split_conv<double, double, false><<<nb, 256, 0, s>>>(x64, x64_natural, g);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
 }
@@ -873,13 +927,13 @@ int scale_by_device(hipStream_t s, float *x, const double *f, uint32_t n)
     return FLUB_OK;
 }
 
-int split_true_residual(hipStream_t s, const Level &L0, float *r, const float *b, const float *x,
+int split_true_residual(hipStream_t s, const Level &L0, float *r, const float *b, const double *x,
                         const uint16_t *code, double *part, int *nblk)
 {
     const SGeo g = make_sgeo(L0.nx, L0.ny, L0.nz);
     const unsigned nb = grid8((uint32_t)L0.nx * L0.ny * g.nq);
     *nblk = (int)nb;
-    k_strue_residual<<<nb, 256, 0, s>>>(r, b, x, code, g, part);
+    k_strue_residual<double><<<nb, 256, 0, s>>>(r, b, x, code, g, part);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
 }

@@ -11,12 +11,14 @@ on FLUID cells), and falls back to MGPCG for good as soon as one iteration
 reduces |r| by less than the factor `stagnation` (MGPCG as automatic
 fallback).  The fallback hands the current residual to the library's fused
 MGPCG solve on the same handle (FP32, or the FP16/BF16 V-cycle of a mixed
-handle) with the absolute tolerance of the original test, and adds its
+handle) with the absolute tolerance of the original test, and adds its FP64
 solution to x; with handover=False it continues this loop with the V-cycle
-as preconditioner instead.  When the recursive residual passes the test, the
-FP64 true residual of x is recomputed with the library's operator (MGPCG as
-judge of the error); if it exceeds the tolerance by more than 4x it replaces
-r and the solve continues on MGPCG.
+as preconditioner instead.  x accumulates in FP64, as in the library's
+colour-split solves.  When the recursive residual passes the test, the FP64
+true residual b - A x is recomputed with the library's operator (MGPCG as
+judge of the error); the solve stops if it passes the same test, else it
+replaces r and the solve continues on MGPCG (the library solves' rule).  The
+returned x is the FP64 solution rounded to FP32.
 """
 from __future__ import annotations
 
@@ -24,6 +26,7 @@ import ctypes
 import math
 import time
 
+import numpy as np
 import torch
 
 from .. import _lib
@@ -43,6 +46,14 @@ class LearnedPCG:
 
     def apply_A(self, x, y):
         _lib.call("flub_mgpcg_apply_A", self.mg.handle, _p(x), _p(y))
+
+    def true_residual(self, x64, b, r_out=None):
+        """||b - A x||^2 over the FLUID cells with the FP64 stencil from the FP64
+        x; r_out (FP32, optional) receives b - A x."""
+        rr = ctypes.c_double()
+        _lib.call("flub_mgpcg_true_residual_f64", self.mg.handle, _p(x64), _p(b),
+                  _p(r_out) if r_out is not None else None, ctypes.byref(rr))
+        return rr.value
 
     def vcycle(self, r, z):
         _lib.call("flub_mgpcg_precondition_into", self.mg.handle, _p(r), _p(z), 1)
@@ -69,24 +80,23 @@ class LearnedPCG:
 
     def mg_solve(self, r, tol, max_iters):
         """The library's fused MGPCG on residual r down to r.r < tol; returns
-        (e, iterations) with e a view of the handle's x (valid until its next solve)."""
-        import torch
+        (e, iterations) with e its FP64 solution."""
         self.mg.init(self._as_device(r), 1.0)
         self.mg.mask_interior()
         its = self.mg.solve(max_iters, abs_tol=tol, rel_tol=0.0, smoother="phases")
-        e = torch.empty_like(r)
-        _lib.call("flub_memcpy_d2d", e.data_ptr(), self.mg.x.ptr, e.numel() * 4, None)
+        e = torch.empty(r.shape, dtype=torch.float64, device=r.device)
+        _lib.call("flub_mgpcg_solution_f64", self.mg.handle, _p(e))
         return e, its
 
     def solve(self, b, fluid, max_iters=-1, rel_tol=1e-12, abs_tol=0.0, eps=1e-12, use_net=True, handover=True):
         """b: FP32 right-hand side (FLUID-masked); fluid: bool mask.  Returns
-        (x, stats)."""
+        (x, stats) with x the FP32 solution (the FP64 one rounded)."""
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        x = torch.zeros_like(b)
+        x = torch.zeros(b.shape, dtype=torch.float64, device=b.device)
         r = b.clone()
         q = torch.empty_like(b)
-        ax = torch.empty_like(b)
+        rt = torch.empty_like(b)
         z = torch.empty_like(b)
         rr0 = self.dot(r, r)
         tol = max(abs_tol, rr0 * rel_tol)
@@ -95,18 +105,18 @@ class LearnedPCG:
         cap = max_iters if max_iters >= 0 else 20000
         it = 0
         rr = rr0
+        true_rr = None
 
         def judge():
-            """FP64 true residual with the library's operator; on drift (> 4x the
-            tolerance) r takes it (at most max_replacements times).  True: stop."""
-            nonlocal rr
-            self.apply_A(x, ax)
-            true_r = b.double() - ax.double()
-            true_rr = float((true_r ** 2).sum())
-            if true_rr <= 4.0 * tol or stats["replacements"] >= self.max_replacements:
+            """FP64 true residual of the FP64 x with the library's operator: the
+            solve stops when it passes the test, else r takes it (at most
+            max_replacements times)."""
+            nonlocal rr, true_rr
+            true_rr = self.true_residual(x, b, rt)
+            if true_rr < tol or stats["replacements"] >= self.max_replacements:
                 return True
             stats["replacements"] += 1
-            r.copy_(true_r.float())
+            r.copy_(rt)
             rr = true_rr
             return False
 
@@ -137,7 +147,7 @@ class LearnedPCG:
                 zr = self.dot(z, r)
                 restart = False
             self.apply_A(p, q)
-            alpha = zr / (self.dot(p, q) + eps)
+            alpha = float(np.float32(zr / (self.dot(p, q) + eps)))   # the FP32 alpha of both updates
             x.add_(p, alpha=alpha)
             r.add_(q, alpha=-alpha)
             rr_new = self.dot(r, r)
@@ -161,9 +171,13 @@ class LearnedPCG:
             beta = -alpha * self.dot(z, q) / (zr + eps)
             zr = self.dot(z, r)
             p = z + beta * p
-        # judge: FP64 true residual with the library's operator
-        self.apply_A(x, ax)
-        true_rr = float(((b.double() - ax.double()) ** 2).sum())
+        if true_rr is None:   # stopped by the iteration cap: judge once for the report
+            true_rr = self.true_residual(x, b)
+        x32 = x.float()
         torch.cuda.synchronize()
-        stats.update(iterations=it, rTr=rr, true_rTr=true_rr, rTr0=rr0, ms=(time.perf_counter() - t0) * 1e3)
-        return x, stats
+        ms = (time.perf_counter() - t0) * 1e3
+        # the accuracy of the FP32 solution handed back (outside the timing)
+        true_rr32 = self.true_residual(x32.double(), b)
+        stats.update(iterations=it, rTr=rr, true_rTr=true_rr, true_rTr_fp32=true_rr32, rTr0=rr0, ms=ms)
+        self.last_x64 = x   # the FP64 solution (x32 is it rounded)
+        return x32, stats

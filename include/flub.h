@@ -214,7 +214,12 @@ int flub_mgpcg_inner(flub_mgpcg *h, int buf_a, int buf_b, int scalar_out);
 /* MGPCG.solve (mgpcg.py:378-462).  max_iters = -1 for no limit; stops when
  * r.r < max(abs_tol, rel_tol * r0.r0) (squared norms).  *iters receives the
  * completed-iteration count, *rTr the last r.r.  verbose prints the residual
- * of every iteration as the reference does. */
+ * of every iteration as the reference does.  The colour-split solves ("phases"
+ * on grids with nz % 8 == 0 and even nx, ny; every precision) accumulate x in
+ * FP64 and, once the recursive r.r passes the test, recompute r = b - A x in
+ * FP64: they stop when that true residual passes the same test (else r is
+ * replaced and the solve goes on, at most 3 times), and *rTr is its squared
+ * norm.  The other solves are the reference's FP32 algorithm. */
 int flub_mgpcg_solve(flub_mgpcg *h, int max_iters, double eps, double abs_tol, double rel_tol,
                      int smoother, int verbose, int *iters, double *rTr);
 /* compute_Ap_mcadams(Ap, p, tag) (mgpcg.py:36-39) on caller buffers of the
@@ -225,6 +230,18 @@ int flub_mgpcg_apply_A(flub_mgpcg *h, const float *x, float *y);
  * over the FLUID cells, the stencil evaluated in FP64 from the FP32 x and b
  * (caller buffers of the level-0 shape).  Synchronizes the handle's stream. */
 int flub_mgpcg_residual_norm(flub_mgpcg *h, const float *x, const float *b, double *rr);
+/* The same check for an FP64 solution (caller buffers of the level-0 shape,
+ * x FP64): *rr = ||b - neighbor_dot_tag(x)||^2 over the FLUID cells, the
+ * stencil in FP64; r (may be NULL) receives b - neighbor_dot_tag(x) rounded to
+ * FP32.  Synchronizes the handle's stream.  No counterpart in the reference. */
+int flub_mgpcg_true_residual_f64(flub_mgpcg *h, const double *x, const float *b, float *r, double *rr);
+/* The last solve's x in FP64 (reference layout, caller buffer of the level-0
+ * shape).  The colour-split solves (the "phases" smoother on grids with nz % 8
+ * == 0 and even nx, ny) accumulate x in FP64 and hand back x rounded to FP32
+ * in the x buffer; this returns the FP64 values until the next init, solve,
+ * update_x or profile call.  Otherwise it returns the FP32 x widened.  No
+ * counterpart in the reference (its x is FP32, mgpcg.py:207-210). */
+int flub_mgpcg_solution_f64(flub_mgpcg *h, double *x);
 /* One preconditioner application on caller buffers: r[0] <- r, V-cycle
  * (apply_preconditioner, mgpcg.py:296-349), z <- z[0]. */
 int flub_mgpcg_precondition_into(flub_mgpcg *h, const float *r, float *z, int smoother);
@@ -242,10 +259,10 @@ int flub_mgpcg_level_shape(flub_mgpcg *h, int level, int shape[3]);
  * kernel's algorithmic HBM bytes per launch.  Kernel ids (FLUB_PROF_*):
  * 0 level-0 RB half-sweep, 1 Ap + p.Ap, 2 r/x update + r.r, 3 whole V-cycle,
  * 4 restriction 0->1, 5 prolongation 1->0, 6 p update, 7 one temporally
- * blocked launch of four level-0 half-sweeps, 8/9/10 the half-sweep,
- * restriction and fused p update + Ap of the colour-split level 0 the FP32
- * solve runs on, 11 the p update and Ap as two kernels (FLUB_ESTATE when the
- * grid has no split level 0).  Leaves the handle's
+ * blocked launch of four level-0 half-sweeps, 8/9 the half-sweep and
+ * restriction of the colour-split level 0 the FP32 solve runs on, 10 its p
+ * step (p and FP64 x update, then Ap and p.Ap), 11 its r step (r -= alpha Ap
+ * and r.r) (8-10: FLUB_ESTATE when the grid has no split level 0).  Leaves the handle's
  * vectors in an unspecified state (call before init / after reading x). */
 int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_ms, double *bytes);
 

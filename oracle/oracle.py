@@ -71,6 +71,7 @@ def _declare(L):
             "orc_mgpcg_config": (None, [_P, _I, _I, _I, _I]),
             "orc_mgpcg_buf": (_P, [_P, _I, _I]),
             "orc_mgpcg_growth": (None, [_P, _I]),
+            "orc_mgpcg_mean_f64": (None, [_P, _I]),
             "orc_mgpcg_tag": (_P, [_P, _I]),
             "orc_restrict_tag": (None, [_P, _P, _I, _I, _I]),
             "orc_mgpcg_set_tags": (None, [_P, _P]),
@@ -281,6 +282,11 @@ class MGPCG:
         if h:
             _fn("orc_mgpcg_destroy", self.real)(h)
             self._h = None
+
+    def set_mean_f64(self, flag=True):
+        """zero_centered inside solve sums its mean in double (the HIP path's
+        accumulation) instead of the reference's serial FP32 sum."""
+        _fn("orc_mgpcg_mean_f64", self.real)(self._h, int(bool(flag)))
 
     def set_growth(self, growth):
         """Smoothing count growth per level (1: the reference's << l; 0: constant)."""

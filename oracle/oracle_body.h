@@ -308,6 +308,21 @@ void FN(orc_zero_centered)(REAL *q, const int32_t *tag, size_t n)
         q[I] = tag[I] == 1 ? q[I] - mean : (REAL)0.0;
 }
 
+/* zero_centered with the sum of the mean accumulated in double (mean_f64):
+ * the same serial order and subtraction; mean = (REAL)sum / (REAL)count, as
+ * the HIP path forms it. */
+static void FN(zero_centered_f64)(REAL *q, const int32_t *tag, size_t n)
+{
+    double sum = 0.0;
+    int cnt = 0;
+    for (size_t I = 0; I < n; ++I)
+        if (tag[I] == 1) { sum += (double)q[I]; cnt += 1; }
+    const REAL mean = (REAL)sum / (REAL)cnt;
+#pragma omp parallel for schedule(static)
+    for (long long I = 0; I < (long long)n; ++I)
+        q[I] = tag[I] == 1 ? q[I] - mean : (REAL)0.0;
+}
+
 /* warp.utils.array_inner (mgpcg.py:5): sum of a*b.  The accumulation type is a
  * parameter: dot_f64 = 1 accumulates in double (the design this build ships),
  * 0 in REAL, both in index order. */
@@ -344,6 +359,7 @@ typedef struct {
     double alpha, beta, sum, old_zTr;   /* device scalars of the reference */
     int dot_f64;       /* 1: inner products accumulate in double */
     int parallel_dot;  /* 1: OpenMP reduction (order depends on threads) */
+    int mean_f64;      /* 1: zero_centered sums the mean in double (0: the reference's REAL sum) */
 } FN(orc_mgpcg);
 
 FN(orc_mgpcg) *FN(orc_mgpcg_create)(int nx, int ny, int nz, int levels)
@@ -380,6 +396,11 @@ void FN(orc_mgpcg_config)(FN(orc_mgpcg) *m, int pre_post, int bottom, int dot_f6
 {
     m->pre_post = pre_post; m->bottom = bottom; m->dot_f64 = dot_f64; m->parallel_dot = parallel_dot;
 }
+
+/* The mean of zero_centered inside solve: 0 (default) sums it in REAL in
+ * serial launch order as the reference's atomic sum (mgpcg.py:191-196), 1 in
+ * double (the accumulation the HIP path uses). */
+void FN(orc_mgpcg_mean_f64)(FN(orc_mgpcg) *m, int mean_f64) { m->mean_f64 = mean_f64; }
 
 /* Smoothing growth per level: 1 = the reference's pre_and_post_smoothing << l
  * (mgpcg.py:304,343), 0 = the same count on every level. */
@@ -520,7 +541,10 @@ int FN(orc_mgpcg_solve)(FN(orc_mgpcg) *m, int max_iters, double eps, double abs_
                         int smoother, int verbose, double *rTr_out)
 {
     const size_t n = m->n[0];
-    if (smoother != 1) FN(orc_zero_centered)(m->r[0], m->tag[0], n);
+    if (smoother != 1) {
+        if (m->mean_f64) FN(zero_centered_f64)(m->r[0], m->tag[0], n);
+        else FN(orc_zero_centered)(m->r[0], m->tag[0], n);
+    }
     const double rTr0 = FN(rs)(m, FN(dot)(m, m->r[0], m->r[0]));
     const double tol = abs_tol > rTr0 * rel_tol ? abs_tol : rTr0 * rel_tol;
     FN(orc_mgpcg_apply_preconditioner)(m, smoother);
@@ -535,7 +559,10 @@ int FN(orc_mgpcg_solve)(FN(orc_mgpcg) *m, int max_iters, double eps, double abs_
         m->sum = FN(rs)(m, FN(dot)(m, m->p, m->Ap));
         m->alpha = FN(rs)(m, m->old_zTr / FN(rs)(m, m->sum + FN(rs)(m, eps)));
         FN(axpy)(m->r[0], (REAL)(-(REAL)m->alpha), m->Ap, n);
-        if (smoother != 1) FN(orc_zero_centered)(m->r[0], m->tag[0], n);
+        if (smoother != 1) {
+            if (m->mean_f64) FN(zero_centered_f64)(m->r[0], m->tag[0], n);
+            else FN(orc_zero_centered)(m->r[0], m->tag[0], n);
+        }
         rTr = FN(rs)(m, FN(dot)(m, m->r[0], m->r[0]));
         if (verbose) printf("after iter %d, |residual|_2=%.9g\n", it + 1, sqrt(rTr));
         if (rTr < tol || !isfinite(rTr)) {

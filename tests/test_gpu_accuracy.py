@@ -1,0 +1,165 @@
+"""Accuracy of the fused solves against the FP64 judge (through the C-ABI).
+
+The colour-split solves accumulate x in FP64 and stop only when the FP64 true
+residual b - A x of that x passes the reference's test
+(||r|| < 1e-6 ||b|| for rel_tol = 1e-12 on squared norms, mgpcg.py:393);
+the FP32 x they hand back is the FP64 x rounded once.  The fused projection
+subtracts the gradient of the FP64 x, so the projected velocity's divergence
+is the solve's residual (plus the FP32 rounding of the velocities).
+
+The residuals are recomputed here on the CPU with the FP64 oracle operator
+(neighbor_dot_tag, mgpcg.py:118-142), from the right-hand side the GPU
+divergence produced (bitwise the oracle's, tested elsewhere).
+
+Config 3 (flow past a sphere, 256^3, FP32 / FP16 / BF16 V-cycles) is checked
+at its own size against the converged FP64 oracle projection computed on the
+host: relative pressure error (bar 1e-5; the FP32 solve also within 1.5x the
+FP32 reference algorithm's own error, 1.38e-6 -- the BF16 V-cycle's solution
+at the same 1e-6 residual measured 7.6e-6), FP64 true residual and
+post-projection divergence.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from flub_amd import _lib
+from flub_amd import scenes
+from flub_amd.project import Projection
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def solution_f64(pr):
+    x = torch.empty(pr.shape, dtype=torch.float64, device="cuda")
+    _lib.call("flub_mgpcg_solution_f64", pr.mgpcg.handle, ctypes.c_void_p(x.data_ptr()))
+    torch.cuda.synchronize()
+    return x.cpu().numpy()
+
+
+def rhs(pr, tag):
+    """The solve's b: FLUID ? div * (-h) : 0 in FP32 (init + mask_interior)."""
+    div = pr.div.f.numpy()
+    b = (div * np.float32(-pr.h)).astype(np.float32)
+    b[tag != 1] = 0.0
+    return b
+
+
+def true_rel(oracle, x, b, tag):
+    ndt = oracle.compute_Ap(np.asarray(x, np.float64), tag, real=np.float64)
+    r = np.where(tag == 1, b.astype(np.float64) - ndt, 0.0)
+    return float(np.linalg.norm(r) / np.linalg.norm(b.astype(np.float64))), float(np.sum(r * r))
+
+
+def post_div_rel(oracle, pr, s):
+    fl = s["tag"] == 1
+    u, v, w = pr.velocity()
+    post = oracle.divergence_tag(u, v, w, s["ur"], s["vr"], s["wr"], s["tag"], real=np.float64)
+    pre = oracle.divergence_tag(s["u"], s["v"], s["w"], s["ur"], s["vr"], s["wr"], s["tag"], real=np.float64)
+    return float(np.linalg.norm(post[fl]) / np.linalg.norm(pre[fl]))
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16", "bf16"])
+@pytest.mark.parametrize("name,make", [("spheres_64", lambda: scenes.dam_break_spheres(64, 1)),
+                                       ("sphere_flow_128", lambda: scenes.flow_past_sphere(128, 2))])
+def test_split_solve_fp64_true_residual(name, make, precision, oracle):
+    s = make()
+    n = s["n"]
+    pr = Projection((n, n, n), n_mg_levels=4, pre_and_post_smoothing=2, bottom_smoothing=6, precision=precision)
+    pr.load(s)
+    it = pr.project(max_iters=-1, rel_tol=1e-12)
+    x64 = solution_f64(pr)
+    x32 = pr.mgpcg.x.numpy()
+    # the FP32 x handed back is the FP64 x rounded once
+    assert np.array_equal(x32.view(np.uint32), x64.astype(np.float32).view(np.uint32))
+    b = rhs(pr, s["tag"])
+    rel64, rr64 = true_rel(oracle, x64, b, s["tag"])
+    rel32, _ = true_rel(oracle, x32, b, s["tag"])
+    post = post_div_rel(oracle, pr, s)
+    print(f"{name} {precision}: {it} iterations, FP64 true rel. residual {rel64:.3e} (of the FP32 x {rel32:.3e}), "
+          f"post-projection divergence {post:.3e}")
+    assert rel64 < 1e-6
+    assert np.isclose(pr.rTr, rr64, rtol=1e-6)            # the solve's report is that residual
+    assert post < 1.5e-6                                    # the field is divergence-free to it
+
+
+def test_true_residual_f64_entry(oracle):
+    """flub_mgpcg_true_residual_f64 against the FP64 oracle operator."""
+    from flub_amd import device as wp
+    from flub_amd.multigrid.mgpcg import MGPCG
+    rng = np.random.default_rng(31)
+    tag = rng.choice(3, size=(32, 24, 16), p=[0.2, 0.7, 0.1]).astype(np.int32)
+    x = rng.normal(0, 1, tag.shape)
+    b = rng.normal(0, 1, tag.shape).astype(np.float32)
+    mg = MGPCG(3, tag.shape, 3)
+    mg.set_tags(wp.from_numpy(tag))
+    xd = torch.from_numpy(x).cuda()
+    bd = torch.from_numpy(b).cuda()
+    rd = torch.empty_like(bd)
+    rr = ctypes.c_double()
+    _lib.call("flub_mgpcg_true_residual_f64", mg.handle, ctypes.c_void_p(xd.data_ptr()),
+              ctypes.c_void_p(bd.data_ptr()), ctypes.c_void_p(rd.data_ptr()), ctypes.byref(rr))
+    ndt = oracle.compute_Ap(x, tag, real=np.float64)
+    res = np.where(tag == 1, b.astype(np.float64) - ndt, 0.0)
+    assert np.isclose(rr.value, float(np.sum(res * res)), rtol=1e-12)
+    # r is the FP64 value rounded to FP32 (the stencil's summation order differs
+    # from the oracle's: equal to within an FP32 rounding)
+    assert np.allclose(rd.cpu().numpy(), res, rtol=2 ** -23, atol=1e-12 * np.abs(res).max())
+
+
+def test_solution_f64_without_split_is_the_fp32_x(monkeypatch):
+    """Solves outside the colour-split layout keep the reference's FP32 x:
+    flub_mgpcg_solution_f64 then returns it widened."""
+    monkeypatch.setenv("FLUB_NO_SPLIT", "1")
+    s = scenes.dam_break(32, 0)
+    pr = Projection((32, 32, 32), n_mg_levels=3, pre_and_post_smoothing=2, bottom_smoothing=6)
+    pr.load(s)
+    pr.project(max_iters=-1, rel_tol=1e-12)
+    assert np.array_equal(solution_f64(pr), pr.mgpcg.x.numpy().astype(np.float64))
+
+
+# ---------------------------------------------------------------- config 3
+@pytest.fixture(scope="module")
+def sphere_flow_256(oracle):
+    """flow_past_sphere(256, 2) and its converged FP64 oracle projection, plus
+    the FP32 reference algorithm's pressure (both on the host CPU)."""
+    n = 256
+    s = scenes.flow_past_sphere(n, 2)
+    o64 = oracle.MGPCG((n,) * 3, 4, real=np.float64, pre_and_post_smoothing=2, bottom_smoothing=6)
+    _, _, _, p64, _, it64, _ = o64.project(s["u"], s["v"], s["w"], s["ur"], s["vr"], s["wr"], s["tag"], 1.0 / n,
+                                           rel_tol=1e-24)
+    del o64
+    o32 = oracle.MGPCG((n,) * 3, 4, real=np.float32, pre_and_post_smoothing=2, bottom_smoothing=6)
+    _, _, _, p32, _, it32, _ = o32.project(s["u"], s["v"], s["w"], s["ur"], s["vr"], s["wr"], s["tag"], 1.0 / n,
+                                           rel_tol=1e-12)
+    del o32
+    return s, p64, p32, it32
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16", "bf16"])
+def test_config3_sphere_flow_256_vs_fp64_oracle(precision, sphere_flow_256, oracle):
+    s, p64, p32, it32 = sphere_flow_256
+    n = s["n"]
+    pr = Projection((n, n, n), n_mg_levels=4, pre_and_post_smoothing=2, bottom_smoothing=6, precision=precision)
+    pr.load(s)
+    it = pr.project(max_iters=-1, rel_tol=1e-12)
+    p = pr.p0.f.numpy()
+    err, err_ref = _rel(p, p64), _rel(p32, p64)
+    x64 = solution_f64(pr)
+    rel64, _ = true_rel(oracle, x64, rhs(pr, s["tag"]), s["tag"])
+    post = post_div_rel(oracle, pr, s)
+    print(f"256^3 {precision}: {it} iterations (FP32 oracle {it32}), rel. pressure error vs FP64 {err:.3e} "
+          f"(FP32 reference algorithm {err_ref:.3e}), FP64 true rel. residual {rel64:.3e}, "
+          f"post-projection divergence {post:.3e}")
+    assert err < 1e-5                   # the config's bar
+    if precision == "fp32":             # as accurate as the reference algorithm itself
+        assert err < 1.5 * err_ref
+    assert rel64 < 1e-6
+    assert post < 1.5e-6

@@ -1,0 +1,141 @@
+"""The learned preconditioner on held-out scenes (SURVEY config 4): the
+committed network (trained on seeds 1000-2999 at 128^3) in flexible PCG with
+MGPCG fallback, on seeds >= 3000 at 128^3 and 256^3.
+
+The judge is the reference's solve (mgpcg.py:378-462) run to convergence in
+FP64 by the CPU oracle: at 128^3 the learned solution and the plain MGPCG
+solution of the same right-hand side are both compared with it, and both
+solvers must reach the same FP64 true residual (||b - A x|| < 1e-6 ||b||, the
+reference's test rel_tol = 1e-12 on squared norms), recomputed here on the
+CPU with the oracle's FP64 operator.  The error is measured in the energy
+norm ||e||_A / ||x*||_A, which a residual bounds (||e||_A^2 = r^T A^-1 r):
+on these scenes a 1e-6 residual leaves 1.1-1.7e-6 there (the FP64 oracle
+stopped at the same test), while the plain 2-norm error of the same
+solutions ranges over 1e-5 to 3e-4 with the smooth modes the residual barely
+sees.  At 256^3 (the oracle would take a minute per seed on the host) the
+same FP64 true residual is the size-independent check.
+"""
+import ctypes
+import os
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+WEIGHTS = os.path.join(ROOT, "flub_amd", "learned", "weights", "unet_128.npz")
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def net():
+    from flub_amd.learned.evaluate import load_net
+    return load_net(WEIGHTS)
+
+
+def _problem(n, seed):
+    import torch
+    from flub_amd import device as wp
+    from flub_amd.learned.data import divergence, scene_torch
+    from flub_amd.multigrid.mgpcg import MGPCG
+    tag, u, v, w = scene_torch(n, seed)
+    F = divergence(tag, u, v, w)
+    mg = MGPCG(3, (n, n, n), 4)
+    mg.pre_and_post_smoothing = 2
+    mg.bottom_smoothing = 6
+    mg.set_tags(wp.from_numpy(tag.cpu().numpy()))
+    torch.cuda.synchronize()
+    return tag, F, mg
+
+
+def _mgpcg_solve(mg, F):
+    """Plain MGPCG (the library's fused FP32 solve) on b = F; returns
+    (x FP64, iterations, true r.r, ms)."""
+    import torch
+    from flub_amd import _lib
+    from flub_amd.learned.data import _dev
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    mg.init(_dev(F), 1.0)
+    mg.mask_interior()
+    it = mg.solve(-1, abs_tol=0.0, rel_tol=1e-12, smoother="phases")
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) * 1e3
+    x = torch.empty(F.shape, dtype=torch.float64, device="cuda")
+    _lib.call("flub_mgpcg_solution_f64", mg.handle, ctypes.c_void_p(x.data_ptr()))
+    return x, it, mg.last_rTr, ms
+
+
+def _energy_err(oracle, x, xs, tag):
+    """||x - x*||_A / ||x*||_A with A = -neighbor_dot_tag (FP64 oracle operator)."""
+    e = np.asarray(x, np.float64) - xs
+    Ae = -oracle.compute_Ap(e, tag, real=np.float64)
+    Axs = -oracle.compute_Ap(xs, tag, real=np.float64)
+    return float(np.sqrt(abs(np.sum(e * Ae)) / np.sum(xs * Axs)))
+
+
+def _true_rel(oracle, x, F, tag):
+    b = np.where(tag == 1, F.astype(np.float64), 0.0)
+    r = np.where(tag == 1, b - oracle.compute_Ap(np.asarray(x, np.float64), tag, real=np.float64), 0.0)
+    return float(np.linalg.norm(r) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("seed", [3000, 3001])
+def test_learned_heldout_128_vs_fp64_oracle(seed, net, oracle):
+    from flub_amd.learned.fpcg import LearnedPCG
+    n = 128
+    tag, F, mg = _problem(n, seed)
+    fluid = tag == 1
+    solver = LearnedPCG(mg, net)
+    x, st = solver.solve(F, fluid, rel_tol=1e-12)
+    x_mg, it_mg, rr_mg, ms_mg = _mgpcg_solve(mg, F)
+    rr0 = st["rTr0"]
+    # the FP64 oracle's converged solution of the same system
+    tag_np = tag.cpu().numpy()
+    Fnp = F.cpu().numpy()
+    o = oracle.MGPCG((n, n, n), 4, real=np.float64, pre_and_post_smoothing=2, bottom_smoothing=6)
+    o.set_tags(tag_np)
+    o.init(Fnp.astype(np.float64), 1.0)
+    o.mask_interior()
+    o.solve(max_iters=-1, eps=0.0, abs_tol=0.0, rel_tol=1e-24, smoother="phases")
+    xs = o.get("x")
+    xl, xm = solver.last_x64.cpu().numpy(), x_mg.cpu().numpy()
+    assert np.array_equal(x.cpu().numpy(), xl.astype(np.float32))   # the FP32 x is the FP64 one rounded
+    res_l, res_m = _true_rel(oracle, xl, Fnp, tag_np), _true_rel(oracle, xm, Fnp, tag_np)
+    ea_l, ea_m = _energy_err(oracle, xl, xs, tag_np), _energy_err(oracle, xm, xs, tag_np)
+    print(f"seed {seed} 128^3: learned {st['iterations']} it ({st['net_iters']} network, fallback at "
+          f"{st['fallback_at']}) {st['ms']:.1f} ms, FP64 true rel. residual {np.sqrt(st['true_rTr'] / rr0):.2e} "
+          f"(CPU: {res_l:.2e}), energy error {ea_l:.2e}, 2-norm error {_rel(xl, xs):.2e}; "
+          f"MGPCG {it_mg} it {ms_mg:.1f} ms, true rel. residual {np.sqrt(rr_mg / rr0):.2e} (CPU: {res_m:.2e}), "
+          f"energy error {ea_m:.2e}, 2-norm error {_rel(xm, xs):.2e}")
+    assert st["net_iters"] >= 1
+    # both solvers stop at the same FP64 true residual criterion, and the CPU
+    # operator confirms it
+    assert st["true_rTr"] < 1e-12 * rr0
+    assert rr_mg < 1e-12 * rr0
+    assert res_m < 1e-6 and res_l < 1e-6
+    # and both are as close to the converged FP64 solution as that residual allows
+    assert ea_l < 1e-5 and ea_m < 1e-5
+
+
+def test_learned_heldout_256_true_residual(net):
+    from flub_amd.learned.fpcg import LearnedPCG
+    n = 256
+    tag, F, mg = _problem(n, 3002)
+    solver = LearnedPCG(mg, net)
+    x, st = solver.solve(F, tag == 1, rel_tol=1e-12)
+    _, it_mg, rr_mg, ms_mg = _mgpcg_solve(mg, F)
+    rr0 = st["rTr0"]
+    print(f"seed 3002 256^3: learned {st['iterations']} it ({st['net_iters']} network) {st['ms']:.1f} ms, "
+          f"true rel. residual {np.sqrt(st['true_rTr'] / rr0):.2e} (FP32 x {np.sqrt(st['true_rTr_fp32'] / rr0):.2e}); "
+          f"MGPCG {it_mg} it {ms_mg:.1f} ms, true rel. residual {np.sqrt(rr_mg / rr0):.2e}")
+    assert st["net_iters"] >= 1
+    assert st["true_rTr"] < 1e-12 * rr0
+    assert rr_mg < 1e-12 * rr0
