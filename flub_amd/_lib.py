@@ -62,6 +62,7 @@ SIGNATURES = {
     "flub_scale": [_P, _F, _Z, _P],
     "flub_subtract_gradient_tag": [_P, _P, _P, _P, _I, _I, _I, _I, _P],
     "flub_inner": [_P, _P, _Z, _P, _P],
+    "flub_inner_noalloc": [_P, _P, _Z, _P, _P],
     "flub_mgpcg_create": [_PP, _I, _I, _I, _I, _I, _P],
     "flub_mgpcg_destroy": [_P],
     "flub_mgpcg_set_smoothing": [_P, _I, _I],

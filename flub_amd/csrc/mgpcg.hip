@@ -11,6 +11,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "common.h"
 #include "grid_ops.h"
 #include "mgpcg_internal.h"
@@ -1504,6 +1506,62 @@ FLUB_API int flub_inner(const float *a, const float *b, size_t n, double *out_de
     if (!part) FLUB_CHECK_HIP(hipMalloc((void **)&part, MAXBLK * sizeof(double)));
     hipStream_t s = as_stream(stream);
     const int nb = red_blocks((uint32_t)n);
+    if (n) {
+        k_dot<<<nb, RBS, 0, s>>>(a, b, (uint32_t)n, part);
+        FLUB_CHECK_LAUNCH();
+    }
+    k_finish<<<1, RBS, 0, s>>>(part, n ? nb : 0, out_dev, FIN_OUT, 0.0);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+// ---------------------------------------------------------------- inner product without allocation
+// flub_inner_noalloc keeps its FP64 partials (at most MAXBLK, one per block of
+// the reduction) in this file-scope device array instead of memory the
+// library allocates: every GPU the library runs on holds one copy of it (16
+// KiB) until the process ends.  In host code `scratch` names a host-side
+// stand-in, not device memory: the kernels receive the per-device address that
+// hipGetSymbolAddress returns, resolved once per device on the entry point's
+// first call there.  Calls on one GPU share its copy (the last call's partials
+// stay in it): calls that overlap on one GPU, from several streams or
+// threads, race.
+static __device__ __align__(256) double scratch[MAXBLK];
+#define FLUB_NOALLOC_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[FLUB_NOALLOC_MAX_DEVICES];
+
+static int noalloc_scratch(double **out)
+{
+    int dev = 0;
+    FLUB_CHECK_HIP(hipGetDevice(&dev));
+    FLUB_REQUIRE(dev >= 0 && dev < FLUB_NOALLOC_MAX_DEVICES, FLUB_EINVAL,
+                 "flub_inner_noalloc: device %d beyond the %d the scratch table holds", dev,
+                 FLUB_NOALLOC_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            flub_set_error("flub_inner_noalloc: hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                           hipGetErrorString(e));
+            return e != hipSuccess ? (int)e : FLUB_ESTATE;
+        }
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return FLUB_OK;
+}
+
+FLUB_API int flub_inner_noalloc(const float *a, const float *b, size_t n, double *out_dev, void *stream)
+{
+    FLUB_REQUIRE(a && b && out_dev, FLUB_EINVAL, "flub_inner_noalloc: null pointer");
+    FLUB_REQUIRE(n <= FLUB_INNER_NOALLOC_MAX_N, FLUB_ESHAPE,
+                 "flub_inner_noalloc: n = %zu exceeds the maximum %llu", n,
+                 (unsigned long long)FLUB_INNER_NOALLOC_MAX_N);
+    double *part = nullptr;
+    TRY(noalloc_scratch(&part));
+    hipStream_t s = as_stream(stream);
+    const int nb = red_blocks((uint32_t)n);   // <= MAXBLK: the scratch holds every partial
     if (n) {
         k_dot<<<nb, RBS, 0, s>>>(a, b, (uint32_t)n, part);
         FLUB_CHECK_LAUNCH();

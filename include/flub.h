@@ -174,6 +174,17 @@ int flub_subtract_gradient_tag(float *uf, const float *u_rigid, const float *p, 
 /* warp.utils.array_inner (mgpcg.py:5,391,...): *out_dev = sum a*b, accumulated
  * in FP64 by a deterministic two-stage reduction. */
 int flub_inner(const float *a, const float *b, size_t n, double *out_dev, void *stream);
+/* flub_inner without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same kernels and
+ * bitwise the same result; the FP64 partials live in a static device array of
+ * 2048 doubles (16 KiB per GPU, held until the process ends) instead of memory
+ * flub_inner allocates.  n is limited to FLUB_INNER_NOALLOC_MAX_N (larger n:
+ * FLUB_ESHAPE).  Every call on one GPU shares that array: calls that overlap on
+ * one GPU (several streams or threads) race.  The stream must belong to the
+ * current device. */
+#define FLUB_INNER_NOALLOC_MAX_N 2147483647u
+int flub_inner_noalloc(const float *a, const float *b, size_t n, double *out_dev, void *stream);
 
 /* ---------------------------------------------------------------------------
  * MGPCG solver handle (multigrid/mgpcg.py:234-490).  The handle owns the level
