@@ -61,6 +61,7 @@ SIGNATURES = {
     "flub_mask_with_tag": [_P, _P, _Z, _P],
     "flub_scale": [_P, _F, _Z, _P],
     "flub_subtract_gradient_tag": [_P, _P, _P, _P, _I, _I, _I, _I, _P],
+    "flub_subtract_gradient_tag_f64": [_P, _P, _P, _P, _I, _I, _I, _I, _P],
     "flub_inner": [_P, _P, _Z, _P, _P],
     "flub_inner_noalloc": [_P, _P, _Z, _P, _P],
     "flub_mgpcg_create": [_PP, _I, _I, _I, _I, _I, _P],
@@ -85,6 +86,8 @@ SIGNATURES = {
     "flub_mgpcg_scalars": [_P, _PD],
     "flub_mgpcg_level_shape": [_P, _I, _PI],
     "flub_mgpcg_profile": [_P, _I, _I, _PF, _PD],
+    "flub_mgpcg_probe": [_P, _I, _I],
+    "flub_mgpcg_probe_read": [_P, _P, _P, _P],
     "flub_mgpcg_apply_A": [_P, _P, _P],
     "flub_mgpcg_residual_norm": [_P, _P, _P, _PD],
     "flub_mgpcg_true_residual_f64": [_P, _P, _P, _P, _PD],

@@ -441,6 +441,12 @@ int flub_subtract_gradient_x_u8(float *uf, const float *u_rigid, const float *x,
     return subgrad_impl<uint8_t, true>(uf, u_rigid, x, tag, nx, ny, nz, d, h, stream);
 }
 
+FLUB_API int flub_subtract_gradient_tag_f64(float *uf, const float *u_rigid, const double *p, const int32_t *tag,
+                                            int nx, int ny, int nz, int d, void *stream)
+{
+    return subgrad_impl<int32_t, false, double>(uf, u_rigid, p, tag, nx, ny, nz, d, 1.0f, stream);
+}
+
 int flub_subtract_gradient_x64_u8(float *uf, const float *u_rigid, const double *x, const uint8_t *tag,
                                   int nx, int ny, int nz, int d, void *stream)
 {

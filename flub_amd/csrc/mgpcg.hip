@@ -595,8 +595,12 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
                 // slower than two single launches
                 TRY(split_sweep_pair(h->s, F, nullptr, ZP, nullptr, R, CD, 0, SWEEP_ZERO));
                 for (int it = 1; it < s; ++it) {
+                    TRY(probe_begin(h));
                     TRY(split_sweep(h->s, F, ZP, ZP, nullptr, R, CD, 0, SWEEP_PLAIN, nullptr, nullptr));
+                    TRY(probe_end(h));
+                    TRY(probe_begin(h));
                     TRY(split_sweep(h->s, F, ZP, ZP, nullptr, R, CD, 1, SWEEP_PLAIN, nullptr, nullptr));
+                    TRY(probe_end(h));
                 }
                 TRY(split_restrict(h->s, F, R, ZP, CD, h->lv[l + 1], l + 1 < h->nsplit));
                 continue;
@@ -694,7 +698,9 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
                 TRY(split_sweep_pair(h->s, F, ZP, Z, zc, R, CD, 1, SWEEP_PROLONG));
                 for (int it = 1; it < s; ++it) {
                     const bool last = it + 1 == s;
+                    TRY(probe_begin(h));
                     TRY(split_sweep(h->s, F, Z, Z, nullptr, R, CD, 1, SWEEP_PLAIN, nullptr, nullptr));
+                    TRY(probe_end(h));
                     TRY(split_sweep(h->s, F, Z, Z, nullptr, R, CD, 0, SWEEP_PLAIN, last ? h->part : nullptr,
                                     last ? &h->vc_dot_nblk : nullptr));
                 }
@@ -959,6 +965,10 @@ FLUB_API int flub_mgpcg_destroy(flub_mgpcg *h)
     hipFree(h->x64n);
     hipFree(h->sp.code); hipFree(h->sp.rh); hipFree(h->sp.zh); hipFree(h->sp.zw); hipFree(h->sp.b);
     for (int l = 1; l < MAXL; ++l) { hipFree(h->sp.lcode[l]); hipFree(h->sp.zt[l]); }
+    if (h->probe) {
+        for (int i = 0; i < 2 * h->probe->max; ++i) hipEventDestroy(h->probe->ev[i]);
+        free(h->probe);
+    }
     if (h->ev) hipEventDestroy(h->ev);
     if (h->hsc) hipHostFree(h->hsc);
     if (h->hpart) hipHostFree(h->hpart);
@@ -1216,9 +1226,10 @@ FLUB_API int flub_mgpcg_solution_f64(flub_mgpcg *h, double *x)
 // mgpcg.py:378-462 for the "phases" smoother on the colour-split level 0:
 // r, x, z, p and Ap of level 0 are colour-split for the whole solve and
 // converted back at the end.  update_p then Ap, and update_x deferred into
-// the next update_p (x += alpha p with the same alpha and p, before p
-// changes; the last one after the loop), so that update_r reads and writes
-// only r.  x accumulates in FP64 (SplitBuf::x).  The V-cycle is queued
+// later p steps (x += alpha p with the same alpha and p; two iterations'
+// updates at a time, p kept in two alternating buffers, see PxState; the last
+// ones after the loop), so that update_r reads and writes only r.  x
+// accumulates in FP64 (SplitBuf::x).  The V-cycle is queued
 // before the host reads the residual norm, into the z buffer the iteration
 // does not use: the GPU keeps working while the host tests for convergence.
 // If the test stops the solve, that speculative z is not used (x, r, z and
@@ -1237,8 +1248,9 @@ static int solve_split(flub_mgpcg *h, int max_iters, double eps, double abs_tol,
     const int max_replacements = 3;
     int replacements = 0;
     int zs = 0;
-    bool x_set = true;
-    TRY(mg_split_enter(h, x_set));
+    PxState px;
+    px.P[0] = h->p; px.P[1] = h->p_alt; px.x = S.x; px.sc = h->sc; px.n = n;
+    TRY(mg_split_enter(h, px.x_set));
     float *const R = S.r;
     // the r.r partials go to the mapped host buffer and the host sums them
     // (no finish kernel on the stream)
@@ -1258,23 +1270,20 @@ static int solve_split(flub_mgpcg *h, int max_iters, double eps, double abs_tol,
     int it = 0;
     double rTr = rTr0, r_prev = rTr0, rate = 1.0;   // rate: last reduction factor of r.r
     int status = FLUB_OK;
-    bool x_pending = false;   // x += alpha p of the last iteration not applied yet
     while (it < cap) {
         int nbp = 0;
         // speculate unless this test is predicted to stop the solve (a wasted
         // speculative cycle would delay the end of the solve)
         const bool speculate = it + 1 < cap && r_prev * rate > 16.0 * tol;
-        // p = z + beta*p (and the deferred x update), then Ap = ndt(p)
-        if (x_pending) TRY(update_px_pending(h->s, h->p, S.x, S.z[zs], h->sc, nullptr, n, x_set, true));
-        else TRY(launch_update_px<XM_NONE>(h->s, h->p, S.x, S.z[zs], h->sc, nullptr, n));
-        x_pending = false;
-        TRY(split_Ap(h->s, L0, h->p, h->Ap, S.code, h->part, &nbp));
+        // p = z + beta*p (with the pending x updates), then Ap = ndt(p)
+        TRY(px_step(h->s, px, S.z[zs], nullptr));
+        TRY(split_Ap(h->s, L0, px.p(), h->Ap, S.code, h->part, &nbp));
         TRY(launch_finish(h, nbp, FIN_ALPHA, eps));
-        // r only: x += alpha p waits for the next p update (or the end)
+        // r only: x += alpha p waits for a later p step (or the end)
         k_update_rx_dot<false><<<nb, RBS, 0, h->s>>>(R, nullptr, h->Ap, nullptr, h->sc, n,
                                                       hsum ? h->hpart_dev : h->part);
         FLUB_CHECK_LAUNCH();
-        x_pending = true;
+        px_after_r_update(px);
         if (hsum) FLUB_CHECK_HIP(hipEventRecord(h->ev, h->s));
         else TRY(finish_rtr_to_host(h, nb));
         if (speculate) TRY(mg_precondition_split(h, zs ^ 1));
@@ -1284,8 +1293,7 @@ static int solve_split(flub_mgpcg *h, int max_iters, double eps, double abs_tol,
         if (!isfinite(rTr)) { status = FLUB_ENONFINITE; flub_set_error("non-finite residual at iteration %d", it + 1); break; }
         bool redo = false;   // the speculative cycle used an r that was then replaced
         if (rTr < tol) {
-            TRY(update_px_pending(h->s, h->p, S.x, nullptr, h->sc, nullptr, n, x_set, false));
-            x_pending = false;
+            TRY(px_flush(h->s, px));
             double rTrue;
             TRY(mg_split_true_residual(h, R, &rTrue));
             if (verbose) printf("  true |residual|_2=%.9g\n", sqrt(rTrue));
@@ -1302,16 +1310,16 @@ static int solve_split(flub_mgpcg *h, int max_iters, double eps, double abs_tol,
         TRY(launch_finish(h, h->vc_dot_nblk, FIN_BETA, eps));
         it += 1;
     }
-    if (x_pending) TRY(update_px_pending(h->s, h->p, S.x, nullptr, h->sc, nullptr, n, x_set, false));
+    TRY(px_flush(h->s, px));
     if (hsum) TRY(mg_rtr_to_device(h, rTr));
     // back to the reference layout (sp.zpre is free scratch now)
-    TRY(mg_split_leave(h, x_set));
+    TRY(mg_split_leave(h, px.x_set));
     if (!h->x_only) {
         TRY(split_convert(h->s, L0, R, L0.r, 4, false));
         TRY(split_convert(h->s, L0, S.z[zs], L0.z, 4, false));
         TRY(split_convert(h->s, L0, h->Ap, S.zpre, 4, false));
         FLUB_CHECK_HIP(hipMemcpyAsync(h->Ap, S.zpre, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
-        TRY(split_convert(h->s, L0, h->p, S.zpre, 4, false));
+        TRY(split_convert(h->s, L0, px.p(), S.zpre, 4, false));
         FLUB_CHECK_HIP(hipMemcpyAsync(h->p, S.zpre, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
     }
     if (status == FLUB_OK && it >= cap && max_iters < 0) {
@@ -1629,6 +1637,56 @@ FLUB_API int flub_project(flub_mgpcg *h, float *u, float *v, float *w, const flo
     return st;
 }
 
+// ---------------------------------------------------------------- in-solve timing
+FLUB_API int flub_mgpcg_probe(flub_mgpcg *h, int kernel, int max_launches)
+{
+    FLUB_REQUIRE(h, FLUB_EINVAL, "null handle");
+    FLUB_REQUIRE(kernel == 8, FLUB_EINVAL, "flub_mgpcg_probe: kernel %d not probed (only 8)", kernel);
+    FLUB_REQUIRE(max_launches >= 1 && max_launches <= PROBE_MAX, FLUB_EINVAL,
+                 "flub_mgpcg_probe: max_launches must be 1..%d", PROBE_MAX);
+    FLUB_REQUIRE(h->split && h->pair_l0, FLUB_ESTATE, "flub_mgpcg_probe: no colour-split FP32 level 0");
+    FLUB_REQUIRE(!h->probe, FLUB_ESTATE, "flub_mgpcg_probe: already armed");
+    Probe *p = (Probe *)calloc(1, sizeof(Probe));
+    FLUB_REQUIRE(p, FLUB_EINVAL, "flub_mgpcg_probe: out of host memory");
+    for (int i = 0; i < 2 * max_launches; ++i) {
+        const hipError_t e = hipEventCreate(&p->ev[i]);
+        if (e != hipSuccess) {
+            for (int j = 0; j < i; ++j) hipEventDestroy(p->ev[j]);
+            free(p);
+            flub_set_error("hipEventCreate: %s", hipGetErrorString(e));
+            return (int)e;
+        }
+    }
+    p->max = max_launches;
+    h->probe = p;
+    return FLUB_OK;
+}
+
+FLUB_API int flub_mgpcg_probe_read(flub_mgpcg *h, float *avg_ms, float *min_ms, int *count)
+{
+    FLUB_REQUIRE(h && avg_ms && count, FLUB_EINVAL, "null argument");
+    FLUB_REQUIRE(h->probe, FLUB_ESTATE, "flub_mgpcg_probe_read: no probe armed");
+    Probe *p = h->probe;
+    h->probe = nullptr;
+    int rc = FLUB_OK;
+    double sum = 0.0;
+    float mn = 0.0f;
+    if (p->n > 0) rc = (int)hipEventSynchronize(p->ev[2 * p->n - 1]);
+    for (int i = 0; i < p->n && rc == FLUB_OK; ++i) {
+        float ms = 0.0f;
+        rc = (int)hipEventElapsedTime(&ms, p->ev[2 * i], p->ev[2 * i + 1]);
+        sum += ms;
+        mn = i == 0 ? ms : fminf(mn, ms);
+    }
+    if (rc != FLUB_OK) flub_set_error("flub_mgpcg_probe_read: %s", hipGetErrorString((hipError_t)rc));
+    *count = p->n;
+    *avg_ms = p->n ? (float)(sum / p->n) : 0.0f;
+    if (min_ms) *min_ms = mn;
+    for (int i = 0; i < 2 * p->max; ++i) hipEventDestroy(p->ev[i]);
+    free(p);
+    return rc;
+}
+
 // ---------------------------------------------------------------- measurement
 FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_ms, double *bytes)
 {
@@ -1691,8 +1749,11 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
         case 10: {   // the split solve's p step: p (and the deferred x) update, then Ap and p.Ap
             FLUB_REQUIRE(h->split, FLUB_ESTATE, "no colour-split level 0 for this handle");
             int nbp = 0;
-            TRY(launch_update_px<XM_ADD>(h->s, h->p, h->sp.x, h->sp.z[0], h->sc, nullptr, L0.n));
-            TRY(split_Ap(h->s, L0, h->p, h->Ap, h->sp.code, h->part, &nbp));
+            // the p step that applies two x updates (every second iteration)
+            PxState px;
+            px.P[0] = h->p; px.P[1] = h->p_alt; px.x = h->sp.x; px.sc = h->sc; px.n = L0.n; px.npend = 2;
+            TRY(px_step(h->s, px, h->sp.z[0], nullptr));
+            TRY(split_Ap(h->s, L0, px.p(), h->Ap, h->sp.code, h->part, &nbp));
             return launch_finish(h, nbp, FIN_SUM, 0.0);
         }
         case 11:   // the split solve's r step: r -= alpha Ap and r.r
@@ -1713,7 +1774,7 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
     case 7: b = 4 * N * (2 * v + 1); break;                  // four half-sweeps
     case 8: b = N * (2 * v + 1); break;
     case 9: b = N * (2 * v + 1) + N / 8 * (v + 1); break;
-    case 10: b = N * (9 * 4 + 2); break;                     // p, z, x (FP64) -> p, x; p, codes -> Ap
+    case 10: b = N * (10 * 4 + 2); break;                    // p, p', z, x (FP64) -> p, x; p, codes -> Ap
     case 11: b = N * 3 * 4; break;                           // r, Ap -> r
     default: break;
     }

@@ -13,8 +13,9 @@
 // Device scalar slots (FP64).
 // SC_ZINV0/1: 1/scale of the mixed-precision V-cycle output held in the split z
 // buffer of that slot (the buffer holds z times the storage scale).
+// SC_APREV: the alpha of a deferred x update (see PxState).
 enum { SC_ALPHA = 0, SC_BETA, SC_SUM, SC_OLDZTR, SC_RTR, SC_MEAN, SC_COUNT, SC_RSCALE, SC_AUX0, SC_AUX1,
-       SC_AUX2, SC_AUX3, SC_ZINV0, SC_ZINV1, SC_NUM = 16 };
+       SC_AUX2, SC_AUX3, SC_ZINV0, SC_ZINV1, SC_APREV, SC_NUM = 16 };
 // k_finish operations.
 enum { FIN_SUM = 0, FIN_ALPHA, FIN_BETA, FIN_OLDZTR, FIN_RTR, FIN_MEAN, FIN_OUT };
 
@@ -52,6 +53,14 @@ struct SplitBuf {
     // levels 1 .. nsplit-1 that sweep in pairs (split_sweep_pair): a third z
     // buffer of the level (nullptr: one launch per half-sweep)
     float *zt[MAXL];
+};
+
+// In-solve timing of the level-0 plain half-sweeps (flub_mgpcg_probe): HIP
+// events recorded around those launches inside the solves' V-cycles.
+#define PROBE_MAX 512
+struct Probe {
+    int n, max;
+    hipEvent_t ev[2 * PROBE_MAX];
 };
 
 struct flub_mgpcg {
@@ -100,37 +109,70 @@ struct flub_mgpcg {
     bool want_x64n;    // the next split solve also writes x64n
     SplitBuf sp;
     MixedState *mx;
+    Probe *probe;      // armed in-solve timing (nullptr: off)
 };
 
-// Elementwise CG vector updates over float4 vectors (n4 = n / 4), any layout.
-// DOP: p = beta p + z (mgpcg.py:229-232), z times zinv[0] when zinv is set
-// (the z buffer holds z times a power-of-two scale: exact).  XM: the x update
-// of the previous iteration (mgpcg.py:207-210), x += alpha p with the p it
-// used, deferred from the r update to this pass, which reads p anyway
-// (XM_ADD), or x = alpha p when x is known to be zero (XM_SET: the first
-// update of a solve from x = 0, which then never reads or clears x); XM_NONE:
-// no x update.  x is FP64: alpha is the FP32 value the r update uses, and
-// (double)alpha * (double)p is exact, so x + alpha p has one rounding in
-// FP64.  The solve applies the last update (DOP false) before x is read or
-// returned.
-enum { XM_NONE = 0, XM_ADD = 1, XM_SET = 2 };
-template <int XM, bool DOP = true>
-__global__ void __launch_bounds__(256) k_update_px(float *__restrict__ p, double *__restrict__ x,
-                                                   const float *__restrict__ z, const double *__restrict__ sc,
+// Event before / after a probed launch (no-ops unless a probe is armed and has room).
+inline int probe_begin(flub_mgpcg *h)
+{
+    Probe *p = h->probe;
+    if (p && p->n < p->max) FLUB_CHECK_HIP(hipEventRecord(p->ev[2 * p->n], h->s));
+    return FLUB_OK;
+}
+inline int probe_end(flub_mgpcg *h)
+{
+    Probe *p = h->probe;
+    if (p && p->n < p->max) {
+        FLUB_CHECK_HIP(hipEventRecord(p->ev[2 * p->n + 1], h->s));
+        ++p->n;
+    }
+    return FLUB_OK;
+}
+
+// The p step of the split solves over float4 vectors (n4 = n / 4), any
+// layout: p_out = beta p_prev + z (mgpcg.py:229-232), z times zinv[0] when
+// zinv is set (the z buffer holds z times a power-of-two scale: exact), with
+// the pending x updates of earlier iterations (mgpcg.py:207-210) folded in.
+// x is FP64: each update adds (double)alpha * (double)p (exact) with alpha the
+// FP32 value the r update used, one rounding in FP64 per update.  p_prev is
+// the previous iteration's p; the solve keeps p in two buffers alternately
+// (p_out != p_prev), so the p before it survives one more step.  XU:
+//   XU_NONE  nothing pending;
+//   XU_DEFER the update alpha p_prev waits for the next step: alpha goes to
+//            SC_APREV (this step moves no x);
+//   XU_ONE   x += alpha p_prev;
+//   XU_TWO   x += alpha' p_prev2 + alpha p_prev with alpha' from SC_APREV and
+//            p_prev2 the p before p_prev, read from p_out before it is
+//            overwritten (the same elements, by the same thread).
+// Deferring every other update moves x (16 bytes a cell) on every second
+// step only.  XSET: x is known to be zero (not read).  DOP false: the x
+// update alone (the solve's last).
+enum { XU_NONE = 0, XU_DEFER, XU_ONE, XU_TWO };
+template <int XU, bool XSET, bool DOP>
+__global__ void __launch_bounds__(256) k_update_pp(float *p_out, const float *p_prev, double *__restrict__ x,
+                                                   const float *__restrict__ z, double *__restrict__ sc,
                                                    const double *__restrict__ zinv, uint32_t n4)
 {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (t >= n4) return;
-    float4 pv = reinterpret_cast<const float4 *>(p)[t];
-    if (XM != XM_NONE) {
+    if (XU == XU_DEFER && t == 0) sc[SC_APREV] = sc[SC_ALPHA];   // read by a later launch only
+    float4 pv = reinterpret_cast<const float4 *>(p_prev)[t];
+    if (XU == XU_ONE || XU == XU_TWO) {
         const double alpha = (double)(float)sc[SC_ALPHA];
-        const float pp[4] = {pv.x, pv.y, pv.z, pv.w};
         double xv[4] = {0.0, 0.0, 0.0, 0.0};
-        if (XM == XM_ADD) {
+        if (!XSET) {
             const double2 a = reinterpret_cast<const double2 *>(x)[2 * t];
             const double2 b = reinterpret_cast<const double2 *>(x)[2 * t + 1];
             xv[0] = a.x; xv[1] = a.y; xv[2] = b.x; xv[3] = b.y;
         }
+        if (XU == XU_TWO) {
+            const double a0 = (double)(float)sc[SC_APREV];
+            const float4 q = reinterpret_cast<const float4 *>(p_out)[t];
+            const float qq[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) xv[e] = fma(a0, (double)qq[e], xv[e]);
+        }
+        const float pp[4] = {pv.x, pv.y, pv.z, pv.w};
 #pragma unroll
         for (int e = 0; e < 4; ++e) xv[e] = fma(alpha, (double)pp[e], xv[e]);
         reinterpret_cast<double2 *>(x)[2 * t] = make_double2(xv[0], xv[1]);
@@ -147,34 +189,80 @@ __global__ void __launch_bounds__(256) k_update_px(float *__restrict__ p, double
         pv.y = fmaf(beta, pv.y, zv.y);
         pv.z = fmaf(beta, pv.z, zv.z);
         pv.w = fmaf(beta, pv.w, zv.w);
-        reinterpret_cast<float4 *>(p)[t] = pv;
+        reinterpret_cast<float4 *>(p_out)[t] = pv;
     }
 }
 
-// k_update_px for n values (n % 4 == 0), on stream s.
-template <int XM, bool DOP = true>
-int launch_update_px(hipStream_t s, float *p, double *x, const float *z, const double *sc, const double *zinv,
-                     uint32_t n)
+// The split solves' p buffers and pending x updates.  P[0], P[1] alternate
+// as the output of the p steps; after the r update of an iteration its
+// alpha p is pending (held in SC_ALPHA and the p buffer `pa`), possibly after
+// an older deferred one (alpha in SC_APREV, p in `pb`).
+struct PxState {
+    float *P[2];
+    int cur = 0;          // index of the latest p in P
+    int npend = 0;        // pending x updates (0, 1, 2)
+    bool deferred = false;  // the one pending update's alpha moved to SC_APREV (p in P[cur])
+    bool x_set = true;    // x holds a value (else it is zero and not yet written)
+    double *x = nullptr;
+    double *sc = nullptr;
+    uint32_t n = 0;
+    float *p() const { return P[cur]; }
+};
+
+template <int XU, bool XSET, bool DOP>
+inline void launch_pp(hipStream_t s, const PxState &st, float *p_out, const float *p_prev, const float *z,
+                      const double *zinv)
 {
-    FLUB_REQUIRE(n % 4 == 0, FLUB_EINVAL, "vector update: n = %u is not a multiple of 4", n);
-    const uint32_t n4 = n / 4;
-    k_update_px<XM, DOP><<<blocks_for(n4, 256), 256, 0, s>>>(p, x, z, sc, zinv, n4);
+    const uint32_t n4 = st.n / 4;
+    k_update_pp<XU, XSET, DOP><<<blocks_for(n4, 256), 256, 0, s>>>(p_out, p_prev, st.x, z, st.sc, zinv, n4);
+}
+
+// The p step of an iteration: p into the other buffer, the pending x update
+// deferred (one pending) or applied together with the deferred one (two).
+inline int px_step(hipStream_t s, PxState &st, const float *z, const double *zinv)
+{
+    FLUB_REQUIRE(st.n % 4 == 0, FLUB_EINVAL, "vector update: n = %u is not a multiple of 4", st.n);
+    float *prev = st.P[st.cur], *out = st.P[st.cur ^ 1];
+    if (st.npend == 0) {
+        launch_pp<XU_NONE, false, true>(s, st, out, prev, z, zinv);
+    } else if (st.npend == 1) {
+        launch_pp<XU_DEFER, false, true>(s, st, out, prev, z, zinv);
+        st.deferred = true;
+    } else {
+        if (st.x_set) launch_pp<XU_TWO, false, true>(s, st, out, prev, z, zinv);
+        else launch_pp<XU_TWO, true, true>(s, st, out, prev, z, zinv);
+        st.x_set = true;
+        st.npend = 0;
+    }
     FLUB_CHECK_LAUNCH();
+    st.cur ^= 1;
     return FLUB_OK;
 }
 
-// The pending x update of a split solve: XM_ADD, or XM_SET while x has not
-// been written yet (x_set false: a solve that starts from x = 0).
-inline int update_px_pending(hipStream_t s, float *p, double *x, const float *z, const double *sc,
-                             const double *zinv, uint32_t n, bool &x_set, bool dop)
+// After the r update of an iteration: its alpha p is pending.
+inline void px_after_r_update(PxState &st)
 {
-    int rc;
-    if (dop) rc = x_set ? launch_update_px<XM_ADD, true>(s, p, x, z, sc, zinv, n)
-                        : launch_update_px<XM_SET, true>(s, p, x, z, sc, zinv, n);
-    else rc = x_set ? launch_update_px<XM_ADD, false>(s, p, x, z, sc, zinv, n)
-                    : launch_update_px<XM_SET, false>(s, p, x, z, sc, zinv, n);
-    if (rc == FLUB_OK) x_set = true;
-    return rc;
+    st.npend = st.deferred ? 2 : 1;
+    st.deferred = false;
+}
+
+// Apply every pending x update (before x is read).
+inline int px_flush(hipStream_t s, PxState &st)
+{
+    if (st.npend == 0) return FLUB_OK;
+    float *pa = st.P[st.cur];
+    if (st.npend == 1) {
+        if (st.x_set) launch_pp<XU_ONE, false, false>(s, st, nullptr, pa, nullptr, nullptr);
+        else launch_pp<XU_ONE, true, false>(s, st, nullptr, pa, nullptr, nullptr);
+    } else {   // the older p in the other buffer, passed as p_out (read only: DOP false)
+        float *pb = st.P[st.cur ^ 1];
+        if (st.x_set) launch_pp<XU_TWO, false, false>(s, st, pb, pa, nullptr, nullptr);
+        else launch_pp<XU_TWO, true, false>(s, st, pb, pa, nullptr, nullptr);
+    }
+    FLUB_CHECK_LAUNCH();
+    st.x_set = true;
+    st.npend = 0;
+    return FLUB_OK;
 }
 
 int mg_dot(flub_mgpcg *h, const float *a, const float *b, int op, double eps);

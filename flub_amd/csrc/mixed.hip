@@ -431,8 +431,9 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
     int replacements = 0;
     SplitBuf &S = h->sp;
     h->rh_ready = -1;
-    bool x_set = true;
-    TRYM(mg_split_enter(h, x_set));
+    PxState px;
+    px.P[0] = h->p; px.P[1] = h->p_alt; px.x = S.x; px.sc = h->sc; px.n = n;
+    TRYM(mg_split_enter(h, px.x_set));
     // the tolerance is first needed after iteration 1: the first V-cycle runs
     // while the initial r.r travels to the host
     TRYM(mg_mixed_rscale_of(h, S.r, true));
@@ -446,15 +447,12 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
     const int cap = max_iters < 0 ? SOLVE_SAFETY_CAP : max_iters;
     int it = 0, status = FLUB_OK;
     double rTr = rTr0, r_prev = rTr0, rate = 1.0;
-    bool x_pending = false;   // x += alpha p of the last iteration not applied yet
     while (it < cap) {
         const bool speculate = it + 1 < cap && r_prev * rate > 16.0 * tol;
         // p = z + beta p (and the deferred x update), then q = ndt(p) and p.q
         int nbp = 0;
-        if (x_pending) TRYM(update_px_pending(h->s, h->p, S.x, S.z[zs], h->sc, h->sc + SC_ZINV0 + zs, n, x_set, true));
-        else TRYM(launch_update_px<XM_NONE>(h->s, h->p, S.x, S.z[zs], h->sc, h->sc + SC_ZINV0 + zs, n));
-        x_pending = false;
-        TRYM(split_Ap(h->s, L0, h->p, h->Ap, S.code, h->part, &nbp));
+        TRYM(px_step(h->s, px, S.z[zs], h->sc + SC_ZINV0 + zs));
+        TRYM(split_Ap(h->s, L0, px.p(), h->Ap, S.code, h->part, &nbp));
         TRYM(mg_finish_alpha(h, nbp, eps));
         // r only: x += alpha p waits for the next p update (or a read of x)
         // BF16: the update also stores the next V-cycle's r (storage scale 1)
@@ -467,7 +465,7 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
         k_mx_update<false><<<nb, RBS, 0, h->s>>>(S.r, nullptr, h->Ap, nullptr, h->sc, n,
                                                  hsum ? h->hpart_dev : h->part, rh_next);
         FLUB_CHECK_LAUNCH();
-        x_pending = true;
+        px_after_r_update(px);
         h->rh_ready = h->mx_unit ? (zs ^ 1) : -1;
         if (hsum) {
             FLUB_CHECK_HIP(hipEventRecord(h->ev, h->s));
@@ -486,8 +484,7 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
         if (rTr < tol) {
             // the reference's test passed on the recursive residual: the FP64
             // true residual replaces r and decides
-            TRYM(update_px_pending(h->s, h->p, S.x, nullptr, h->sc, nullptr, n, x_set, false));
-            x_pending = false;
+            TRYM(px_flush(h->s, px));
             int nbt = 0;
             TRYM(split_true_residual(h->s, L0, S.r, S.b, S.x, S.code, h->part, &nbt));
             k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nbt, h->sc, h->mx_unit ? -1 : 0,
@@ -512,17 +509,17 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
         FLUB_CHECK_LAUNCH();
         it += 1;
     }
-    if (x_pending) TRYM(update_px_pending(h->s, h->p, S.x, nullptr, h->sc, nullptr, n, x_set, false));
+    TRYM(px_flush(h->s, px));
     if (h->mx_unit && h->hpart_dev) TRYM(mg_rtr_to_device(h, rTr));
     // back to the reference layout (S.zw is free scratch now)
-    TRYM(mg_split_leave(h, x_set));
+    TRYM(mg_split_leave(h, px.x_set));
     if (!h->x_only) {
         TRYM(split_convert(h->s, L0, S.r, L0.r, 4, false));
         TRYM(split_convert(h->s, L0, S.z[zs], L0.z, 4, false));
         TRYM(scale_by_device(h->s, L0.z, h->sc + SC_ZINV0 + zs, n));   // z buffers keep z * scale
         TRYM(split_convert(h->s, L0, h->Ap, S.zw, 4, false));
         FLUB_CHECK_HIP(hipMemcpyAsync(h->Ap, S.zw, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
-        TRYM(split_convert(h->s, L0, h->p, S.zw, 4, false));
+        TRYM(split_convert(h->s, L0, px.p(), S.zw, 4, false));
         FLUB_CHECK_HIP(hipMemcpyAsync(h->p, S.zw, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
     }
     h->rh_ready = -1;

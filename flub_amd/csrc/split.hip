@@ -196,7 +196,8 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
             else { km = w[e]; kp = e < CPT - 1 ? w[e + 1] : edge; }
             // neighbours hold +0 on non-FLUID cells except in the prolongating
             // sweep, which adds the coarse correction to every cell on the fly
-            out[e] = gs_update<MODE != SWEEP_PROLONG>(cd[e], c[e], rv[e], xp[e], xm[e], yp[e], ym[e], kp, km);
+            out[e] = gs_update<MODE != SWEEP_PROLONG, sizeof(TR) == 2>(cd[e], c[e], rv[e], xp[e], xm[e], yp[e], ym[e],
+                                                                      kp, km);
         }
         if (OUT == SWEEP_OUT_FINAL) {
             const float inv = 1.0f / s0;   // a power of two: exact

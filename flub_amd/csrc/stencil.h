@@ -181,13 +181,18 @@ __device__ __forceinline__ bool code_fluid(uint32_t cd) { return (cd >> 15) != 0
 
 // One red-black Gauss-Seidel update of a cell (mgpcg.py:72-86):
 // z + (r + ndt(z)) / divd on FLUID cells, +0 elsewhere (divd is 0 there; the
-// quotient computed for them is discarded).
-template <bool AIR0 = false>
+// quotient computed for them is discarded).  RCP: the quotient as a multiply
+// by the hardware reciprocal of divd (within an ulp of the IEEE quotient; the
+// reduced-precision sweeps, whose results are rounded to 16 bits or feed a
+// V-cycle with 16-bit storage, use it: it has no reference counterpart to
+// match bitwise, and the IEEE division is most of their instruction count).
+template <bool AIR0 = false, bool RCP = false>
 __device__ __forceinline__ float gs_update(uint32_t cd, float z, float r, float xip, float xim, float xjp, float xjm,
                                            float xkp, float xkm)
 {
     const float s = AIR0 ? nsum_code_air0(cd, z, xip, xim, xjp, xjm, xkp, xkm)
                          : nsum_code(cd, z, xip, xim, xjp, xjm, xkp, xkm);
     const uint32_t divd = (cd >> 12) & 7u;
-    return keep_if(z + (r + s) / (float)divd, 0u - (uint32_t)(divd != 0u));
+    const float q = RCP ? (r + s) * __builtin_amdgcn_rcpf((float)divd) : (r + s) / (float)divd;
+    return keep_if(z + q, 0u - (uint32_t)(divd != 0u));
 }

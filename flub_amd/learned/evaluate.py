@@ -1,7 +1,8 @@
 """Evaluate the learned preconditioner on held-out scenes (seeds >= 3000):
 iterations and solve time of (a) the plain FP32 MGPCG of the library, (b) the
 learned flexible PCG with V-cycle fallback, (c) the same flexible PCG with the
-V-cycle only, all to the same relative residual.
+V-cycle only, all to the same FP64 true relative residual (1e-6: x in FP64,
+the true residual recomputed and replacing r when the recursive one passes).
 
     python -m flub_amd.learned.evaluate --weights weights/unet_128.npz --n 128 --seeds 3000 3001
 """
@@ -51,6 +52,9 @@ def evaluate(net, n, seeds, levels=4, rel_tol=1e-12, log=print):
             it = mg.solve(-1, abs_tol=0.0, rel_tol=rel_tol, smoother="phases")
             torch.cuda.synchronize()
             res["mgpcg"] = {"iterations": it, "ms": (time.perf_counter() - t) * 1e3}
+        rr0 = float(torch.dot(F.view(-1).double(), F.view(-1).double()))
+        # the FP64 true residual of the solve's FP64 x (the split solves' stopping rule)
+        res["mgpcg"]["true_rel_res"] = float(np.sqrt(mg.last_rTr / rr0)) if rr0 > 0 else 0.0
         # (b) learned with fallback, (c) flexible PCG, V-cycle only
         for name, use_net in (("learned", True), ("fpcg_vcycle", False)):
             for rep in range(2):

@@ -178,10 +178,17 @@ class Projection:
         _lib.call("flub_memcpy_d2d", b.data_ptr(), mg.r[0].ptr, mg.r[0].nbytes, None)
         x, st = solver.solve(b, fluid, rel_tol=rel_tol, abs_tol=abs_tol)
         _lib.call("flub_memcpy_d2d", mg.x.ptr, x.data_ptr(), mg.x.nbytes, None)
-        self.iterations, self.rTr = st["iterations"], st["rTr"]
+        self.iterations, self.rTr = st["iterations"], st["true_rTr"]
         wp.copy(self.p0.f, mg.x)
         launch(scale_by_3d, dim=self.shape, inputs=[self.p0, self.h])
-        self._subtract_gradient()
+        # the gradient of the solver's FP64 x (p0 / h of the scale chain), as
+        # the fused projection subtracts it
+        x64 = solver.last_x64
+        g = self.shape
+        for d, uf, ur in ((1, self.vel0.u, self.vel_rigid.u), (2, self.vel0.v, self.vel_rigid.v),
+                          (3, self.vel0.w, self.vel_rigid.w)):
+            _lib.call("flub_subtract_gradient_tag_f64", uf.f.ptr, ur.f.ptr, x64.data_ptr(), self.tag.f.ptr, *g, d,
+                      None)
         return st
 
     def project_stepwise(self, max_iters=-1, rel_tol=1e-12, abs_tol=0.0, smoother="phases", verbose=False):

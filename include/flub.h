@@ -171,6 +171,11 @@ int flub_scale(float *q, float s, size_t n, void *stream);
  * uf/u_rigid have the face shape of that axis; p and tag are cell fields. */
 int flub_subtract_gradient_tag(float *uf, const float *u_rigid, const float *p, const int32_t *tag,
                                int nx, int ny, int nz, int d, void *stream);
+/* The same face update from an FP64 pressure p (a solver's FP64 solution x:
+ * the reference's p0 / h), computed in FP64 and rounded once; no FP64 path in
+ * the reference. */
+int flub_subtract_gradient_tag_f64(float *uf, const float *u_rigid, const double *p, const int32_t *tag,
+                                   int nx, int ny, int nz, int d, void *stream);
 /* warp.utils.array_inner (mgpcg.py:5,391,...): *out_dev = sum a*b, accumulated
  * in FP64 by a deterministic two-stage reduction. */
 int flub_inner(const float *a, const float *b, size_t n, double *out_dev, void *stream);
@@ -276,6 +281,14 @@ int flub_mgpcg_level_shape(flub_mgpcg *h, int level, int shape[3]);
  * and r.r) (8-10: FLUB_ESTATE when the grid has no split level 0).  Leaves the handle's
  * vectors in an unspecified state (call before init / after reading x). */
 int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_ms, double *bytes);
+/* In-solve timing (the same hook, inside the solve): arms HIP events around
+ * the launches of kernel 8 (the colour-split level-0 plain half-sweep) that
+ * the following split FP32 solves make, up to max_launches (<= 512) of them;
+ * flub_mgpcg_probe_read returns their mean and minimum event-to-event time
+ * (the kernel plus its dispatch) and the count, and disarms.  FLUB_ESTATE when
+ * the handle has no colour-split FP32 level 0 or is already armed. */
+int flub_mgpcg_probe(flub_mgpcg *h, int kernel, int max_launches);
+int flub_mgpcg_probe_read(flub_mgpcg *h, float *avg_ms, float *min_ms, int *count);
 
 /* ---------------------------------------------------------------------------
  * Whole projection (example_fluid.py:120-147 with water_3d.py:58-68 call_mgpcg):

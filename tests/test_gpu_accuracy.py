@@ -163,3 +163,25 @@ def test_config3_sphere_flow_256_vs_fp64_oracle(precision, sphere_flow_256, orac
         assert err < 1.5 * err_ref
     assert rel64 < 1e-6
     assert post < 1.5e-6
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (24, 40, 16), (17, 9, 13)])
+def test_subtract_gradient_f64_bitwise(shape, oracle):
+    """flub_subtract_gradient_tag_f64: the FP64 oracle's face update on the same
+    (FP32) velocities and FP64 pressure, rounded to FP32 -- bitwise, on the
+    vector kernels (nz % 4 == 0) and the scalar one."""
+    rng = np.random.default_rng(sum(shape))
+    nx, ny, nz = shape
+    tag = rng.choice(3, size=shape, p=[0.2, 0.6, 0.2]).astype(np.int32)
+    p = rng.normal(0, 1, shape)
+    for d, fshape in ((1, (nx + 1, ny, nz)), (2, (nx, ny + 1, nz)), (3, (nx, ny, nz + 1))):
+        u = rng.normal(0, 1, fshape).astype(np.float32)
+        ur = rng.normal(0, 1, fshape).astype(np.float32)
+        # every device tensor stays referenced until the call has run
+        ud, urd, pd, td = (torch.from_numpy(a).cuda() for a in (u, ur, p, tag))
+        _lib.call("flub_subtract_gradient_tag_f64", ud.data_ptr(), urd.data_ptr(), pd.data_ptr(), td.data_ptr(),
+                  nx, ny, nz, d, None)
+        torch.cuda.synchronize()
+        ref = oracle.subtract_gradient_tag(u.astype(np.float64), ur.astype(np.float64), p, tag, d,
+                                           real=np.float64).astype(np.float32)
+        assert np.array_equal(ud.cpu().numpy().view(np.uint32), ref.view(np.uint32)), d

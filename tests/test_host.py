@@ -82,3 +82,13 @@ def test_div_small_is_ieee_division(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True, env={**os.environ, "OMP_NUM_THREADS": "8"})
     assert r.returncode == 0, r.stdout + r.stderr
     assert "mismatches 0" in r.stdout
+
+
+def test_oracle_under_address_and_ub_sanitizers():
+    """The CPU oracle built with -fsanitize=address,undefined (make -C oracle
+    sanitize) runs its driver over every routine family on cubic and
+    non-cubic grids without a report."""
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "sanitize"], capture_output=True, text=True,
+                       env={**os.environ, "OMP_NUM_THREADS": "2"})
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    assert "sanitized oracle run complete" in r.stdout
