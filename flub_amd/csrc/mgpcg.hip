@@ -907,6 +907,16 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
             ALLOC(h->sp.zpre, n0 * 4);
             ALLOC(h->sp.x, n0 * 8);
             ALLOC(h->x64n, n0 * 8);
+            {   // the p ring: p, p_alt and up to 30 more, within 8 GiB
+                const size_t budget = (size_t)8 << 30;
+                size_t np = budget / (n0 * 4);
+                np = np < 3 ? 3 : (np > MAXPH ? MAXPH : np);
+                const char *ev = getenv("FLUB_PH");   // ring size override (3..32)
+                if (ev) np = (size_t)(atoi(ev) < 3 ? 3 : (atoi(ev) > MAXPH ? MAXPH : atoi(ev)));
+                h->sp.nph = (int)np;
+                for (int i = 2; i < h->sp.nph; ++i) ALLOC(h->sp.ph[i], n0 * 4);
+                ALLOC(h->sp.ahist, MAXPH * sizeof(double));
+            }
             ALLOC(h->sp.b, n0 * 4);
             ALLOC(h->sp.code, n0 * 2);
             h->sp.lcode[0] = h->sp.code;
@@ -963,6 +973,8 @@ FLUB_API int flub_mgpcg_destroy(flub_mgpcg *h)
     hipFree(h->tb_order);
     hipFree(h->sp.r); hipFree(h->sp.z[0]); hipFree(h->sp.z[1]); hipFree(h->sp.zpre); hipFree(h->sp.x);
     hipFree(h->x64n);
+    for (int i = 2; i < MAXPH; ++i) hipFree(h->sp.ph[i]);
+    hipFree(h->sp.ahist);
     hipFree(h->sp.code); hipFree(h->sp.rh); hipFree(h->sp.zh); hipFree(h->sp.zw); hipFree(h->sp.b);
     for (int l = 1; l < MAXL; ++l) { hipFree(h->sp.lcode[l]); hipFree(h->sp.zt[l]); }
     if (h->probe) {
@@ -1225,11 +1237,10 @@ FLUB_API int flub_mgpcg_solution_f64(flub_mgpcg *h, double *x)
 
 // mgpcg.py:378-462 for the "phases" smoother on the colour-split level 0:
 // r, x, z, p and Ap of level 0 are colour-split for the whole solve and
-// converted back at the end.  update_p then Ap, and update_x deferred into
-// later p steps (x += alpha p with the same alpha and p; two iterations'
-// updates at a time, p kept in two alternating buffers, see PxState; the last
-// ones after the loop), so that update_r reads and writes only r.  x
-// accumulates in FP64 (SplitBuf::x).  The V-cycle is queued
+// converted back at the end.  update_p then Ap; update_x (x += alpha p with
+// the same alpha and p) is deferred until x is needed, every p kept in a ring
+// of buffers meanwhile (PxState), so that update_r reads and writes only r and
+// x (FP64, SplitBuf::x) moves once per flush.  The V-cycle is queued
 // before the host reads the residual norm, into the z buffer the iteration
 // does not use: the GPU keeps working while the host tests for convergence.
 // If the test stops the solve, that speculative z is not used (x, r, z and
@@ -1249,7 +1260,7 @@ static int solve_split(flub_mgpcg *h, int max_iters, double eps, double abs_tol,
     int replacements = 0;
     int zs = 0;
     PxState px;
-    px.P[0] = h->p; px.P[1] = h->p_alt; px.x = S.x; px.sc = h->sc; px.n = n;
+    px_init(h, px, S.x, n);
     TRY(mg_split_enter(h, px.x_set));
     float *const R = S.r;
     // the r.r partials go to the mapped host buffer and the host sums them
@@ -1749,9 +1760,9 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
         case 10: {   // the split solve's p step: p (and the deferred x) update, then Ap and p.Ap
             FLUB_REQUIRE(h->split, FLUB_ESTATE, "no colour-split level 0 for this handle");
             int nbp = 0;
-            // the p step that applies two x updates (every second iteration)
+            // the p step: p into a ring buffer, then Ap
             PxState px;
-            px.P[0] = h->p; px.P[1] = h->p_alt; px.x = h->sp.x; px.sc = h->sc; px.n = L0.n; px.npend = 2;
+            px_init(h, px, h->sp.x, L0.n);
             TRY(px_step(h->s, px, h->sp.z[0], nullptr));
             TRY(split_Ap(h->s, L0, px.p(), h->Ap, h->sp.code, h->part, &nbp));
             return launch_finish(h, nbp, FIN_SUM, 0.0);
@@ -1774,7 +1785,7 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
     case 7: b = 4 * N * (2 * v + 1); break;                  // four half-sweeps
     case 8: b = N * (2 * v + 1); break;
     case 9: b = N * (2 * v + 1) + N / 8 * (v + 1); break;
-    case 10: b = N * (10 * 4 + 2); break;                    // p, p', z, x (FP64) -> p, x; p, codes -> Ap
+    case 10: b = N * (5 * 4 + 2); break;                     // p, z -> p'; p', codes -> Ap
     case 11: b = N * 3 * 4; break;                           // r, Ap -> r
     default: break;
     }

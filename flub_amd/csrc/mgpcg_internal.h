@@ -13,9 +13,8 @@
 // Device scalar slots (FP64).
 // SC_ZINV0/1: 1/scale of the mixed-precision V-cycle output held in the split z
 // buffer of that slot (the buffer holds z times the storage scale).
-// SC_APREV: the alpha of a deferred x update (see PxState).
 enum { SC_ALPHA = 0, SC_BETA, SC_SUM, SC_OLDZTR, SC_RTR, SC_MEAN, SC_COUNT, SC_RSCALE, SC_AUX0, SC_AUX1,
-       SC_AUX2, SC_AUX3, SC_ZINV0, SC_ZINV1, SC_APREV, SC_NUM = 16 };
+       SC_AUX2, SC_AUX3, SC_ZINV0, SC_ZINV1, SC_NUM = 16 };
 // k_finish operations.
 enum { FIN_SUM = 0, FIN_ALPHA, FIN_BETA, FIN_OLDZTR, FIN_RTR, FIN_MEAN, FIN_OUT };
 
@@ -53,6 +52,11 @@ struct SplitBuf {
     // levels 1 .. nsplit-1 that sweep in pairs (split_sweep_pair): a third z
     // buffer of the level (nullptr: one launch per half-sweep)
     float *zt[MAXL];
+    // p buffers of the solve's ring beyond p and p_alt (PxState), and the
+    // alpha saved per ring slot (device)
+    float *ph[32];
+    int nph;           // ring size, p and p_alt included
+    double *ahist;
 };
 
 // In-solve timing of the level-0 plain half-sweeps (flub_mgpcg_probe): HIP
@@ -131,138 +135,153 @@ inline int probe_end(flub_mgpcg *h)
 
 // The p step of the split solves over float4 vectors (n4 = n / 4), any
 // layout: p_out = beta p_prev + z (mgpcg.py:229-232), z times zinv[0] when
-// zinv is set (the z buffer holds z times a power-of-two scale: exact), with
-// the pending x updates of earlier iterations (mgpcg.py:207-210) folded in.
-// x is FP64: each update adds (double)alpha * (double)p (exact) with alpha the
-// FP32 value the r update used, one rounding in FP64 per update.  p_prev is
-// the previous iteration's p; the solve keeps p in two buffers alternately
-// (p_out != p_prev), so the p before it survives one more step.  XU:
-//   XU_NONE  nothing pending;
-//   XU_DEFER the update alpha p_prev waits for the next step: alpha goes to
-//            SC_APREV (this step moves no x);
-//   XU_ONE   x += alpha p_prev;
-//   XU_TWO   x += alpha' p_prev2 + alpha p_prev with alpha' from SC_APREV and
-//            p_prev2 the p before p_prev, read from p_out before it is
-//            overwritten (the same elements, by the same thread).
-// Deferring every other update moves x (16 bytes a cell) on every second
-// step only.  XSET: x is known to be zero (not read).  DOP false: the x
-// update alone (the solve's last).
-enum { XU_NONE = 0, XU_DEFER, XU_ONE, XU_TWO };
-template <int XU, bool XSET, bool DOP>
-__global__ void __launch_bounds__(256) k_update_pp(float *p_out, const float *p_prev, double *__restrict__ x,
-                                                   const float *__restrict__ z, double *__restrict__ sc,
-                                                   const double *__restrict__ zinv, uint32_t n4)
+// zinv is set (the z buffer holds z times a power-of-two scale: exact).  The
+// solve keeps every p since x was last updated (PxState): with asave set,
+// thread 0 also stores alpha (SC_ALPHA, the previous iteration's, which the
+// r update used) into *asave for the later x update of p_prev.
+static __global__ void __launch_bounds__(256) k_update_pp(float *__restrict__ p_out, const float *__restrict__ p_prev,
+                                                   const float *__restrict__ z, const double *__restrict__ sc,
+                                                   const double *__restrict__ zinv, double *__restrict__ asave,
+                                                   uint32_t n4)
 {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (t >= n4) return;
-    if (XU == XU_DEFER && t == 0) sc[SC_APREV] = sc[SC_ALPHA];   // read by a later launch only
+    if (asave && t == 0) asave[0] = sc[SC_ALPHA];   // read by a later launch only
     float4 pv = reinterpret_cast<const float4 *>(p_prev)[t];
-    if (XU == XU_ONE || XU == XU_TWO) {
-        const double alpha = (double)(float)sc[SC_ALPHA];
-        double xv[4] = {0.0, 0.0, 0.0, 0.0};
-        if (!XSET) {
-            const double2 a = reinterpret_cast<const double2 *>(x)[2 * t];
-            const double2 b = reinterpret_cast<const double2 *>(x)[2 * t + 1];
-            xv[0] = a.x; xv[1] = a.y; xv[2] = b.x; xv[3] = b.y;
-        }
-        if (XU == XU_TWO) {
-            const double a0 = (double)(float)sc[SC_APREV];
-            const float4 q = reinterpret_cast<const float4 *>(p_out)[t];
-            const float qq[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-            for (int e = 0; e < 4; ++e) xv[e] = fma(a0, (double)qq[e], xv[e]);
-        }
-        const float pp[4] = {pv.x, pv.y, pv.z, pv.w};
-#pragma unroll
-        for (int e = 0; e < 4; ++e) xv[e] = fma(alpha, (double)pp[e], xv[e]);
-        reinterpret_cast<double2 *>(x)[2 * t] = make_double2(xv[0], xv[1]);
-        reinterpret_cast<double2 *>(x)[2 * t + 1] = make_double2(xv[2], xv[3]);
+    const float beta = (float)sc[SC_BETA];
+    float4 zv = reinterpret_cast<const float4 *>(z)[t];
+    if (zinv) {
+        const float zi = (float)zinv[0];
+        zv.x *= zi; zv.y *= zi; zv.z *= zi; zv.w *= zi;
     }
-    if (DOP) {
-        const float beta = (float)sc[SC_BETA];
-        float4 zv = reinterpret_cast<const float4 *>(z)[t];
-        if (zinv) {
-            const float zi = (float)zinv[0];
-            zv.x *= zi; zv.y *= zi; zv.z *= zi; zv.w *= zi;
-        }
-        pv.x = fmaf(beta, pv.x, zv.x);
-        pv.y = fmaf(beta, pv.y, zv.y);
-        pv.z = fmaf(beta, pv.z, zv.z);
-        pv.w = fmaf(beta, pv.w, zv.w);
-        reinterpret_cast<float4 *>(p_out)[t] = pv;
-    }
+    pv.x = fmaf(beta, pv.x, zv.x);
+    pv.y = fmaf(beta, pv.y, zv.y);
+    pv.z = fmaf(beta, pv.z, zv.z);
+    pv.w = fmaf(beta, pv.w, zv.w);
+    reinterpret_cast<float4 *>(p_out)[t] = pv;
 }
 
-// The split solves' p buffers and pending x updates.  P[0], P[1] alternate
-// as the output of the p steps; after the r update of an iteration its
-// alpha p is pending (held in SC_ALPHA and the p buffer `pa`), possibly after
-// an older deferred one (alpha in SC_APREV, p in `pb`).
+// The deferred x updates of the split solves (mgpcg.py:207-210): x += alpha_i
+// p_i for the p of every iteration since the last flush, oldest first, in
+// FP64: each term (double)alpha_i * (double)p_i is exact (alpha_i the FP32
+// value the r update used) and each addition rounds once, so x is the FP64
+// sum of the updates the reference makes in FP32.  alpha_i is *a[i] (the last
+// one may be sc + SC_ALPHA).  XSET: x is zero (not read).
+#define MAXPH 32
+struct XFlush {
+    const float *p[MAXPH];
+    const double *a[MAXPH];
+    int n;
+};
+template <bool XSET>
+__global__ void __launch_bounds__(256) k_flush_x(double *__restrict__ x, XFlush f, uint32_t n4)
+{
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= n4) return;
+    double xv[4] = {0.0, 0.0, 0.0, 0.0};
+    if (!XSET) {
+        const double2 a = reinterpret_cast<const double2 *>(x)[2 * t];
+        const double2 b = reinterpret_cast<const double2 *>(x)[2 * t + 1];
+        xv[0] = a.x; xv[1] = a.y; xv[2] = b.x; xv[3] = b.y;
+    }
+    for (int i = 0; i < f.n; ++i) {
+        const double al = (double)(float)f.a[i][0];
+        const float4 q = reinterpret_cast<const float4 *>(f.p[i])[t];
+        xv[0] = fma(al, (double)q.x, xv[0]);
+        xv[1] = fma(al, (double)q.y, xv[1]);
+        xv[2] = fma(al, (double)q.z, xv[2]);
+        xv[3] = fma(al, (double)q.w, xv[3]);
+    }
+    reinterpret_cast<double2 *>(x)[2 * t] = make_double2(xv[0], xv[1]);
+    reinterpret_cast<double2 *>(x)[2 * t + 1] = make_double2(xv[2], xv[3]);
+}
+
+// The split solves' p buffers and deferred x updates.  Each p step writes the
+// new p into a free buffer of the ring (the previous p stays readable), and
+// the x update alpha p of every iteration waits until x is needed (the true
+// residual, the end) or the ring is full: then one pass adds them all.  x
+// (FP64, 16 bytes a cell read and written) then moves once per flush instead
+// of once per iteration, and each p once more (4 bytes).
 struct PxState {
-    float *P[2];
-    int cur = 0;          // index of the latest p in P
-    int npend = 0;        // pending x updates (0, 1, 2)
-    bool deferred = false;  // the one pending update's alpha moved to SC_APREV (p in P[cur])
-    bool x_set = true;    // x holds a value (else it is zero and not yet written)
+    float *ring[MAXPH];
+    int nring = 0;
+    double *ahist = nullptr;   // device: the alpha saved for each ring slot
+    int cur = 0;               // slot of the latest p
+    int pend[MAXPH];           // slots whose x update is pending, oldest first
+    int npend = 0;
+    bool newest_saved = false; // the newest pending alpha was copied into ahist
+    bool x_set = true;         // x holds a value (else it is zero and not yet written)
     double *x = nullptr;
     double *sc = nullptr;
     uint32_t n = 0;
-    float *p() const { return P[cur]; }
+    float *p() const { return ring[cur]; }
 };
-
-template <int XU, bool XSET, bool DOP>
-inline void launch_pp(hipStream_t s, const PxState &st, float *p_out, const float *p_prev, const float *z,
-                      const double *zinv)
-{
-    const uint32_t n4 = st.n / 4;
-    k_update_pp<XU, XSET, DOP><<<blocks_for(n4, 256), 256, 0, s>>>(p_out, p_prev, st.x, z, st.sc, zinv, n4);
-}
-
-// The p step of an iteration: p into the other buffer, the pending x update
-// deferred (one pending) or applied together with the deferred one (two).
-inline int px_step(hipStream_t s, PxState &st, const float *z, const double *zinv)
-{
-    FLUB_REQUIRE(st.n % 4 == 0, FLUB_EINVAL, "vector update: n = %u is not a multiple of 4", st.n);
-    float *prev = st.P[st.cur], *out = st.P[st.cur ^ 1];
-    if (st.npend == 0) {
-        launch_pp<XU_NONE, false, true>(s, st, out, prev, z, zinv);
-    } else if (st.npend == 1) {
-        launch_pp<XU_DEFER, false, true>(s, st, out, prev, z, zinv);
-        st.deferred = true;
-    } else {
-        if (st.x_set) launch_pp<XU_TWO, false, true>(s, st, out, prev, z, zinv);
-        else launch_pp<XU_TWO, true, true>(s, st, out, prev, z, zinv);
-        st.x_set = true;
-        st.npend = 0;
-    }
-    FLUB_CHECK_LAUNCH();
-    st.cur ^= 1;
-    return FLUB_OK;
-}
-
-// After the r update of an iteration: its alpha p is pending.
-inline void px_after_r_update(PxState &st)
-{
-    st.npend = st.deferred ? 2 : 1;
-    st.deferred = false;
-}
 
 // Apply every pending x update (before x is read).
 inline int px_flush(hipStream_t s, PxState &st)
 {
     if (st.npend == 0) return FLUB_OK;
-    float *pa = st.P[st.cur];
-    if (st.npend == 1) {
-        if (st.x_set) launch_pp<XU_ONE, false, false>(s, st, nullptr, pa, nullptr, nullptr);
-        else launch_pp<XU_ONE, true, false>(s, st, nullptr, pa, nullptr, nullptr);
-    } else {   // the older p in the other buffer, passed as p_out (read only: DOP false)
-        float *pb = st.P[st.cur ^ 1];
-        if (st.x_set) launch_pp<XU_TWO, false, false>(s, st, pb, pa, nullptr, nullptr);
-        else launch_pp<XU_TWO, true, false>(s, st, pb, pa, nullptr, nullptr);
+    FLUB_REQUIRE(st.n % 4 == 0, FLUB_EINVAL, "x update: n = %u is not a multiple of 4", st.n);
+    XFlush f{};
+    f.n = st.npend;
+    for (int i = 0; i < st.npend; ++i) {
+        const int sl = st.pend[i];
+        f.p[i] = st.ring[sl];
+        f.a[i] = (i + 1 < st.npend || st.newest_saved) ? st.ahist + sl : st.sc + SC_ALPHA;
     }
+    const uint32_t n4 = st.n / 4;
+    if (st.x_set) k_flush_x<false><<<blocks_for(n4, 256), 256, 0, s>>>(st.x, f, n4);
+    else k_flush_x<true><<<blocks_for(n4, 256), 256, 0, s>>>(st.x, f, n4);
     FLUB_CHECK_LAUNCH();
     st.x_set = true;
     st.npend = 0;
+    st.newest_saved = false;
     return FLUB_OK;
+}
+
+// The p step of an iteration: p into a free ring buffer; the pending update of
+// the previous p gets its alpha saved.
+inline int px_step(hipStream_t s, PxState &st, const float *z, const double *zinv)
+{
+    FLUB_REQUIRE(st.n % 4 == 0, FLUB_EINVAL, "vector update: n = %u is not a multiple of 4", st.n);
+    if (st.npend + 1 >= st.nring) {   // keep a buffer free besides the pending ones
+        const int rc = px_flush(s, st);
+        if (rc) return rc;
+    }
+    bool used[MAXPH] = {};
+    used[st.cur] = true;
+    for (int i = 0; i < st.npend; ++i) used[st.pend[i]] = true;
+    int out = 0;
+    while (used[out]) ++out;
+    const bool save = st.npend > 0 && st.pend[st.npend - 1] == st.cur && !st.newest_saved;
+    const uint32_t n4 = st.n / 4;
+    k_update_pp<<<blocks_for(n4, 256), 256, 0, s>>>(st.ring[out], st.ring[st.cur], z, st.sc, zinv,
+                                                    save ? st.ahist + st.cur : nullptr, n4);
+    FLUB_CHECK_LAUNCH();
+    if (save) st.newest_saved = true;
+    st.cur = out;
+    return FLUB_OK;
+}
+
+// The ring of a split solve: p (zero at the start of a solve, as init and
+// the fused projection leave it), p_alt and the extra buffers.
+inline void px_init(flub_mgpcg *h, PxState &st, double *x, uint32_t n)
+{
+    st.nring = h->sp.nph;
+    st.ring[0] = h->p;
+    st.ring[1] = h->p_alt;
+    for (int i = 2; i < st.nring; ++i) st.ring[i] = h->sp.ph[i];
+    st.ahist = h->sp.ahist;
+    st.x = x;
+    st.sc = h->sc;
+    st.n = n;
+}
+
+// After the r update of an iteration: its alpha p is pending.
+inline void px_after_r_update(PxState &st)
+{
+    st.pend[st.npend++] = st.cur;
+    st.newest_saved = false;
 }
 
 int mg_dot(flub_mgpcg *h, const float *a, const float *b, int op, double eps);

@@ -432,7 +432,7 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
     SplitBuf &S = h->sp;
     h->rh_ready = -1;
     PxState px;
-    px.P[0] = h->p; px.P[1] = h->p_alt; px.x = S.x; px.sc = h->sc; px.n = n;
+    px_init(h, px, S.x, n);
     TRYM(mg_split_enter(h, px.x_set));
     // the tolerance is first needed after iteration 1: the first V-cycle runs
     // while the initial r.r travels to the host

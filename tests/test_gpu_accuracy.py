@@ -185,3 +185,26 @@ def test_subtract_gradient_f64_bitwise(shape, oracle):
         ref = oracle.subtract_gradient_tag(u.astype(np.float64), ur.astype(np.float64), p, tag, d,
                                            real=np.float64).astype(np.float32)
         assert np.array_equal(ud.cpu().numpy().view(np.uint32), ref.view(np.uint32)), d
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_x_update_ring_sizes(precision, monkeypatch):
+    """The deferred x updates (every p kept in a ring of buffers until x is
+    needed): a ring of 3 buffers, which flushes x at every iteration, against
+    the default ring, which flushes once: the same iterations, residuals equal
+    to FP64 rounding and the FP32 x equal to within one rounding."""
+    s = scenes.dam_break_spheres(64, 1)
+    res = []
+    for ring in (None, "3"):
+        if ring:
+            monkeypatch.setenv("FLUB_PH", ring)
+        pr = Projection((64, 64, 64), n_mg_levels=4, pre_and_post_smoothing=2, bottom_smoothing=6,
+                        precision=precision)
+        pr.load(s)
+        it = pr.project(max_iters=-1, rel_tol=1e-12)
+        res.append((it, solution_f64(pr), pr.mgpcg.x.numpy(), pr.rTr))
+    (i0, x0, f0, r0), (i1, x1, f1, r1) = res
+    assert i0 == i1
+    assert _rel(x0, x1) < 1e-13
+    assert np.isclose(r0, r1, rtol=1e-9)
+    assert np.max(np.abs(f0 - f1)) <= 2.0 ** -23 * np.max(np.abs(f0))
