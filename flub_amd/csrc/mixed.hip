@@ -56,6 +56,50 @@ __global__ void __launch_bounds__(RBS) k_mx_update(float *__restrict__ r, float 
     }
 }
 
+// k_mx_update without x (the split solve), four values a thread: r -= alpha q,
+// the BF16 r when rh is set, partials of r.r (FP64) and max|r|.
+__global__ void __launch_bounds__(RBS) k_mx_update4(float *__restrict__ r, const float *__restrict__ q,
+                                                    const double *__restrict__ sc, uint32_t n4,
+                                                    double *__restrict__ part, __hip_bfloat16 *__restrict__ rh)
+{
+    __shared__ double lds[RBS / 64];
+    const float alpha = (float)sc[SC_ALPHA];
+    double acc = 0.0;
+    float mx = 0.0f;
+    for (uint32_t t = blockIdx.x * RBS + threadIdx.x; t < n4; t += gridDim.x * RBS) {
+        const float4 qv = reinterpret_cast<const float4 *>(q)[t];
+        float4 rv = reinterpret_cast<const float4 *>(r)[t];
+        rv.x = fmaf(-alpha, qv.x, rv.x);
+        rv.y = fmaf(-alpha, qv.y, rv.y);
+        rv.z = fmaf(-alpha, qv.z, rv.z);
+        rv.w = fmaf(-alpha, qv.w, rv.w);
+        reinterpret_cast<float4 *>(r)[t] = rv;
+        if (rh) {
+            uint2 hv;
+            __hip_bfloat16 *e = reinterpret_cast<__hip_bfloat16 *>(&hv);
+            e[0] = __float2bfloat16(rv.x); e[1] = __float2bfloat16(rv.y);
+            e[2] = __float2bfloat16(rv.z); e[3] = __float2bfloat16(rv.w);
+            reinterpret_cast<uint2 *>(rh)[t] = hv;
+        }
+        acc += (double)rv.x * (double)rv.x;
+        acc += (double)rv.y * (double)rv.y;
+        acc += (double)rv.z * (double)rv.z;
+        acc += (double)rv.w * (double)rv.w;
+        mx = fmaxf(mx, fmaxf(fmaxf(fabsf(rv.x), fabsf(rv.y)), fmaxf(fabsf(rv.z), fabsf(rv.w))));
+    }
+    const double s = block_sum<RBS>(acc, lds);
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_down(mx, off, 64));
+    __shared__ float lm[RBS / 64];
+    if ((threadIdx.x & 63) == 0) lm[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = 0.0f;
+        for (int w = 0; w < RBS / 64; ++w) m = fmaxf(m, lm[w]);
+        part[blockIdx.x] = s;
+        part[gridDim.x + blockIdx.x] = (double)m;
+    }
+}
+
 // r = b - ndt(x) evaluated in FP64 (true residual) from the FP32 or FP64 x,
 // stored in FP32 (r may be null); partials of |r|^2 (FP64, from the FP64
 // values) and max|r|.
@@ -462,8 +506,7 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
         // BF16 (scale 1: no max|r| needed on the device): the r.r partials go
         // to the mapped host buffer and the host sums them
         const bool hsum = h->mx_unit && h->hpart_dev;
-        k_mx_update<false><<<nb, RBS, 0, h->s>>>(S.r, nullptr, h->Ap, nullptr, h->sc, n,
-                                                 hsum ? h->hpart_dev : h->part, rh_next);
+        k_mx_update4<<<nb, RBS, 0, h->s>>>(S.r, h->Ap, h->sc, n / 4, hsum ? h->hpart_dev : h->part, rh_next);
         FLUB_CHECK_LAUNCH();
         px_after_r_update(px);
         h->rh_ready = h->mx_unit ? (zs ^ 1) : -1;

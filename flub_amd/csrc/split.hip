@@ -200,28 +200,30 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
                                                                       kp, km);
         }
         if (OUT == SWEEP_OUT_FINAL) {
-            const float inv = 1.0f / s0;   // a power of two: exact
-            float zo[CPT], zw[CPT], ro[CPT], rw[CPT], qo[CPT], qw[CPT];
-#pragma unroll
-            for (int e = 0; e < CPT; ++e) { zo[e] = out[e] * inv; zw[e] = w[e] * inv; }
+            // z and r over the storage scale s0 (a power of two: exact); the
+            // products scale by s0^2, applied once to the thread's sums
+            float rw[CPT], qo[CPT], qw[CPT];
             stn<TZO, CPT>(reinterpret_cast<TZO *>(a.zout) + own + base, out);
             // r from the sweep's stored r (r times the storage scale, rounded to
-            // TR; inv is a power of two): the dot's rounding errors average out
-            // over the level, and the other colour costs sizeof(TR) bytes a cell
+            // TR): the dot's rounding errors average out over the level, and the
+            // other colour costs sizeof(TR) bytes a cell
             ldn<TR, CPT>(reinterpret_cast<const TR *>(a.r) + oth + base, rw);
-#pragma unroll
-            for (int e = 0; e < CPT; ++e) { ro[e] = rv[e] * inv; rw[e] = rw[e] * inv; }
-            // r_old as the previous V-cycle stored it, over its own scale
-            const float iold = (float)a.zinv_old[0];
+            // r_old as the previous V-cycle stored it (over its own scale)
             ldn<TR, CPT>(reinterpret_cast<const TR *>(a.rold) + own + base, qo);
             ldn<TR, CPT>(reinterpret_cast<const TR *>(a.rold) + oth + base, qw);
-#pragma unroll
-            for (int e = 0; e < CPT; ++e) { qo[e] = qo[e] * iold; qw[e] = qw[e] * iold; }
+            // the vector's terms summed in FP32, the sums accumulated in FP64
+            // (this is the flexible CG's beta of a reduced-precision V-cycle:
+            // an FP32 rounding per term is far below the V-cycle's own error)
+            float s1 = 0.0f, s2 = 0.0f;
 #pragma unroll
             for (int e = 0; e < CPT; ++e) {
-                acc1 += (double)zo[e] * (double)ro[e] + (double)zw[e] * (double)rw[e];
-                acc2 += (double)zo[e] * (double)qo[e] + (double)zw[e] * (double)qw[e];
+                s1 = fmaf(out[e], rv[e], s1);
+                s1 = fmaf(w[e], rw[e], s1);
+                s2 = fmaf(out[e], qo[e], s2);
+                s2 = fmaf(w[e], qw[e], s2);
             }
+            acc1 += (double)s1;
+            acc2 += (double)s2;
         } else if (OUT == SWEEP_OUT_NATURAL) {
             // cells k = 2 m0 .. 2 m0 + 2 CPT - 1 of row (i, j): this colour at
             // k = 2m + o, the other one (unchanged, +0 on non-FLUID cells) at 2m + 1 - o
@@ -250,6 +252,11 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
                 for (int e = 0; e < CPT; ++e) acc1 += (double)out[e] * (double)rv[e] + (double)w[e] * (double)ro[e];
             }
         }
+    }
+    if (OUT == SWEEP_OUT_FINAL) {   // (z / s0).(r / s0) and (z / s0).(r_old / s_old): powers of two
+        const double inv = 1.0 / (double)s0;
+        acc1 *= inv * inv;
+        acc2 *= inv * a.zinv_old[0];
     }
     if (OUT == SWEEP_OUT_DOT || OUT == SWEEP_OUT_FINAL) {   // the only modes with partials (part != nullptr)
         const double s1 = block_sum<256>(acc1, lds);
