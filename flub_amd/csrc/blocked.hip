@@ -64,7 +64,7 @@ __device__ __forceinline__ float tb_update(uint32_t cd, float z, float r, float 
         const uint32_t mm = (uint32_t)((int32_t)(nzb << (29 - 4 * a)) >> 31);
         s = s + (keep_if(xs[2 * a] - z, mp) + keep_if(xs[2 * a + 1] - z, mm));
     }
-    const float q = div_small(r + s, (float)((cd >> 12) & 7u), y);
+    const float q = div_small_wave(r + s, (float)((cd >> 12) & 7u), y);
     return keep_if(z + q, (uint32_t)((int32_t)(cd << 16) >> 31));   // +0 unless FLUID
 }
 

@@ -25,3 +25,23 @@ FLUB_HD float div_small(float x, float df, float y)
     if ((ax < 0x1p-123f && ax != 0.0f) || !(ax <= 0x1.fffffep127f)) q = x / df;
     return q;
 }
+
+#if defined(__HIPCC__)
+// The same quotient in device code with the IEEE division behind a
+// wavefront-uniform branch: written as above, the compiler computes both
+// quotients for every lane and selects, and the IEEE division is most of
+// the instructions of a cell update.
+__device__ __forceinline__ float div_small_wave(float x, float df, float y)
+{
+    const float q0 = x * y;
+    float q = copysignf(fmaf(fmaf(-q0, df, x), y, q0), x);
+    const float ax = fabsf(x);
+    const bool slow = (ax < 0x1p-123f && ax != 0.0f) || !(ax <= 0x1.fffffep127f);
+    if (__builtin_amdgcn_ballot_w64(slow) != 0ull) {
+        float xs = x;
+        asm volatile("" : "+v"(xs));   // keeps the division inside the branch
+        if (slow) q = xs / df;
+    }
+    return q;
+}
+#endif

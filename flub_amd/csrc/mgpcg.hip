@@ -574,8 +574,11 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
             void *RH = static_cast<uint16_t *>(S.rh) + (size_t)zslot * F.n;
             m.code = S.code; m.rh = RH; m.r32 = S.r; m.sc = h->sc; m.zin = S.zh; m.zout = S.zh;
             const bool have_rh = h->rh_ready == zslot;   // the r update stored this cycle's T r
+            // ... and ran its first half-sweep (colour 1 of zh from z = 0)
+            const bool have_zh0 = have_rh && h->zh0_ready == zslot;
             for (int it = 0; it < s; ++it)
                 for (int ph = 0; ph < 2; ++ph) {
+                    if (it == 0 && ph == 0 && have_zh0) continue;
                     m.phase = ph;
                     m.kind = it > 0 ? SMX_PRE
                                     : ph == 0 ? (have_rh ? SMX_PRE_ZERO_RH : SMX_PRE_ZERO_SRC)
@@ -816,7 +819,7 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
     FLUB_REQUIRE(h, FLUB_EINVAL, "out of host memory");
     h->L = levels;
     h->prec = precision;
-    h->rh_ready = -1;
+    h->rh_ready = h->zh0_ready = -1;
     h->pre_post = 6;
     h->bottom = 6;
     h->no_graph = getenv("FLUB_NO_GRAPH") != nullptr;
@@ -925,6 +928,7 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
             h->pair_l0 = precision == FLUB_PREC_FP32 && split_pair_supported(h->lv[0]) &&
                          getenv("FLUB_NO_PAIR") == nullptr && getenv("FLUB_NO_PAIR_L0") == nullptr;
             h->mx_unit = precision == FLUB_PREC_BF16 && getenv("FLUB_MX_SRC32") == nullptr;
+            h->mx_fuse0 = h->mx_unit && getenv("FLUB_MX_NO_FUSE0") == nullptr;
             // The next levels are colour-split too while they hold at least
             // min_cells (default 2^20): there one launch per half-sweep on the
             // split layout streams faster than the blocked natural-layout
@@ -1704,6 +1708,7 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
     FLUB_REQUIRE(h && avg_ms && reps > 0, FLUB_EINVAL, "flub_mgpcg_profile: bad argument");
     FLUB_REQUIRE(h->tags_set, FLUB_ESTATE, "profile before set_tags");
     h->x64_valid = false;
+    h->rh_ready = h->zh0_ready = -1;   // the replayed V-cycle converts r itself
     Level &L0 = h->lv[0];
     const double N = (double)L0.n, v = h->prec == FLUB_PREC_FP32 ? 4.0 : 2.0;
     double b = 0.0;
@@ -1826,7 +1831,7 @@ FLUB_API int flub_mgpcg_precondition_into(flub_mgpcg *h, const float *r, float *
     if (h->split && smoother == FLUB_SMOOTHER_PHASES) {
         // the cycle converts the caller's r: no BF16 r stored by an earlier
         // solve's r update (one that returned early) may stand in for it
-        h->rh_ready = -1;
+        h->rh_ready = h->zh0_ready = -1;
         TRY(split_convert(h->s, L0, r, h->sp.r, 4, true));
         if (h->prec != FLUB_PREC_FP32) TRY(mg_mixed_rscale_of(h, h->sp.r));
         TRY(mg_precondition_split(h, 0));

@@ -108,6 +108,8 @@ struct flub_mgpcg {
     bool pair_l0;      // FP32 split level 0: the first pre- and post-smoothing half-sweeps in pairs
     bool mx_unit;      // BF16 split solve: storage scale 1, the r update stores the next cycle's BF16 r
     int rh_ready;      // z slot whose BF16 r the r update stored for the next V-cycle (-1: none)
+    bool mx_fuse0;     // BF16 split solve: the r update also runs the next cycle's first half-sweep
+    int zh0_ready;     // z slot whose first pre-smoothing half-sweep the r update ran (-1: none)
     bool x64_valid;    // sp.x holds the FP64 x of the last solve (h->x is it rounded to FP32)
     double *x64n;      // the fused projection: the FP64 x in the reference layout, for the gradient
     bool want_x64n;    // the next split solve also writes x64n

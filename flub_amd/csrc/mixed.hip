@@ -22,6 +22,7 @@
 
 #include "mgpcg_internal.h"
 #include "smooth.h"
+#include "stencil.h"
 
 // r -= alpha q; with DOX x += alpha p; partials of r.r and max|r|; with rh
 // also the BF16 r (storage scale 1) for the next V-cycle.
@@ -57,10 +58,17 @@ __global__ void __launch_bounds__(RBS) k_mx_update(float *__restrict__ r, float 
 }
 
 // k_mx_update without x (the split solve), four values a thread: r -= alpha q,
-// the BF16 r when rh is set, partials of r.r (FP64) and max|r|.
+// the BF16 r when rh is set, partials of r.r (FP64) and max|r|.  With zh
+// set (and rh), it also runs the next V-cycle's first half-sweep on the
+// cells from vector zh_from on (the colour that sweep updates, the second
+// half of the split layout): from z = 0 every neighbour term is +0, so that
+// sweep's z is gs_update of the cell's code and its BF16 r alone, stored as
+// BF16 in zh -- the values the ZERO launch computes from the same BF16 r.
 __global__ void __launch_bounds__(RBS) k_mx_update4(float *__restrict__ r, const float *__restrict__ q,
                                                     const double *__restrict__ sc, uint32_t n4,
-                                                    double *__restrict__ part, __hip_bfloat16 *__restrict__ rh)
+                                                    double *__restrict__ part, __hip_bfloat16 *__restrict__ rh,
+                                                    const uint16_t *__restrict__ code = nullptr,
+                                                    __hip_bfloat16 *__restrict__ zh = nullptr, uint32_t zh_from = 0)
 {
     __shared__ double lds[RBS / 64];
     const float alpha = (float)sc[SC_ALPHA];
@@ -69,6 +77,9 @@ __global__ void __launch_bounds__(RBS) k_mx_update4(float *__restrict__ r, const
     for (uint32_t t = blockIdx.x * RBS + threadIdx.x; t < n4; t += gridDim.x * RBS) {
         const float4 qv = reinterpret_cast<const float4 *>(q)[t];
         float4 rv = reinterpret_cast<const float4 *>(r)[t];
+        uint2 c4 = make_uint2(0u, 0u);
+        const bool sweep = zh && t >= zh_from;
+        if (sweep) c4 = reinterpret_cast<const uint2 *>(code)[t];
         rv.x = fmaf(-alpha, qv.x, rv.x);
         rv.y = fmaf(-alpha, qv.y, rv.y);
         rv.z = fmaf(-alpha, qv.z, rv.z);
@@ -80,6 +91,16 @@ __global__ void __launch_bounds__(RBS) k_mx_update4(float *__restrict__ r, const
             e[0] = __float2bfloat16(rv.x); e[1] = __float2bfloat16(rv.y);
             e[2] = __float2bfloat16(rv.z); e[3] = __float2bfloat16(rv.w);
             reinterpret_cast<uint2 *>(rh)[t] = hv;
+            if (sweep) {
+                const uint32_t cd[4] = {c4.x & 0xffffu, c4.x >> 16, c4.y & 0xffffu, c4.y >> 16};
+                uint2 zv;
+                __hip_bfloat16 *ze = reinterpret_cast<__hip_bfloat16 *>(&zv);
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    ze[u] = __float2bfloat16(gs_update<true, true>(cd[u], 0.0f, __bfloat162float(e[u]), 0.0f, 0.0f,
+                                                                   0.0f, 0.0f, 0.0f, 0.0f));
+                reinterpret_cast<uint2 *>(zh)[t] = zv;
+            }
         }
         acc += (double)rv.x * (double)rv.x;
         acc += (double)rv.y * (double)rv.y;
@@ -474,7 +495,7 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
     const int max_replacements = 3;
     int replacements = 0;
     SplitBuf &S = h->sp;
-    h->rh_ready = -1;
+    h->rh_ready = h->zh0_ready = -1;
     PxState px;
     px_init(h, px, S.x, n);
     TRYM(mg_split_enter(h, px.x_set));
@@ -506,10 +527,15 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
         // BF16 (scale 1: no max|r| needed on the device): the r.r partials go
         // to the mapped host buffer and the host sums them
         const bool hsum = h->mx_unit && h->hpart_dev;
-        k_mx_update4<<<nb, RBS, 0, h->s>>>(S.r, h->Ap, h->sc, n / 4, hsum ? h->hpart_dev : h->part, rh_next);
+        // BF16: the update also runs that cycle's first half-sweep (from z = 0)
+        const bool fuse0 = rh_next && h->mx_fuse0;
+        k_mx_update4<<<nb, RBS, 0, h->s>>>(S.r, h->Ap, h->sc, n / 4, hsum ? h->hpart_dev : h->part, rh_next,
+                                           fuse0 ? S.code : nullptr,
+                                           fuse0 ? static_cast<__hip_bfloat16 *>(S.zh) : nullptr, n / 8);
         FLUB_CHECK_LAUNCH();
         px_after_r_update(px);
         h->rh_ready = h->mx_unit ? (zs ^ 1) : -1;
+        h->zh0_ready = fuse0 ? (zs ^ 1) : -1;
         if (hsum) {
             FLUB_CHECK_HIP(hipEventRecord(h->ev, h->s));
         } else {
@@ -532,7 +558,7 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
             TRYM(split_true_residual(h->s, L0, S.r, S.b, S.x, S.code, h->part, &nbt));
             k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nbt, h->sc, h->mx_unit ? -1 : 0,
                                                h->hsc_dev ? h->hsc_dev + SC_RTR : nullptr);
-            h->rh_ready = -1;   // r replaced: the next cycle converts it again
+            h->rh_ready = h->zh0_ready = -1;   // r replaced: the next cycle converts it again
             FLUB_CHECK_LAUNCH();
             TRYM(mg_rtr_sent(h));
             FLUB_CHECK_HIP(hipEventSynchronize(h->ev));
@@ -565,7 +591,7 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
         TRYM(split_convert(h->s, L0, px.p(), S.zw, 4, false));
         FLUB_CHECK_HIP(hipMemcpyAsync(h->p, S.zw, (size_t)n * 4, hipMemcpyDeviceToDevice, h->s));
     }
-    h->rh_ready = -1;
+    h->rh_ready = h->zh0_ready = -1;
     if (status == FLUB_OK && it >= cap && max_iters < 0) {
         status = FLUB_ENOTCONV;
         flub_set_error("no convergence within %d iterations", cap);

@@ -231,7 +231,8 @@ def test_split_coarse_vcycle_bitexact(shape, levels, variant, oracle, monkeypatc
     assert bits_equal(zd.numpy(), orc.get("z", 0))
 
 
-@pytest.mark.parametrize("variant", ["FLUB_SPLIT_GRAPH", "FLUB_RTR_COPY", "FLUB_NO_PAIR", "FLUB_MX_SRC32"])
+@pytest.mark.parametrize("variant", ["FLUB_SPLIT_GRAPH", "FLUB_RTR_COPY", "FLUB_NO_PAIR", "FLUB_MX_SRC32",
+                                     "FLUB_MX_NO_FUSE0"])
 @pytest.mark.parametrize("precision", ["fp32", "fp16", "bf16"])
 def test_split_launch_variants_bitwise(precision, variant, monkeypatch):
     """How the fused solves queue their work changes no value: replaying the
@@ -239,8 +240,9 @@ def test_split_launch_variants_bitwise(precision, variant, monkeypatch):
     the residual norm to the host instead of storing it there from the finish
     kernel, one launch per coarse half-sweep instead of pairs, or (BF16) the
     first sweeps converting the scaled FP32 r instead of reading the BF16 r
-    the r update stored at scale 1, gives bitwise the same pressure in the
-    same iterations (coarse levels split as on large grids)."""
+    the r update stored at scale 1, or (BF16) the first half-sweep launched on
+    its own instead of run by the r update, gives bitwise the same pressure in
+    the same iterations (coarse levels split as on large grids)."""
     s = scenes.dam_break_spheres(64, 1)
     monkeypatch.setenv("FLUB_SPLIT_MIN_CELLS", "1")
     res = []
