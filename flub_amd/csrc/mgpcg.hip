@@ -585,7 +585,22 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
                                               : (have_rh ? SMX_PRE_OWN0_RH : SMX_PRE_SRC);
                     TRY(split_sweep_mixed(h->s, F, h->prec, m));
                 }
-            TRY(split_restrict_mixed(h->s, F, h->prec, RH, S.zh, S.code, h->lv[1], h->nsplit > 1));
+            TRY(split_restrict_mixed(h->s, F, h->prec, RH, S.zh, S.code, h->lv[1], h->nsplit > 1,
+                                     h->mx_l1 ? S.r1h : nullptr));
+            continue;
+        }
+        if (l == 1 && split && h->mx_l1) {
+            // BF16 pre-smoothing of level 1: the pairs alternate between its two
+            // BF16 z buffers, the last into zp1h; then the restriction into
+            // level 2's FP32 r
+            const void *in = nullptr;
+            for (int it = 0; it < s; ++it) {
+                void *out = ((s - 1 - it) & 1) ? S.zt1h : S.zp1h;
+                TRY(split_sweep_pair(h->s, F, in, out, nullptr, S.r1h, S.lcode[1], 0, it == 0 ? SWEEP_ZERO : SWEEP_PLAIN,
+                                     false, PAIR_BF16));
+                in = out;
+            }
+            TRY(split_restrict_mixed(h->s, F, h->prec, S.r1h, S.zp1h, S.lcode[1], h->lv[2], 2 < h->nsplit));
             continue;
         }
         if (split && l < h->nsplit) {   // level 0 in sp.*, the next split levels in their own buffers
@@ -684,13 +699,18 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
             const uint16_t *CD = S.lcode[l];
             if (l > 0 && S.zt[l] && s >= 2) {
                 // the sweeps in pairs from ZP through zt and z (alternating),
-                // the last pair writing the whole z, reference layout, into ZP
-                const float *in = ZP;
+                // the last pair writing the whole z, reference layout, into ZP;
+                // a BF16 level 1 starts from its BF16 pre-smoothed z and reads
+                // its BF16 r
+                const bool h1 = l == 1 && h->mx_l1;
+                const void *in = h1 ? S.zp1h : (const void *)ZP;
+                const void *RR = h1 ? S.r1h : (const void *)R;
                 for (int it = 0; it < s; ++it) {
                     const bool last = it + 1 == s;
                     float *out = last ? ZP : ((it & 1) ? Z : S.zt[l]);
-                    TRY(split_sweep_pair(h->s, F, in, out, it == 0 ? zc : nullptr, R, CD, 1,
-                                         it == 0 ? SWEEP_PROLONG : SWEEP_PLAIN, last));
+                    const int ty = !h1 ? PAIR_F32 : (it == 0 ? PAIR_BF16_IN : PAIR_BF16_R);
+                    TRY(split_sweep_pair(h->s, F, in, out, it == 0 ? zc : nullptr, RR, CD, 1,
+                                         it == 0 ? SWEEP_PROLONG : SWEEP_PLAIN, last, ty));
                     in = out;
                 }
                 continue;
@@ -953,6 +973,15 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
                 if (e_ == hipSuccess) e_ = hipStreamSynchronize(h->s);
                 if (e_ != hipSuccess) { flub_set_error("init: %s", hipGetErrorString(e_)); rc = (int)e_; goto fail; }
             }
+            // BF16: a split level 1 that sweeps in pairs pre-smooths in BF16 too
+            // (its r and pre-smoothed z; the post-smoothing stays FP32)
+            h->mx_l1 = h->mx_unit && h->nsplit > 1 && h->sp.zt[1] && getenv("FLUB_MX_NO_L1") == nullptr;
+            if (h->mx_l1) {
+                const size_t n1 = h->lv[1].n;
+                ALLOC(h->sp.r1h, n1 * 2);
+                ALLOC(h->sp.zp1h, n1 * 2);
+                ALLOC(h->sp.zt1h, n1 * 2);
+            }
         }
     }
 #undef ALLOC
@@ -981,6 +1010,7 @@ FLUB_API int flub_mgpcg_destroy(flub_mgpcg *h)
     hipFree(h->sp.ahist);
     hipFree(h->sp.code); hipFree(h->sp.rh); hipFree(h->sp.zh); hipFree(h->sp.zw); hipFree(h->sp.b);
     for (int l = 1; l < MAXL; ++l) { hipFree(h->sp.lcode[l]); hipFree(h->sp.zt[l]); }
+    hipFree(h->sp.r1h); hipFree(h->sp.zp1h); hipFree(h->sp.zt1h);
     if (h->probe) {
         for (int i = 0; i < 2 * h->probe->max; ++i) hipEventDestroy(h->probe->ev[i]);
         free(h->probe);

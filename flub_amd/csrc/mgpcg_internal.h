@@ -57,6 +57,9 @@ struct SplitBuf {
     float *ph[32];
     int nph;           // ring size, p and p_alt included
     double *ahist;
+    // BF16 solve: level 1's r and its pre-smoothed z (and the pairs' second z
+    // buffer) in BF16, split layout
+    void *r1h, *zp1h, *zt1h;
 };
 
 // In-solve timing of the level-0 plain half-sweeps (flub_mgpcg_probe): HIP
@@ -109,6 +112,7 @@ struct flub_mgpcg {
     bool mx_unit;      // BF16 split solve: storage scale 1, the r update stores the next cycle's BF16 r
     int rh_ready;      // z slot whose BF16 r the r update stored for the next V-cycle (-1: none)
     bool mx_fuse0;     // BF16 split solve: the r update also runs the next cycle's first half-sweep
+    bool mx_l1;        // BF16 split solve: level 1 pre-smooths in BF16 (sp.r1h, zp1h, zt1h)
     int zh0_ready;     // z slot whose first pre-smoothing half-sweep the r update ran (-1: none)
     bool x64_valid;    // sp.x holds the FP64 x of the last solve (h->x is it rounded to FP32)
     double *x64n;      // the fused projection: the FP64 x in the reference layout, for the gradient

@@ -61,8 +61,8 @@ __global__ void __launch_bounds__(RBS) k_mx_update(float *__restrict__ r, float 
 // the BF16 r when rh is set, partials of r.r (FP64) and max|r|.  With zh
 // set (and rh), it also runs the next V-cycle's first half-sweep on the
 // cells from vector zh_from on (the colour that sweep updates, the second
-// half of the split layout): from z = 0 every neighbour term is +0, so that
-// sweep's z is gs_update of the cell's code and its BF16 r alone, stored as
+// half of the split layout): from z = 0 every neighbour holds +0, so that
+// sweep's z is gs_direct of the cell's code and its BF16 r alone, stored as
 // BF16 in zh -- the values the ZERO launch computes from the same BF16 r.
 __global__ void __launch_bounds__(RBS) k_mx_update4(float *__restrict__ r, const float *__restrict__ q,
                                                     const double *__restrict__ sc, uint32_t n4,
@@ -97,8 +97,8 @@ __global__ void __launch_bounds__(RBS) k_mx_update4(float *__restrict__ r, const
                 __hip_bfloat16 *ze = reinterpret_cast<__hip_bfloat16 *>(&zv);
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
-                    ze[u] = __float2bfloat16(gs_update<true, true>(cd[u], 0.0f, __bfloat162float(e[u]), 0.0f, 0.0f,
-                                                                   0.0f, 0.0f, 0.0f, 0.0f));
+                    ze[u] = __float2bfloat16(gs_direct(cd[u], __bfloat162float(e[u]), 0.0f, 0.0f, 0.0f, 0.0f, 0.0f,
+                                                       0.0f));
                 reinterpret_cast<uint2 *>(zh)[t] = zv;
             }
         }

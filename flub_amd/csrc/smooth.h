@@ -109,10 +109,14 @@ int split_sweep(hipStream_t s, const Level &L0, const float *zin, float *zout, c
 // SWEEP_PROLONG (from zin + zc[parent]) for the first, SWEEP_PLAIN for the
 // second; from zin into zout (a different buffer; both colours, or with
 // natural_out the whole z in the reference layout).  Bitwise the two
-// split_sweep launches.
+// split_sweep launches.  `types` selects the storage: PAIR_F32 (z and r FP32),
+// PAIR_BF16 (z in and out and r BF16), PAIR_BF16_IN (BF16 z in and r, FP32 z
+// out), PAIR_BF16_R (FP32 z, BF16 r); the 16-bit forms divide through the
+// hardware reciprocal like the level-0 16-bit sweeps.
+enum { PAIR_F32 = 0, PAIR_BF16 = 1, PAIR_BF16_IN = 2, PAIR_BF16_R = 3 };
 bool split_pair_supported(const Level &L);
-int split_sweep_pair(hipStream_t s, const Level &L, const float *zin, float *zout, const float *zc, const float *r,
-                     const uint16_t *code, int phase, int mode, bool natural_out = false);
+int split_sweep_pair(hipStream_t s, const Level &L, const void *zin, void *zout, const float *zc, const void *r,
+                     const uint16_t *code, int phase, int mode, bool natural_out = false, int types = PAIR_F32);
 // Mixed-precision level-0 sweeps on the split layout (T = FP16 or BF16 by
 // prec): pre-smoothing stores z and r as T (the first two sweeps take r from
 // the FP32 CG residual times the storage scale), post-smoothing keeps z in
@@ -138,10 +142,10 @@ struct SplitMixedSweep {
     int *nblk;
 };
 int split_sweep_mixed(hipStream_t s, const Level &L0, int prec, const SplitMixedSweep &m);
-// Restriction of the split level 0 (r, z stored as T) into level 1's FP32 r,
-// colour-split when csplit.
+// Restriction of a split level (r, z stored as T) into the next level's FP32
+// r, colour-split when csplit; with rc16 (BF16 only) into that BF16 buffer.
 int split_restrict_mixed(hipStream_t s, const Level &F, int prec, const void *rh, const void *zh,
-                         const uint16_t *code, const Level &C, bool csplit = false);
+                         const uint16_t *code, const Level &C, bool csplit = false, void *rc16 = nullptr);
 // Restriction of a split level (r, z) into the next level's r, colour-split
 // when csplit.
 int split_restrict(hipStream_t s, const Level &F, const float *r, const float *z, const uint16_t *code,

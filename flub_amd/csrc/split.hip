@@ -113,9 +113,21 @@ struct SSweepArgs {
 // stores like STORE (z stays in storage scale; 1/scale goes to zinv) and
 // accumulates (z / scale).(r / scale) (r as stored, TR) and (z / scale).r_old
 // over both colours; NATURAL stores both colours in the reference layout.
+// Zeros standing in for the neighbour rows outside the domain in the direct-form
+// sweeps (a row of at most ZROW_FLOATS values of the z type).
+#define ZROW_FLOATS 4096
+static __device__ __align__(16) float g_zero_row[ZROW_FLOATS];
+
 template <int MODE, int OUT, typename TZI, typename TZO, typename TR, bool SRC32, int CPT>
 __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
 {
+    // the reduced-precision pre-smoothing sweeps (16-bit z) update in direct
+    // form (gs_direct): every neighbour holds +0 on non-FLUID cells, and the
+    // rows outside the domain are read from a row of zeros.  The FP32 z of the
+    // post-smoothing keeps the incremental form: there z holds the coarse
+    // correction, and the direct form's sum of six such values rounds worse
+    // (at 256^3 it left ten times the pressure error at the same residual).
+    constexpr bool DIRECT = sizeof(TR) == 2 && sizeof(TZO) == 2 && MODE != SWEEP_PROLONG;
     __shared__ double lds[4];
     const SGeo g = a.g;
     const uint32_t total = (uint32_t)g.nx * g.ny * g.nq;
@@ -153,17 +165,26 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
         // neighbour rows clamped to the domain (a wall neighbour has stencil
         // class 0 and is never used): no branches, so all loads issue at once
         float c[CPT], w[CPT], xp[CPT], xm[CPT], yp[CPT], ym[CPT];
-        if (MODE == SWEEP_PLAIN_OWN0) {
+        if (MODE == SWEEP_PLAIN_OWN0 || DIRECT) {   // the direct form does not read the old value
 #pragma unroll
             for (int e = 0; e < CPT; ++e) c[e] = 0.0f;
         } else {
             zq(own, i, j, c);
         }
         zq(oth, i, j, w);
-        zq(oth, min(i + 1, g.nx - 1), j, xp);
-        zq(oth, max(i - 1, 0), j, xm);
-        zq(oth, i, min(j + 1, g.ny - 1), yp);
-        zq(oth, i, max(j - 1, 0), ym);
+        if constexpr (DIRECT && MODE != SWEEP_ZERO) {
+            const TZI *zr = reinterpret_cast<const TZI *>(g_zero_row) + m0;
+            auto rowp = [&](bool in, int ii, int jj) { return in ? zin + oth + g.row(ii, jj) + m0 : zr; };
+            ldn<TZI, CPT>(rowp(i + 1 < g.nx, i + 1, j), xp);
+            ldn<TZI, CPT>(rowp(i > 0, i - 1, j), xm);
+            ldn<TZI, CPT>(rowp(j + 1 < g.ny, i, j + 1), yp);
+            ldn<TZI, CPT>(rowp(j > 0, i, j - 1), ym);
+        } else {
+            zq(oth, min(i + 1, g.nx - 1), j, xp);
+            zq(oth, max(i - 1, 0), j, xm);
+            zq(oth, i, min(j + 1, g.ny - 1), yp);
+            zq(oth, i, max(j - 1, 0), ym);
+        }
         // the k-neighbour beyond the vector: other colour at m0-1 (o = 0) or m0+CPT (o = 1)
         const float left = __shfl_up(w[CPT - 1], 1, 64), right = __shfl_down(w[0], 1, 64);
         float edge = 0.0f;   // outside the domain: stencil class 0, unused
@@ -196,8 +217,9 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
             else { km = w[e]; kp = e < CPT - 1 ? w[e + 1] : edge; }
             // neighbours hold +0 on non-FLUID cells except in the prolongating
             // sweep, which adds the coarse correction to every cell on the fly
-            out[e] = gs_update<MODE != SWEEP_PROLONG, sizeof(TR) == 2>(cd[e], c[e], rv[e], xp[e], xm[e], yp[e], ym[e],
-                                                                      kp, km);
+            if constexpr (DIRECT) out[e] = gs_direct(cd[e], rv[e], xp[e], xm[e], yp[e], ym[e], kp, km);
+            else out[e] = gs_update<MODE != SWEEP_PROLONG, sizeof(TR) == 2>(cd[e], c[e], rv[e], xp[e], xm[e], yp[e],
+                                                                            ym[e], kp, km);
         }
         if (OUT == SWEEP_OUT_FINAL) {
             // z and r over the storage scale s0 (a power of two: exact); the
@@ -284,9 +306,9 @@ __global__ void __launch_bounds__(256) k_ssweep(SSweepArgs a)
 #define P2_TI 8
 #define P2_TJ 4
 #define P2_NHMAX 128
-template <int MODE1, bool NAT, int NT>
-__global__ void __launch_bounds__(NT) k_ssweep2(const float *__restrict__ zin, float *__restrict__ zout,
-                                                 const float *__restrict__ zc, const float *__restrict__ r,
+template <int MODE1, bool NAT, int NT, typename TZI = float, typename TZO = float, typename TR = float>
+__global__ void __launch_bounds__(NT) k_ssweep2(const TZI *__restrict__ zin, TZO *__restrict__ zout,
+                                                 const float *__restrict__ zc, const TR *__restrict__ r,
                                                  const uint16_t *__restrict__ code, SGeo g, int cuA)
 {
     constexpr int HJ = P2_TJ + 2, HR = (P2_TI + 2) * HJ;
@@ -304,7 +326,7 @@ __global__ void __launch_bounds__(NT) k_ssweep2(const float *__restrict__ zin, f
             for (int e = 0; e < 4; ++e) v[e] = 0.0f;
             return;
         }
-        ldn<float, 4>(zin + h + g.row(ii, jj) + m0, v);
+        ldn<TZI, 4>(zin + h + g.row(ii, jj) + m0, v);
         if (MODE1 == SWEEP_PROLONG) {
             float pc[4];
             ldn<float, 4>(zc + cidx(ii >> 1, jj >> 1, m0, g.ny >> 1, nh), pc);
@@ -330,7 +352,7 @@ __global__ void __launch_bounds__(NT) k_ssweep2(const float *__restrict__ zin, f
         if (MODE1 != SWEEP_ZERO) {
             const int me = o == 0 ? m0 - 1 : m0 + 4;
             if (me >= 0 && me < nh) {
-                edge = zin[hB + g.row(i, j) + me];
+                edge = ldv<TZI>(zin[hB + g.row(i, j) + me]);
                 if (MODE1 == SWEEP_PROLONG) edge = edge + zc[cidx(i >> 1, j >> 1, me, g.ny >> 1, nh)];
             }
         }
@@ -338,13 +360,17 @@ __global__ void __launch_bounds__(NT) k_ssweep2(const float *__restrict__ zin, f
         uint32_t cd[4];
         codesn<4>(code, hA + base, cd);
         float rv[4], out[4];
-        ldn<float, 4>(r + hA + base, rv);
+        ldn<TR, 4>(r + hA + base, rv);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             float kp, km;
             if (o == 0) { kp = w[e]; km = e > 0 ? w[e - 1] : edge; }
             else { km = w[e]; kp = e < 3 ? w[e + 1] : edge; }
-            out[e] = gs_update<MODE1 != SWEEP_PROLONG>(cd[e], c[e], rv[e], xp[e], xm[e], yp[e], ym[e], kp, km);
+            out[e] = gs_update<MODE1 != SWEEP_PROLONG, sizeof(TR) == 2>(cd[e], c[e], rv[e], xp[e], xm[e], yp[e], ym[e], kp, km);
+        }
+        if constexpr (sizeof(TZO) == 2) {   // the second sweep reads the first's stored (rounded) values
+#pragma unroll
+            for (int e = 0; e < 4; ++e) out[e] = ldv<TZO>(stv<TZO>(out[e]));
         }
         A4[(hr * nh + m0) >> 2] = make_float4(out[0], out[1], out[2], out[3]);
     }
@@ -379,22 +405,22 @@ __global__ void __launch_bounds__(NT) k_ssweep2(const float *__restrict__ zin, f
         uint32_t cd[4];
         codesn<4>(code, hB + base, cd);
         float rv[4], out[4];
-        ldn<float, 4>(r + hB + base, rv);
+        ldn<TR, 4>(r + hB + base, rv);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             float kp, km;
             if (o == 0) { kp = w[e]; km = e > 0 ? w[e - 1] : edge; }
             else { km = w[e]; kp = e < 3 ? w[e + 1] : edge; }
-            out[e] = gs_update<true>(cd[e], c[e], rv[e], xp[e], xm[e], yp[e], ym[e], kp, km);
+            out[e] = gs_update<true, sizeof(TR) == 2>(cd[e], c[e], rv[e], xp[e], xm[e], yp[e], ym[e], kp, km);
         }
         if (NAT) {
             float nat[8];
 #pragma unroll
             for (int e = 0; e < 4; ++e) { nat[2 * e + o] = out[e]; nat[2 * e + 1 - o] = w[e]; }
-            stn<float, 8>(zout + cidx(i, j, 2 * m0, g.ny, 2 * nh), nat);
+            stn<TZO, 8>(zout + cidx(i, j, 2 * m0, g.ny, 2 * nh), nat);
         } else {
-            stn<float, 4>(zout + hB + base, out);
-            stn<float, 4>(zout + hA + base, w);
+            stn<TZO, 4>(zout + hB + base, out);
+            stn<TZO, 4>(zout + hA + base, w);
         }
     }
 }
@@ -404,9 +430,9 @@ __global__ void __launch_bounds__(NT) k_ssweep2(const float *__restrict__ zin, f
 // (I, J, K) are, per child row (2I+a, 2J+b), the cells m = K of both halves.
 // Residual r - (-ndt(z)) per fine cell (0 unless FLUID); a FLUID parent sums
 // its children in the serial launch order (a, b, kk), others get 0.
-template <typename TR, typename TZ, int CPT>
+template <typename TR, typename TZ, int CPT, typename TO = float>
 __global__ void __launch_bounds__(256) k_srestrict(const TR *__restrict__ r, const TZ *__restrict__ z,
-                                                   const uint16_t *__restrict__ code, float *__restrict__ rc,
+                                                   const uint16_t *__restrict__ code, TO *__restrict__ rc,
                                                    const uint8_t *__restrict__ tagc, SGeo g, SGeo cg, int csplit)
 {
     // cg: the coarse grid in thread vectors of CPT cells (nx, ny coarse; nh = coarse nz = fine nz/2)
@@ -504,10 +530,10 @@ __global__ void __launch_bounds__(256) k_srestrict(const TR *__restrict__ r, con
             float ev[CPT / 2], od[CPT / 2];
 #pragma unroll
             for (int e = 0; e < CPT / 2; ++e) { ev[e] = out[2 * e]; od[e] = out[2 * e + 1]; }
-            stn<float, CPT / 2>(rc + c0 * halfc + b, ev);
-            stn<float, CPT / 2>(rc + (c0 ^ 1u) * halfc + b, od);
+            stn<TO, CPT / 2>(rc + c0 * halfc + b, ev);
+            stn<TO, CPT / 2>(rc + (c0 ^ 1u) * halfc + b, od);
         } else {
-            stn<float, CPT>(rc + cb, out);
+            stn<TO, CPT>(rc + cb, out);
         }
     }
 }
@@ -748,6 +774,7 @@ int split_sweep_mixed_t(hipStream_t s, const Level &L0, const SplitMixedSweep &m
     const bool pre = m.kind == SMX_PRE_ZERO_SRC || m.kind == SMX_PRE_SRC || m.kind == SMX_PRE ||
                      m.kind == SMX_PRE_ZERO_RH || m.kind == SMX_PRE_OWN0_RH;
     const int cpt = pre && (L0.nz / 2) % 8 == 0 ? 8 : 4;
+    FLUB_REQUIRE(L0.nz / 2 <= ZROW_FLOATS, FLUB_ESHAPE, "mixed sweep: nz = %d above %d", L0.nz, 2 * ZROW_FLOATS);
     a.g = make_sgeo(L0.nx, L0.ny, L0.nz, cpt);
     const unsigned nb = grid8((uint32_t)L0.nx * L0.ny * a.g.nq);
     if (m.nblk) *m.nblk = (int)nb;
@@ -838,16 +865,24 @@ int split_sweep(hipStream_t s, const Level &L0, const float *zin, float *zout, c
     return FLUB_OK;
 }
 
-template <int NT>
-static void pair_launch(hipStream_t s, unsigned nb, int mode, bool nat, const float *zin, float *zout, const float *zc,
-                 const float *r, const uint16_t *code, const SGeo &g, int cu)
+template <int NT, typename TZI, typename TZO, typename TR>
+static void pair_launch(hipStream_t s, unsigned nb, int mode, bool nat, const void *zin_, void *zout_, const float *zc,
+                        const void *r_, const uint16_t *code, const SGeo &g, int cu)
 {
-    if (mode == SWEEP_ZERO && !nat) k_ssweep2<SWEEP_ZERO, false, NT><<<nb, NT, 0, s>>>(zin, zout, zc, r, code, g, cu);
-    else if (mode == SWEEP_PROLONG && !nat) k_ssweep2<SWEEP_PROLONG, false, NT><<<nb, NT, 0, s>>>(zin, zout, zc, r, code, g, cu);
-    else if (mode == SWEEP_PLAIN && !nat) k_ssweep2<SWEEP_PLAIN, false, NT><<<nb, NT, 0, s>>>(zin, zout, zc, r, code, g, cu);
-    else if (mode == SWEEP_PLAIN) k_ssweep2<SWEEP_PLAIN, true, NT><<<nb, NT, 0, s>>>(zin, zout, zc, r, code, g, cu);
-    else if (mode == SWEEP_PROLONG) k_ssweep2<SWEEP_PROLONG, true, NT><<<nb, NT, 0, s>>>(zin, zout, zc, r, code, g, cu);
-    else k_ssweep2<SWEEP_ZERO, true, NT><<<nb, NT, 0, s>>>(zin, zout, zc, r, code, g, cu);
+    const TZI *zin = static_cast<const TZI *>(zin_);
+    TZO *zout = static_cast<TZO *>(zout_);
+    const TR *r = static_cast<const TR *>(r_);
+    if (mode == SWEEP_ZERO && !nat)
+        k_ssweep2<SWEEP_ZERO, false, NT, TZI, TZO, TR><<<nb, NT, 0, s>>>(zin, zout, zc, r, code, g, cu);
+    else if (mode == SWEEP_PROLONG && !nat)
+        k_ssweep2<SWEEP_PROLONG, false, NT, TZI, TZO, TR><<<nb, NT, 0, s>>>(zin, zout, zc, r, code, g, cu);
+    else if (mode == SWEEP_PLAIN && !nat)
+        k_ssweep2<SWEEP_PLAIN, false, NT, TZI, TZO, TR><<<nb, NT, 0, s>>>(zin, zout, zc, r, code, g, cu);
+    else if (mode == SWEEP_PLAIN)
+        k_ssweep2<SWEEP_PLAIN, true, NT, TZI, TZO, TR><<<nb, NT, 0, s>>>(zin, zout, zc, r, code, g, cu);
+    else if (mode == SWEEP_PROLONG)
+        k_ssweep2<SWEEP_PROLONG, true, NT, TZI, TZO, TR><<<nb, NT, 0, s>>>(zin, zout, zc, r, code, g, cu);
+    else k_ssweep2<SWEEP_ZERO, true, NT, TZI, TZO, TR><<<nb, NT, 0, s>>>(zin, zout, zc, r, code, g, cu);
 }
 
 bool split_pair_supported(const Level &L)
@@ -856,20 +891,32 @@ bool split_pair_supported(const Level &L)
     return split_supported(L) && L.nx % P2_TI == 0 && L.ny % P2_TJ == 0 && g.lq >= 0 && g.nh <= P2_NHMAX;
 }
 
-int split_sweep_pair(hipStream_t s, const Level &L, const float *zin, float *zout, const float *zc, const float *r,
-                     const uint16_t *code, int phase, int mode, bool natural_out)
+int split_sweep_pair(hipStream_t s, const Level &L, const void *zin, void *zout, const float *zc, const void *r,
+                     const uint16_t *code, int phase, int mode, bool natural_out, int types)
 {
     FLUB_REQUIRE(split_pair_supported(L), FLUB_ESHAPE, "sweep pair: level %dx%dx%d not supported", L.nx, L.ny, L.nz);
     FLUB_REQUIRE(zin != zout || mode == SWEEP_ZERO, FLUB_EINVAL, "sweep pair: zin and zout must differ");
     FLUB_REQUIRE(mode == SWEEP_ZERO || mode == SWEEP_PLAIN || mode == SWEEP_PROLONG, FLUB_EINVAL,
                  "sweep pair: bad mode %d", mode);
+    FLUB_REQUIRE(types == PAIR_F32 || types == PAIR_BF16_R || !natural_out, FLUB_EINVAL,
+                 "sweep pair: the reference-layout output is FP32");
     const SGeo g = make_sgeo(L.nx, L.ny, L.nz);
     const unsigned nb = (unsigned)((L.nx / P2_TI) * (L.ny / P2_TJ));
     const int cu = phase ^ 1;
-    static const int nt = getenv("FLUB_PAIR_NT") ? atoi(getenv("FLUB_PAIR_NT")) : 1024;
-    if (nt == 256) pair_launch<256>(s, nb, mode, natural_out, zin, zout, zc, r, code, g, cu);
-    else if (nt == 512) pair_launch<512>(s, nb, mode, natural_out, zin, zout, zc, r, code, g, cu);
-    else pair_launch<1024>(s, nb, mode, natural_out, zin, zout, zc, r, code, g, cu);
+    using BF = __hip_bfloat16;
+    if (types == PAIR_BF16) {   // the BF16 pre-smoothing (1024 threads)
+        pair_launch<1024, BF, BF, BF>(s, nb, mode, false, zin, zout, zc, r, code, g, cu);
+    } else if (types == PAIR_BF16_IN) {   // BF16 z and r in, FP32 z out (the prolongating pair)
+        pair_launch<1024, BF, float, BF>(s, nb, mode, false, zin, zout, zc, r, code, g, cu);
+    } else if (types == PAIR_BF16_R) {   // FP32 z, BF16 r
+        if (natural_out) pair_launch<1024, float, float, BF>(s, nb, mode, true, zin, zout, zc, r, code, g, cu);
+        else pair_launch<1024, float, float, BF>(s, nb, mode, false, zin, zout, zc, r, code, g, cu);
+    } else {
+        static const int nt = getenv("FLUB_PAIR_NT") ? atoi(getenv("FLUB_PAIR_NT")) : 1024;
+        if (nt == 256) pair_launch<256, float, float, float>(s, nb, mode, natural_out, zin, zout, zc, r, code, g, cu);
+        else if (nt == 512) pair_launch<512, float, float, float>(s, nb, mode, natural_out, zin, zout, zc, r, code, g, cu);
+        else pair_launch<1024, float, float, float>(s, nb, mode, natural_out, zin, zout, zc, r, code, g, cu);
+    }
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
 }
@@ -892,7 +939,7 @@ int split_restrict(hipStream_t s, const Level &F, const float *r, const float *z
 }
 
 int split_restrict_mixed(hipStream_t s, const Level &F, int prec, const void *rh, const void *zh,
-                         const uint16_t *code, const Level &C, bool csplit)
+                         const uint16_t *code, const Level &C, bool csplit, void *rc16)
 {
     // 4 coarse cells per thread: at 8 (16-byte loads) the kernel needs 256
     // VGPRs and runs 53 us instead of 40 us at 256^3
@@ -900,7 +947,10 @@ int split_restrict_mixed(hipStream_t s, const Level &F, int prec, const void *rh
     SGeo cg = make_sgeo(C.nx, C.ny, 2 * C.nz);
     const unsigned nb = grid8((uint32_t)C.nx * C.ny * cg.nq);
     using BF = __hip_bfloat16;
-    if (prec == FLUB_PREC_BF16)
+    if (prec == FLUB_PREC_BF16 && rc16)   // the coarse r in BF16 (the BF16 level-1 pre-smoothing)
+        k_srestrict<BF, BF, 4, BF><<<nb, 256, 0, s>>>((const BF *)rh, (const BF *)zh, code, (BF *)rc16, C.tag, g, cg,
+                                                      (int)csplit);
+    else if (prec == FLUB_PREC_BF16)
         k_srestrict<BF, BF, 4><<<nb, 256, 0, s>>>((const BF *)rh, (const BF *)zh, code, C.r, C.tag, g, cg,
                                                   (int)csplit);
     else

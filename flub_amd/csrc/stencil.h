@@ -179,6 +179,23 @@ __device__ __forceinline__ float nsum_code_air0(uint32_t cd, float xI, float xip
 __device__ __forceinline__ int code_divd(uint32_t cd) { return (int)((cd >> 12) & 7u); }
 __device__ __forceinline__ bool code_fluid(uint32_t cd) { return (cd >> 15) != 0; }
 
+// The red-black update in direct form, for neighbour values that hold +0 on
+// every non-FLUID cell and outside the domain: z' = (r + sum of the six
+// neighbours) * rcp(divd) on FLUID cells, +0 elsewhere.  divd counts the
+// neighbours that are neither SOLID nor outside (mgpcg.py:145-160) and AIR
+// neighbours hold 0, so this is algebraically the reference's
+// z + (r + ndt(z)) / divd (mgpcg.py:72-86, 119-142) without the cell's own old
+// value and without a mask per neighbour.  Only the reduced-precision sweeps
+// use it: they have no reference result to match bitwise, and the masks were
+// most of their instructions.
+__device__ __forceinline__ float gs_direct(uint32_t cd, float r, float xip, float xim, float xjp, float xjm, float xkp,
+                                           float xkm)
+{
+    const float s = ((xip + xim) + (xjp + xjm)) + (xkp + xkm);
+    const uint32_t divd = (cd >> 12) & 7u;
+    return keep_if((r + s) * __builtin_amdgcn_rcpf((float)divd), 0u - (uint32_t)(divd != 0u));
+}
+
 // One red-black Gauss-Seidel update of a cell (mgpcg.py:72-86):
 // z + (r + ndt(z)) / divd on FLUID cells, +0 elsewhere (divd is 0 there; the
 // quotient computed for them is discarded).  RCP: the quotient as a multiply

@@ -245,6 +245,9 @@ def test_split_launch_variants_bitwise(precision, variant, monkeypatch):
     the same iterations (coarse levels split as on large grids)."""
     s = scenes.dam_break_spheres(64, 1)
     monkeypatch.setenv("FLUB_SPLIT_MIN_CELLS", "1")
+    if variant in ("FLUB_NO_PAIR", "FLUB_MX_SRC32"):
+        # the BF16 level-1 pre-smoothing needs the pairs and the BF16 r update
+        monkeypatch.setenv("FLUB_MX_NO_L1", "1")
     res = []
     for on in (False, True):
         if on:
@@ -301,8 +304,9 @@ def test_split_solve_nonfinite(precision):
 def test_split_coarse_solve_bitwise(precision, monkeypatch):
     """Colour-split coarse levels change where values are stored, not the
     arithmetic: the solve gives bitwise the same pressure in the same
-    iterations with and without them."""
+    iterations with and without them (BF16: with level 1 kept in FP32)."""
     s = scenes.dam_break_spheres(64, 1)
+    monkeypatch.setenv("FLUB_MX_NO_L1", "1")
     res = []
     for cells in ("1", str(1 << 30)):
         monkeypatch.setenv("FLUB_SPLIT_MIN_CELLS", cells)
@@ -314,6 +318,36 @@ def test_split_coarse_solve_bitwise(precision, monkeypatch):
     (i0, p0), (i1, p1) = res
     assert i0 == i1
     assert bits_equal(p0, p1)
+
+
+@pytest.mark.parametrize("name,make,cells", [("spheres_64", lambda: scenes.dam_break_spheres(64, 1), "1"),
+                                             ("sphere_flow_256", lambda: scenes.flow_past_sphere(256, 2), None)])
+def test_bf16_level1_presmoothing(name, make, cells, monkeypatch):
+    """The BF16 solve pre-smooths a split level 1 in BF16 (r and z; the
+    post-smoothing stays FP32): against the same solve with level 1 in FP32,
+    the same iterations (at most one more) and the same FP64 true residual
+    test passed; the pressures agree to the solves' accuracy."""
+    s = make()
+    n = s["n"]
+    if cells:
+        monkeypatch.setenv("FLUB_SPLIT_MIN_CELLS", cells)
+    res = []
+    for off in (False, True):
+        if off:
+            monkeypatch.setenv("FLUB_MX_NO_L1", "1")
+        pr = Projection((n, n, n), n_mg_levels=4, pre_and_post_smoothing=2, bottom_smoothing=6, precision="bf16")
+        pr.load(s)
+        it = pr.project(max_iters=-1, rel_tol=1e-12)
+        div = pr.div.f.numpy()
+        b = np.where(s["tag"] == 1, div * np.float32(-pr.h), np.float32(0)).astype(np.float64)
+        res.append((it, pr.p0.f.numpy().astype(np.float64), pr.rTr, float(np.sum(b * b))))
+    (i1, p1, r1, bb), (i0, p0, r0, _) = res
+    rel = float(np.linalg.norm(p1 - p0) / np.linalg.norm(p0))
+    print(f"{name}: BF16 level 1 {i1} iterations, FP32 level 1 {i0}; true rel. residual "
+          f"{np.sqrt(r1 / bb):.3e} / {np.sqrt(r0 / bb):.3e}; pressures differ by {rel:.2e}")
+    assert i0 <= i1 <= i0 + 1
+    assert r1 < 1e-12 * bb and r0 < 1e-12 * bb
+    assert rel < 1e-5
 
 
 def test_split_solve_matches_reference_layout(monkeypatch, oracle):
