@@ -16,11 +16,12 @@ preconditioner in flexible PCG with V-cycle fallback, weights
 flub_amd/learned/weights/unet_128.npz).  Every solver keeps x in FP64 and
 stops when the FP64 true residual ||b - A x|| of that x is below 1e-6 ||b||
 (the reference's test, rel_tol = 1e-12 on squared norms): one accuracy rule
-for all legs.  `value` is the fastest solver's ms per projection; every
+for all legs.  "bf16" also keeps level 1's pre-smoothing in BF16.  `value` is the fastest solver's ms per projection; every
 solver's figures are in "solvers", with its FP64 true residual, the true
 residual of the FP32 pressure it hands back, and (the FP64 judge) its
-relative pressure error against the converged FP64 CPU oracle projection and
-the post-projection divergence of its velocity field.  "learned_heldout":
+relative pressure error against the converged FP64 CPU oracle projection, the
+energy-norm error of its FP64 x, and the post-projection divergence of its
+velocity field.  "learned_heldout":
 the learned solver against plain MGPCG and V-cycle FPCG on held-out seeds at
 128^3 and 256^3.
 
@@ -140,7 +141,7 @@ def main():
                     f"rTr {pr.rTr:.3e}")
         solvers[name] = {"ms": round(float(np.mean(times)), 4), "ms_min": round(float(np.min(times)), 4),
                          "iterations": int(np.median(iters)), **accuracy(pr, s["tag"], pr.rTr)}
-        outputs[name] = (pr.p0.f.numpy(), *pr.velocity())
+        outputs[name] = (pr.p0.f.numpy(), *pr.velocity(), solution_f64(pr.mgpcg.handle, n))
         if name == "fp32":
             kernels = profile_kernels(pr)
             in_solve = probe_in_solve(pr, src)
@@ -262,9 +263,11 @@ def accuracy(pr, tag, true_rr):
 
 def fp64_judge(s, n, L, outputs, solvers, threads):
     """The converged FP64 CPU oracle projection of the same scene (rel_tol
-    1e-24): each solver's relative pressure error against it, and the
-    post-projection divergence of each solver's velocity field relative to
-    the divergence before the projection (FLUID cells, FP64)."""
+    1e-24): each solver's relative pressure error against it (2-norm of the
+    FP32 pressure, and the energy norm ||x - x*||_A / ||x*||_A of its FP64 x,
+    the error a residual test bounds), and the post-projection divergence of
+    each solver's velocity field relative to the divergence before the
+    projection (FLUID cells, FP64)."""
     from oracle import oracle as O
     cores = _cpu_cores(threads)
     O.set_threads(cores)
@@ -278,11 +281,16 @@ def fp64_judge(s, n, L, outputs, solvers, threads):
     pre = O.divergence_tag(s["u"], s["v"], s["w"], s["ur"], s["vr"], s["wr"], s["tag"], real=np.float64)
     pre_n = float(np.linalg.norm(pre[fl]))
     pn = float(np.linalg.norm(p64))
-    for name, (p, u, v, w) in outputs.items():
+    xs = p64 * n   # x* (the pressure is x h, h = 1/n)
+    xa = float(np.sum(xs * -O.compute_Ap(xs, s["tag"], real=np.float64)))
+    for name, (p, u, v, w, x64) in outputs.items():
         post = O.divergence_tag(u, v, w, s["ur"], s["vr"], s["wr"], s["tag"], real=np.float64)
         solvers[name]["p_rel_err_fp64"] = float(np.linalg.norm(p.astype(np.float64) - p64) / pn)
+        e = x64 - xs
+        solvers[name]["x_err_energy_fp64"] = float(np.sqrt(abs(np.sum(e * -O.compute_Ap(e, s["tag"], real=np.float64))) / xa))
         solvers[name]["post_div_rel"] = float(np.linalg.norm(post[fl]) / pre_n)
-        log(f"{name}: pressure error vs FP64 {solvers[name]['p_rel_err_fp64']:.3e}, "
+        log(f"{name}: pressure error vs FP64 {solvers[name]['p_rel_err_fp64']:.3e} "
+            f"(energy norm {solvers[name]['x_err_energy_fp64']:.3e}), "
             f"post-projection divergence {solvers[name]['post_div_rel']:.3e}")
 
 
@@ -373,7 +381,17 @@ def learned_leg(s, n, L, src, steps, warmup):
            "true_rel_res_fp32_x": float(np.sqrt(st["true_rTr_fp32"] / st["rTr0"])),
            "unet_forward_ms": round(net_ms, 3), "unet_tflops": round(tflops, 2),
            "unet_mfma_frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4)}
-    return ent, (pr.p0.f.numpy(), *pr.velocity())
+    return ent, (pr.p0.f.numpy(), *pr.velocity(), solver.last_x64.cpu().numpy())
+
+
+def solution_f64(handle, n):
+    """The FP64 x of the handle's last solve (flub_mgpcg_solution_f64)."""
+    import torch
+    from flub_amd import _lib
+    x = torch.empty((n, n, n), dtype=torch.float64, device="cuda")
+    _lib.call("flub_mgpcg_solution_f64", handle, ctypes.c_void_p(x.data_ptr()))
+    torch.cuda.synchronize()
+    return x.cpu().numpy()
 
 
 def learned_heldout(sizes=(128, 256), seeds=(3000,)):

@@ -14,9 +14,12 @@ divergence produced (bitwise the oracle's, tested elsewhere).
 Config 3 (flow past a sphere, 256^3, FP32 / FP16 / BF16 V-cycles) is checked
 at its own size against the converged FP64 oracle projection computed on the
 host: relative pressure error (bar 1e-5; the FP32 solve also within 1.5x the
-FP32 reference algorithm's own error, 1.38e-6 -- the BF16 V-cycle's solution
-at the same 1e-6 residual measured 7.6e-6), FP64 true residual and
-post-projection divergence.
+FP32 reference algorithm's own error, 1.38e-6), the error in the energy norm
+(which the residual bounds: 1.6e-6 FP32 and FP16, 2.2e-6 BF16), FP64 true
+residual and post-projection divergence.  The 2-norm error of this system's
+lowest modes (outflow through one face) is not fixed by the residual test:
+small changes of the BF16 V-cycle moved it between 1.5e-6 and 7.9e-6 at the
+same residual and an energy-norm error within 2.2e-6.
 """
 import ctypes
 
@@ -155,12 +158,22 @@ def test_config3_sphere_flow_256_vs_fp64_oracle(precision, sphere_flow_256, orac
     x64 = solution_f64(pr)
     rel64, _ = true_rel(oracle, x64, rhs(pr, s["tag"]), s["tag"])
     post = post_div_rel(oracle, pr, s)
+    # the error in the energy norm ||e||_A / ||x*||_A (what the residual test
+    # bounds), of the FP64 x against the FP64 oracle's x* = p64 / h
+    xs = p64 * n
+    e = x64 - xs
+    ea = float(np.sqrt(abs(np.sum(e * -oracle.compute_Ap(e, s["tag"], real=np.float64))) /
+                       np.sum(xs * -oracle.compute_Ap(xs, s["tag"], real=np.float64))))
     print(f"256^3 {precision}: {it} iterations (FP32 oracle {it32}), rel. pressure error vs FP64 {err:.3e} "
-          f"(FP32 reference algorithm {err_ref:.3e}), FP64 true rel. residual {rel64:.3e}, "
-          f"post-projection divergence {post:.3e}")
+          f"(FP32 reference algorithm {err_ref:.3e}), energy-norm error {ea:.3e}, FP64 true rel. residual "
+          f"{rel64:.3e}, post-projection divergence {post:.3e}")
     assert err < 1e-5                   # the config's bar
     if precision == "fp32":             # as accurate as the reference algorithm itself
         assert err < 1.5 * err_ref
+    # the energy-norm error is what the residual test bounds: within twice the
+    # FP32 solve's (1.6e-6) for every precision, while the 2-norm error of the
+    # lowest modes moves with small changes of the V-cycle (1.4e-6 to 7.9e-6)
+    assert ea < 3.2e-6
     assert rel64 < 1e-6
     assert post < 1.5e-6
 
