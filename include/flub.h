@@ -33,7 +33,12 @@ typedef struct flub_mgpcg flub_mgpcg;
  * "fraction"= "phases" plus mean removal (zero_centered) as in solve(). */
 enum { FLUB_SMOOTHER_DEFAULT = 0, FLUB_SMOOTHER_PHASES = 1, FLUB_SMOOTHER_FRACTION = 2 };
 
-/* Storage precision of the V-cycle (the outer CG is always FP32 with FP64 dots). */
+/* Storage precision of the V-cycle (the outer CG is always FP32 with FP64 dots).
+ * FP32 is the reference V-cycle, bitwise.  FP16 / BF16 (no counterpart in the
+ * reference) store level 0's r and pre-smoothed z in 16 bits, and the BF16
+ * solve also level 1's when that level is colour-split; the 16-bit
+ * pre-smoothing sweeps update in direct form, (r + sum of the neighbours) /
+ * divd, with the hardware reciprocal; the outer loop is flexible CG. */
 enum { FLUB_PREC_FP32 = 0, FLUB_PREC_FP16 = 1, FLUB_PREC_BF16 = 2 };
 
 /* ---------------------------------------------------------------------------
