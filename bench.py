@@ -378,7 +378,7 @@ def learned_leg(s, n, L, src, steps, warmup):
     ent = {"ms": round(float(np.mean(times)), 3), "iterations": st["iterations"], "net_iters": st["net_iters"],
            "vcycle_iters": st["vcycle_iters"], "fallback_at": st["fallback_at"],
            "true_rel_res": float(np.sqrt(st["true_rTr"] / st["rTr0"])),
-           "true_rel_res_fp32_x": float(np.sqrt(st["true_rTr_fp32"] / st["rTr0"])),
+           "true_rel_res_fp32_x": float(np.sqrt(solver.fp32_true_residual() / st["rTr0"])),
            "unet_forward_ms": round(net_ms, 3), "unet_tflops": round(tflops, 2),
            "unet_mfma_frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4)}
     return ent, (pr.p0.f.numpy(), *pr.velocity(), solver.last_x64.cpu().numpy())

@@ -168,6 +168,93 @@ __global__ void __launch_bounds__(RBS) k_true_residual(float *__restrict__ r, co
     }
 }
 
+// k_true_residual four cells a thread along k (nz % 4 == 0): the same FP64
+// terms in the same order per cell, with the tags, x and b of a quad and of
+// its four neighbour quads loaded as vectors.
+template <typename XT>
+__global__ void __launch_bounds__(RBS) k_true_residual4(float *__restrict__ r, const float *__restrict__ b,
+                                                        const XT *__restrict__ x, const uint8_t *__restrict__ tag,
+                                                        int nx, int ny, int nz, double *__restrict__ part)
+{
+    __shared__ double lds[RBS / 64];
+    const uint32_t n4 = (uint32_t)nx * ny * nz / 4;
+    double acc = 0.0, mx = 0.0;
+    for (uint32_t q = blockIdx.x * RBS + threadIdx.x; q < n4; q += gridDim.x * RBS) {
+        const uint32_t t = 4 * q;
+        const int k0 = (int)(t % (uint32_t)nz), j = (int)((t / (uint32_t)nz) % (uint32_t)ny),
+                  i = (int)(t / ((uint32_t)nz * ny));
+        const uint32_t sI = (uint32_t)ny * nz, sJ = (uint32_t)nz;
+        const bool in[6] = {i + 1 < nx, i > 0, j + 1 < ny, j > 0, true, true};
+        const uint32_t nb[4] = {t + sI, t - sI, t + sJ, t - sJ};
+        const uint32_t tg = *reinterpret_cast<const uint32_t *>(tag + t);
+        uint32_t tn[4];
+        double xn[4][4], xc[6];
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            tn[d] = in[d] ? *reinterpret_cast<const uint32_t *>(tag + nb[d]) : 0x02020202u;   // outside: skipped
+#pragma unroll
+            for (int e = 0; e < 4; ++e) xn[d][e] = in[d] ? (double)x[nb[d] + e] : 0.0;
+        }
+        // the row's x from k0 - 1 to k0 + 4 (0 and SOLID beyond the domain)
+        const int tkm = k0 > 0 ? tag[t - 1] : 2, tkp = k0 + 4 < nz ? tag[t + 4] : 2;
+        xc[0] = k0 > 0 ? (double)x[t - 1] : 0.0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) xc[1 + e] = (double)x[t + e];
+        xc[5] = k0 + 4 < nz ? (double)x[t + 4] : 0.0;
+        const float4 bv = *reinterpret_cast<const float4 *>(b + t);
+        const float bq[4] = {bv.x, bv.y, bv.z, bv.w};
+        float rq[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int tI = (int)((tg >> (8 * e)) & 0xffu);
+            double rv = 0.0;
+            if (tI == 1) {
+                const double xI = xc[1 + e];
+                double sum = 0.0;
+#pragma unroll
+                for (int d = 0; d < 4; ++d) {   // +i, -i, +j, -j
+                    const int tN = (int)((tn[d] >> (8 * e)) & 0xffu);
+                    if (tN == 2) continue;
+                    sum += (tN == 0 ? 0.0 : xn[d][e]) - xI;
+                }
+                // +k, -k
+                const int tP = e < 3 ? (int)((tg >> (8 * (e + 1))) & 0xffu) : tkp;
+                const int tM = e > 0 ? (int)((tg >> (8 * (e - 1))) & 0xffu) : tkm;
+                if (tP != 2) sum += (tP == 0 ? 0.0 : xc[2 + e]) - xI;
+                if (tM != 2) sum += (tM == 0 ? 0.0 : xc[e]) - xI;
+                rv = (double)bq[e] - sum;
+            }
+            rq[e] = (float)rv;
+            acc += rv * rv;
+            mx = fmax(mx, fabs(rv));
+        }
+        if (r) *reinterpret_cast<float4 *>(r + t) = make_float4(rq[0], rq[1], rq[2], rq[3]);
+    }
+    const double s = block_sum<RBS>(acc, lds);
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
+    __shared__ double lm[RBS / 64];
+    if ((threadIdx.x & 63) == 0) lm[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int w = 0; w < RBS / 64; ++w) m = fmax(m, lm[w]);
+        part[blockIdx.x] = s;
+        part[gridDim.x + blockIdx.x] = m;
+    }
+}
+
+// The true residual of level 0's x into r (may be null), partials in part:
+// four cells a thread when nz % 4 == 0.
+template <typename XT>
+static void launch_true_residual(flub_mgpcg *h, float *r, const float *b, const XT *x, int nb)
+{
+    const Level &L0 = h->lv[0];
+    if (L0.nz % 4 == 0)
+        k_true_residual4<XT><<<nb, RBS, 0, h->s>>>(r, b, x, L0.tag, L0.nx, L0.ny, L0.nz, h->part);
+    else
+        k_true_residual<XT><<<nb, RBS, 0, h->s>>>(r, b, x, L0.tag, L0.nx, L0.ny, L0.nz, h->part);
+}
+
 // Sum of nblk partials (fixed order) into out[0].
 __global__ void __launch_bounds__(RBS) k_sum_partials(const double *__restrict__ part, int nblk, double *__restrict__ out)
 {
@@ -467,7 +554,7 @@ static int read_sc(flub_mgpcg *h, int idx, double *out)
 static int true_residual(flub_mgpcg *h, int nb)
 {
     Level &L0 = h->lv[0];
-    k_true_residual<float><<<nb, RBS, 0, h->s>>>(L0.r, h->mx->b, h->x, L0.tag, L0.nx, L0.ny, L0.nz, h->part);
+    launch_true_residual<float>(h, L0.r, h->mx->b, h->x, nb);
     FLUB_CHECK_LAUNCH();
     k_mx_finish_r<<<1, RBS, 0, h->s>>>(h->part, nb, h->sc, h->mx->shift);
     FLUB_CHECK_LAUNCH();
@@ -675,7 +762,7 @@ FLUB_API int flub_mgpcg_residual_norm(flub_mgpcg *h, const float *x, const float
     Level &L0 = h->lv[0];
     const int nb = red_nb(L0.n);
     double *out = h->part + 2 * MAXPART - 1;   // beyond the 2*nb partials
-    k_true_residual<float><<<nb, RBS, 0, h->s>>>(h->p_alt, b, x, L0.tag, L0.nx, L0.ny, L0.nz, h->part);
+    launch_true_residual<float>(h, h->p_alt, b, x, nb);
     FLUB_CHECK_LAUNCH();
     k_sum_partials<<<1, RBS, 0, h->s>>>(h->part, nb, out);
     FLUB_CHECK_LAUNCH();
@@ -692,7 +779,7 @@ FLUB_API int flub_mgpcg_true_residual_f64(flub_mgpcg *h, const double *x, const 
     Level &L0 = h->lv[0];
     const int nb = red_nb(L0.n);
     double *out = h->part + 2 * MAXPART - 1;   // beyond the 2*nb partials
-    k_true_residual<double><<<nb, RBS, 0, h->s>>>(r, b, x, L0.tag, L0.nx, L0.ny, L0.nz, h->part);
+    launch_true_residual<double>(h, r, b, x, nb);
     FLUB_CHECK_LAUNCH();
     k_sum_partials<<<1, RBS, 0, h->s>>>(h->part, nb, out);
     FLUB_CHECK_LAUNCH();

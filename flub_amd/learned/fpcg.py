@@ -176,8 +176,12 @@ class LearnedPCG:
         x32 = x.float()
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) * 1e3
-        # the accuracy of the FP32 solution handed back (outside the timing)
-        true_rr32 = self.true_residual(x32.double(), b)
-        stats.update(iterations=it, rTr=rr, true_rTr=true_rr, true_rTr_fp32=true_rr32, rTr0=rr0, ms=ms)
+        stats.update(iterations=it, rTr=rr, true_rTr=true_rr, rTr0=rr0, ms=ms)
         self.last_x64 = x   # the FP64 solution (x32 is it rounded)
+        self.last_b = b
         return x32, stats
+
+    def fp32_true_residual(self):
+        """||b - A x32||^2 of the last solve's FP32 solution (its FP64 x rounded),
+        with the FP64 stencil: the accuracy of what the solve hands back."""
+        return self.true_residual(self.last_x64.float().double(), self.last_b)
