@@ -70,9 +70,16 @@ class LearnedPCG:
             return torch.zeros_like(r)
         return self.net.precondition(r, self._fluid_u8, s)
 
-    @staticmethod
-    def dot(a, b):
-        return float(torch.dot(a.view(-1).double(), b.view(-1).double()))
+    def dot(self, a, b):
+        """a.b of two contiguous FP32 device tensors, accumulated in FP64 by the
+        library's two-stage reduction (flub_inner_noalloc: its partials live in
+        a static device array; the solve's calls run one after another on the
+        null stream)."""
+        out = getattr(self, "_dot_out", None)
+        if out is None or out.device != a.device:
+            out = self._dot_out = torch.zeros(1, dtype=torch.float64, device=a.device)
+        _lib.call("flub_inner_noalloc", _p(a), _p(b), a.numel(), _p(out), None)
+        return float(out.item())
 
     def _as_device(self, t):
         from ..device import DeviceArray
@@ -98,6 +105,7 @@ class LearnedPCG:
         q = torch.empty_like(b)
         rt = torch.empty_like(b)
         z = torch.empty_like(b)
+        p = torch.empty_like(b)
         rr0 = self.dot(r, r)
         tol = max(abs_tol, rr0 * rel_tol)
         mode = "net" if use_net and self.net is not None else "vcycle"
@@ -143,7 +151,7 @@ class LearnedPCG:
                 continue
             if restart:
                 precond()
-                p = z.clone()
+                p.copy_(z)
                 zr = self.dot(z, r)
                 restart = False
             self.apply_A(p, q)
@@ -170,7 +178,7 @@ class LearnedPCG:
             precond()
             beta = -alpha * self.dot(z, q) / (zr + eps)
             zr = self.dot(z, r)
-            p = z + beta * p
+            torch.add(z, p, alpha=beta, out=p)   # p = z + beta p
         if true_rr is None:   # stopped by the iteration cap: judge once for the report
             true_rr = self.true_residual(x, b)
         x32 = x.float()
