@@ -790,10 +790,19 @@ int mg_precondition(flub_mgpcg *h, int smoother, bool use_graph)
 
 // The V-cycle on the colour-split level 0 (sp.r -> sp.z[zslot]), as a graph
 // in slot 3 + zslot.
+// Which of its first sweeps the mixed V-cycle into slot zslot runs (they
+// depend on what the r update stored): a captured cycle is replayed only in
+// the state it was captured in.
+static int mx_state(const flub_mgpcg *h, int zslot)
+{
+    return h->prec == FLUB_PREC_FP32 ? 0 : (h->rh_ready == zslot ? 1 : 0) + (h->zh0_ready == zslot ? 2 : 0);
+}
+
 int mg_precondition_split(flub_mgpcg *h, int zslot)
 {
     if (h->no_graph || !h->split_graph) return vcycle_fast(h, true, zslot);
     const int gs = 3 + zslot;
+    if (h->graph_ok[gs] && h->graph_rh[gs] != mx_state(h, zslot)) return vcycle_fast(h, true, zslot);
     if (!h->graph_ok[gs]) {
         hipGraph_t g;
         hipStream_t work = h->s;
@@ -807,6 +816,7 @@ int mg_precondition_split(flub_mgpcg *h, int zslot)
         FLUB_CHECK_HIP(hipGraphInstantiate(&h->graph[gs], g, nullptr, nullptr, 0));
         FLUB_CHECK_HIP(hipGraphDestroy(g));
         h->graph_ok[gs] = true;
+        h->graph_rh[gs] = mx_state(h, zslot);
     }
     FLUB_CHECK_HIP(hipGraphLaunch(h->graph[gs], h->s));
     return FLUB_OK;

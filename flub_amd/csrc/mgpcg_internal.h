@@ -102,6 +102,7 @@ struct flub_mgpcg {
     uint32_t *tb_order;  // update-order tables of the blocked smoother
     hipGraphExec_t graph[5];   // per smoother; [3], [4]: the V-cycle on the split level 0 into sp.z[0], sp.z[1]
     bool graph_ok[5];
+    int graph_rh[5];           // [3], [4]: the mixed V-cycle's 16-bit r state at capture (mx_state)
     hipEvent_t ev;     // marks the residual norm's copy to the host
     bool split;        // level 0 of the FP32 solve runs in the colour-split layout
     int nsplit;        // levels 0 .. nsplit-1 of the split V-cycle are colour-split (0: none)

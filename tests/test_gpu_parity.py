@@ -263,6 +263,33 @@ def test_split_launch_variants_bitwise(precision, variant, monkeypatch):
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_split_graph_across_solves(precision, monkeypatch):
+    """Captured split V-cycles reused across solves on one handle (a capped
+    solve, then full solves of two scenes): the BF16 cycle's first sweeps
+    depend on what the r update stored, so a captured cycle is replayed only
+    in the state it was captured in; the pressures equal the eager path's
+    bitwise."""
+    monkeypatch.setenv("FLUB_SPLIT_MIN_CELLS", "1")
+    scs = [scenes.dam_break_spheres(64, 1), scenes.dam_break(64, 3), scenes.dam_break_spheres(64, 2)]
+    caps = [3, -1, -1]
+    res = []
+    for graph in (False, True):
+        if graph:
+            monkeypatch.setenv("FLUB_SPLIT_GRAPH", "1")
+        pr = Projection((64, 64, 64), n_mg_levels=4, pre_and_post_smoothing=2, bottom_smoothing=6,
+                        precision=precision)
+        out = []
+        for sc, cap in zip(scs, caps):
+            pr.load(sc)
+            it = pr.project(max_iters=cap, rel_tol=1e-12)
+            out.append((it, pr.p0.f.numpy().copy()))
+        res.append(out)
+    for (i0, p0), (i1, p1) in zip(*res):
+        assert i0 == i1
+        assert bits_equal(p0, p1)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
 def test_split_solve_iteration_cap(precision, monkeypatch):
     """A fused solve stopped by max_iters returns that count; its pressure and
     the r.r it reports (summed on the host from the r update's partials) are
