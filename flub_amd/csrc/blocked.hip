@@ -43,6 +43,7 @@ struct TbArgs {
     int ti, tj, tk;       // tiles per axis
     int nsteps;           // 1..G (the configuration's halo width)
     uint32_t phases;      // bit s: phase of step s (updates (i+j+k)&1 == phase^1)
+    int zmasked;          // zin is +0 on every non-FLUID cell (an earlier launch of the chain wrote it)
     int cnt[TB_GMAX];     // cnt[d]: pairs within distance d of the core
 };
 
@@ -119,8 +120,13 @@ __global__ void __launch_bounds__(NT) k_tb(TbArgs a)
             if (q < NQ && gi >= 0 && gi < a.nx && gj >= 0 && gj < a.ny && gk >= 0 && gk < a.nz) {
                 const uint32_t gx = cidx(gi, gj, gk, a.ny, a.nz);
                 ld4<T>(reinterpret_cast<const T *>(a.zin) + gx, zq[u]);
-                const uint2 c4 = *reinterpret_cast<const uint2 *>(a.code + gx);
-                fq[u] = ((c4.x >> 15) & 1u) | ((c4.x >> 30) & 2u) | (((c4.y >> 15) & 1u) << 2) | ((c4.y >> 28) & 8u);
+                if (a.zmasked) {
+                    fq[u] = 15u;   // masking is the identity: skip the codes
+                } else {
+                    const uint2 c4 = *reinterpret_cast<const uint2 *>(a.code + gx);
+                    fq[u] = ((c4.x >> 15) & 1u) | ((c4.x >> 30) & 2u) | (((c4.y >> 15) & 1u) << 2) |
+                            ((c4.y >> 28) & 8u);
+                }
                 if (MODE == SWEEP_PROLONG)
                     ld2<T>(reinterpret_cast<const T *>(a.zc) + cidx(gi >> 1, gj >> 1, gk >> 1, a.ny >> 1, a.nz >> 1),
                            cq[u]);
@@ -252,13 +258,17 @@ __global__ void __launch_bounds__(NT) k_tb(TbArgs a)
 // per launch).  The G = 8 configurations (8 half-sweeps per launch on levels
 // of at most 64^3 cells) are optional: measured on a 256^3 solve, their larger
 // halos cost more than the launches they save.
-enum { TB_CFG_16 = 0, TB_CFG_8x16 = 1, TB_CFG_8 = 2, TB_CFG_8x16_G8 = 3, TB_CFG_8_G8 = 4, TB_NCFG = 5 };
+enum { TB_CFG_16 = 0, TB_CFG_8x16 = 1, TB_CFG_8 = 2, TB_CFG_8x16_G8 = 3, TB_CFG_8_G8 = 4, TB_CFG_4x8 = 5, TB_NCFG = 6 };
 struct TbGeom {
     int ci, cj, ck, nt, g;
 };
 const TbGeom kGeom[TB_NCFG] = {{16, 16, 16, 1024, 4}, {8, 8, 16, 1024, 4}, {8, 8, 8, 1024, 4},
-                               {8, 8, 16, 512, 8}, {8, 8, 8, 512, 8}};
+                               {8, 8, 16, 512, 8}, {8, 8, 8, 512, 8}, {4, 4, 8, 1024, 4}};
 #define TB_G8_MAX_CELLS (1u << 18)
+// Levels of at most 32^3 cells take 4 x 4 x 8 cores: on a 32^3 level they
+// give 256 workgroups (32 with 8 x 8 x 16 cores), so the launch is spread
+// over every CU, at three times the halo work per core cell.
+#define TB_SMALL_MAX_CELLS (1u << 15)
 
 bool fits(const Level &L, int c) { return L.nx % kGeom[c].ci == 0 && L.ny % kGeom[c].cj == 0 && L.nz % kGeom[c].ck == 0; }
 
@@ -269,6 +279,7 @@ int tb_cfg(const Level &L, bool allow_g8)
         if (fits(L, TB_CFG_8_G8)) return TB_CFG_8_G8;
     }
     if (L.n >= (1u << 21) && fits(L, TB_CFG_16)) return TB_CFG_16;
+    if (L.n <= TB_SMALL_MAX_CELLS && fits(L, TB_CFG_4x8)) return TB_CFG_4x8;
     if (fits(L, TB_CFG_8x16)) return TB_CFG_8x16;
     if (fits(L, TB_CFG_8)) return TB_CFG_8;
     return -1;
@@ -327,6 +338,7 @@ void tb_launch(hipStream_t s, int cfg, TbArgs a)
     if (cfg == TB_CFG_16) k_tb<T, 16, 16, 16, 1024, 4, MODE, DOT><<<nb, 1024, 0, s>>>(a);
     else if (cfg == TB_CFG_8x16) k_tb<T, 8, 8, 16, 1024, 4, MODE, DOT><<<nb, 1024, 0, s>>>(a);
     else if (cfg == TB_CFG_8) k_tb<T, 8, 8, 8, 1024, 4, MODE, DOT><<<nb, 1024, 0, s>>>(a);
+    else if (cfg == TB_CFG_4x8) k_tb<T, 4, 4, 8, 1024, 4, MODE, DOT><<<nb, 1024, 0, s>>>(a);
     else if (cfg == TB_CFG_8x16_G8) k_tb<T, 8, 8, 16, 512, 8, MODE, DOT><<<nb, 512, 0, s>>>(a);
     else k_tb<T, 8, 8, 8, 512, 8, MODE, DOT><<<nb, 512, 0, s>>>(a);
 }
@@ -401,6 +413,7 @@ int tb_chain(hipStream_t s, const uint32_t *tables, const Level &L, int prec, in
         for (int d = 0; d < G; ++d) a.cnt[d] = tab.cnt[d];
         a.nx = L.nx; a.ny = L.ny; a.nz = L.nz;
         a.nsteps = cnt;
+        a.zmasked = c > 0;   // the previous launch stored the whole level with +0 on non-FLUID cells
         uint32_t seen = 0;
         for (int q = 0; q < cnt; ++q) {
             a.phases |= (uint32_t)(phases[st + q] & 1u) << q;

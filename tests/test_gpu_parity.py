@@ -181,6 +181,24 @@ def test_vcycle_bitexact(smoother, shape, levels, oracle):
     assert bits_equal(mg.z[0].numpy(), orc.get("z", 0))
 
 
+@pytest.mark.parametrize("shape,levels", [((64, 64, 64), 4), ((32, 48, 64), 4)])
+def test_vcycle_blocked_g8_bitexact(shape, levels, oracle, monkeypatch):
+    """The optional 8-half-sweep blocked configurations (FLUB_TB_G8) on the
+    coarse levels of the reference-layout V-cycle: bitwise the oracle."""
+    monkeypatch.setenv("FLUB_TB_G8", "1")
+    tag = random_tags(shape, 6, (0.1, 0.85, 0.05))
+    r = np.random.default_rng(14).normal(0, 1, shape).astype(np.float32)
+    r[tag != 1] = 0.0
+    mg = _loaded_mgpcg(tag, levels)
+    mg.r[0].assign(r)
+    orc = oracle.MGPCG(shape, levels, pre_and_post_smoothing=2, bottom_smoothing=6)
+    orc.set_tags(tag)
+    orc.put("r", r, 0)
+    orc.apply_preconditioner("phases")
+    mg.apply_preconditioner("phases")
+    assert bits_equal(mg.z[0].numpy(), orc.get("z", 0))
+
+
 @pytest.mark.parametrize("shape,levels", [((64, 64, 64), 4), ((128, 128, 128), 4), ((64, 32, 48), 3),
                                           ((256, 256, 256), 4)])
 def test_split_vcycle_bitexact(shape, levels, oracle):
