@@ -26,7 +26,14 @@ the learned solver against plain MGPCG and V-cycle FPCG on held-out seeds at
 128^3 and 256^3.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--solvers fp32 fp16 bf16 learned]
-                    [--no-cpu] [--no-judge] [--no-heldout]
+                    [--no-cpu] [--no-judge] [--no-heldout] [--no-configs]
+
+"configs" (after the headline, unless --no-configs) holds two more records of
+the same run: "dam_break_spheres_128", plain FP32 MGPCG at 128^3 on the
+dam-break block union 32 fluid spheres (radius 2-8 cells, seed 1) with its
+per-kernel HBM GB/s and a bit-for-bit check of the cell flags and coarse
+masks, and "droplets_box_512", the second workload below at 512^3 (two timed
+steps per solver).
 
 Second workload (--workload droplets_box, default 512^3): the dam-break block
 plus 200 droplets of radius 2-12 cells (seed 3) and a solid 0.1^3 box at a
@@ -78,6 +85,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-judge", action="store_true", help="skip the FP64 CPU oracle projection (error report)")
     ap.add_argument("--no-heldout", action="store_true", help="skip the held-out learned evaluation")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 128^3 dam-break + spheres and the 512^3 droplets + box records")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--kernels-only", action="store_true",
                     help="only replay the level-0 half-sweep (for counter collection)")
@@ -179,6 +188,11 @@ def main():
     if "learned" in args.solvers and not args.no_heldout:
         heldout = learned_heldout()
     cpu = None if args.no_cpu else cpu_baseline(s, n, L, args.cpu_threads)
+    configs = None
+    if not args.no_configs:
+        del last_pr, outputs
+        configs = {"dam_break_spheres_128": dam_break_spheres_128(L, 5, 2),
+                   "droplets_box_512": droplets_box(512, L, 2, 1)}
     out = {
         "metric": "ms per pressure projection to 1e-6 rel. residual @256^3: learned/mixed vs MGPCG",
         "value": ms, "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -196,6 +210,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,
+        "configs": configs,
     }
     if "fp32" in solvers:
         out["speedup_vs_mgpcg_fp32"] = {k: round(solvers["fp32"]["ms"] / v["ms"], 3) for k, v in solvers.items()}
@@ -418,6 +433,13 @@ def _timed(fn):
 
 
 def droplets_box_main(args):
+    print(json.dumps(droplets_box(args.n or 512, args.levels, max(1, args.steps), args.warmup)), flush=True)
+
+
+def droplets_box(n, L, steps, warmup):
+    """The dam-break + droplets + moving box workload at n^3 (module
+    docstring): one record with every solver's time, the geometry,
+    divergence, extrapolation and advection kernels and the U-Net."""
     import torch
     from flub_amd import device, scenes
     from flub_amd.learned.evaluate import load_net
@@ -425,7 +447,6 @@ def droplets_box_main(args):
     from flub_amd.learned.unet import flops_per_voxel
     from flub_amd.project import Projection
 
-    n, L = args.n or 512, args.levels
     t0 = time.time()
     s = scenes.dam_break_droplets_box(n, 3)
     tag = s["tag"]
@@ -434,7 +455,6 @@ def droplets_box_main(args):
     fluid = torch.from_numpy(tag == 1).cuda()
     net = load_net(os.path.join(ROOT, "flub_amd", "learned", "weights", "unet_128.npz"))
     out = {}
-    steps = max(1, args.steps)
     for name, prec, learned in (("fp32", "fp32", False), ("bf16", "bf16", False), ("learned+bf16", "bf16", True)):
         pr = Projection((n, n, n), n_mg_levels=L, pre_and_post_smoothing=2, bottom_smoothing=6, precision=prec)
         pr.load(s)
@@ -442,11 +462,11 @@ def droplets_box_main(args):
         solver = LearnedPCG(pr.mgpcg, net) if learned else None
         dst = [pr.vel0.u.f, pr.vel0.v.f, pr.vel0.w.f]
         times, st = [], None
-        for i in range(max(1, args.warmup) + steps):
+        for i in range(max(1, warmup) + steps):
             for a, b in zip(dst, src):
                 device.copy(a, b)
             ms, st = _timed(lambda: pr.project_fractional(solver, fluid if learned else None, rel_tol=1e-12))
-            if i >= max(1, args.warmup):
+            if i >= max(1, warmup):
                 times.append(ms)
                 log(f"{name} step: {ms:.2f} ms, " + (f"{st['iterations']} iterations ({st['net_iters']} net, "
                     f"{st['vcycle_iters']} MGPCG, fallback at {st['fallback_at']})" if learned
@@ -507,18 +527,73 @@ def droplets_box_main(args):
     tflops = flops_per_voxel() * n ** 3 / (net_ms * 1e-3) / 1e12
     out["unet"] = {"forward_ms": round(net_ms, 2), "tflops": round(tflops, 1),
                    "mfma_frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4)}
+    del r
+    torch.cuda.empty_cache()
     best = min(("fp32", "bf16", "learned+bf16"), key=lambda k: out[k]["ms"])
-    print(json.dumps({"metric": f"ms per pressure projection to 1e-6 rel. residual @{n}^3 with a moving solid box "
-                                f"(face-fraction RHS): learned+mixed vs MGPCG",
-                      "value": out[best]["ms"], "unit": "ms", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
-                      "higher_is_better": False, "data": "synthetic (seeded scene)",
-                      "config": {"workload": f"dam break + 200 droplets + rotated moving box {n}^3: face fractions, "
-                                             f"rigid face velocities, fractional divergence, solve, gradient "
-                                             f"subtraction", "grid": n, "best_solver": best},
-                      "solvers": {k: out[k] for k in ("fp32", "bf16", "learned+bf16")},
-                      "geometry": out["geometry"], "divergence_fractional": out["divergence_fractional"],
-                      "extrapolate": out["extrapolate"], "advect": out["advect"],
-                      "unet": out["unet"], "kernels": out["kernels"]}), flush=True)
+    return {"metric": f"ms per pressure projection to 1e-6 rel. residual @{n}^3 with a moving solid box "
+                      f"(face-fraction RHS): learned+mixed vs MGPCG",
+            "value": out[best]["ms"], "unit": "ms", "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "higher_is_better": False, "data": "synthetic (seeded scene)",
+            "config": {"workload": f"dam break + 200 droplets + rotated moving box {n}^3: face fractions, "
+                                   f"rigid face velocities, fractional divergence, solve, gradient "
+                                   f"subtraction", "grid": n, "best_solver": best},
+            "solvers": {k: out[k] for k in ("fp32", "bf16", "learned+bf16")},
+            "speedup_vs_mgpcg_fp32": {k: round(out["fp32"]["ms"] / out[k]["ms"], 3)
+                                      for k in ("fp32", "bf16", "learned+bf16")},
+            "geometry": out["geometry"], "divergence_fractional": out["divergence_fractional"],
+            "extrapolate": out["extrapolate"], "advect": out["advect"],
+            "unet": out["unet"], "kernels": out["kernels"]}
+
+
+def dam_break_spheres_128(L, steps, warmup):
+    """Plain FP32 MGPCG at 128^3 on the dam-break block union 32 fluid spheres
+    (radius 2-8 cells, seed 1), fluid faces N(0, 0.5): ms per projection,
+    iterations, FP64 true residual, the per-kernel HBM GB/s of the solve's
+    kernels at this size, and whether the GPU's cell flags and coarse masks
+    (tag_dirichlet / tag_neumann, set_tags) equal the oracle's bit for bit
+    (the oracle only as the checker)."""
+    from flub_amd import device, scenes
+    from flub_amd.const_param import Field, IntField
+    from flub_amd.geometry import tag_dirichlet, tag_neumann
+    from flub_amd.project import Projection
+    from oracle import oracle as O
+    n = 128
+    s = scenes.dam_break_spheres(n, 1)
+    src = [device.from_numpy(s[c]) for c in ("u", "v", "w")]
+    pr = Projection((n, n, n), n_mg_levels=L, pre_and_post_smoothing=2, bottom_smoothing=6)
+    pr.load(s)
+    dst = [pr.vel0.u.f, pr.vel0.v.f, pr.vel0.w.f]
+    times, iters = [], []
+    for i in range(warmup + steps):
+        for a, b in zip(dst, src):
+            device.copy(a, b)
+        ms, it = _timed(lambda: pr.project(max_iters=-1, rel_tol=1e-12))
+        if i >= warmup:
+            times.append(ms)
+            iters.append(it)
+    ent = {"ms": round(float(np.mean(times)), 4), "ms_min": round(float(np.min(times)), 4),
+           "iterations": int(np.median(iters)), **accuracy(pr, s["tag"], pr.rTr)}
+    ent["kernels"] = profile_kernels(pr)
+    # flags from the level sets on the GPU, and the coarse masks of set_tags
+    tag = IntField(f=device.zeros((n, n, n), int))
+    tag_dirichlet(tag, Field(f=device.from_numpy(s["phi"])))
+    tag_neumann(tag, Field(f=device.from_numpy(s["phi_rigid"])))
+    t = O.tag_neumann(s["phi_rigid"], O.tag_dirichlet(s["phi"]))
+    flags_ok = bool(np.array_equal(tag.f.numpy(), t))
+    pr.mgpcg.set_tags(tag.f)
+    masks_ok = True
+    for l in range(L):
+        masks_ok &= bool(np.array_equal(pr.mgpcg.tag[l].numpy().astype(np.int32), t))
+        masks_ok &= bool(np.array_equal(pr.mgpcg.divd[l].numpy().view(np.uint32),
+                                        O.compute_divd(t).view(np.uint32)))
+        if l + 1 < L:
+            t = O.restrict_tag(t)
+    ent["flags_bitexact"] = flags_ok
+    ent["coarse_masks_and_divd_bitexact"] = masks_ok
+    ent["fluid_cells"] = int((s["tag"] == 1).sum())
+    log(f"128^3 dam break + 32 spheres: {ent['ms']} ms, {ent['iterations']} iterations, flags {flags_ok}, "
+        f"masks {masks_ok}")
+    return ent
 
 
 def _cpu_cores(threads):

@@ -48,6 +48,7 @@ SIGNATURES = {
     "flub_phi_prims": [_P, _P, _I, _I, _I, _F, _F, _F, _F, _I, _P, _P, _P, _P, _P],
     "flub_phi_to_fraction": [_P, _P, _I, _I, _I, _I, _P],
     "flub_face_fractions": [_P] * 6 + [_I, _I, _I, _F, _I, _P, _P, _P, _P, _P],
+    "flub_update_argmin": [_P, _P, _I, _I, _I, _P, _I, _I, _I, _I, _P, _P, _F, _P, _P, _F, _P],
     "flub_solid_vel": [_P, _P, _I, _I, _I, _I, _F, _P, _P, _P, _P],
     "flub_divergence_fractional_3d": [_P] * 10 + [_I, _I, _I, _P],
     "flub_set_unknown": [_P, _P, _P, _P, _I, _I, _I, _P],

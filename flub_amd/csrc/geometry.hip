@@ -13,6 +13,7 @@
 // (x, y, z, w), gtype[n] (0 none, 1 quad, 2 circle, 3 rect, 4 "x", 5 mesh),
 // radius[3n], v[3n], omega[3n].
 #include "common.h"
+#include "interp.h"
 
 namespace {
 
@@ -243,6 +244,37 @@ __global__ void __launch_bounds__(256) k_face_fractions3(float *__restrict__ fu,
     }
 }
 
+// geometry.py:365-388 update_argmin: a body given as a cached level set
+// (grid spacing h_ls, its cell ijk at the body origin) resampled at point
+// (i h + st h, ...) in the body frame -- (p - x) rotated by q^-1, divided by
+// h_ls, less ijk -- by the tricubic interpolation of utils/util.py:98-104
+// (trilerp_stagger with stagger 0); the point takes the level set's value,
+// and the body's id as its owner (staggered arrays only), where it is below
+// the value already there.
+struct Pose {
+    float x[3], q[4], st[3], ijk[3];
+};
+
+__global__ void __launch_bounds__(256) k_update_argmin(float *__restrict__ phi, int32_t *__restrict__ owner, int sx,
+                                                       int sy, int sz, Arr ls, int id, Pose P, float h_ls, float h)
+{
+    const uint32_t n = (uint32_t)sx * sy * sz;
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= n) return;
+    int i, j, k;
+    decode3(t, sy, sz, i, j, k);
+    const float p[3] = {(float)i * h + P.st[0] * h - P.x[0], (float)j * h + P.st[1] * h - P.x[1],
+                        (float)k * h + P.st[2] * h - P.x[2]};
+    float b[3];
+    quat_rotate_inv(P.q, p, b);
+    const float s = b[0] / h_ls - P.ijk[0], tt = b[1] / h_ls - P.ijk[1], r = b[2] / h_ls - P.ijk[2];
+    const float m = tricubic(ls, s - 0.0f, tt - 0.0f, r - 0.0f);
+    if (m < phi[t]) {
+        phi[t] = m;
+        if (owner) owner[t] = id;
+    }
+}
+
 // rigid_coupling.py:51-70 precompute_solid_vel for axis d: component d of
 // v + omega x (p - x) of the owning body on faces with h0/100 < i*h0 <
 // 1 - h0/100 (the test reads i for every axis, as the reference does), else 0.
@@ -344,6 +376,25 @@ FLUB_API int flub_face_fractions(float *fu, float *fv, float *fw, int32_t *owner
     FLUB_REQUIRE((uint64_t)ti * tj * tk < (1ull << 31), FLUB_EINVAL, "flub_face_fractions: grid too large");
     k_face_fractions3<<<(unsigned)(ti * tj * tk), 256, 0, as_stream(stream)>>>(fu, fv, fw, owner_u, owner_v, owner_w,
                                                                                nx, ny, nz, h, B, tj, tk);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+FLUB_API int flub_update_argmin(float *phi, int32_t *owner, int sx, int sy, int sz, const float *levelset, int lx,
+                                int ly, int lz, int id, const float *x, const float *q, float h_levelset,
+                                const float *stagger, const float *ijk, float h, void *stream)
+{
+    FLUB_REQUIRE(phi && dims_ok(sx, sy, sz), FLUB_EINVAL, "flub_update_argmin: bad array");
+    FLUB_REQUIRE(levelset && dims_ok(lx, ly, lz) && (uint64_t)lx * ly * lz * 4u < (1ull << 32), FLUB_EINVAL,
+                 "flub_update_argmin: bad level-set array (at most 2^30 values)");
+    FLUB_REQUIRE(x && q && stagger && ijk, FLUB_EINVAL, "flub_update_argmin: null pose");
+    Pose P;
+    for (int a = 0; a < 3; ++a) { P.x[a] = x[a]; P.st[a] = stagger[a]; P.ijk[a] = ijk[a]; }
+    for (int a = 0; a < 4; ++a) P.q[a] = q[a];
+    const bool staggered = stagger[0] != 0.0f || stagger[1] != 0.0f || stagger[2] != 0.0f;
+    const uint32_t n = (uint32_t)sx * sy * sz;
+    k_update_argmin<<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(phi, staggered ? owner : nullptr, sx, sy, sz,
+                                                                       Arr{levelset, lx, ly, lz}, id, P, h_levelset, h);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
 }

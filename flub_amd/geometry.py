@@ -9,9 +9,12 @@ the HIP C-ABI:
   ``sdf_argmin_prims``);
 * ``precompute_grid_point_phi_prims`` (:225-239) and ``phi_to_fraction``
   (:264-290) as launchable kernels;
+* ``update_argmin`` (:365-388): a body given as a cached level set (a mesh
+  body's signed distance on a grid of its own), resampled tricubically in the
+  body frame into a distance array;
 * ``PrecomputeFaceFraction`` (:401-462): tags, rigid level sets and face
-  fractions with their owners.  Bodies given as cached mesh level sets
-  (``rc.cached_levelset``) are not supported and raise.
+  fractions with their owners, primitive bodies and cached level sets
+  (``rc.cached_levelset``: body index -> (level set, ijk, h_levelset)).
 """
 from __future__ import annotations
 
@@ -113,12 +116,40 @@ phi_to_fraction = Kernel("phi_to_fraction", _phi_to_fraction,
                          lambda phi, frac, dim: cp.get_uf(frac, dim).f.shape)
 
 
+def _host3(a, n, what):
+    v = np.ascontiguousarray(np.asarray(a, np.float32).reshape(-1))
+    if v.size != n:
+        raise ValueError(f"update_argmin: {what} must hold {n} floats")
+    return v
+
+
+def _update_argmin(phi, owner, levelset, body_id, x, q, h_levelset, stagger, ijk, h=None):
+    """geometry.py:365-388 (owner written only for a staggered array of phi's
+    shape, as precompute_grid_point_phi_prims does)."""
+    ls = levelset if isinstance(levelset, wp_dev.DeviceArray) else wp_dev.from_numpy(
+        np.ascontiguousarray(levelset, np.float32))
+    if len(ls.shape) != 3:
+        raise ValueError("update_argmin: the level set must be a 3D array")
+    xs, qs, st, ij = (_host3(x, 3, "x"), _host3(q, 4, "q"), _host3(stagger, 3, "stagger"), _host3(ijk, 3, "ijk"))
+    staggered = any(float(v) != 0.0 for v in st)
+    own = owner.ptr if (staggered and owner is not None and owner.size == phi.size) else None
+    _lib.call("flub_update_argmin", phi.ptr, own, *phi.shape, ls.ptr, *ls.shape, int(body_id), xs.ctypes.data,
+              qs.ctypes.data, float(h_levelset), st.ctypes.data, ij.ctypes.data,
+              float(cp.h if h is None else h), None)
+
+
+update_argmin = Kernel("update_argmin", _update_argmin,
+                       lambda phi, owner, ls, i, x, q, hl, st, ijk, h=None: phi.shape)
+
+
 class PrecomputeFaceFraction:
-    """geometry.py:401-462 for primitive bodies (boxes, spheres, "x")."""
+    """geometry.py:401-462: primitive bodies (boxes, spheres, "x") and bodies
+    given as cached level sets.  ``vertices`` / ``edges`` are kept as the
+    reference keeps them; no part of the projection path reads them."""
 
     def __init__(self, use_mesh=False, vertices=(), edges=(), h=None):
-        if use_mesh:
-            raise NotImplementedError("mesh bodies are not supported; give primitives")
+        self.vertices = np.array(vertices, dtype=np.float32)
+        self.edges = np.array(edges, dtype=int)
         self.use_mesh = use_mesh
         self.h = h   # grid spacing; None: the module grid's h (const_param.h)
 
@@ -131,30 +162,40 @@ class PrecomputeFaceFraction:
     def tag_neumann(self, tag, phi):
         _tag_neumann(tag, phi)
 
-    @staticmethod
-    def _check_rc(rc):
-        if getattr(rc, "cached_levelset", None):
-            raise NotImplementedError("cached mesh level sets are not supported")
-
     def precompute_phi_with_cached_levelset(self, phi, rc, owner=None, stagger=(0.0, 0.0, 0.0)):
-        """geometry.py:415-436 (primitive bodies only)."""
-        self._check_rc(rc)
+        """geometry.py:417-433: the primitive bodies, then each cached level
+        set in the dict's order (update_argmin at the body's current pose)."""
         n, xp, qp, gp, rp = _bodies(rc.x_pair.cur, rc.theta_pair.cur, rc.rigid_geometry)
         staggered = any(float(s) != 0.0 for s in stagger)
         own = owner.ptr if (staggered and owner is not None and owner.size == phi.size) else None
         _lib.call("flub_phi_prims", phi.ptr, own, *phi.shape, *(float(s) for s in stagger), self._h(), n, xp, qp, gp,
                   rp, None)
+        cached = getattr(rc, "cached_levelset", None) or {}
+        if cached:
+            xs = rc.x_pair.cur.numpy().reshape(-1, 3)
+            qs = rc.theta_pair.cur.numpy().reshape(-1, 4)
+            for i, (levelset, ijk, h_levelset) in cached.items():
+                _update_argmin(phi, owner, levelset, i, xs[i], qs[i], h_levelset, stagger, ijk, self._h())
 
     def precompute_fij_with_cache(self, frac, vel_owner, rc):
-        """geometry.py:438-462: fractions and owners of all faces.  One fused
-        pass per axis, bitwise the reference's staged corner/centre/fraction
-        launches."""
-        self._check_rc(rc)
+        """geometry.py:438-462: fractions and owners of all faces.  With
+        primitive bodies only, one fused pass per axis, bitwise the reference's
+        staged corner/centre/fraction launches; with cached level sets, those
+        staged launches (corner distances, face-centre distances and owners,
+        phi_to_fraction per axis)."""
         fu, fv, fw = frac.u.f, frac.v.f, frac.w.f
         nx, ny, nz = fv.shape[0], fu.shape[1], fu.shape[2]
         for d, a in ((1, fu), (2, fv), (3, fw)):
             if a.shape != cp.field_shape(d, (nx, ny, nz)) or cp.get_uf(vel_owner, d).f.shape != a.shape:
                 raise ValueError("precompute_fij_with_cache: face array shapes differ from the grid")
+        if getattr(rc, "cached_levelset", None):
+            corner = wp_dev.empty((nx + 1, ny + 1, nz + 1), float)
+            self.precompute_phi_with_cached_levelset(corner, rc)
+            for d, a, st in ((1, fu, (0.0, 0.5, 0.5)), (2, fv, (0.5, 0.0, 0.5)), (3, fw, (0.5, 0.5, 0.0))):
+                self.precompute_phi_with_cached_levelset(a, rc, cp.get_uf(vel_owner, d).f, stagger=st)
+            for d, a in ((1, fu), (2, fv), (3, fw)):
+                _lib.call("flub_phi_to_fraction", corner.ptr, a.ptr, nx, ny, nz, d, None)
+            return
         n, xp, qp, gp, rp = _bodies(rc.x_pair.cur, rc.theta_pair.cur, rc.rigid_geometry)
         _lib.call("flub_face_fractions", fu.ptr, fv.ptr, fw.ptr, vel_owner.u.f.ptr, vel_owner.v.f.ptr,
                   vel_owner.w.f.ptr, nx, ny, nz, self._h(), n, xp, qp, gp, rp, None)

@@ -99,6 +99,17 @@ int flub_phi_to_fraction(const float *phi, float *frac, int nx, int ny, int nz, 
 int flub_face_fractions(float *fu, float *fv, float *fw, int32_t *owner_u, int32_t *owner_v, int32_t *owner_w,
                         int nx, int ny, int nz, float h, int n_bodies, const float *x, const float *q,
                         const int32_t *gtype, const float *radius, void *stream);
+/* geometry.py:365-388 update_argmin, launched per cached level set by
+ * PrecomputeFaceFraction.precompute_phi_with_cached_levelset (geometry.py:417-433):
+ * the body's level set (lx x ly x lz values, spacing h_levelset) is sampled
+ * tricubically (utils/util.py:98-104, 130-159) at p' = q^-1 (p - x) / h_levelset - ijk
+ * for every point p = ((i + st_x) h, ...) of the sx x sy x sz array; where the sample
+ * is below phi it replaces it, and owner (staggered arrays only; may be NULL) gets id.
+ * x[3], q[4] (x, y, z, w), stagger[3] and ijk[3] are host arrays read during the call;
+ * the level set holds at most 2^30 values. */
+int flub_update_argmin(float *phi, int32_t *owner, int sx, int sy, int sz, const float *levelset, int lx, int ly,
+                       int lz, int id, const float *x, const float *q, float h_levelset, const float *stagger,
+                       const float *ijk, float h, void *stream);
 /* rigid_coupling.py:51-70 precompute_solid_vel for axis d (1 u, 2 v, 3 w) over its
  * face array (sx, sy, sz): vel = (v + omega x (p - x))[d-1] of the owning body where
  * h0/100 < i*h0 < 1 - h0/100 and owner != -1, else 0. */

@@ -101,6 +101,7 @@ def _declare(L):
         getattr(L, name).argtypes = [_F] * n
         getattr(L, name).restype = _F
     L.orc_phi_to_fraction.argtypes = [_P, _P, _I, _I, _I, _I]
+    L.orc_update_argmin.argtypes = [_P, _P, _I, _I, _I, _P, _I, _I, _I, _I, _P, _P, _F, _P, _P, _F]
     L.orc_solid_vel.argtypes = [_P, _P, _I, _I, _I, _I, _F, _F, _F, _F, _P, _P, _P]
     L.orc_divergence_fractional.argtypes = [_P] * 10 + [_I, _I, _I]
     L.orc_set_unknown.argtypes = [_P, _P, _P, _P, _I, _I, _I]
@@ -392,6 +393,18 @@ def phi_prims(shape, stagger, h, rb: Rigids, owner=True):
     own = np.empty(shape, np.int32) if owner else None
     lib().orc_phi_prims(_ptr(phi), _ptr(own) if owner else None, *shape, *(float(s) for s in stagger), float(h),
                         rb.n, _ptr(rb.x), _ptr(rb.q), _ptr(rb.gtype), _ptr(rb.radius))
+    return phi, own
+
+
+def update_argmin(phi, owner, levelset, body_id, x, q, h_levelset, stagger, ijk, h):
+    """geometry.py:365-388 on copies of phi and owner (owner may be None);
+    returns (phi, owner)."""
+    phi = _arr(phi, np.float32).copy()
+    own = None if owner is None else _arr(owner, np.int32).copy()
+    ls = _arr(levelset, np.float32)
+    x, q, st, ijk = (_arr(np.reshape(a, -1), np.float32) for a in (x, q, stagger, ijk))
+    lib().orc_update_argmin(_ptr(phi), None if own is None else _ptr(own), *phi.shape, _ptr(ls), *ls.shape,
+                            int(body_id), _ptr(x), _ptr(q), float(h_levelset), _ptr(st), _ptr(ijk), float(h))
     return phi, own
 
 

@@ -4,6 +4,8 @@
  * velocities and the fraction-weighted divergence of the reference Warp code:
  *   geometry.py:17-32 (_sdf_cube), :82-113 (compute_sdf_atom_prims),
  *   :148-169 (sdf_argmin_prims), :225-239 (precompute_grid_point_phi_prims),
+ *   :365-388 (update_argmin: cached level sets, sampled by the tricubic
+ *   interpolation of oracle_advect.c),
  *   :264-290 (phi_to_fraction), :401-462 (PrecomputeFaceFraction);
  *   utils/util.py:168-228 (fraction_edge / fraction_triangle / fraction_quad),
  *   :231-235 (compute_v_solid_3d);
@@ -24,6 +26,9 @@
  * the solid velocity against v + omega x (p - x); the fractional divergence
  * against the plain flux divergence (all fractions 1), the rigid flux
  * divergence (all fractions 0) and u == u_rigid (any fractions); the
+ * cached-level-set resampling against a level set that is a polynomial of degree
+ * <= 3 per axis (reproduced exactly by the tricubic interpolation away from
+ * the clamped border), evaluated in FP64 at the rotated, translated point; the
  * extrapolation against a vectorised mean over the known 27-neighbourhood,
  * constant fields (kept exactly), idempotence and unchanged tags.
  *
@@ -133,6 +138,40 @@ void orc_phi_prims(float *phi, int32_t *owner, int sx, int sy, int sz, float stx
                 const size_t t = GIDX(i, j, k, sy, sz);
                 phi[t] = orc_sdf_argmin_prims(u, v, w, n, x, q, gtype, radius, h, &am);
                 if (staggered && owner) owner[t] = am;
+            }
+}
+
+void orc_tricubic(const float *q, int gx, int gy, int gz, const float *str, float *out, int n);
+
+/* geometry.py:365-388 update_argmin over an (sx, sy, sz) array: the point
+ * (i h + st_x h, ...) less the body position x, rotated by q^-1, divided by
+ * h_ls, less ijk, samples the level set (lx, ly, lz) tricubically
+ * (trilerp_stagger with stagger 0, utils/util.py:98-104); a sample below phi
+ * replaces it and, on a staggered array, sets owner to id. */
+void orc_update_argmin(float *phi, int32_t *owner, int sx, int sy, int sz, const float *ls, int lx, int ly, int lz,
+                       int id, const float *x, const float *q, float h_ls, const float *st, const float *ijk,
+                       float h)
+{
+    const int staggered = st[0] != 0.0f || st[1] != 0.0f || st[2] != 0.0f;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 0; i < sx; ++i)
+        for (int j = 0; j < sy; ++j)
+            for (int k = 0; k < sz; ++k) {
+                float p[3] = {(float)i * h, (float)j * h, (float)k * h};
+                for (int a = 0; a < 3; ++a) p[a] = p[a] + st[a] * h;
+                for (int a = 0; a < 3; ++a) p[a] = p[a] - x[a];
+                float b[3];
+                quat_rotate_inv(q, p, b);
+                float s3[3];
+                for (int a = 0; a < 3; ++a) s3[a] = b[a] / h_ls - ijk[a];
+                for (int a = 0; a < 3; ++a) s3[a] = s3[a] - 0.0f;
+                float m;
+                orc_tricubic(ls, lx, ly, lz, s3, &m, 1);
+                const size_t t = GIDX(i, j, k, sy, sz);
+                if (m < phi[t]) {
+                    phi[t] = m;
+                    if (staggered && owner) owner[t] = id;
+                }
             }
 }
 

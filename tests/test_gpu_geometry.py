@@ -160,3 +160,85 @@ def test_divergence_fractional_rejects_bad_shapes():
     div = wp.empty((8, 8, 8), float)
     with pytest.raises(ValueError):
         launch(divergence_fractional_3d, dim=(8, 8, 8), inputs=[a, a, a, div, a, a, a, a, a, a])
+
+
+def _box_levelset(lshape, h_ls, half):
+    """The signed distance of a box (half-extents ``half``) centred on the body
+    origin, sampled on a grid of its own: index a sits at body coordinate
+    (a + ijk) h_ls with ijk = -(l - 1) / 2 -- the form of a mesh body's cached
+    level set (geometry.py:427-433)."""
+    ijk = -(np.array(lshape, np.float64) - 1.0) / 2.0
+    ax = [(np.arange(n) + o) * h_ls for n, o in zip(lshape, ijk)]
+    p = np.stack(np.meshgrid(*ax, indexing="ij"), -1)
+    m = np.abs(p) - np.array(half)
+    out = np.where(m.max(-1) < 0, m.max(-1), np.linalg.norm(np.maximum(m, 0.0), axis=-1))
+    return out.astype(np.float32), ijk.astype(np.float32)
+
+
+def _mesh_body_scene(seed, nprim=3):
+    """nprim primitive bodies plus one mesh body (kind 5, the last) whose
+    distance comes from a cached level set."""
+    rb, arrs = random_bodies(seed, nprim + 1, kinds=(GEOM_QUAD, GEOM_CIRCLE, GEOM_X, GEOM_MESH))
+    x, q, kinds, rad, v, om = arrs
+    kinds[-1] = GEOM_MESH
+    geoms = [GeometryType3D(int(k), tuple(float(r) for r in rr)) for k, rr in zip(kinds, rad)]
+    rb = RigidBodies(x, q, geoms, v, om)
+    ls, ijk = _box_levelset((26, 22, 30), 0.02, (0.16, 0.12, 0.2))
+    rb.cached_levelset = {nprim: (wp.from_numpy(ls), ijk, 0.02)}
+    return rb, (x, q, kinds, rad, v, om), ls, ijk
+
+
+@pytest.mark.parametrize("shape,seed", [((32, 32, 32), 21), ((40, 24, 56), 22)])
+def test_update_argmin_bitexact(shape, seed, grid, oracle):
+    """geometry.py:365-388: a cached level set resampled into distance arrays
+    of every stagger (owners on the staggered ones), bitwise the oracle; the
+    level set covers part of the grid, so clamped samples are included."""
+    from flub_amd.geometry import update_argmin
+    grid(shape)
+    rb, arrs, ls, ijk = _mesh_body_scene(seed)
+    ob = oracle_bodies(oracle, arrs)
+    x, q = arrs[0][-1], arrs[1][-1]
+    changed = 0
+    for d, st in STAGGERS.items():
+        fs = cp.field_shape(d, shape)
+        phi0, own0 = oracle.phi_prims(fs, st, cp.h, ob, owner=True)
+        phi, own = wp.from_numpy(phi0), wp.from_numpy(own0)
+        launch(update_argmin, dim=fs, inputs=[phi, own, wp.from_numpy(ls), 3, x, q, 0.02, st, ijk, cp.h])
+        want_phi, want_own = oracle.update_argmin(phi0, own0, ls, 3, x, q, 0.02, st, ijk, cp.h)
+        assert bits_equal(phi.numpy(), want_phi), f"phi differs on array {d}"
+        assert np.array_equal(own.numpy(), want_own), f"owner differs on array {d}"
+        changed += int((want_phi != phi0).sum())
+        if d:
+            assert (want_own == 3).any()
+    assert changed > 100
+
+
+@pytest.mark.parametrize("shape,seed", [((32, 32, 32), 23), ((48, 40, 24), 24)])
+def test_face_fractions_cached_levelset_bitexact(shape, seed, grid, oracle):
+    """precompute_fij_with_cache with a cached mesh level set (the staged
+    corner / face-centre / phi_to_fraction launches of geometry.py:438-462)
+    against the oracle's same sequence, fractions and owners bitwise."""
+    grid(shape)
+    rb, arrs, ls, ijk = _mesh_body_scene(seed)
+    ob = oracle_bodies(oracle, arrs)
+    x, q = arrs[0][-1], arrs[1][-1]
+    nx, ny, nz = shape
+    cs = (nx + 1, ny + 1, nz + 1)
+    corner, _ = oracle.phi_prims(cs, (0.0, 0.0, 0.0), cp.h, ob, owner=False)
+    corner, _ = oracle.update_argmin(corner, None, ls, 3, x, q, 0.02, (0.0, 0.0, 0.0), ijk, cp.h)
+    frac = make_mac_grid(float, shape)
+    owner = make_mac_grid(int, shape)
+    PrecomputeFaceFraction().precompute_fij_with_cache(frac, owner, rb)
+    plain = make_mac_grid(float, shape)
+    rb_plain = RigidBodies(*arrs[:2], [GeometryType3D(int(k), tuple(float(r) for r in rr))
+                                       for k, rr in zip(arrs[2], arrs[3])])
+    PrecomputeFaceFraction().precompute_fij_with_cache(plain, make_mac_grid(int, shape), rb_plain)
+    for d in (1, 2, 3):
+        fs = cp.field_shape(d, shape)
+        cen, own = oracle.phi_prims(fs, STAGGERS[d], cp.h, ob, owner=True)
+        cen, own = oracle.update_argmin(cen, own, ls, 3, x, q, 0.02, STAGGERS[d], ijk, cp.h)
+        want = oracle.phi_to_fraction(corner, cen, d)
+        got = cp.get_uf(frac, d).f.numpy()
+        assert bits_equal(got, want), f"fractions differ on axis {d}"
+        assert np.array_equal(cp.get_uf(owner, d).f.numpy(), own), f"owners differ on axis {d}"
+        assert (own == 3).any() and not bits_equal(got, cp.get_uf(plain, d).f.numpy())

@@ -12,6 +12,10 @@ of the same formula:
   rule reproduces exactly for linear level sets;
 * phi_to_fraction (geometry.py:264-290) per axis against that exact area on
   every face of a grid cut by a plane;
+* update_argmin (geometry.py:365-388, cached mesh level sets) against a level
+  set that is a cubic polynomial per axis, which the tricubic sampling
+  reproduces exactly away from the clamped border, evaluated in FP64 at the
+  point moved into the body frame by scipy's Rotation;
 * precompute_solid_vel (rigid_coupling.py:51-70) against v + omega x (p - x);
 * divergence_fractional_3d (divergence.py:5-41) against the plain flux
   divergence (all fractions 1), the rigid flux divergence (all fractions 0),
@@ -253,3 +257,43 @@ def test_divergence_fractional_special_cases():
     db = O.divergence_fractional(*zeros, *fr, ur, vr, wr)
     dc = O.divergence_fractional(u, v, w, *fr, ur, vr, wr)
     np.testing.assert_allclose(dc, da + db, atol=5e-6)
+
+
+def _cubic_levelset(a, b, c):
+    """A tensor-product polynomial of degree 3 per axis in level-set index units."""
+    return (0.05 + 0.004 * a - 0.003 * b + 0.002 * c + 1e-4 * a * b - 2e-5 * c * c
+            + 3e-6 * a ** 3 - 1e-6 * b ** 2 * c + 1e-7 * a * b * c ** 2)
+
+
+@pytest.mark.parametrize("stagger", [(0.0, 0.0, 0.0), (0.0, 0.5, 0.5), (0.5, 0.5, 0.0)])
+def test_update_argmin_cached_levelset_polynomial(stagger):
+    rng = np.random.default_rng(11)
+    shape, h = (24, 20, 28), 1.0 / 24
+    lshape, h_ls = (30, 30, 30), 0.025
+    ijk = np.array([-14.0, -15.5, -13.0], F32)
+    x = np.array([0.47, 0.41, 0.56], F32)
+    q = _rand_quat(rng, 1)[0]
+    a, b, c = np.meshgrid(*(np.arange(n, dtype=np.float64) for n in lshape), indexing="ij")
+    ls = _cubic_levelset(a, b, c).astype(F32)
+    phi0 = rng.uniform(-0.05, 0.25, shape).astype(F32)
+    own0 = rng.integers(-1, 3, shape).astype(np.int32)
+    phi, own = O.update_argmin(phi0, own0, ls, 7, x, q, h_ls, stagger, ijk, h)
+
+    i, j, k = np.meshgrid(*(np.arange(n, dtype=np.float64) for n in shape), indexing="ij")
+    pts = np.stack([i + stagger[0], j + stagger[1], k + stagger[2]], -1) * h - x.astype(np.float64)
+    body = Rotation.from_quat(q.astype(np.float64)).inv().apply(pts.reshape(-1, 3)).reshape(pts.shape)
+    s = body / h_ls - ijk.astype(np.float64)
+    inside = np.all((s >= 1.0) & (s <= np.array(lshape) - 3.0), axis=-1)
+    want = _cubic_levelset(s[..., 0], s[..., 1], s[..., 2])
+    clear = inside & (np.abs(want - phi0) > 1e-4)
+    assert clear.sum() > 500 and (clear & (want < phi0)).sum() > 100 and (clear & (want > phi0)).sum() > 100
+    np.testing.assert_allclose(phi[clear], np.minimum(want, phi0)[clear], atol=2e-6)
+    upd = clear & (want < phi0)
+    keep = clear & (want > phi0)
+    assert np.array_equal(phi[keep], phi0[keep])
+    staggered = any(v != 0.0 for v in stagger)
+    assert np.all(own[upd] == (7 if staggered else own0[upd]))
+    assert np.array_equal(own[keep], own0[keep])
+    # no owner array: the same distances
+    phi2, _ = O.update_argmin(phi0, None, ls, 7, x, q, h_ls, stagger, ijk, h)
+    assert np.array_equal(phi2.view(np.uint32), phi.view(np.uint32))
