@@ -68,6 +68,8 @@ SIGNATURES = {
     "flub_mgpcg_create": [_PP, _I, _I, _I, _I, _I, _P],
     "flub_mgpcg_destroy": [_P],
     "flub_mgpcg_set_smoothing": [_P, _I, _I],
+    "flub_mgpcg_set_coarse_scale": [_P, _F],
+    "flub_mgpcg_set_sweep_cap": [_P, _I],
     "flub_mgpcg_set_tags": [_P, _P],
     "flub_mgpcg_init": [_P, _P, _F],
     "flub_mgpcg_mask_interior": [_P],

@@ -162,7 +162,7 @@ __global__ void k_smooth_def(float *__restrict__ z, const float *__restrict__ r,
 template <bool TAGGED>
 __global__ void k_restrict(const float *__restrict__ r, const float *__restrict__ z,
                            const uint8_t *__restrict__ tag, float *__restrict__ rc,
-                           const uint8_t *__restrict__ tagc, int nx, int ny, int nz)
+                           const uint8_t *__restrict__ tagc, int nx, int ny, int nz, float cs)
 {
     const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;
     const uint32_t n = (uint32_t)cx * cy * cz;
@@ -185,7 +185,7 @@ __global__ void k_restrict(const float *__restrict__ r, const float *__restrict_
                 else a = nsum_stencil(GX{z, ny, nz}, i, j, k, nx, ny, nz, zc);
                 s += r[c] - (-a);
             }
-    rc[t] = s;
+    rc[t] = s * cs;
 }
 
 // mgpcg.py:113-116 _prolongate.
@@ -517,7 +517,7 @@ static int vcycle_fp32(flub_mgpcg *h, int smoother)
     k_fill_f32<<<nb256(L0.n), 256, 0, h->s>>>(L0.z, 0.0f, L0.n);
     FLUB_CHECK_LAUNCH();
     for (int l = 0; l + 1 < h->L; ++l) {
-        for (int it = 0; it < (h->pre_post << l); ++it) {
+        for (int it = 0; it < mg_sweeps(h, l); ++it) {
             TRY(half_sweep(h, l, tagged, 0));
             TRY(half_sweep(h, l, tagged, 1));
         }
@@ -525,8 +525,8 @@ static int vcycle_fp32(flub_mgpcg *h, int smoother)
         k_fill_f32<<<nb256(C.n), 256, 0, h->s>>>(C.z, 0.0f, C.n);
         FLUB_CHECK_LAUNCH();
         Level &F = h->lv[l];
-        if (tagged) k_restrict<true><<<nb256(C.n), 256, 0, h->s>>>(F.r, F.z, F.tag, C.r, C.tag, F.nx, F.ny, F.nz);
-        else k_restrict<false><<<nb256(C.n), 256, 0, h->s>>>(F.r, F.z, F.tag, C.r, C.tag, F.nx, F.ny, F.nz);
+        if (tagged) k_restrict<true><<<nb256(C.n), 256, 0, h->s>>>(F.r, F.z, F.tag, C.r, C.tag, F.nx, F.ny, F.nz, C.cs);
+        else k_restrict<false><<<nb256(C.n), 256, 0, h->s>>>(F.r, F.z, F.tag, C.r, C.tag, F.nx, F.ny, F.nz, C.cs);
         FLUB_CHECK_LAUNCH();
     }
     bottom_solve(h, tagged);
@@ -534,7 +534,7 @@ static int vcycle_fp32(flub_mgpcg *h, int smoother)
         Level &F = h->lv[l];
         k_prolong<<<nb256(F.n), 256, 0, h->s>>>(F.z, h->lv[l + 1].z, F.nx, F.ny, F.nz);
         FLUB_CHECK_LAUNCH();
-        for (int it = 0; it < (h->pre_post << l); ++it) {
+        for (int it = 0; it < mg_sweeps(h, l); ++it) {
             TRY(half_sweep(h, l, tagged, 1));
             TRY(half_sweep(h, l, tagged, 0));
         }
@@ -563,9 +563,12 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
     const void *pre_z[MAXL];
     uint8_t ph[MG_MAX_PHASES];
     const SplitBuf &S = h->sp;
-    for (int l = 0; l + 1 < h->L; ++l) {
+    // the coarsest levels, from level T on, in one workgroup (tail.hip)
+    const int T = tail_first_level(h, split ? std::max(h->nsplit, h->mx_l1 ? 2 : 1) : 1);
+    const int lend = T > 0 ? T : h->L - 1;
+    for (int l = 0; l < lend; ++l) {
         Level &F = h->lv[l];
-        const int s = h->pre_post << l;
+        const int s = mg_sweeps(h, l);
         if (l == 0 && split && h->prec != FLUB_PREC_FP32) {
             // FP16/BF16 pre-smoothing (z and r), the first two sweeps converting r
             SplitMixedSweep m{};
@@ -660,16 +663,18 @@ static int vcycle_fast(flub_mgpcg *h, bool split, int zslot = 0)
         TRY(fast_restrict(h->s, F, (const float *)pre_z[l], h->lv[l + 1]));
     }
     const int lb = h->L - 1;
-    if (h->L > 1 && mg_use_tb(h, lb, 2 * (h->bottom / 2) * 2)) {
+    if (T > 0) {
+        TRY(tail_cycle(h->s, h, T));
+    } else if (h->L > 1 && mg_use_tb(h, lb, 2 * (h->bottom / 2) * 2)) {
         Level &B = h->lv[lb];
         const int n = mg_bottom_phases(ph, h->bottom / 2);
         TRY(tb_chain(h->s, h->tb_order, B, FLUB_PREC_FP32, SWEEP_ZERO, nullptr, nullptr, 1.0f, B.r, ph, n, B.z, B.zpre, nullptr, nullptr, h->tb_g8));
     } else {
         bottom_solve(h, 1);
     }
-    for (int l = h->L - 2; l >= 0; --l) {
+    for (int l = lend - 1; l >= 0; --l) {
         Level &F = h->lv[l];
-        const int s = h->pre_post << l;
+        const int s = mg_sweeps(h, l);
         // the coarse z in the reference layout: a split level leaves it in its zpre
         const float *zc = split && l + 1 < h->nsplit ? h->lv[l + 1].zpre : h->lv[l + 1].z;
         if (l == 0 && split && h->prec != FLUB_PREC_FP32) {
@@ -857,6 +862,7 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
     h->no_tb = getenv("FLUB_NO_TB") != nullptr;
     h->no_tb_l0 = getenv("FLUB_NO_TB_L0") != nullptr;
     h->tb_g8 = getenv("FLUB_TB_G8") != nullptr;
+    h->no_tail = getenv("FLUB_NO_TAIL") != nullptr;
     int rc = FLUB_OK;
 #define ALLOC(ptr, bytes)                                              \
     do {                                                               \
@@ -879,6 +885,7 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
             Level &L = h->lv[l];
             L.nx = nx >> l; L.ny = ny >> l; L.nz = nz >> l;
             L.n = (uint32_t)L.nx * L.ny * L.nz;
+            L.cs = 1.0f;
             ALLOC(L.tag, L.n);
             ALLOC(L.code, (size_t)L.n * 2);
             ALLOC(L.r, (size_t)L.n * 4);
@@ -1044,6 +1051,30 @@ FLUB_API int flub_mgpcg_set_smoothing(flub_mgpcg *h, int pre_post, int bottom)
     }
     h->pre_post = pre_post;
     h->bottom = bottom;
+    return FLUB_OK;
+}
+
+FLUB_API int flub_mgpcg_set_coarse_scale(flub_mgpcg *h, float cs)
+{
+    FLUB_REQUIRE(h, FLUB_EINVAL, "null handle");
+    FLUB_REQUIRE(cs > 0.0f && cs <= 4.0f, FLUB_EINVAL, "coarse-grid correction scale %g outside (0, 4]", (double)cs);
+    if (h->L > 1 && cs != h->lv[1].cs) {
+        FLUB_CHECK_HIP(hipStreamSynchronize(h->s));
+        drop_graphs(h);   // captured restrictions hold the old scale
+    }
+    for (int l = 1; l < h->L; ++l) h->lv[l].cs = cs;
+    return FLUB_OK;
+}
+
+FLUB_API int flub_mgpcg_set_sweep_cap(flub_mgpcg *h, int cap)
+{
+    FLUB_REQUIRE(h, FLUB_EINVAL, "null handle");
+    FLUB_REQUIRE(cap >= 0 && cap <= 1024, FLUB_EINVAL, "sweep cap %d outside [0, 1024]", cap);
+    if (cap != h->sweep_cap) {
+        FLUB_CHECK_HIP(hipStreamSynchronize(h->s));
+        drop_graphs(h);
+    }
+    h->sweep_cap = cap;
     return FLUB_OK;
 }
 
@@ -1776,7 +1807,7 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
         case 4: {
             Level &C = h->lv[1];
             if (h->fast) return fast_restrict(h->s, L0, L0.zpre, C);
-            k_restrict<true><<<nb256(C.n), 256, 0, h->s>>>(L0.r, L0.z, L0.tag, C.r, C.tag, L0.nx, L0.ny, L0.nz);
+            k_restrict<true><<<nb256(C.n), 256, 0, h->s>>>(L0.r, L0.z, L0.tag, C.r, C.tag, L0.nx, L0.ny, L0.nz, C.cs);
             FLUB_CHECK_LAUNCH();
             return FLUB_OK;
         }

@@ -26,6 +26,10 @@ struct Level {
     float *r, *z;   // FP32 storage
     float *zpre;    // pre-smoothed z of the fast V-cycle (post-smoothing lands in z)
     void *rh, *zh;  // FP16/BF16 storage (mixed precision only)
+    // The restriction into this level stores cs times the sum of the children's
+    // residuals (1, set at creation: the reference's un-normalised sum,
+    // mgpcg.py:94-101).
+    float cs;
 };
 
 struct MixedState;
@@ -121,7 +125,28 @@ struct flub_mgpcg {
     SplitBuf sp;
     MixedState *mx;
     Probe *probe;      // armed in-solve timing (nullptr: off)
+    int sweep_cap;     // > 0: at most this many pre/post iterations on any level (0: the reference's doubling)
+    bool no_tail;      // the coarsest levels launch kernel by kernel (no one-workgroup tail, tail.hip)
 };
+
+// The one-workgroup lower part of the V-cycle (tail.hip): levels of at most
+// TAIL_MAX_CELLS cells, at most TAIL_MAXL of them.
+constexpr uint32_t TAIL_MAX_CELLS = 4096;
+constexpr int TAIL_MAXL = 8;
+// First level (>= lowest, >= 1) the tail can take, or -1.
+int tail_first_level(const flub_mgpcg *h, int lowest);
+// The V-cycle from level `first` (r in its lv[].r) down to the bottom and back:
+// z of every level from `first` on into lv[].z, the restricted r into lv[].r.
+int tail_cycle(hipStream_t s, flub_mgpcg *h, int first);
+
+// Red-black iterations of the pre- and post-smoothing of level l: the
+// reference doubles them per level (pre_and_post_smoothing << l,
+// mgpcg.py:301,341); a handle may cap them.
+inline int mg_sweeps(const flub_mgpcg *h, int l)
+{
+    const int s = h->pre_post << l;
+    return h->sweep_cap > 0 && s > h->sweep_cap ? h->sweep_cap : s;
+}
 
 // Event before / after a probed launch (no-ops unless a probe is armed and has room).
 inline int probe_begin(flub_mgpcg *h)

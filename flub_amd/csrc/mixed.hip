@@ -458,7 +458,7 @@ static int vcycle_mixed(flub_mgpcg *h)
     uint8_t ph[MG_MAX_PHASES];
     for (int l = 0; l + 1 < L; ++l) {
         void *zp = h->mx->zph[l];
-        const int s = h->pre_post << l;
+        const int s = mg_sweeps(h, l);
         if (mg_use_tb(h, l, 2 * s)) {
             Level &F = h->lv[l];
             const int n = mg_phase_pairs(ph, s, 0);
@@ -505,7 +505,7 @@ static int vcycle_mixed(flub_mgpcg *h)
         void *zp = h->mx->zph[l];
         void *z = h->lv[l].zh;
         const void *zc = h->lv[l + 1].zh;
-        const int s = h->pre_post << l;
+        const int s = mg_sweeps(h, l);
         if (mg_use_tb(h, l, 2 * s)) {
             const int n = mg_phase_pairs(ph, s, 1);
             TRYM(tb_chain(h->s, h->tb_order, h->lv[l], h->prec, SWEEP_PROLONG, pre_z[l], zc, 8.0f, h->lv[l].rh, ph, n, z, zp, nullptr, nullptr, h->tb_g8));

@@ -167,7 +167,7 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgs a)
 // quad of the four child rows (2I+a, 2J+b).  Residual r - (-ndt(z)) per fine
 // cell; a FLUID parent sums its 8 children in the serial launch order
 // (ii, jj, kk), others get 0.  The coarse value is stored times cscale
-// (1 in FP32; 1/8 for the mixed level scaling).
+// (the coarse level's cs in FP32; cs/8 for the mixed level scaling).
 template <typename T>
 __global__ void __launch_bounds__(256) k_restrict(const T *__restrict__ r, const T *__restrict__ z,
                                                   const uint16_t *__restrict__ code, T *__restrict__ rc,
@@ -437,12 +437,12 @@ int restrict_level(hipStream_t s, const Level &F, const void *rf, const void *zf
     const uint32_t items = (uint32_t)C.nx * C.ny * gc.nq;
     const unsigned nb = grid_for(items);
     if (prec == FLUB_PREC_FP16)
-        k_restrict<__half><<<nb, 256, 0, s>>>((const __half *)rf, (const __half *)zf, F.code, (__half *)rc, C.tag, gc, 0.125f);
+        k_restrict<__half><<<nb, 256, 0, s>>>((const __half *)rf, (const __half *)zf, F.code, (__half *)rc, C.tag, gc, 0.125f * C.cs);
     else if (prec == FLUB_PREC_BF16)
         k_restrict<__hip_bfloat16><<<nb, 256, 0, s>>>((const __hip_bfloat16 *)rf, (const __hip_bfloat16 *)zf, F.code,
-                                                      (__hip_bfloat16 *)rc, C.tag, gc, 0.125f);
+                                                      (__hip_bfloat16 *)rc, C.tag, gc, 0.125f * C.cs);
     else
-        k_restrict<float><<<nb, 256, 0, s>>>((const float *)rf, (const float *)zf, F.code, (float *)rc, C.tag, gc, 1.0f);
+        k_restrict<float><<<nb, 256, 0, s>>>((const float *)rf, (const float *)zf, F.code, (float *)rc, C.tag, gc, C.cs);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
 }

@@ -433,7 +433,8 @@ __global__ void __launch_bounds__(NT) k_ssweep2(const TZI *__restrict__ zin, TZO
 template <typename TR, typename TZ, int CPT, typename TO = float>
 __global__ void __launch_bounds__(256) k_srestrict(const TR *__restrict__ r, const TZ *__restrict__ z,
                                                    const uint16_t *__restrict__ code, TO *__restrict__ rc,
-                                                   const uint8_t *__restrict__ tagc, SGeo g, SGeo cg, int csplit)
+                                                   const uint8_t *__restrict__ tagc, SGeo g, SGeo cg, int csplit,
+                                                   float cs)
 {
     // cg: the coarse grid in thread vectors of CPT cells (nx, ny coarse; nh = coarse nz = fine nz/2)
     const uint32_t total = (uint32_t)cg.nx * cg.ny * cg.nq;
@@ -522,7 +523,7 @@ __global__ void __launch_bounds__(256) k_srestrict(const TR *__restrict__ r, con
         for (int q = 0; q < CPT / 4; ++q) tg[q] = reinterpret_cast<const uint32_t *>(tagc + cb)[q];
         float out[CPT];
 #pragma unroll
-        for (int e = 0; e < CPT; ++e) out[e] = ((tg[e >> 2] >> (8 * (e & 3))) & 0xffu) == 1u ? s[e] : 0.0f;
+        for (int e = 0; e < CPT; ++e) out[e] = ((tg[e >> 2] >> (8 * (e & 3))) & 0xffu) == 1u ? s[e] * cs : 0.0f;
         if (csplit) {   // colour-split coarse r: even e to half (I + J) & 1, odd e to the other (K0 even)
             const uint32_t nhc = (uint32_t)cg.nh >> 1, halfc = (uint32_t)cg.nx * (uint32_t)cg.ny * nhc;
             const uint32_t b = ((uint32_t)I * cg.ny + J) * nhc + (uint32_t)(K0 >> 1);
@@ -933,7 +934,7 @@ int split_restrict(hipStream_t s, const Level &F, const float *r, const float *z
     const SGeo g = make_sgeo(F.nx, F.ny, F.nz);
     SGeo cg = make_sgeo(C.nx, C.ny, 2 * C.nz);   // nh = C.nz
     const unsigned nb = grid8((uint32_t)C.nx * C.ny * cg.nq);
-    k_srestrict<float, float, 4><<<nb, 256, 0, s>>>(r, z, code, C.r, C.tag, g, cg, (int)csplit);
+    k_srestrict<float, float, 4><<<nb, 256, 0, s>>>(r, z, code, C.r, C.tag, g, cg, (int)csplit, C.cs);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
 }
@@ -949,13 +950,13 @@ int split_restrict_mixed(hipStream_t s, const Level &F, int prec, const void *rh
     using BF = __hip_bfloat16;
     if (prec == FLUB_PREC_BF16 && rc16)   // the coarse r in BF16 (the BF16 level-1 pre-smoothing)
         k_srestrict<BF, BF, 4, BF><<<nb, 256, 0, s>>>((const BF *)rh, (const BF *)zh, code, (BF *)rc16, C.tag, g, cg,
-                                                      (int)csplit);
+                                                      (int)csplit, C.cs);
     else if (prec == FLUB_PREC_BF16)
         k_srestrict<BF, BF, 4><<<nb, 256, 0, s>>>((const BF *)rh, (const BF *)zh, code, C.r, C.tag, g, cg,
-                                                  (int)csplit);
+                                                  (int)csplit, C.cs);
     else
         k_srestrict<__half, __half, 4><<<nb, 256, 0, s>>>((const __half *)rh, (const __half *)zh, code, C.r, C.tag,
-                                                          g, cg, (int)csplit);
+                                                          g, cg, (int)csplit, C.cs);
     FLUB_CHECK_LAUNCH();
     return FLUB_OK;
 }

@@ -67,6 +67,8 @@ class MGPCG:
         self._h = h.value
         self._pre_post = 6
         self._bottom = 6
+        self._coarse_scale = 1.0
+        self._sweep_cap = 0
         self.r = [self._view("r", l, np.float32) for l in range(self.n_mg_levels)]
         self.z = [self._view("z", l, np.float32) for l in range(self.n_mg_levels)]
         self.tag = [self._view("tag", l, np.uint8) for l in range(self.n_mg_levels)]
@@ -124,6 +126,29 @@ class MGPCG:
     def bottom_smoothing(self, v):
         self._bottom = int(v)
         _lib.call("flub_mgpcg_set_smoothing", self._h, self._pre_post, self._bottom)
+
+    @property
+    def coarse_scale(self):
+        """Scale of the coarse-grid correction (no reference counterpart): each
+        restriction stores coarse_scale times the children's residual sum.
+        1.0 (the default) is the reference's V-cycle, bitwise."""
+        return self._coarse_scale
+
+    @coarse_scale.setter
+    def coarse_scale(self, v):
+        _lib.call("flub_mgpcg_set_coarse_scale", self._h, float(v))
+        self._coarse_scale = float(v)
+
+    @property
+    def sweep_cap(self):
+        """At most this many pre/post red-black iterations on any level (no
+        reference counterpart; 0, the default: the reference's doubling)."""
+        return self._sweep_cap
+
+    @sweep_cap.setter
+    def sweep_cap(self, v):
+        _lib.call("flub_mgpcg_set_sweep_cap", self._h, int(v))
+        self._sweep_cap = int(v)
 
     def scalars(self):
         """(alpha, beta, sum, old_zTr, rTr) from the device, as floats."""

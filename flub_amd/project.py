@@ -33,12 +33,16 @@ class Projection:
     """Device state of one projection problem on an (nx, ny, nz) grid."""
 
     def __init__(self, shape, n_mg_levels=4, pre_and_post_smoothing=2, bottom_smoothing=6,
-                 precision="fp32", h=None):
+                 precision="fp32", h=None, coarse_scale=1.0, sweep_cap=0):
         self.shape = tuple(int(s) for s in shape)
         self.h = float(h) if h is not None else 1.0 / self.shape[0]
         self.mgpcg = MGPCG(3, self.shape, n_mg_levels, float, precision=precision)
         self.mgpcg.pre_and_post_smoothing = pre_and_post_smoothing
         self.mgpcg.bottom_smoothing = bottom_smoothing
+        if coarse_scale != 1.0:
+            self.mgpcg.coarse_scale = coarse_scale
+        if sweep_cap:
+            self.mgpcg.sweep_cap = sweep_cap
         g = self.shape
         self.vel0 = MACGrid(*(Field(f=wp.zeros(field_shape(d, g)), d=d) for d in (1, 2, 3)))
         self.vel_rigid = MACGrid(*(Field(f=wp.zeros(field_shape(d, g)), d=d) for d in (1, 2, 3)))

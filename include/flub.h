@@ -219,6 +219,19 @@ int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int n_mg_levels,
 int flub_mgpcg_destroy(flub_mgpcg *h);
 /* Attributes pre_and_post_smoothing / bottom_smoothing (mgpcg.py:238-239). */
 int flub_mgpcg_set_smoothing(flub_mgpcg *h, int pre_and_post_smoothing, int bottom_smoothing);
+/* No counterpart in the reference: the scale cs of the coarse-grid correction.
+ * Every restriction stores cs times the sum of the eight children's residuals
+ * (cs = 1, the default, is the reference's un-normalised sum, mgpcg.py:94-101,
+ * and leaves every result bitwise unchanged); with cs < 1 each level's
+ * prolongated correction shrinks by cs (cs = 0.25 is the Galerkin scale of
+ * the piecewise-constant transfers).  Applies to every V-cycle of the handle
+ * (FP32 and mixed, eager, graph and fused solves).  cs in (0, 4]. */
+int flub_mgpcg_set_coarse_scale(flub_mgpcg *h, float cs);
+/* No counterpart in the reference: level l of the V-cycle pre- and
+ * post-smooths min(pre_and_post_smoothing << l, cap) red-black iterations
+ * (cap 0, the default: the reference's doubling per level, mgpcg.py:301,341).
+ * cap in [0, 1024]. */
+int flub_mgpcg_set_sweep_cap(flub_mgpcg *h, int cap);
 /* MGPCG.set_tags (mgpcg.py:360-369): tag (int32, may be NULL to keep level 0)
  * -> packed tags of every level (restrict_tag, mgpcg.py:162-174). */
 int flub_mgpcg_set_tags(flub_mgpcg *h, const int32_t *tag);

@@ -71,6 +71,7 @@ def _declare(L):
             "orc_mgpcg_config": (None, [_P, _I, _I, _I, _I]),
             "orc_mgpcg_buf": (_P, [_P, _I, _I]),
             "orc_mgpcg_growth": (None, [_P, _I]),
+            "orc_mgpcg_cycle": (None, [_P, ctypes.c_double, _I]),
             "orc_mgpcg_mean_f64": (None, [_P, _I]),
             "orc_mgpcg_tag": (_P, [_P, _I]),
             "orc_restrict_tag": (None, [_P, _P, _I, _I, _I]),
@@ -292,6 +293,12 @@ class MGPCG:
     def set_growth(self, growth):
         """Smoothing count growth per level (1: the reference's << l; 0: constant)."""
         _fn("orc_mgpcg_growth", self.real)(self._h, int(growth))
+
+    def set_cycle(self, coarse_scale=1.0, sweep_cap=0):
+        """The V-cycle variant without a reference counterpart: restricted
+        residuals times coarse_scale, at most sweep_cap sweeps per level
+        (1.0 and 0: the reference)."""
+        _fn("orc_mgpcg_cycle", self.real)(self._h, float(coarse_scale), int(sweep_cap))
 
     def level_shape(self, l):
         return tuple(s >> l for s in self.shape)

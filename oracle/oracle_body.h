@@ -351,6 +351,8 @@ static double FN(inner_par)(const REAL *a, const REAL *b, size_t n)
  * ------------------------------------------------------------------------- */
 typedef struct {
     int L, pre_post, bottom, growth;   /* sweeps at level l: pre_post << (growth * l) */
+    int cap;           /* > 0: at most cap sweeps on any level (no reference counterpart) */
+    REAL cs;           /* restricted residuals stored times cs (1: the reference) */
     int nx[ORC_MAXL], ny[ORC_MAXL], nz[ORC_MAXL];
     size_t n[ORC_MAXL];
     REAL *r[ORC_MAXL], *z[ORC_MAXL], *divd[ORC_MAXL];
@@ -370,6 +372,7 @@ FN(orc_mgpcg) *FN(orc_mgpcg_create)(int nx, int ny, int nz, int levels)
     m->pre_post = 6;
     m->bottom = 6;
     m->growth = 1;
+    m->cs = (REAL)1.0;
     m->dot_f64 = 1;
     for (int l = 0; l < levels; ++l) {
         m->nx[l] = nx >> l; m->ny[l] = ny >> l; m->nz[l] = nz >> l;
@@ -405,6 +408,19 @@ void FN(orc_mgpcg_mean_f64)(FN(orc_mgpcg) *m, int mean_f64) { m->mean_f64 = mean
 /* Smoothing growth per level: 1 = the reference's pre_and_post_smoothing << l
  * (mgpcg.py:304,343), 0 = the same count on every level. */
 void FN(orc_mgpcg_growth)(FN(orc_mgpcg) *m, int growth) { m->growth = growth; }
+
+/* No reference counterpart: the V-cycle variant of the HIP path's
+ * flub_mgpcg_set_coarse_scale / flub_mgpcg_set_sweep_cap.  Each restricted
+ * residual is stored times cs (one rounding of the parent's sum); level l
+ * smooths min(pre_post << (growth * l), cap) iterations when cap > 0.
+ * cs = 1, cap = 0 is the reference's V-cycle. */
+void FN(orc_mgpcg_cycle)(FN(orc_mgpcg) *m, double cs, int cap) { m->cs = (REAL)cs; m->cap = cap; }
+
+static int FN(sweeps)(const FN(orc_mgpcg) *m, int l)
+{
+    const int s = m->pre_post << (m->growth * l);
+    return m->cap > 0 && s > m->cap ? m->cap : s;
+}
 
 REAL *FN(orc_mgpcg_buf)(FN(orc_mgpcg) *m, int which, int level)
 {
@@ -479,7 +495,7 @@ void FN(orc_mgpcg_apply_preconditioner)(FN(orc_mgpcg) *m, int smoother)
 {
     memset(m->z[0], 0, m->n[0] * sizeof(REAL));
     for (int l = 0; l + 1 < m->L; ++l) {
-        for (int it = 0; it < (m->pre_post << (m->growth * l)); ++it) {
+        for (int it = 0; it < FN(sweeps)(m, l); ++it) {
             FN(half_sweep)(m, l, smoother, 0);
             FN(half_sweep)(m, l, smoother, 1);
         }
@@ -490,6 +506,8 @@ void FN(orc_mgpcg_apply_preconditioner)(FN(orc_mgpcg) *m, int smoother)
         else
             FN(orc_restrict_mcadams)(m->r[l], m->z[l], m->tag[l], m->r[l + 1], m->tag[l + 1],
                                      m->nx[l], m->ny[l], m->nz[l]);
+        if (m->cs != (REAL)1.0)
+            for (size_t I = 0; I < m->n[l + 1]; ++I) m->r[l + 1][I] = m->r[l + 1][I] * m->cs;
     }
     const int lb = m->L - 1;
     for (int it = 0; it < m->bottom / 2; ++it) {
@@ -502,7 +520,7 @@ void FN(orc_mgpcg_apply_preconditioner)(FN(orc_mgpcg) *m, int smoother)
     }
     for (int l = m->L - 2; l >= 0; --l) {
         FN(orc_prolongate)(m->z[l], m->z[l + 1], m->nx[l], m->ny[l], m->nz[l]);
-        for (int it = 0; it < (m->pre_post << (m->growth * l)); ++it) {
+        for (int it = 0; it < FN(sweeps)(m, l); ++it) {
             FN(half_sweep)(m, l, smoother, 1);
             FN(half_sweep)(m, l, smoother, 0);
         }
