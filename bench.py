@@ -11,7 +11,11 @@ before every step.
 Solvers, each timed over the same steps: "fp32" (the reference MGPCG:
 4-level V-cycle, 2 RB-GS pre/post sweeps with the reference's doubling per
 level, 6 bottom sweeps), "fp16" / "bf16" (the same V-cycle with FP16/BF16
-storage inside FP32 flexible CG with FP64 reductions) and "learned" (U-Net
+storage inside FP32 flexible CG with FP64 reductions), "fp32-tuned" /
+"bf16-tuned" (the tuned V-cycle, no reference counterpart: levels down to
+4^3, i.e. 7 at 256^3, the coarse-grid correction scaled by 1/2, at most 3
+red-black iterations per level, the smallest levels in one workgroup; the
+same smoother, transfers and stop rule) and "learned" (U-Net
 preconditioner in flexible PCG with V-cycle fallback, weights
 flub_amd/learned/weights/unet_128.npz).  Every solver keeps x in FP64 and
 stops when the FP64 true residual ||b - A x|| of that x is below 1e-6 ||b||
@@ -25,7 +29,8 @@ velocity field.  "learned_heldout":
 the learned solver against plain MGPCG and V-cycle FPCG on held-out seeds at
 128^3 and 256^3.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--solvers fp32 fp16 bf16 learned]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
+                    [--solvers fp32 fp16 bf16 fp32-tuned bf16-tuned learned]
                     [--no-cpu] [--no-judge] [--no-heldout] [--no-configs]
 
 "configs" (after the headline, unless --no-configs) holds two more records of
@@ -80,7 +85,7 @@ def main():
     ap.add_argument("--workload", choices=["flow_past_sphere", "droplets_box"], default="flow_past_sphere")
     ap.add_argument("--n", type=int, default=0, help="grid size (default 256, or 512 for droplets_box)")
     ap.add_argument("--levels", type=int, default=4)
-    ap.add_argument("--solvers", nargs="+", default=["fp32", "fp16", "bf16", "learned"])
+    ap.add_argument("--solvers", nargs="+", default=["fp32", "fp16", "bf16", "fp32-tuned", "bf16-tuned", "learned"])
     ap.add_argument("--learned-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-judge", action="store_true", help="skip the FP64 CPU oracle projection (error report)")
@@ -98,7 +103,7 @@ def main():
         sys.exit(2)
 
     from flub_amd import _lib, device, scenes
-    from flub_amd.project import Projection
+    from flub_amd.project import Projection, tuned_levels
 
     _lib.lib()
     if args.workload == "droplets_box":
@@ -132,7 +137,7 @@ def main():
     for name in args.solvers:
         if name == "learned":
             continue
-        pr = Projection((n, n, n), n_mg_levels=L, pre_and_post_smoothing=2, bottom_smoothing=6, precision=name)
+        pr = make_projection(name, n, L)
         pr.load(s)
         dst = [pr.vel0.u.f, pr.vel0.v.f, pr.vel0.w.f]
         times, iters = [], []
@@ -197,11 +202,13 @@ def main():
         "metric": "ms per pressure projection to 1e-6 rel. residual @256^3: learned/mixed vs MGPCG",
         "value": ms, "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32" if best in ("fp32",) else ("bf16+f32" if best in ("bf16", "learned") else "f16+f32"),
+        "dtype": ("f32" if best.startswith("fp32") else
+                  ("bf16+f32" if best.startswith("bf16") or best == "learned" else "f16+f32")),
         "data": "synthetic (seeded scene)",
         "config": {"workload": f"flow past sphere {n}^3: divergence, pressure solve to 1e-6 rel. residual, "
-                               f"gradient subtraction; MGPCG {L} levels, 2 RB-GS pre/post (doubling per level), "
-                               f"6 bottom", "grid": n, "best_solver": best},
+                               f"gradient subtraction; reference MGPCG {L} levels, 2 RB-GS pre/post (doubling per "
+                               f"level), 6 bottom; tuned cycle {tuned_levels((n, n, n))} levels, coarse correction "
+                               f"x1/2, at most 3 RB-GS per level", "grid": n, "best_solver": best},
         "solvers": solvers,
         "accuracy_rule": "every leg: x in FP64, stop when the FP64 true residual ||b - A x|| < 1e-6 ||b|| "
                          "(the reference's test; r replaced by the true residual and the solve continued "
@@ -215,6 +222,19 @@ def main():
     if "fp32" in solvers:
         out["speedup_vs_mgpcg_fp32"] = {k: round(solvers["fp32"]["ms"] / v["ms"], 3) for k, v in solvers.items()}
     print(json.dumps(out), flush=True)
+
+
+def make_projection(name, n, L):
+    """The projection of a solver leg: "fp32" / "fp16" / "bf16" run the
+    reference V-cycle (L levels, 2 pre/post sweeps doubling per level, 6
+    bottom); "<precision>-tuned" the tuned cycle (flub_amd.project.TUNED_CYCLE,
+    levels down to 4^3)."""
+    from flub_amd.project import TUNED_CYCLE, Projection, tuned_levels
+    prec, _, variant = name.partition("-")
+    if variant == "tuned":
+        return Projection((n, n, n), n_mg_levels=tuned_levels((n, n, n)), pre_and_post_smoothing=2,
+                          bottom_smoothing=6, precision=prec, **TUNED_CYCLE)
+    return Projection((n, n, n), n_mg_levels=L, pre_and_post_smoothing=2, bottom_smoothing=6, precision=prec)
 
 
 def kernels_only_other(args, s, n, reps=3):
@@ -455,8 +475,10 @@ def droplets_box(n, L, steps, warmup):
     fluid = torch.from_numpy(tag == 1).cuda()
     net = load_net(os.path.join(ROOT, "flub_amd", "learned", "weights", "unet_128.npz"))
     out = {}
-    for name, prec, learned in (("fp32", "fp32", False), ("bf16", "bf16", False), ("learned+bf16", "bf16", True)):
-        pr = Projection((n, n, n), n_mg_levels=L, pre_and_post_smoothing=2, bottom_smoothing=6, precision=prec)
+    legs = ("fp32", "bf16", "bf16-tuned", "learned+bf16")
+    for name in legs:
+        learned = name.startswith("learned")
+        pr = make_projection(name.split("+")[-1], n, L)
         pr.load(s)
         pr.set_bodies(s["bodies"])
         solver = LearnedPCG(pr.mgpcg, net) if learned else None
@@ -529,7 +551,7 @@ def droplets_box(n, L, steps, warmup):
                    "mfma_frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4)}
     del r
     torch.cuda.empty_cache()
-    best = min(("fp32", "bf16", "learned+bf16"), key=lambda k: out[k]["ms"])
+    best = min(legs, key=lambda k: out[k]["ms"])
     return {"metric": f"ms per pressure projection to 1e-6 rel. residual @{n}^3 with a moving solid box "
                       f"(face-fraction RHS): learned+mixed vs MGPCG",
             "value": out[best]["ms"], "unit": "ms", "n_gpus": 1, "steps": steps, "warmup": warmup,
@@ -537,9 +559,8 @@ def droplets_box(n, L, steps, warmup):
             "config": {"workload": f"dam break + 200 droplets + rotated moving box {n}^3: face fractions, "
                                    f"rigid face velocities, fractional divergence, solve, gradient "
                                    f"subtraction", "grid": n, "best_solver": best},
-            "solvers": {k: out[k] for k in ("fp32", "bf16", "learned+bf16")},
-            "speedup_vs_mgpcg_fp32": {k: round(out["fp32"]["ms"] / out[k]["ms"], 3)
-                                      for k in ("fp32", "bf16", "learned+bf16")},
+            "solvers": {k: out[k] for k in legs},
+            "speedup_vs_mgpcg_fp32": {k: round(out["fp32"]["ms"] / out[k]["ms"], 3) for k in legs},
             "geometry": out["geometry"], "divergence_fractional": out["divergence_fractional"],
             "extrapolate": out["extrapolate"], "advect": out["advect"],
             "unet": out["unet"], "kernels": out["kernels"]}

@@ -66,7 +66,9 @@ __device__ __forceinline__ float tb_update(uint32_t cd, float z, float r, float 
         s = s + (keep_if(xs[2 * a] - z, mp) + keep_if(xs[2 * a + 1] - z, mm));
     }
     const float q = div_small_wave(r + s, (float)((cd >> 12) & 7u), y);
-    return keep_if(z + q, (uint32_t)((int32_t)(cd << 16) >> 31));   // +0 unless FLUID
+    // +0 unless divd != 0: a FLUID cell with no AIR/FLUID neighbour has divd 0
+    // and is set to 0 as well (mgpcg.py:80-83), whatever a prolongation added
+    return keep_if(z + q, 0u - (uint32_t)(((cd >> 12) & 7u) != 0u));
 }
 
 // Kernel layout of a pair entry in `order`: LDS index of its even cell (bits

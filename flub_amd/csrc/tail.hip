@@ -23,6 +23,7 @@
 #include "mgpcg_internal.h"
 #include "smooth.h"
 #include "stencil.h"
+#include "div_small.h"
 
 namespace {
 
@@ -35,6 +36,7 @@ struct TailArgs {
     float *rg[TAIL_MAXL];   // every level's r (the restricted ones written back)
     const uint16_t *code[TAIL_MAXL];
     int nx[TAIL_MAXL], ny[TAIL_MAXL], nz[TAIL_MAXL];
+    int ly[TAIL_MAXL], lz[TAIL_MAXL];   // log2 of ny, nz (every extent a power of two)
     uint32_t off[TAIL_MAXL];   // LDS offset of each level's cells
     int sweeps[TAIL_MAXL];     // pre/post red-black iterations (all levels but the last)
     float cs[TAIL_MAXL];       // scale of the restriction into each level (from index 1)
@@ -44,24 +46,45 @@ struct TailArgs {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// One half-sweep of a level held in LDS.
-__device__ void tail_half(float *z, const float *r, const uint16_t *cd, int nx, int ny, int nz, int phase)
+// One half-sweep of a level held in LDS (extents powers of two: ny = 2^ly,
+// nz = 2^lz).  A thread takes up to TAIL_CPT cells of the colour, all their
+// LDS reads issued before the first update.  The quotient by divd is
+// div_small_wave with y = RN(1/divd), bitwise the IEEE division of gs_update.
+constexpr int TAIL_CPT = (int)(TAIL_MAX_CELLS / 2 / TAIL_NT);
+
+__device__ __forceinline__ void tail_half(float *z, const float *r, const uint16_t *cd, const float *ytab, int nx,
+                                          int ny, int nz, int ly, int lz, int phase)
 {
-    const int nzh = nz >> 1;
-    const uint32_t n = (uint32_t)nx * ny * nzh;
-    for (uint32_t t = threadIdx.x; t < n; t += TAIL_NT) {
-        const int kk = (int)(t % (uint32_t)nzh);
-        const uint32_t row = t / (uint32_t)nzh;
-        const int j = (int)(row % (uint32_t)ny), i = (int)(row / (uint32_t)ny);
+    const uint32_t n = (uint32_t)nx << (ly + lz - 1);
+    uint32_t I[TAIL_CPT], c[TAIL_CPT];
+    float v[TAIL_CPT][8];
+#pragma unroll
+    for (int e = 0; e < TAIL_CPT; ++e) {
+        const uint32_t t = threadIdx.x + (uint32_t)e * TAIL_NT;
+        if (t >= n) break;
+        const int kk = (int)(t & ((1u << (lz - 1)) - 1u));
+        const int j = (int)((t >> (lz - 1)) & (uint32_t)(ny - 1)), i = (int)(t >> (ly + lz - 1));
         const int k = 2 * kk + ((((i + j) & 1) == phase) ? 1 : 0);
-        const uint32_t I = cidx(i, j, k, ny, nz);
-        const float xip = z[cidx(clampi(i + 1, 0, nx - 1), j, k, ny, nz)];
-        const float xim = z[cidx(clampi(i - 1, 0, nx - 1), j, k, ny, nz)];
-        const float xjp = z[cidx(i, clampi(j + 1, 0, ny - 1), k, ny, nz)];
-        const float xjm = z[cidx(i, clampi(j - 1, 0, ny - 1), k, ny, nz)];
-        const float xkp = z[cidx(i, j, clampi(k + 1, 0, nz - 1), ny, nz)];
-        const float xkm = z[cidx(i, j, clampi(k - 1, 0, nz - 1), ny, nz)];
-        z[I] = gs_update<false, false>(cd[I], z[I], r[I], xip, xim, xjp, xjm, xkp, xkm);
+        const uint32_t x = ((uint32_t)i << (ly + lz)) + ((uint32_t)j << lz) + (uint32_t)k;
+        I[e] = x;
+        c[e] = cd[x];
+        v[e][0] = z[x];
+        v[e][1] = z[i + 1 < nx ? x + (1u << (ly + lz)) : x];
+        v[e][2] = z[i > 0 ? x - (1u << (ly + lz)) : x];
+        v[e][3] = z[j + 1 < ny ? x + (1u << lz) : x];
+        v[e][4] = z[j > 0 ? x - (1u << lz) : x];
+        v[e][5] = z[k + 1 < nz ? x + 1u : x];
+        v[e][6] = z[k > 0 ? x - 1u : x];
+        v[e][7] = r[x];
+    }
+#pragma unroll
+    for (int e = 0; e < TAIL_CPT; ++e) {
+        const uint32_t t = threadIdx.x + (uint32_t)e * TAIL_NT;
+        if (t >= n) break;
+        const float s = nsum_code(c[e], v[e][0], v[e][1], v[e][2], v[e][3], v[e][4], v[e][5], v[e][6]);
+        const uint32_t divd = (c[e] >> 12) & 7u;
+        const float q = div_small_wave(v[e][7] + s, (float)divd, ytab[divd]);
+        z[I[e]] = keep_if(v[e][0] + q, 0u - (uint32_t)(divd != 0u));
     }
 }
 
@@ -69,14 +92,40 @@ __global__ void __launch_bounds__(TAIL_NT) k_tail(TailArgs a)
 {
     __shared__ float zs[TAIL_LDS], rs[TAIL_LDS];
     __shared__ uint16_t cs[TAIL_LDS];
-    // stage the codes of every level, r of level T; z of level T starts at 0
-    for (int l = 0; l < a.nl; ++l) {
-        const uint32_t n = (uint32_t)a.nx[l] * a.ny[l] * a.nz[l];
-        for (uint32_t t = threadIdx.x; t < n; t += TAIL_NT) cs[a.off[l] + t] = a.code[l][t];
+    __shared__ float ytab[8];
+    if (threadIdx.x < 8) {   // RN(1/d), d = 0..7 (d = 0: the quotient is discarded)
+        const float t[8] = {0.0f, 1.0f, 0.5f, 1.0f / 3.0f, 0.25f, 0.2f, 1.0f / 6.0f, 1.0f / 7.0f};
+        ytab[threadIdx.x] = t[threadIdx.x];
     }
+    // stage the codes of every level, r of level T (z of level T starts at 0),
+    // every global load of a thread issued before its first LDS store
     {
-        const uint32_t n = (uint32_t)a.nx[0] * a.ny[0] * a.nz[0];
-        for (uint32_t t = threadIdx.x; t < n; t += TAIL_NT) { rs[t] = a.r0[t]; zs[t] = 0.0f; }
+        constexpr int NL = (int)(TAIL_MAX_CELLS / TAIL_NT);
+        const uint32_t n0 = (uint32_t)a.nx[0] * a.ny[0] * a.nz[0];
+        float rv[NL];
+        uint32_t cv[NL];
+#pragma unroll
+        for (int e = 0; e < NL; ++e) {
+            const uint32_t t = threadIdx.x + (uint32_t)e * TAIL_NT;
+            if (t < n0) { rv[e] = a.r0[t]; cv[e] = a.code[0][t]; }
+        }
+        static_assert(TAIL_MAX_CELLS / 8 <= (uint32_t)TAIL_NT, "one coarse cell per thread");
+        uint32_t cw[TAIL_MAXL];
+#pragma unroll
+        for (int l = 1; l < TAIL_MAXL; ++l) {   // the coarser levels: at most TAIL_MAX_CELLS / 8 cells
+            const uint32_t t = threadIdx.x;
+            if (l < a.nl && t < (uint32_t)a.nx[l] * a.ny[l] * a.nz[l]) cw[l] = a.code[l][t];
+        }
+#pragma unroll
+        for (int e = 0; e < NL; ++e) {
+            const uint32_t t = threadIdx.x + (uint32_t)e * TAIL_NT;
+            if (t < n0) { rs[t] = rv[e]; zs[t] = 0.0f; cs[t] = (uint16_t)cv[e]; }
+        }
+#pragma unroll
+        for (int l = 1; l < TAIL_MAXL; ++l) {
+            const uint32_t t = threadIdx.x;
+            if (l < a.nl && t < (uint32_t)a.nx[l] * a.ny[l] * a.nz[l]) cs[a.off[l] + t] = (uint16_t)cw[l];
+        }
     }
     __syncthreads();
     // down: pre-smoothing, restriction into the next level (its z zeroed)
@@ -86,9 +135,9 @@ __global__ void __launch_bounds__(TAIL_NT) k_tail(TailArgs a)
         const uint16_t *cd = cs + a.off[l];
         const int nx = a.nx[l], ny = a.ny[l], nz = a.nz[l];
         for (int it = 0; it < a.sweeps[l]; ++it) {
-            tail_half(z, r, cd, nx, ny, nz, 0);
+            tail_half(z, r, cd, ytab, nx, ny, nz, a.ly[l], a.lz[l], 0);
             __syncthreads();
-            tail_half(z, r, cd, nx, ny, nz, 1);
+            tail_half(z, r, cd, ytab, nx, ny, nz, a.ly[l], a.lz[l], 1);
             __syncthreads();
         }
         const int cy = a.ny[l + 1], cz = a.nz[l + 1];
@@ -96,10 +145,10 @@ __global__ void __launch_bounds__(TAIL_NT) k_tail(TailArgs a)
         float *zc = zs + a.off[l + 1], *rc = rs + a.off[l + 1];
         const uint16_t *cdc = cs + a.off[l + 1];
         const float scale = a.cs[l + 1];
+        const int lcy = a.ly[l + 1], lcz = a.lz[l + 1];
         for (uint32_t t = threadIdx.x; t < nc; t += TAIL_NT) {
-            const int K = (int)(t % (uint32_t)cz);
-            const uint32_t row = t / (uint32_t)cz;
-            const int J = (int)(row % (uint32_t)cy), I = (int)(row / (uint32_t)cy);
+            const int K = (int)(t & (uint32_t)(cz - 1));
+            const int J = (int)((t >> lcz) & (uint32_t)(cy - 1)), I = (int)(t >> (lcy + lcz));
             float s = 0.0f;
             if (code_fluid(cdc[t])) {
                 for (int ii = 0; ii < 2; ++ii)
@@ -133,9 +182,9 @@ __global__ void __launch_bounds__(TAIL_NT) k_tail(TailArgs a)
         const uint16_t *cd = cs + a.off[l];
         for (int it = 0; it < 2 * a.nb; ++it) {
             const int first = it < a.nb ? 0 : 1;
-            tail_half(z, r, cd, a.nx[l], a.ny[l], a.nz[l], first);
+            tail_half(z, r, cd, ytab, a.nx[l], a.ny[l], a.nz[l], a.ly[l], a.lz[l], first);
             __syncthreads();
-            tail_half(z, r, cd, a.nx[l], a.ny[l], a.nz[l], first ^ 1);
+            tail_half(z, r, cd, ytab, a.nx[l], a.ny[l], a.nz[l], a.ly[l], a.lz[l], first ^ 1);
             __syncthreads();
         }
     }
@@ -147,17 +196,17 @@ __global__ void __launch_bounds__(TAIL_NT) k_tail(TailArgs a)
         const float *zc = zs + a.off[l + 1];
         const int nx = a.nx[l], ny = a.ny[l], nz = a.nz[l];
         const uint32_t n = (uint32_t)nx * ny * nz;
+        const int ly = a.ly[l], lz = a.lz[l];
         for (uint32_t t = threadIdx.x; t < n; t += TAIL_NT) {
-            const int k = (int)(t % (uint32_t)nz);
-            const uint32_t row = t / (uint32_t)nz;
-            const int j = (int)(row % (uint32_t)ny), i = (int)(row / (uint32_t)ny);
-            z[t] = z[t] + zc[cidx(i >> 1, j >> 1, k >> 1, ny >> 1, nz >> 1)];
+            const int k = (int)(t & (uint32_t)(nz - 1));
+            const int j = (int)((t >> lz) & (uint32_t)(ny - 1)), i = (int)(t >> (ly + lz));
+            z[t] = z[t] + zc[((uint32_t)(i >> 1) << (ly + lz - 2)) + ((uint32_t)(j >> 1) << (lz - 1)) + (uint32_t)(k >> 1)];
         }
         __syncthreads();
         for (int it = 0; it < a.sweeps[l]; ++it) {
-            tail_half(z, r, cd, nx, ny, nz, 1);
+            tail_half(z, r, cd, ytab, nx, ny, nz, a.ly[l], a.lz[l], 1);
             __syncthreads();
-            tail_half(z, r, cd, nx, ny, nz, 0);
+            tail_half(z, r, cd, ytab, nx, ny, nz, a.ly[l], a.lz[l], 0);
             __syncthreads();
         }
     }
@@ -179,8 +228,14 @@ int tail_first_level(const flub_mgpcg *h, int lowest)
     // the tail starts at the first level (>= lowest, >= 1) of at most
     // TAIL_MAX_CELLS cells, if the levels from there down fit its tables
     if (h->no_tail || h->L < 2) return -1;
+    auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
     for (int l = lowest < 1 ? 1 : lowest; l < h->L; ++l)
-        if (h->lv[l].n <= TAIL_MAX_CELLS) return h->L - l <= TAIL_MAXL ? l : -1;
+        if (h->lv[l].n <= TAIL_MAX_CELLS) {
+            if (h->L - l > TAIL_MAXL) return -1;
+            for (int q = l; q < h->L; ++q)
+                if (!pow2(h->lv[q].nx) || !pow2(h->lv[q].ny) || !pow2(h->lv[q].nz)) return -1;
+            return l;
+        }
     return -1;
 }
 
@@ -199,6 +254,7 @@ int tail_cycle(hipStream_t s, flub_mgpcg *h, int first)
         a.rg[q] = L.r;
         a.code[q] = L.code;
         a.nx[q] = L.nx; a.ny[q] = L.ny; a.nz[q] = L.nz;
+        a.ly[q] = __builtin_ctz((unsigned)L.ny); a.lz[q] = __builtin_ctz((unsigned)L.nz);
         a.off[q] = off;
         a.sweeps[q] = mg_sweeps(h, first + q);
         a.cs[q] = L.cs;

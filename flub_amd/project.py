@@ -29,6 +29,25 @@ def _field(arr, d):
     return Field(f=wp.from_numpy(np.asarray(arr, np.float32)), d=d)
 
 
+# The tuned V-cycle (no reference counterpart): levels down to 4 cells on the
+# shortest axis, the coarse-grid correction scaled by 1/2 (each restriction
+# stores half the children's residual sum), at most 3 red-black iterations of
+# pre- and post-smoothing on any level; same smoother, transfers and bottom
+# schedule as the reference's MGPCG otherwise.
+TUNED_CYCLE = {"coarse_scale": 0.5, "sweep_cap": 3}
+
+
+def tuned_levels(shape):
+    """Levels of the tuned cycle for a grid: halve until the shortest axis has
+    4 cells (every restricted level even, as the V-cycle requires)."""
+    m = min(int(s) for s in shape)
+    levels = 1
+    while levels < 12 and m % 2 == 0 and m // 2 >= 4 and all((int(s) >> (levels - 1)) % 2 == 0 for s in shape):
+        m //= 2
+        levels += 1
+    return levels
+
+
 class Projection:
     """Device state of one projection problem on an (nx, ny, nz) grid."""
 

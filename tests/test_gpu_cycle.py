@@ -96,3 +96,43 @@ def test_variant_cycle_projection(name, make, precision, oracle):
           f"variant {res['variant'][0]} (error {res['variant'][1]:.2e})")
     assert res["variant"][1] < 1e-5
     assert res["variant"][0] < res["reference"][0]
+
+
+def _isolated_pocket_tags(n=64):
+    """SOLID everywhere but a FLUID block (i < n/2) with an AIR face (i = 0),
+    a 2x2x2 FLUID pocket whose level-1 cell P = (20, 20, 20) has only SOLID
+    face neighbours (divd 0 on level 1), and a line of FLUID level-1 cells
+    (16..21, 21, 21) that ties P's level-2 parent to the block."""
+    tag = np.full((n, n, n), 2, np.int32)
+    tag[: n // 2] = 1
+    tag[0] = 0
+    tag[40:42, 40:42, 40:42] = 1                      # the pocket: level-1 cell (20, 20, 20)
+    tag[32:44, 42:44, 42:44] = 1                      # level-1 cells (16..21, 21, 21)
+    return tag
+
+
+def test_isolated_coarse_fluid_cell_bitexact(oracle):
+    """A FLUID cell with no AIR/FLUID neighbour has divd 0 and the reference
+    sets its z to 0 on every sweep (mgpcg.py:80-83), also after a
+    prolongation added its parent's value; the children of that cell then
+    receive 0.  The V-cycle paths (split, reference layout, blocked and tail
+    levels) follow it bitwise."""
+    tag = _isolated_pocket_tags()
+    t1 = oracle.restrict_tag(tag)
+    assert t1[20, 20, 20] == 1 and oracle.compute_divd(t1)[20, 20, 20] == 0
+    assert oracle.restrict_tag(t1)[10, 10, 10] == 1
+    r = np.random.default_rng(23).normal(0, 1, tag.shape).astype(np.float32)
+    r[tag != 1] = 0.0
+    orc = oracle.MGPCG(tag.shape, 4, pre_and_post_smoothing=2, bottom_smoothing=6)
+    orc.set_tags(tag)
+    orc.put("r", r, 0)
+    orc.apply_preconditioner("phases")
+    want = orc.get("z", 0)
+    assert orc.get("z", 2)[10, 10, 10] != 0.0      # the parent's value the prolongation passes down
+    assert bits_equal(_precondition(tag, r, 4, 1.0, 0), want)
+    mg = MGPCG(3, tag.shape, 4)
+    mg.pre_and_post_smoothing, mg.bottom_smoothing = 2, 6
+    mg.set_tags(wp.from_numpy(tag))
+    mg.r[0].assign(r)
+    mg.apply_preconditioner("phases")
+    assert bits_equal(mg.z[0].numpy(), want)
