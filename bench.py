@@ -66,11 +66,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 MFMA_BF16_PEAK_TFLOPS = 2500.0
-# HBM bytes per launch of the level-0 FP32 half-sweeps at 256^3, from two
+# HBM bytes per launch of the level-0 half-sweeps at 256^3, from two
 # rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of `bench.py --kernels-only
 # [--kernel ...]`, calibrated on a 67.4 MB device copy
-# (profiles/rb_half_sweep_l0_split_traffic_256.md, profiles/rb_half_sweep_l0_traffic_256.md)
-MEASURED_TRAFFIC_L0 = {("rb_half_sweep_l0_split", 256): 153.8e6, ("rb_half_sweep_l0", 256): 242.7e6}
+# (profiles/rb_half_sweep_l0_split_traffic_256.md, profiles/rb_half_sweep_l0_traffic_256.md,
+# profiles/rb_half_sweep_l0_split_bf16r_traffic_256.md)
+MEASURED_TRAFFIC_L0 = {("rb_half_sweep_l0_split", 256): 153.8e6, ("rb_half_sweep_l0", 256): 242.7e6,
+                       ("rb_half_sweep_l0_split_bf16r", 256): 137.4e6}
 
 
 def log(*a):
@@ -95,8 +97,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--kernels-only", action="store_true",
                     help="only replay the level-0 half-sweep (for counter collection)")
-    ap.add_argument("--kernel", choices=["split", "reference-layout", "unet", "advect"], default="split",
-                    help="what --kernels-only replays: a level-0 half-sweep, one U-Net forward, or advect_3d")
+    ap.add_argument("--kernel", choices=["split", "split-bf16", "reference-layout", "unet", "advect"], default="split",
+                    help="what --kernels-only replays: a level-0 half-sweep (FP32 solve, BF16 solve's "
+                         "post-smoothing, reference layout), one U-Net forward, or advect_3d")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
@@ -117,20 +120,24 @@ def main():
     if args.kernels_only and args.kernel in ("unet", "advect"):
         return kernels_only_other(args, s, n)
     if args.kernels_only:
-        pr = Projection((n, n, n), n_mg_levels=L, pre_and_post_smoothing=2, bottom_smoothing=6)
+        mixed = args.kernel == "split-bf16"
+        pr = Projection((n, n, n), n_mg_levels=L, pre_and_post_smoothing=2, bottom_smoothing=6,
+                        precision="bf16" if mixed else "fp32")
         pr.load(s)
         pr.mgpcg.set_tags(pr.tag.f)
         # a copy of known size (calibrates FETCH_SIZE / WRITE_SIZE)
         device.copy(pr.vel0.u.f, src[0])
         device.synchronize()
         avg, byt = ctypes.c_float(), ctypes.c_double()
-        kid, kname = (8, "rb_half_sweep_l0_split") if args.kernel == "split" else (0, "rb_half_sweep_l0")
+        kid, kname = {"split": (8, "rb_half_sweep_l0_split"), "split-bf16": (12, "rb_half_sweep_l0_split_bf16r"),
+                      "reference-layout": (0, "rb_half_sweep_l0")}[args.kernel]
         _lib.call("flub_mgpcg_profile", pr.mgpcg.handle, kid, 20, ctypes.byref(avg), ctypes.byref(byt))
         print(json.dumps({"kernel": kname, "ms": avg.value, "bytes": byt.value, "copy_bytes": src[0].nbytes}))
         return
 
     solvers = {}
     kernels = {}
+    mixed_sweep = {}   # per BF16 leg: its level-0 post-smoothing half-sweep replayed
     outputs = {}      # per solver: pressure and velocities, for the FP64 judge
     in_solve = None
     last_pr = None
@@ -159,6 +166,8 @@ def main():
         if name == "fp32":
             kernels = profile_kernels(pr)
             in_solve = probe_in_solve(pr, src)
+        if name.startswith("bf16"):
+            mixed_sweep[name] = profile_one(pr, 12)
         last_pr = pr
 
     if "learned" in args.solvers:
@@ -169,26 +178,41 @@ def main():
 
     best = min(solvers, key=lambda k: solvers[k]["ms"])
     ms = solvers[best]["ms"]
-    # the dominant kernel: the level-0 half-sweep the solve runs (colour-split when available)
+    # the dominant kernel of the fastest leg: the level-0 half-sweep its solve
+    # runs most (colour-split; in the BF16 solves the FP32 post-smoothing sweep
+    # with BF16 r), replayed with HIP events on the solver's stream (it agrees
+    # with the rocprofv3 kernel-trace average of those replays).  fp32_sweep:
+    # the FP32 solve's half-sweep, with in_solve = HIP events around its
+    # launches inside a projection's V-cycles (each interval also holds the
+    # dispatch after the previous kernel, so it bounds the in-solve kernel time
+    # from above).
     dom_name = "rb_half_sweep_l0_split" if "rb_half_sweep_l0_split" in kernels else "rb_half_sweep_l0"
     dom = kernels.get(dom_name)
-    roofline = None
+    fp32_sweep = None
     if dom:
-        # frac: HIP events around back-to-back replays of the kernel on the
-        # solver's stream (the kernel alone; it agrees with the rocprofv3
-        # kernel-trace average of those replays).  in_solve: HIP events around
-        # its launches inside a projection's V-cycles -- each interval also
-        # holds the dispatch after the previous kernel, so it bounds the
-        # in-solve kernel time from above.
-        roofline = {"bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC_L0.get((dom_name, n)),
-                    "kernel": ("k_ssweep level-0 red-black half-sweep, colour-split layout"
-                               if dom_name.endswith("split") else "k_sweep<float> level-0 red-black half-sweep")
-                              + " (9 B/cell algorithmic)",
-                    "bytes_per_launch": dom["bytes"], "avg_ms": dom["ms"]}
+        fp32_sweep = {"kernel": ("k_ssweep level-0 red-black half-sweep, colour-split layout"
+                                 if dom_name.endswith("split") else "k_sweep<float> level-0 red-black half-sweep")
+                                + " (9 B/cell algorithmic)", "bytes_per_launch": dom["bytes"], "avg_ms": dom["ms"],
+                      "achieved": dom["GBs"], "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4),
+                      "traffic": MEASURED_TRAFFIC_L0.get((dom_name, n))}
         if in_solve:
             gbs = dom["bytes"] / (in_solve["avg_ms"] * 1e-3) / 1e9
-            roofline["in_solve"] = {**in_solve, "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+            fp32_sweep["in_solve"] = {**in_solve, "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    roofline = None
+    if best in mixed_sweep and mixed_sweep[best]:
+        d = mixed_sweep[best]
+        roofline = {"bound": "hbm", "achieved": d["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(d["GBs"] / HBM_PEAK_GBS, 4),
+                    "traffic": MEASURED_TRAFFIC_L0.get(("rb_half_sweep_l0_split_bf16r", n)),
+                    "kernel": "k_ssweep<PLAIN, STORE, float, float, bf16> level-0 post-smoothing red-black "
+                              "half-sweep of the BF16 solve, colour-split layout (8 B/cell algorithmic: FP32 z, "
+                              "BF16 r)", "leg": best, "bytes_per_launch": d["bytes"], "avg_ms": d["ms"],
+                    "fp32_sweep": fp32_sweep}
+    elif fp32_sweep:
+        roofline = {"bound": "hbm", "achieved": fp32_sweep["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": fp32_sweep["frac"], "traffic": fp32_sweep["traffic"], "kernel": fp32_sweep["kernel"],
+                    "leg": "fp32", "bytes_per_launch": fp32_sweep["bytes_per_launch"], "avg_ms": fp32_sweep["avg_ms"],
+                    "in_solve": fp32_sweep.get("in_solve")}
     heldout = None
     if "learned" in args.solvers and not args.no_heldout:
         heldout = learned_heldout()
@@ -347,6 +371,21 @@ def probe_in_solve(pr, src):
     out = {"avg_ms": round(avg.value, 5), "min_ms": round(mn.value, 5), "launches": cnt.value}
     log(f"in-solve level-0 half-sweep: {out}")
     return out
+
+
+def profile_one(pr, kid, reps=20):
+    """One kernel of a handle replayed with HIP events (flub_mgpcg_profile):
+    ms per launch, algorithmic bytes, GB/s."""
+    from flub_amd import _lib
+    avg, byt = ctypes.c_float(), ctypes.c_double()
+    try:
+        _lib.call("flub_mgpcg_profile", pr.mgpcg.handle, kid, reps, ctypes.byref(avg), ctypes.byref(byt))
+    except _lib.FlubError as e:
+        log(f"kernel {kid}: {e}")
+        return None
+    ent = {"ms": round(avg.value, 5), "bytes": byt.value, "GBs": round(byt.value / (avg.value * 1e-3) / 1e9, 1)}
+    log(f"kernel {kid}: {ent}")
+    return ent
 
 
 def profile_kernels(pr):

@@ -1847,6 +1847,20 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
             k_update_rx_dot<false><<<nb, RBS, 0, h->s>>>(L0.r, nullptr, h->Ap, nullptr, h->sc, L0.n, h->part);
             FLUB_CHECK_LAUNCH();
             return launch_finish(h, nb, FIN_RTR, 0.0);
+        case 12: {   // the mixed split solve's level-0 post-smoothing half-sweep: FP32 z, 16-bit r
+            FLUB_REQUIRE(h->split && h->prec != FLUB_PREC_FP32, FLUB_ESTATE,
+                         "no mixed-precision colour-split level 0 for this handle");
+            SplitMixedSweep m{};
+            m.kind = SMX_POST;
+            m.phase = 0;
+            m.zin = h->sp.z[0];
+            m.zout = h->sp.z[0];
+            m.rh = h->sp.rh;
+            m.code = h->sp.code;
+            m.r32 = h->sp.r;
+            m.sc = h->sc;
+            return split_sweep_mixed(h->s, L0, h->prec, m);
+        }
         default: flub_set_error("unknown kernel id %d", kernel); return FLUB_EINVAL;
         }
     };
@@ -1863,6 +1877,7 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
     case 9: b = N * (2 * v + 1) + N / 8 * (v + 1); break;
     case 10: b = N * (5 * 4 + 2); break;                     // p, z -> p'; p', codes -> Ap
     case 11: b = N * 3 * 4; break;                           // r, Ap -> r
+    case 12: b = N * 8; break;                               // as 8 with the 16-bit r (own colour)
     default: break;
     }
     int rc = launch_once();   // warm-up (captures the graph for id 3)

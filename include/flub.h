@@ -307,7 +307,9 @@ int flub_mgpcg_level_shape(flub_mgpcg *h, int level, int shape[3]);
  * blocked launch of four level-0 half-sweeps, 8/9 the half-sweep and
  * restriction of the colour-split level 0 the FP32 solve runs on, 10 its p
  * step (p and FP64 x update, then Ap and p.Ap), 11 its r step (r -= alpha Ap
- * and r.r) (8-10: FLUB_ESTATE when the grid has no split level 0).  Leaves the handle's
+ * and r.r) (8-10: FLUB_ESTATE when the grid has no split level 0), 12 the
+ * level-0 post-smoothing half-sweep of the mixed-precision split solve (FP32 z,
+ * FP16/BF16 r; FLUB_ESTATE on an FP32 handle).  Leaves the handle's
  * vectors in an unspecified state (call before init / after reading x). */
 int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_ms, double *bytes);
 /* In-solve timing (the same hook, inside the solve): arms HIP events around
