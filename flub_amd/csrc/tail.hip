@@ -55,33 +55,35 @@ constexpr int TAIL_CPT = (int)(TAIL_MAX_CELLS / 2 / TAIL_NT);
 __device__ __forceinline__ void tail_half(float *z, const float *r, const uint16_t *cd, const float *ytab, int nx,
                                           int ny, int nz, int ly, int lz, int phase)
 {
-    const uint32_t n = (uint32_t)nx << (ly + lz - 1);
+    const uint32_t n = (uint32_t)nx << (ly + lz - 1), last = 2u * n - 1u;
+    const uint32_t si = 1u << (ly + lz), sj = 1u << lz;
     uint32_t I[TAIL_CPT], c[TAIL_CPT];
     float v[TAIL_CPT][8];
 #pragma unroll
     for (int e = 0; e < TAIL_CPT; ++e) {
         const uint32_t t = threadIdx.x + (uint32_t)e * TAIL_NT;
         if (t >= n) break;
-        const int kk = (int)(t & ((1u << (lz - 1)) - 1u));
-        const int j = (int)((t >> (lz - 1)) & (uint32_t)(ny - 1)), i = (int)(t >> (ly + lz - 1));
-        const int k = 2 * kk + ((((i + j) & 1) == phase) ? 1 : 0);
-        const uint32_t x = ((uint32_t)i << (ly + lz)) + ((uint32_t)j << lz) + (uint32_t)k;
+        const uint32_t row = t >> (lz - 1);   // (i, j) row; the cell's k = 2 kk + (1 if (i + j) & 1 == phase)
+        const uint32_t ij = (row >> ly) + (row & (uint32_t)(ny - 1));
+        const uint32_t x = (t << 1) + ((ij & 1u) == (uint32_t)phase ? 1u : 0u);
         I[e] = x;
         c[e] = cd[x];
+        // neighbours outside the level read some cell of it: their class is 0
+        // (masked), so any in-bounds value will do
         v[e][0] = z[x];
-        v[e][1] = z[i + 1 < nx ? x + (1u << (ly + lz)) : x];
-        v[e][2] = z[i > 0 ? x - (1u << (ly + lz)) : x];
-        v[e][3] = z[j + 1 < ny ? x + (1u << lz) : x];
-        v[e][4] = z[j > 0 ? x - (1u << lz) : x];
-        v[e][5] = z[k + 1 < nz ? x + 1u : x];
-        v[e][6] = z[k > 0 ? x - 1u : x];
+        v[e][1] = z[min(x + si, last)];
+        v[e][2] = z[x >= si ? x - si : 0u];
+        v[e][3] = z[min(x + sj, last)];
+        v[e][4] = z[x >= sj ? x - sj : 0u];
+        v[e][5] = z[min(x + 1u, last)];
+        v[e][6] = z[x >= 1u ? x - 1u : 0u];
         v[e][7] = r[x];
     }
 #pragma unroll
     for (int e = 0; e < TAIL_CPT; ++e) {
         const uint32_t t = threadIdx.x + (uint32_t)e * TAIL_NT;
         if (t >= n) break;
-        const float s = nsum_code(c[e], v[e][0], v[e][1], v[e][2], v[e][3], v[e][4], v[e][5], v[e][6]);
+        const float s = nsum_code_air0(c[e], v[e][0], v[e][1], v[e][2], v[e][3], v[e][4], v[e][5], v[e][6]);
         const uint32_t divd = (c[e] >> 12) & 7u;
         const float q = div_small_wave(v[e][7] + s, (float)divd, ytab[divd]);
         z[I[e]] = keep_if(v[e][0] + q, 0u - (uint32_t)(divd != 0u));
@@ -146,6 +148,8 @@ __global__ void __launch_bounds__(TAIL_NT) k_tail(TailArgs a)
         const uint16_t *cdc = cs + a.off[l + 1];
         const float scale = a.cs[l + 1];
         const int lcy = a.ly[l + 1], lcz = a.lz[l + 1];
+        const uint32_t si = 1u << (a.ly[l] + a.lz[l]), sj = 1u << a.lz[l];
+        const uint32_t last = (uint32_t)nx * ny * nz - 1u;
         for (uint32_t t = threadIdx.x; t < nc; t += TAIL_NT) {
             const int K = (int)(t & (uint32_t)(cz - 1));
             const int J = (int)((t >> lcz) & (uint32_t)(cy - 1)), I = (int)(t >> (lcy + lcz));
@@ -159,12 +163,9 @@ __global__ void __launch_bounds__(TAIL_NT) k_tail(TailArgs a)
                             const uint32_t cd0 = cd[c];
                             float nd = 0.0f;
                             if (code_fluid(cd0))
-                                nd = nsum_code(cd0, z[c], z[cidx(clampi(i + 1, 0, nx - 1), j, k, ny, nz)],
-                                               z[cidx(clampi(i - 1, 0, nx - 1), j, k, ny, nz)],
-                                               z[cidx(i, clampi(j + 1, 0, ny - 1), k, ny, nz)],
-                                               z[cidx(i, clampi(j - 1, 0, ny - 1), k, ny, nz)],
-                                               z[cidx(i, j, clampi(k + 1, 0, nz - 1), ny, nz)],
-                                               z[cidx(i, j, clampi(k - 1, 0, nz - 1), ny, nz)]);
+                                nd = nsum_code_air0(cd0, z[c], z[min(c + si, last)], z[c >= si ? c - si : 0u],
+                                                    z[min(c + sj, last)], z[c >= sj ? c - sj : 0u],
+                                                    z[min(c + 1u, last)], z[c >= 1u ? c - 1u : 0u]);
                             s += r[c] - (-nd);
                         }
                 s = s * scale;
@@ -197,10 +198,17 @@ __global__ void __launch_bounds__(TAIL_NT) k_tail(TailArgs a)
         const int nx = a.nx[l], ny = a.ny[l], nz = a.nz[l];
         const uint32_t n = (uint32_t)nx * ny * nz;
         const int ly = a.ly[l], lz = a.lz[l];
+        // With post-smoothing the non-FLUID cells keep +0 here (the sweeps
+        // write +0 to them anyway and no sweep reads their value), which keeps
+        // the invariant of nsum_code_air0; without it they take the sum as in
+        // the reference.
+        const bool all = a.sweeps[l] == 0;
         for (uint32_t t = threadIdx.x; t < n; t += TAIL_NT) {
             const int k = (int)(t & (uint32_t)(nz - 1));
             const int j = (int)((t >> lz) & (uint32_t)(ny - 1)), i = (int)(t >> (ly + lz));
-            z[t] = z[t] + zc[((uint32_t)(i >> 1) << (ly + lz - 2)) + ((uint32_t)(j >> 1) << (lz - 1)) + (uint32_t)(k >> 1)];
+            if (all || code_fluid(cd[t]))
+                z[t] = z[t] + zc[((uint32_t)(i >> 1) << (ly + lz - 2)) + ((uint32_t)(j >> 1) << (lz - 1)) +
+                                 (uint32_t)(k >> 1)];
         }
         __syncthreads();
         for (int it = 0; it < a.sweeps[l]; ++it) {
