@@ -26,3 +26,9 @@ int flub_subtract_gradient_x_u8(float *uf, const float *u_rigid, const float *x,
 // and rounded once.
 int flub_subtract_gradient_x64_u8(float *uf, const float *u_rigid, const double *x, const uint8_t *tag,
                                   int nx, int ny, int nz, int d, void *stream);
+// The three axes of that update in one pass from the FP64 x in the
+// colour-split layout (nz % 4 == 0; u, v and their rigid velocities 16-byte
+// aligned: subtract_gradient3_split_ok).
+bool subtract_gradient3_split_ok(const float *u, const float *v, const float *ur, const float *vr, int nz);
+int subtract_gradient3_split(float *u, float *v, float *w, const float *ur, const float *vr, const float *wr,
+                             const double *xs, const uint8_t *tag, int nx, int ny, int nz, void *stream);

@@ -406,6 +406,141 @@ __global__ void __launch_bounds__(256) k_subtract_gradient4w(float *__restrict__
     if (q == nq - 1) uf[t + 4] = 0.0f;   // k = nz
 }
 
+// The three face updates of subtract_gradient_tag (subtract_gradient.py:38-68,
+// d = 1, 2, 3) in one pass from the fused projection's FP64 solution in the
+// colour-split layout (cell (i, j, k) of colour c = (i + j + k) & 1 at
+// c * half + (i * ny + j) * nz / 2 + k / 2).  A thread takes 4 cells along k
+// and updates the u, v and w faces on their low sides (each face rounded once
+// from the FP64 difference, as flub_subtract_gradient_x64_u8 does); the
+// threads of the last cells of a row / column / plane also zero the
+// boundary faces i = nx, j = ny, k = nz.  x is read once instead of once per
+// axis, and no reference-layout copy of it is written.
+__global__ void __launch_bounds__(256) k_subtract_gradient3_split(float *__restrict__ u, float *__restrict__ v,
+                                                                  float *__restrict__ w,
+                                                                  const float *__restrict__ ur,
+                                                                  const float *__restrict__ vr,
+                                                                  const float *__restrict__ wr,
+                                                                  const double *__restrict__ xs,
+                                                                  const uint8_t *__restrict__ tag, int nx, int ny,
+                                                                  int nz)
+{
+    const uint32_t nq = (uint32_t)nz >> 2, nh = (uint32_t)nz >> 1;
+    const uint32_t half = (uint32_t)nx * (uint32_t)ny * nh;
+    const uint32_t tq = blockIdx.x * 256u + threadIdx.x;
+    if (tq >= (uint32_t)nx * ny * nq) return;
+    const uint32_t row = tq / nq, q = tq - row * nq;
+    const int i = (int)(row / (uint32_t)ny), j = (int)(row - (uint32_t)i * ny);
+    const int k0 = 4 * (int)q;
+    const uint32_t I = row * (uint32_t)nz + (uint32_t)k0;
+    // the 4 cells k0 .. k0 + 3 of row r: colours c0, c0 ^ 1 alternate
+    auto ldx = [&](uint32_t r, int ci, int cj, double x[4]) {
+        const uint32_t c0 = (uint32_t)((ci + cj) & 1);
+        const uint32_t b = r * nh + (uint32_t)(k0 >> 1);
+        const double2 a = *reinterpret_cast<const double2 *>(xs + c0 * half + b);
+        const double2 o = *reinterpret_cast<const double2 *>(xs + (c0 ^ 1u) * half + b);
+        x[0] = a.x; x[1] = o.x; x[2] = a.y; x[3] = o.y;
+    };
+    double X[4];
+    int T[4];
+    ldx(row, i, j, X);
+    ldt4(tag + I, T);
+    // u faces (i, j, k0..k0+3): between cells i - 1 and i
+    {
+        float out[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        const uint32_t f = I;   // u is (nx+1) x ny x nz: face (i, j, k) at (i * ny + j) * nz + k
+        if (i > 0) {
+            double XL[4];
+            int TL[4];
+            ldx(row - (uint32_t)ny, i - 1, j, XL);
+            ldt4(tag + I - (uint32_t)ny * nz, TL);
+            float UF[4], UR[4];
+            ldf4(u + f, UF);
+            bool solid = false;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) solid = solid || TL[e] == 2 || T[e] == 2;
+            if (solid) ldf4(ur + f, UR);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if (TL[e] == 2 || T[e] == 2) out[e] = UR[e];
+                else if (TL[e] == 1 || T[e] == 1) out[e] = (float)((double)UF[e] - (X[e] - XL[e]));
+            }
+        }
+        stf4(u + f, out);
+        if (i == nx - 1) {
+            const float zero[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            stf4(u + f + (uint32_t)ny * nz, zero);   // face i = nx
+        }
+    }
+    // v faces (i, j, k0..k0+3): between cells j - 1 and j; v is nx x (ny+1) x nz
+    {
+        float out[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        const uint32_t f = ((uint32_t)i * (ny + 1) + (uint32_t)j) * nz + (uint32_t)k0;
+        if (j > 0) {
+            double XL[4];
+            int TL[4];
+            ldx(row - 1u, i, j - 1, XL);
+            ldt4(tag + I - (uint32_t)nz, TL);
+            float VF[4], VR[4];
+            ldf4(v + f, VF);
+            bool solid = false;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) solid = solid || TL[e] == 2 || T[e] == 2;
+            if (solid) ldf4(vr + f, VR);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if (TL[e] == 2 || T[e] == 2) out[e] = VR[e];
+                else if (TL[e] == 1 || T[e] == 1) out[e] = (float)((double)VF[e] - (X[e] - XL[e]));
+            }
+        }
+        stf4(v + f, out);
+        if (j == ny - 1) {
+            const float zero[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            stf4(v + f + (uint32_t)nz, zero);   // face j = ny
+        }
+    }
+    // w faces (i, j, k0..k0+3): between cells k - 1 and k; w rows hold nz + 1 faces
+    {
+        const uint32_t f = row * (uint32_t)(nz + 1) + (uint32_t)k0;
+        double xm = 0.0;
+        int tm = 0;
+        if (k0 > 0) {
+            const uint32_t c = (uint32_t)((i + j + k0 - 1) & 1);
+            xm = xs[c * half + row * nh + (uint32_t)((k0 - 1) >> 1)];
+            tm = tag[I - 1];
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int k = k0 + e;
+            float out = 0.0f;
+            if (k > 0) {
+                const double pl = e > 0 ? X[e - 1] : xm;
+                const int tl = e > 0 ? T[e - 1] : tm, tr = T[e];
+                if (tl == 2 || tr == 2) out = wr[f + e];
+                else if (tl == 1 || tr == 1) out = (float)((double)w[f + e] - (X[e] - pl));
+            }
+            w[f + e] = out;
+        }
+        if (q == nq - 1) w[f + 4] = 0.0f;   // face k = nz
+    }
+}
+
+int subtract_gradient3_split(float *u, float *v, float *w, const float *ur, const float *vr, const float *wr,
+                             const double *xs, const uint8_t *tag, int nx, int ny, int nz, void *stream)
+{
+    FLUB_REQUIRE(u && v && w && ur && vr && wr && xs && tag, FLUB_EINVAL, "subtract_gradient3_split: null pointer");
+    FLUB_REQUIRE(grid_ok(nx, ny, nz) && nz % 4 == 0, FLUB_ESHAPE, "subtract_gradient3_split: bad grid");
+    const size_t n4 = (size_t)nx * ny * (nz / 4);
+    k_subtract_gradient3_split<<<blocks_for(n4, 256), 256, 0, as_stream(stream)>>>(u, v, w, ur, vr, wr, xs, tag, nx,
+                                                                                   ny, nz);
+    FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+bool subtract_gradient3_split_ok(const float *u, const float *v, const float *ur, const float *vr, int nz)
+{
+    return nz % 4 == 0 && aligned16(u) && aligned16(v) && aligned16(ur) && aligned16(vr);
+}
+
 template <typename TagT, bool PS, typename PT = float>
 static int subgrad_impl(float *uf, const float *urig, const PT *p, const TagT *tag, int nx, int ny,
                         int nz, int d, float h, void *stream)

@@ -1287,7 +1287,8 @@ int mg_split_leave(flub_mgpcg *h, bool x_set)
 {
     Level &L0 = h->lv[0];
     if (!x_set) FLUB_CHECK_HIP(hipMemsetAsync(h->sp.x, 0, (size_t)L0.n * sizeof(double), h->s));
-    TRY(split_x_to_natural(h->s, L0, h->sp.x, h->x, h->want_x64n ? h->x64n : nullptr));
+    TRY(split_x_to_natural(h->s, L0, h->sp.x, h->x, h->want_x64n ? h->x64n : nullptr, h->p_out, h->p_h));
+    h->p_written = h->p_out != nullptr;
     h->x64_valid = true;
     return FLUB_OK;
 }
@@ -1699,16 +1700,24 @@ FLUB_API int flub_project(flub_mgpcg *h, float *u, float *v, float *w, const flo
     // solver keeps them (the next init resets them)
     h->x_only = true;
     h->presplit = presplit;
-    h->want_x64n = h->x64n != nullptr;
+    // the gradient reads the split FP64 x directly when it can (no reference-layout copy)
+    const bool grad3 = h->split && subtract_gradient3_split_ok(u, v, ur, vr, nz) && getenv("FLUB_GRAD_PER_AXIS") == nullptr;
+    h->want_x64n = h->x64n != nullptr && !grad3;
+    h->p_out = p;
+    h->p_h = h_cell;
+    h->p_written = false;
     int st = flub_mgpcg_solve(h, max_iters, 1e-12, abs_tol, rel_tol, smoother, 0, iters, rTr);
     h->x_only = false;
     h->presplit = false;
     h->want_x64n = false;
+    h->p_out = nullptr;
     if (st != FLUB_OK && st != FLUB_ENOTCONV) return st;
-    if (h->x64_valid) {
+    if (h->x64_valid && grad3) {
         // a split solve: the gradient of its FP64 x (the scale chain p = x h,
         // p / h is x itself), each face update rounded once; the projected
         // field's divergence is then the solve's residual
+        TRY(subtract_gradient3_split(u, v, w, ur, vr, wr, h->sp.x, L0.tag, nx, ny, nz, h->s));
+    } else if (h->x64_valid) {
         TRY(flub_subtract_gradient_x64_u8(u, ur, h->x64n, L0.tag, nx, ny, nz, 1, h->s));
         TRY(flub_subtract_gradient_x64_u8(v, vr, h->x64n, L0.tag, nx, ny, nz, 2, h->s));
         TRY(flub_subtract_gradient_x64_u8(w, wr, h->x64n, L0.tag, nx, ny, nz, 3, h->s));
@@ -1717,9 +1726,12 @@ FLUB_API int flub_project(flub_mgpcg *h, float *u, float *v, float *w, const flo
         TRY(flub_subtract_gradient_x_u8(v, vr, h->x, L0.tag, nx, ny, nz, 2, h_cell, h->s));
         TRY(flub_subtract_gradient_x_u8(w, wr, h->x, L0.tag, nx, ny, nz, 3, h_cell, h->s));
     }
-    const float inv_h = (float)(1.0 / (double)h_cell);
-    k_store_pressure<<<nb256(L0.n), 256, 0, h->s>>>(p, h->x, h_cell, inv_h, L0.n);
-    FLUB_CHECK_LAUNCH();
+    if (!h->p_written) {
+        const float inv_h = (float)(1.0 / (double)h_cell);
+        k_store_pressure<<<nb256(L0.n), 256, 0, h->s>>>(p, h->x, h_cell, inv_h, L0.n);
+        FLUB_CHECK_LAUNCH();
+    }
+    h->p_written = false;
     return st;
 }
 

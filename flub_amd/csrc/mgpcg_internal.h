@@ -122,6 +122,9 @@ struct flub_mgpcg {
     bool x64_valid;    // sp.x holds the FP64 x of the last solve (h->x is it rounded to FP32)
     double *x64n;      // the fused projection: the FP64 x in the reference layout, for the gradient
     bool want_x64n;    // the next split solve also writes x64n
+    float *p_out;      // the fused projection: the split solve also writes its pressure here ...
+    float p_h;         // ... from the FP32 x through the scale chain with this cell size
+    bool p_written;    // the last split solve wrote p_out
     SplitBuf sp;
     MixedState *mx;
     Probe *probe;      // armed in-solve timing (nullptr: off)

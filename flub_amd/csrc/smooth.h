@@ -162,4 +162,5 @@ int split_true_residual(hipStream_t s, const Level &L0, float *r, const float *b
 // x (FP32, reference layout) -> x64 (split); x64 (split) -> x (FP32, rounded
 // once) and/or x64_natural (FP64, reference layout), either may be null.
 int split_x_from_natural(hipStream_t s, const Level &L0, const float *x, double *x64);
-int split_x_to_natural(hipStream_t s, const Level &L0, const double *x64, float *x, double *x64_natural);
+int split_x_to_natural(hipStream_t s, const Level &L0, const double *x64, float *x, double *x64_natural,
+                       float *p = nullptr, float hh = 1.0f);

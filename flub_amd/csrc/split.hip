@@ -703,9 +703,12 @@ __global__ void __launch_bounds__(256) k_split_conv(const TS *__restrict__ src, 
 }
 
 // The split FP64 solution to the reference layout, rounded to FP32 into x32
-// and as it is into x64 (k_split_conv's mapping, both outputs in one pass).
+// and as it is into x64 (k_split_conv's mapping, both outputs in one pass;
+// x64 may be null); with pout also the projection's pressure from the FP32 x
+// through the reference's scale chain, ((x h) / h) h as (x h) * inv_h * h.
 __global__ void __launch_bounds__(256) k_split_x_out(const double *__restrict__ src, float *__restrict__ x32,
-                                                     double *__restrict__ x64, SGeo g)
+                                                     double *__restrict__ x64, SGeo g, float *__restrict__ pout,
+                                                     float hh, float inv_h)
 {
     const uint32_t nzq = (uint32_t)g.nh / 2;
     const uint32_t total = (uint32_t)g.nx * g.ny * nzq;
@@ -718,9 +721,16 @@ __global__ void __launch_bounds__(256) k_split_x_out(const double *__restrict__ 
         const uint32_t s0 = c0 * g.half + row * (uint32_t)g.nh + (uint32_t)(k0 >> 1);
         const uint32_t s1 = (c0 ^ 1u) * g.half + row * (uint32_t)g.nh + (uint32_t)(k0 >> 1);
         const double v0 = src[s0], v2 = src[s0 + 1], v1 = src[s1], v3 = src[s1 + 1];
-        *reinterpret_cast<float4 *>(x32 + nat) = make_float4((float)v0, (float)v1, (float)v2, (float)v3);
-        reinterpret_cast<double2 *>(x64 + nat)[0] = make_double2(v0, v1);
-        reinterpret_cast<double2 *>(x64 + nat)[1] = make_double2(v2, v3);
+        const float f0 = (float)v0, f1 = (float)v1, f2 = (float)v2, f3 = (float)v3;
+        *reinterpret_cast<float4 *>(x32 + nat) = make_float4(f0, f1, f2, f3);
+        if (x64) {
+            reinterpret_cast<double2 *>(x64 + nat)[0] = make_double2(v0, v1);
+            reinterpret_cast<double2 *>(x64 + nat)[1] = make_double2(v2, v3);
+        }
+        if (pout)
+            *reinterpret_cast<float4 *>(pout + nat) =
+                make_float4(((f0 * hh) * inv_h) * hh, ((f1 * hh) * inv_h) * hh, ((f2 * hh) * inv_h) * hh,
+                            ((f3 * hh) * inv_h) * hh);
     }
 }
 
@@ -836,11 +846,13 @@ int split_x_from_natural(hipStream_t s, const Level &L0, const float *x, double 
     return FLUB_OK;
 }
 
-int split_x_to_natural(hipStream_t s, const Level &L0, const double *x64, float *x, double *x64_natural)
+int split_x_to_natural(hipStream_t s, const Level &L0, const double *x64, float *x, double *x64_natural, float *p,
+                       float hh)
 {
     const SGeo g = make_sgeo(L0.nx, L0.ny, L0.nz);
     const unsigned nb = blocks_for(L0.n / 4, 256) < 4096 ? blocks_for(L0.n / 4, 256) : 4096;
-    if (x && x64_natural) k_split_x_out<<<nb, 256, 0, s>>>(x64, x, x64_natural, g);
+    if (x && (x64_natural || p))
+        k_split_x_out<<<nb, 256, 0, s>>>(x64, x, x64_natural, g, p, hh, (float)(1.0 / (double)hh));
     else if (x) k_split_conv<double, float, false><<<nb, 256, 0, s>>>(x64, x, g);
     else if (x64_natural) k_split_conv<double, double, false><<<nb, 256, 0, s>>>(x64, x64_natural, g);
     FLUB_CHECK_LAUNCH();

@@ -514,3 +514,34 @@ def test_residual_norm_fp64(oracle):
     ndt = oracle.compute_Ap(x.astype(np.float64), tag, real=np.float64)   # neighbor_dot_tag(x), 0 off FLUID
     res = np.where(tag == 1, b.astype(np.float64) - ndt, 0.0)
     assert np.isclose(rr.value, float(np.sum(res * res)), rtol=1e-12)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+@pytest.mark.parametrize("shape", [(64, 64, 64), (48, 32, 48)])
+def test_fused_gradient_equals_per_axis(shape, precision, monkeypatch):
+    """The fused projection's gradient subtraction in one pass over the three
+    axes from the colour-split FP64 x equals the per-axis passes over its
+    reference-layout copy (FLUB_GRAD_PER_AXIS=1) bitwise, boundary faces
+    included."""
+    n = shape[0]
+    s = scenes.dam_break_spheres(n, 1) if len(set(shape)) == 1 else None
+    if s is None:
+        rng = np.random.default_rng(41)
+        tag = rng.choice(3, size=shape, p=[0.15, 0.75, 0.10]).astype(np.int32)
+        nx, ny, nz = shape
+        s = {"tag": tag, "u": rng.normal(0, 1, (nx + 1, ny, nz)).astype(np.float32),
+             "v": rng.normal(0, 1, (nx, ny + 1, nz)).astype(np.float32),
+             "w": rng.normal(0, 1, (nx, ny, nz + 1)).astype(np.float32),
+             "ur": rng.normal(0, 1, (nx + 1, ny, nz)).astype(np.float32),
+             "vr": rng.normal(0, 1, (nx, ny + 1, nz)).astype(np.float32),
+             "wr": rng.normal(0, 1, (nx, ny, nz + 1)).astype(np.float32)}
+    res = []
+    for env in (None, "1"):
+        if env:
+            monkeypatch.setenv("FLUB_GRAD_PER_AXIS", env)
+        pr = Projection(shape, n_mg_levels=3, pre_and_post_smoothing=2, bottom_smoothing=6, precision=precision)
+        pr.load(s)
+        pr.project(max_iters=-1, rel_tol=1e-12)
+        res.append([a.copy() for a in pr.velocity()] + [pr.p0.f.numpy()])
+    for a, b in zip(*res):
+        assert bits_equal(a, b)
