@@ -39,10 +39,13 @@ TUNED_CYCLE = {"coarse_scale": 0.5, "sweep_cap": 3}
 
 def tuned_levels(shape):
     """Levels of the tuned cycle for a grid: halve until the shortest axis has
-    4 cells (every restricted level even, as the V-cycle requires)."""
-    m = min(int(s) for s in shape)
+    4 cells, every restricted level even (as the V-cycle requires) and every
+    level's z extent a multiple of 4 (as the 16-bit storage requires)."""
+    shape = tuple(int(s) for s in shape)
+    m = min(shape)
     levels = 1
-    while levels < 12 and m % 2 == 0 and m // 2 >= 4 and all((int(s) >> (levels - 1)) % 2 == 0 for s in shape):
+    while (levels < 12 and m % 2 == 0 and m // 2 >= 4 and all((s >> (levels - 1)) % 2 == 0 for s in shape)
+           and (shape[2] >> levels) % 4 == 0):
         m //= 2
         levels += 1
     return levels

@@ -136,3 +136,42 @@ def test_isolated_coarse_fluid_cell_bitexact(oracle):
     mg.r[0].assign(r)
     mg.apply_preconditioner("phases")
     assert bits_equal(mg.z[0].numpy(), want)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16", "bf16"])
+@pytest.mark.parametrize("shape", [(64, 64, 64), (128, 64, 96)])
+def test_tuned_preset_projection(shape, precision, oracle):
+    """The preset the bench's tuned legs use (flub_amd.project.TUNED_CYCLE,
+    tuned_levels: levels down to 4 cells on the shortest axis) on a cubic and a
+    non-cubic grid (the latter's coarsest levels are not powers of two and run
+    launch by launch): pressure within 1e-5 of the converged FP64 oracle, in
+    every storage precision, stopping on the FP64 true residual -- or, where the
+    1e-6 residual leaves a larger error in the lowest modes, as close as the
+    reference cycle's solve of the same scene."""
+    from flub_amd.project import TUNED_CYCLE, tuned_levels
+    rng = np.random.default_rng(51)
+    nx, ny, nz = shape
+    tag = np.ones(shape, np.int32)
+    tag[:, ny // 2:, :] = 0                       # a free surface
+    c = np.array(shape) * np.array([0.3, 0.25, 0.6])
+    i, j, k = np.meshgrid(*(np.arange(s) for s in shape), indexing="ij")
+    tag[(i - c[0]) ** 2 + (j - c[1]) ** 2 + (k - c[2]) ** 2 < (min(shape) / 6) ** 2] = 2
+    u = rng.normal(0, 1, (nx + 1, ny, nz)).astype(np.float32)
+    v = rng.normal(0, 1, (nx, ny + 1, nz)).astype(np.float32)
+    w = rng.normal(0, 1, (nx, ny, nz + 1)).astype(np.float32)
+    zr = [np.zeros_like(a) for a in (u, v, w)]
+    s = {"tag": tag, "u": u, "v": v, "w": w, "ur": zr[0], "vr": zr[1], "wr": zr[2]}
+    levels = tuned_levels(shape)
+    o64 = oracle.MGPCG(shape, 4, real=np.float64, pre_and_post_smoothing=2, bottom_smoothing=6)
+    _, _, _, p64, _, _, _ = o64.project(u, v, w, *zr, tag, 1.0 / nx, rel_tol=1e-24)
+    errs = {}
+    for cycle in ("tuned", "reference"):
+        kw = dict(n_mg_levels=levels, **TUNED_CYCLE) if cycle == "tuned" else dict(n_mg_levels=3)
+        pr = Projection(shape, pre_and_post_smoothing=2, bottom_smoothing=6, precision=precision, **kw)
+        pr.load(s)
+        it = pr.project(max_iters=-1, rel_tol=1e-12)
+        errs[cycle] = (it, _rel(pr.p0.f.numpy(), p64))
+    print(f"{shape} {precision}: tuned {levels} levels, {errs['tuned'][0]} iterations, pressure error vs FP64 "
+          f"{errs['tuned'][1]:.2e}; reference cycle {errs['reference'][0]} iterations, {errs['reference'][1]:.2e}")
+    assert errs["tuned"][1] < max(1e-5, 2.0 * errs["reference"][1])
+    assert errs["tuned"][0] <= errs["reference"][0]
