@@ -392,7 +392,7 @@ def profile_kernels(pr):
     """Mean duration of single kernels of the FP32 solve, replayed with HIP
     events on the solver's stream, and their algorithmic bytes."""
     from flub_amd import _lib
-    names = {8: "rb_half_sweep_l0_split", 9: "restrict_l0_split", 10: "px_update_then_Ap_dot_split",
+    names = {8: "rb_half_sweep_l0_split", 9: "restrict_l0_split", 10: "p_step_Ap_dot_split",
              11: "update_r_dot",
              0: "rb_half_sweep_l0", 7: "blocked4_l0", 5: "prolong_sweep_l0", 4: "restrict_l0",
              1: "p_update_Ap_dot", 2: "update_rx_dot", 6: "update_p", 3: "vcycle"}

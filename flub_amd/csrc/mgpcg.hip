@@ -863,6 +863,10 @@ FLUB_API int flub_mgpcg_create(flub_mgpcg **out, int nx, int ny, int nz, int lev
     h->no_tb_l0 = getenv("FLUB_NO_TB_L0") != nullptr;
     h->tb_g8 = getenv("FLUB_TB_G8") != nullptr;
     h->no_tail = getenv("FLUB_NO_TAIL") != nullptr;
+    {
+        const char *e = getenv("FLUB_PAP_IB");
+        h->pap_ib = getenv("FLUB_NO_PAP") ? -1 : (e ? atoi(e) : 0);
+    }
     int rc = FLUB_OK;
 #define ALLOC(ptr, bytes)                                              \
     do {                                                               \
@@ -1363,8 +1367,7 @@ static int solve_split(flub_mgpcg *h, int max_iters, double eps, double abs_tol,
         // speculative cycle would delay the end of the solve)
         const bool speculate = it + 1 < cap && r_prev * rate > 16.0 * tol;
         // p = z + beta*p (with the pending x updates), then Ap = ndt(p)
-        TRY(px_step(h->s, px, S.z[zs], nullptr));
-        TRY(split_Ap(h->s, L0, px.p(), h->Ap, S.code, h->part, &nbp));
+        TRY(split_pstep_Ap(h->s, px, S.z[zs], nullptr, L0, h->Ap, S.code, h->part, &nbp, h->pap_ib));
         TRY(launch_finish(h, nbp, FIN_ALPHA, eps));
         // r only: x += alpha p waits for a later p step (or the end)
         k_update_rx_dot<false><<<nb, RBS, 0, h->s>>>(R, nullptr, h->Ap, nullptr, h->sc, n,
@@ -1851,8 +1854,7 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
             // the p step: p into a ring buffer, then Ap
             PxState px;
             px_init(h, px, h->sp.x, L0.n);
-            TRY(px_step(h->s, px, h->sp.z[0], nullptr));
-            TRY(split_Ap(h->s, L0, px.p(), h->Ap, h->sp.code, h->part, &nbp));
+            TRY(split_pstep_Ap(h->s, px, h->sp.z[0], nullptr, L0, h->Ap, h->sp.code, h->part, &nbp, h->pap_ib));
             return launch_finish(h, nbp, FIN_SUM, 0.0);
         }
         case 11:   // the split solve's r step: r -= alpha Ap and r.r
@@ -1887,7 +1889,7 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
     case 7: b = 4 * N * (2 * v + 1); break;                  // four half-sweeps
     case 8: b = N * (2 * v + 1); break;
     case 9: b = N * (2 * v + 1) + N / 8 * (v + 1); break;
-    case 10: b = N * (5 * 4 + 2); break;                     // p, z -> p'; p', codes -> Ap
+    case 10: b = N * (4 * 4 + 2); break;                     // p, z, codes -> p', Ap
     case 11: b = N * 3 * 4; break;                           // r, Ap -> r
     case 12: b = N * 8; break;                               // as 8 with the 16-bit r (own colour)
     default: break;

@@ -130,6 +130,8 @@ struct flub_mgpcg {
     Probe *probe;      // armed in-solve timing (nullptr: off)
     int sweep_cap;     // > 0: at most this many pre/post iterations on any level (0: the reference's doubling)
     bool no_tail;      // the coarsest levels launch kernel by kernel (no one-workgroup tail, tail.hip)
+    int pap_ib;        // split solves' p step: -1 p update and Ap in two passes; 0 one pass, rows
+                       // per segment chosen by grid size; > 0 one pass with this many rows
 };
 
 // The one-workgroup lower part of the V-cycle (tail.hip): levels of at most
@@ -275,8 +277,10 @@ inline int px_flush(hipStream_t s, PxState &st)
 }
 
 // The p step of an iteration: p into a free ring buffer; the pending update of
-// the previous p gets its alpha saved.
-inline int px_step(hipStream_t s, PxState &st, const float *z, const double *zinv)
+// the previous p gets its alpha saved.  px_step_slot picks that buffer (*out)
+// and says whether the kernel writing it also saves alpha into
+// ahist[cur] (*save); px_step_done makes it the current p.
+inline int px_step_slot(hipStream_t s, PxState &st, int *out, bool *save)
 {
     FLUB_REQUIRE(st.n % 4 == 0, FLUB_EINVAL, "vector update: n = %u is not a multiple of 4", st.n);
     if (st.npend + 1 >= st.nring) {   // keep a buffer free besides the pending ones
@@ -286,15 +290,28 @@ inline int px_step(hipStream_t s, PxState &st, const float *z, const double *zin
     bool used[MAXPH] = {};
     used[st.cur] = true;
     for (int i = 0; i < st.npend; ++i) used[st.pend[i]] = true;
-    int out = 0;
-    while (used[out]) ++out;
-    const bool save = st.npend > 0 && st.pend[st.npend - 1] == st.cur && !st.newest_saved;
+    int o = 0;
+    while (used[o]) ++o;
+    *out = o;
+    *save = st.npend > 0 && st.pend[st.npend - 1] == st.cur && !st.newest_saved;
+    return FLUB_OK;
+}
+inline void px_step_done(PxState &st, int out, bool save)
+{
+    if (save) st.newest_saved = true;
+    st.cur = out;
+}
+inline int px_step(hipStream_t s, PxState &st, const float *z, const double *zinv)
+{
+    int out;
+    bool save;
+    const int rc = px_step_slot(s, st, &out, &save);
+    if (rc) return rc;
     const uint32_t n4 = st.n / 4;
     k_update_pp<<<blocks_for(n4, 256), 256, 0, s>>>(st.ring[out], st.ring[st.cur], z, st.sc, zinv,
                                                     save ? st.ahist + st.cur : nullptr, n4);
     FLUB_CHECK_LAUNCH();
-    if (save) st.newest_saved = true;
-    st.cur = out;
+    px_step_done(st, out, save);
     return FLUB_OK;
 }
 
@@ -318,6 +335,13 @@ inline void px_after_r_update(PxState &st)
     st.pend[st.npend++] = st.cur;
     st.newest_saved = false;
 }
+
+// The split solves' p step followed by Ap = ndt(p) and the partials of p.Ap
+// (*nblk of them): one pass (split.hip, k_sp_pAp) when ib >= 0 and the grid
+// fits it, else px_step then split_Ap.  p and Ap are bitwise the same either
+// way; the partials sum the same products in another grouping.
+int split_pstep_Ap(hipStream_t s, PxState &st, const float *z, const double *zinv, const Level &L0, float *Ap,
+                   const uint16_t *code, double *part, int *nblk, int ib);
 
 int mg_dot(flub_mgpcg *h, const float *a, const float *b, int op, double eps);
 int mg_zero_centered(flub_mgpcg *h);

@@ -603,8 +603,8 @@ static int solve_mixed_split(flub_mgpcg *h, int max_iters, double eps, double ab
         const bool speculate = it + 1 < cap && r_prev * rate > 16.0 * tol;
         // p = z + beta p (and the deferred x update), then q = ndt(p) and p.q
         int nbp = 0;
-        TRYM(px_step(h->s, px, S.z[zs], h->sc + SC_ZINV0 + zs));
-        TRYM(split_Ap(h->s, L0, px.p(), h->Ap, S.code, h->part, &nbp));
+        TRYM(split_pstep_Ap(h->s, px, S.z[zs], h->sc + SC_ZINV0 + zs, L0, h->Ap, S.code, h->part, &nbp,
+                            h->pap_ib));
         TRYM(mg_finish_alpha(h, nbp, eps));
         // r only: x += alpha p waits for the next p update (or a read of x)
         // BF16: the update also stores the next V-cycle's r (storage scale 1)

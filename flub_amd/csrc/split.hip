@@ -598,6 +598,171 @@ __global__ void __launch_bounds__(256) k_sp_Ap(const float *__restrict__ p, floa
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// The p step and Ap of the split solves in one pass: p = beta p_prev + z (z
+// times zinv[0] when set; k_update_pp's arithmetic) and Ap = ndt(p) with the
+// FP64 partials of p.Ap (k_sp_Ap's arithmetic, so p and Ap are bitwise those
+// of the two-pass step).  A workgroup owns a tile of jb = 256 / nq rows along
+// j, each row whole along k (nq vectors of 4 cells per half-row), and marches
+// along i over the rows [i0, i1) of its segment.  Each step computes p of row
+// i + 1 (both colours) from the loads issued one step earlier, keeps rows
+// i - 1, i, i + 1 of its own (j, vector) in registers and puts row i + 1 into
+// an LDS plane together with the tile's two halo rows along j (computed again
+// by helper threads: 4 nq vectors); the i and j neighbours of row i then come
+// from registers and LDS, and the k neighbour beyond a vector from the
+// adjacent lane (a half-row's vectors sit in consecutive lanes of one
+// wavefront).  Every value of p_prev and z is thus loaded about
+// (ib + 2) / ib * (jb + 2) / jb times (mostly from L2) instead of five times
+// from L1.  Rows i0 - 1 and i1 are computed for their neighbours only; row
+// indices are clamped to the grid as in k_sp_Ap.  Three planes in rotation
+// need one barrier a step: a step writes plane (i + 1) % 3 and reads plane
+// i % 3, and plane (i + 1) % 3 was last read two steps earlier.  Block b runs
+// tile (b & 7) * tpx + (b >> 3), so that the segments of a tile (which share
+// their halo rows along i) run on one XCD; blocks past the last tile write a
+// zero partial.
+struct PApArgs {
+    float *p_out;
+    const float *p_prev, *z;
+    const double *sc, *zinv;
+    double *asave;   // receives sc[SC_ALPHA] (block 0), as in k_update_pp
+    float *Ap;
+    const uint16_t *code;
+    double *part;
+    SGeo g;
+    int lq;          // log2 of nq (2..6)
+    int jb;          // rows of a tile along j: 256 / nq
+    int ib;          // rows of a segment along i
+    int nseg;        // segments along i
+    uint32_t ntiles, tpx;   // tiles, tiles per XCD
+};
+
+__global__ void __launch_bounds__(256) k_sp_pAp(PApArgs a)
+{
+    extern __shared__ float4 pl[];   // 3 planes of (jb + 2) rows x 2 colours x nq vectors
+    __shared__ double lds[4];
+    const SGeo &g = a.g;
+    const int tid = threadIdx.x;
+    const uint32_t T = (blockIdx.x & 7u) * a.tpx + (blockIdx.x >> 3);
+    if (T >= a.ntiles) {
+        if (tid == 0) a.part[blockIdx.x] = 0.0;
+        return;
+    }
+    if (a.asave && blockIdx.x == 0 && tid == 0) a.asave[0] = a.sc[SC_ALPHA];   // read by a later launch only
+    const int nq = g.nq, lq = a.lq;
+    const int seg = (int)(T % (uint32_t)a.nseg), jt = (int)(T / (uint32_t)a.nseg);
+    const int i0 = seg * a.ib, i1 = min(i0 + a.ib, g.nx);
+    const int j0 = jt * a.jb;
+    const int q = tid & (nq - 1), jl = tid >> lq, j = j0 + jl, m0 = 4 * q;
+    // helper threads: halo row j0 - 1 (hr 0) or j0 + jb (hr 1), clamped, colour hc
+    const bool helper = tid < 4 * nq;
+    const int hr = (tid >> (lq + 1)) & 1, hc = (tid >> lq) & 1;
+    const int hj = hr == 0 ? max(j0 - 1, 0) : min(j0 + a.jb, g.ny - 1);
+    const uint32_t plane = (uint32_t)(a.jb + 2) * 2u * (uint32_t)nq;
+    const uint32_t own_slot = (uint32_t)((jl + 1) * 2) * nq + q;   // + c * nq
+    const uint32_t halo_slot = (uint32_t)(((hr == 0 ? 0 : a.jb + 1) * 2 + hc) * nq + q);
+    const float beta = (float)a.sc[SC_BETA];
+    const bool scaled = a.zinv != nullptr;
+    const float zi = scaled ? (float)a.zinv[0] : 1.0f;
+    auto ld = [&](const float *b, int c, int ii, int jj) -> float4 {
+        return *reinterpret_cast<const float4 *>(b + (uint32_t)c * g.half + g.row(ii, jj) + m0);
+    };
+    auto pval = [&](float4 pp, float4 zz) -> float4 {
+        if (scaled) { zz.x *= zi; zz.y *= zi; zz.z *= zi; zz.w *= zi; }
+        return make_float4(fmaf(beta, pp.x, zz.x), fmaf(beta, pp.y, zz.y), fmaf(beta, pp.z, zz.z),
+                           fmaf(beta, pp.w, zz.w));
+    };
+    float4 P0[2], P1[2], P2[2];   // p of rows i - 1, i, i + 1 (clamped) at (j, m0), both colours
+    float4 PP[2], ZZ[2], HP, HZ;  // loads of the next row: own, helper
+    uint2 CD[2];                  // stencil codes of row i (loaded one step ahead, like p_prev and z:
+                                  // a wait for a load also waits for every load issued before it)
+    auto ldc = [&](int c, int ii) -> uint2 {
+        return *reinterpret_cast<const uint2 *>(a.code + (uint32_t)c * g.half + g.row(ii, j) + m0);
+    };
+    {
+        const int im = max(i0 - 1, 0);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            P0[c] = pval(ld(a.p_prev, c, im, j), ld(a.z, c, im, j));
+            P1[c] = pval(ld(a.p_prev, c, i0, j), ld(a.z, c, i0, j));
+            pl[(uint32_t)(i0 % 3) * plane + own_slot + (uint32_t)c * nq] = P1[c];
+        }
+        if (helper) pl[(uint32_t)(i0 % 3) * plane + halo_slot] = pval(ld(a.p_prev, hc, i0, hj), ld(a.z, hc, i0, hj));
+        const int in = min(i0 + 1, g.nx - 1);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) { CD[c] = ldc(c, i0); PP[c] = ld(a.p_prev, c, in, j); ZZ[c] = ld(a.z, c, in, j); }
+        HP = ld(a.p_prev, hc, in, hj);
+        HZ = ld(a.z, hc, in, hj);
+    }
+    double acc = 0.0;
+    for (int i = i0; i < i1; ++i) {
+        const uint32_t b1 = (uint32_t)((i + 1) % 3) * plane, b0 = (uint32_t)(i % 3) * plane;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            P2[c] = pval(PP[c], ZZ[c]);
+            pl[b1 + own_slot + (uint32_t)c * nq] = P2[c];
+        }
+        if (helper) pl[b1 + halo_slot] = pval(HP, HZ);
+        uint32_t cd[2][4];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            cd[c][0] = CD[c].x & 0xffffu; cd[c][1] = CD[c].x >> 16; cd[c][2] = CD[c].y & 0xffffu; cd[c][3] = CD[c].y >> 16;
+        }
+        if (i + 1 < i1) {   // the next step's codes (row i + 1) and rows i + 2 (clamped)
+            const int in = min(i + 2, g.nx - 1);
+#pragma unroll
+            for (int c = 0; c < 2; ++c) CD[c] = ldc(c, i + 1);
+#pragma unroll
+            for (int c = 0; c < 2; ++c) { PP[c] = ld(a.p_prev, c, in, j); ZZ[c] = ld(a.z, c, in, j); }
+            // every thread loads a halo vector (hj, hc are in the grid for all): with no
+            // branch around the loads the waits for them are exact
+            HP = ld(a.p_prev, hc, in, hj);
+            HZ = ld(a.z, hc, in, hj);
+        }
+        const uint32_t base = g.row(i, j) + m0;
+        __syncthreads();
+        float4 YP[2], YM[2];   // rows j + 1 and j - 1 (clamped through the halo rows)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            YP[c] = pl[b0 + own_slot + (uint32_t)(2 * nq) + (uint32_t)c * nq];
+            YM[c] = pl[b0 + own_slot - (uint32_t)(2 * nq) + (uint32_t)c * nq];
+        }
+        const float l0 = __shfl_up(P1[0].w, 1, 64), r0 = __shfl_down(P1[0].x, 1, 64);
+        const float l1 = __shfl_up(P1[1].w, 1, 64), r1 = __shfl_down(P1[1].x, 1, 64);
+#pragma unroll
+        for (int cc = 0; cc < 2; ++cc) {
+            const int o = (cc - i - j) & 1;
+            const float4 wv = P1[cc ^ 1], own4 = P1[cc];
+            const float w[4] = {wv.x, wv.y, wv.z, wv.w}, pn[4] = {own4.x, own4.y, own4.z, own4.w};
+            const float xp[4] = {P2[cc ^ 1].x, P2[cc ^ 1].y, P2[cc ^ 1].z, P2[cc ^ 1].w};
+            const float xm[4] = {P0[cc ^ 1].x, P0[cc ^ 1].y, P0[cc ^ 1].z, P0[cc ^ 1].w};
+            const float yp[4] = {YP[cc ^ 1].x, YP[cc ^ 1].y, YP[cc ^ 1].z, YP[cc ^ 1].w};
+            const float ym[4] = {YM[cc ^ 1].x, YM[cc ^ 1].y, YM[cc ^ 1].z, YM[cc ^ 1].w};
+            float edge = 0.0f;
+            if (o == 0) {
+                if (m0 > 0) edge = cc ? l0 : l1;
+            } else {
+                if (m0 + 4 < g.nh) edge = cc ? r0 : r1;
+            }
+            float out[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                float kp, km;
+                if (o == 0) { kp = w[e]; km = e > 0 ? w[e - 1] : edge; }
+                else { km = w[e]; kp = e < 3 ? w[e + 1] : edge; }
+                const float s = nsum_code(cd[cc][e], pn[e], xp[e], xm[e], yp[e], ym[e], kp, km);
+                out[e] = code_fluid(cd[cc][e]) ? s : 0.0f;
+                acc += (double)pn[e] * (double)out[e];
+            }
+            const uint32_t at = (uint32_t)cc * g.half + base;
+            st_f4(a.Ap + at, out);
+            *reinterpret_cast<float4 *>(a.p_out + at) = own4;
+        }
+#pragma unroll
+        for (int c = 0; c < 2; ++c) { P0[c] = P1[c]; P1[c] = P2[c]; }
+    }
+    const double s = block_sum<256>(acc, lds);
+    if (tid == 0) a.part[blockIdx.x] = s;
+}
+
 // True residual r = b - ndt(x) with the stencil evaluated in FP64 (from the
 // FP32 or FP64 x), stored as FP32 r; 0 off FLUID cells.  Partials: sum of r^2
 // (FP64 values) in part[block], max |r| in part[nblk + block].  Split layout,
@@ -980,6 +1145,78 @@ int split_Ap(hipStream_t s, const Level &L0, const float *p, float *Ap, const ui
     *nblk = (int)nb;
     k_sp_Ap<<<nb, 256, 0, s>>>(p, Ap, code, g, part);
     FLUB_CHECK_LAUNCH();
+    return FLUB_OK;
+}
+
+// Launch plan of k_sp_pAp: nq a power of two in [4, 64] (256 threads hold
+// whole half-rows of jb = 256 / nq rows), ny a multiple of jb, at most MAXBLK
+// blocks.  ib <= 0 picks the rows per segment: from 16, doubled while the
+// tiles exceed MAXBLK, halved (to 2) while they are fewer than 512.  (At 256^3,
+// 8 x 32-row tiles: the step with its finish takes 81 us in two passes, 67,
+// 60, 54.7 and 58.8 us with 4, 8, 16 and 32 rows per segment.)
+struct PApPlan {
+    int lq, jb, ib, nseg;
+    uint32_t ntiles, tpx, nblocks;
+    size_t shmem;
+};
+static bool pap_plan(const Level &L0, int ib, PApPlan &pl)
+{
+    const SGeo g = make_sgeo(L0.nx, L0.ny, L0.nz);
+    if (g.nq < 4 || g.nq > 64 || g.lq < 0 || g.nh != 4 * g.nq || L0.nx < 2) return false;
+    pl.lq = g.lq;
+    pl.jb = 256 / g.nq;
+    if (L0.ny % pl.jb != 0) return false;
+    const uint64_t njb = (uint64_t)(L0.ny / pl.jb);
+    auto tiles = [&](int b) { return (uint64_t)((L0.nx + b - 1) / b) * njb; };
+    if (ib <= 0) {
+        ib = 16;
+        while (ib < L0.nx && tiles(ib) > MAXBLK) ib *= 2;
+        while (ib > 2 && tiles(ib) < 512) ib /= 2;
+    }
+    pl.ib = ib;
+    pl.nseg = (L0.nx + ib - 1) / ib;
+    pl.ntiles = (uint32_t)tiles(ib);
+    pl.tpx = (pl.ntiles + 7u) / 8u;
+    pl.nblocks = 8u * pl.tpx;
+    if (pl.nblocks > MAXBLK) return false;
+    pl.shmem = 3u * (size_t)(pl.jb + 2) * 2u * (size_t)g.nq * sizeof(float4);
+    return true;
+}
+
+int split_pstep_Ap(hipStream_t s, PxState &st, const float *z, const double *zinv, const Level &L0, float *Ap,
+                   const uint16_t *code, double *part, int *nblk, int ib)
+{
+    PApPlan pl;
+    if (ib < 0 || !pap_plan(L0, ib, pl)) {
+        const int rc = px_step(s, st, z, zinv);
+        if (rc) return rc;
+        return split_Ap(s, L0, st.p(), Ap, code, part, nblk);
+    }
+    int out;
+    bool save;
+    const int rc = px_step_slot(s, st, &out, &save);
+    if (rc) return rc;
+    PApArgs a{};
+    a.p_out = st.ring[out];
+    a.p_prev = st.ring[st.cur];
+    a.z = z;
+    a.sc = st.sc;
+    a.zinv = zinv;
+    a.asave = save ? st.ahist + st.cur : nullptr;
+    a.Ap = Ap;
+    a.code = code;
+    a.part = part;
+    a.g = make_sgeo(L0.nx, L0.ny, L0.nz);
+    a.lq = pl.lq;
+    a.jb = pl.jb;
+    a.ib = pl.ib;
+    a.nseg = pl.nseg;
+    a.ntiles = pl.ntiles;
+    a.tpx = pl.tpx;
+    k_sp_pAp<<<pl.nblocks, 256, pl.shmem, s>>>(a);
+    FLUB_CHECK_LAUNCH();
+    px_step_done(st, out, save);
+    *nblk = (int)pl.nblocks;
     return FLUB_OK;
 }
 
