@@ -70,9 +70,9 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 # rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of `bench.py --kernels-only
 # [--kernel ...]`, calibrated on a 67.4 MB device copy
 # (profiles/rb_half_sweep_l0_split_traffic_256.md, profiles/rb_half_sweep_l0_traffic_256.md,
-# profiles/rb_half_sweep_l0_split_bf16r_traffic_256.md)
+# profiles/rb_half_sweep_l0_split_bf16r_traffic_256.md, profiles/p_step_Ap_split_traffic_256.md)
 MEASURED_TRAFFIC_L0 = {("rb_half_sweep_l0_split", 256): 153.8e6, ("rb_half_sweep_l0", 256): 242.7e6,
-                       ("rb_half_sweep_l0_split_bf16r", 256): 137.4e6}
+                       ("rb_half_sweep_l0_split_bf16r", 256): 137.4e6, ("p_step_Ap_dot_split", 256): 330.3e6}
 
 
 def log(*a):
@@ -97,9 +97,11 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--kernels-only", action="store_true",
                     help="only replay the level-0 half-sweep (for counter collection)")
-    ap.add_argument("--kernel", choices=["split", "split-bf16", "reference-layout", "unet", "advect"], default="split",
+    ap.add_argument("--kernel", choices=["split", "split-bf16", "reference-layout", "pstep", "unet", "advect"],
+                    default="split",
                     help="what --kernels-only replays: a level-0 half-sweep (FP32 solve, BF16 solve's "
-                         "post-smoothing, reference layout), one U-Net forward, or advect_3d")
+                         "post-smoothing, reference layout), the split solve's one-pass p step with Ap, "
+                         "one U-Net forward, or advect_3d")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
@@ -130,7 +132,7 @@ def main():
         device.synchronize()
         avg, byt = ctypes.c_float(), ctypes.c_double()
         kid, kname = {"split": (8, "rb_half_sweep_l0_split"), "split-bf16": (12, "rb_half_sweep_l0_split_bf16r"),
-                      "reference-layout": (0, "rb_half_sweep_l0")}[args.kernel]
+                      "reference-layout": (0, "rb_half_sweep_l0"), "pstep": (10, "p_step_Ap_dot_split")}[args.kernel]
         _lib.call("flub_mgpcg_profile", pr.mgpcg.handle, kid, 20, ctypes.byref(avg), ctypes.byref(byt))
         print(json.dumps({"kernel": kname, "ms": avg.value, "bytes": byt.value, "copy_bytes": src[0].nbytes}))
         return
@@ -138,6 +140,7 @@ def main():
     solvers = {}
     kernels = {}
     mixed_sweep = {}   # per BF16 leg: its level-0 post-smoothing half-sweep replayed
+    pstep = {}         # per leg: its one-pass p step with Ap (k_sp_pAp) replayed
     outputs = {}      # per solver: pressure and velocities, for the FP64 judge
     in_solve = None
     last_pr = None
@@ -168,6 +171,7 @@ def main():
             in_solve = probe_in_solve(pr, src)
         if name.startswith("bf16"):
             mixed_sweep[name] = profile_one(pr, 12)
+        pstep[name] = profile_one(pr, 10)
         last_pr = pr
 
     if "learned" in args.solvers:
@@ -178,14 +182,15 @@ def main():
 
     best = min(solvers, key=lambda k: solvers[k]["ms"])
     ms = solvers[best]["ms"]
-    # the dominant kernel of the fastest leg: the level-0 half-sweep its solve
-    # runs most (colour-split; in the BF16 solves the FP32 post-smoothing sweep
-    # with BF16 r), replayed with HIP events on the solver's stream (it agrees
-    # with the rocprofv3 kernel-trace average of those replays).  fp32_sweep:
-    # the FP32 solve's half-sweep, with in_solve = HIP events around its
-    # launches inside a projection's V-cycles (each interval also holds the
-    # dispatch after the previous kernel, so it bounds the in-solve kernel time
-    # from above).
+    # the dominant kernel of the fastest leg (the one-pass p step with Ap, 8
+    # launches of some 50 us in a tuned BF16 projection), replayed with HIP
+    # events on the solver's stream (it agrees with the rocprofv3 kernel-trace
+    # average of those replays); beside it the level-0 half-sweeps: bf16_sweep,
+    # the BF16 solve's FP32 post-smoothing sweep with BF16 r (16 launches of
+    # some 23 us), and fp32_sweep, the FP32 solve's half-sweep, with in_solve =
+    # HIP events around its launches inside a projection's V-cycles (each
+    # interval also holds the dispatch after the previous kernel, so it bounds
+    # the in-solve kernel time from above).
     dom_name = "rb_half_sweep_l0_split" if "rb_half_sweep_l0_split" in kernels else "rb_half_sweep_l0"
     dom = kernels.get(dom_name)
     fp32_sweep = None
@@ -199,15 +204,25 @@ def main():
             gbs = dom["bytes"] / (in_solve["avg_ms"] * 1e-3) / 1e9
             fp32_sweep["in_solve"] = {**in_solve, "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
     roofline = None
+    bf16_sweep = None
     if best in mixed_sweep and mixed_sweep[best]:
         d = mixed_sweep[best]
+        bf16_sweep = {"kernel": "k_ssweep<PLAIN, STORE, float, float, bf16> level-0 post-smoothing red-black "
+                                "half-sweep of the BF16 solve, colour-split layout (8 B/cell algorithmic: FP32 z, "
+                                "BF16 r)", "bytes_per_launch": d["bytes"], "avg_ms": d["ms"], "achieved": d["GBs"],
+                      "frac": round(d["GBs"] / HBM_PEAK_GBS, 4),
+                      "traffic": MEASURED_TRAFFIC_L0.get(("rb_half_sweep_l0_split_bf16r", n))}
+    if pstep.get(best):
+        # the kernel with the largest share of the fastest leg's solve: the
+        # one-pass p step with Ap (once per PCG iteration, 18 B/cell)
+        d = pstep[best]
         roofline = {"bound": "hbm", "achieved": d["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(d["GBs"] / HBM_PEAK_GBS, 4),
-                    "traffic": MEASURED_TRAFFIC_L0.get(("rb_half_sweep_l0_split_bf16r", n)),
-                    "kernel": "k_ssweep<PLAIN, STORE, float, float, bf16> level-0 post-smoothing red-black "
-                              "half-sweep of the BF16 solve, colour-split layout (8 B/cell algorithmic: FP32 z, "
-                              "BF16 r)", "leg": best, "bytes_per_launch": d["bytes"], "avg_ms": d["ms"],
-                    "fp32_sweep": fp32_sweep}
+                    "traffic": MEASURED_TRAFFIC_L0.get(("p_step_Ap_dot_split", n)),
+                    "kernel": "k_sp_pAp: the split solve's p step (p = z + beta p into the ring) with Ap = ndt(p) "
+                              "and the FP64 p.Ap partials in one pass, colour-split layout (18 B/cell algorithmic: "
+                              "p_old, z, p, Ap FP32, codes)", "leg": best, "bytes_per_launch": d["bytes"],
+                    "avg_ms": d["ms"], "bf16_sweep": bf16_sweep, "fp32_sweep": fp32_sweep}
     elif fp32_sweep:
         roofline = {"bound": "hbm", "achieved": fp32_sweep["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": fp32_sweep["frac"], "traffic": fp32_sweep["traffic"], "kernel": fp32_sweep["kernel"],

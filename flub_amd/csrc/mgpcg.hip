@@ -1848,10 +1848,16 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
         case 9:
             FLUB_REQUIRE(h->split, FLUB_ESTATE, "no colour-split level 0 for this handle");
             return split_restrict(h->s, L0, h->sp.r, h->sp.zpre, h->sp.code, h->lv[1]);
-        case 10: {   // the split solve's p step: p (and the deferred x) update, then Ap and p.Ap
+        case 10: {   // the split solve's p step: p into a ring buffer, then Ap and the p.Ap partials
             FLUB_REQUIRE(h->split, FLUB_ESTATE, "no colour-split level 0 for this handle");
             int nbp = 0;
-            // the p step: p into a ring buffer, then Ap
+            PxState px;
+            px_init(h, px, h->sp.x, L0.n);
+            return split_pstep_Ap(h->s, px, h->sp.z[0], nullptr, L0, h->Ap, h->sp.code, h->part, &nbp, h->pap_ib);
+        }
+        case 14: {   // kernel 10 followed by the finish that sums the partials into sc[SC_SUM]
+            FLUB_REQUIRE(h->split, FLUB_ESTATE, "no colour-split level 0 for this handle");
+            int nbp = 0;
             PxState px;
             px_init(h, px, h->sp.x, L0.n);
             TRY(split_pstep_Ap(h->s, px, h->sp.z[0], nullptr, L0, h->Ap, h->sp.code, h->part, &nbp, h->pap_ib));
@@ -1889,7 +1895,7 @@ FLUB_API int flub_mgpcg_profile(flub_mgpcg *h, int kernel, int reps, float *avg_
     case 7: b = 4 * N * (2 * v + 1); break;                  // four half-sweeps
     case 8: b = N * (2 * v + 1); break;
     case 9: b = N * (2 * v + 1) + N / 8 * (v + 1); break;
-    case 10: b = N * (4 * 4 + 2); break;                     // p, z, codes -> p', Ap
+    case 10: case 14: b = N * (4 * 4 + 2); break;            // p, z, codes -> p', Ap
     case 11: b = N * 3 * 4; break;                           // r, Ap -> r
     case 12: b = N * 8; break;                               // as 8 with the 16-bit r (own colour)
     default: break;

@@ -306,8 +306,9 @@ int flub_mgpcg_level_shape(flub_mgpcg *h, int level, int shape[3]);
  * 4 restriction 0->1, 5 prolongation 1->0, 6 p update, 7 one temporally
  * blocked launch of four level-0 half-sweeps, 8/9 the half-sweep and
  * restriction of the colour-split level 0 the FP32 solve runs on, 10 its p
- * step (p = beta p + z, then Ap and p.Ap: one pass where the grid allows,
- * else two), 11 its r step (r -= alpha Ap
+ * step (p = beta p + z, then Ap and the partials of p.Ap: one pass where the
+ * grid allows, else two; 14 the same followed by the finish that sums the
+ * partials into the `sum` scalar), 11 its r step (r -= alpha Ap
  * and r.r) (8-10: FLUB_ESTATE when the grid has no split level 0), 12 the
  * level-0 post-smoothing half-sweep of the mixed-precision split solve (FP32 z,
  * FP16/BF16 r; FLUB_ESTATE on an FP32 handle).  Leaves the handle's

@@ -1,8 +1,9 @@
 """The split solves' p step and Ap in one pass (csrc/split.hip, k_sp_pAp:
 p = beta p + z, then Ap = ndt(p) and p.Ap) against the FP32 oracle's
 compute_Ap and against the two-pass step (FLUB_NO_PAP=1: k_update_pp, then
-k_sp_Ap), through flub_mgpcg_profile's p-step entry (kernel 10: p of the
-handle and the split z of the last V-cycle, Ap into the handle's Ap):
+k_sp_Ap), through flub_mgpcg_profile's p-step entry (kernel 14: p of the
+handle and the split z of the last V-cycle, Ap into the handle's Ap, p.Ap
+into the `sum` scalar):
 
 * Ap is bitwise the oracle's Ap of beta p + z (beta 0.5, so that beta p + z
   rounds once in numpy as in the kernel's fused multiply-add);
@@ -67,7 +68,7 @@ def _pstep(tag, r, p, beta):
     mg.p.assign(to_split(p))
     mg.beta.assign(np.array([beta], np.float64))
     avg, byt = ctypes.c_float(), ctypes.c_double()
-    _lib.call("flub_mgpcg_profile", mg.handle, 10, 1, ctypes.byref(avg), ctypes.byref(byt))
+    _lib.call("flub_mgpcg_profile", mg.handle, 14, 1, ctypes.byref(avg), ctypes.byref(byt))
     wp.synchronize()
     return from_split(mg.Ap.numpy(), tag.shape), float(mg.sum.numpy()[0]), zd.numpy()
 
