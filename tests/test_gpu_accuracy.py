@@ -11,7 +11,7 @@ The residuals are recomputed here on the CPU with the FP64 oracle operator
 (neighbor_dot_tag, mgpcg.py:118-142), from the right-hand side the GPU
 divergence produced (bitwise the oracle's, tested elsewhere).
 
-Config 3 (flow past a sphere, 256^3, FP32 / FP16 / BF16 V-cycles) is checked
+The 256^3 flow past a sphere (FP32 / FP16 / BF16 V-cycles) is checked
 at its own size against the converged FP64 oracle projection computed on the
 host: relative pressure error (bar 1e-5; the FP32 solve also within 1.5x the
 FP32 reference algorithm's own error, 1.38e-6), the error in the energy norm
@@ -128,7 +128,7 @@ def test_solution_f64_without_split_is_the_fp32_x(monkeypatch):
     assert np.array_equal(solution_f64(pr), pr.mgpcg.x.numpy().astype(np.float64))
 
 
-# ---------------------------------------------------------------- config 3
+# ------------------------------------------------- flow past a sphere, 256^3
 @pytest.fixture(scope="module")
 def sphere_flow_256(oracle):
     """flow_past_sphere(256, 2) and its converged FP64 oracle projection, plus
@@ -147,7 +147,7 @@ def sphere_flow_256(oracle):
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp16", "bf16"])
-def test_config3_sphere_flow_256_vs_fp64_oracle(precision, sphere_flow_256, oracle):
+def test_sphere_flow_256_vs_fp64_oracle(precision, sphere_flow_256, oracle):
     s, p64, p32, it32 = sphere_flow_256
     n = s["n"]
     pr = Projection((n, n, n), n_mg_levels=4, pre_and_post_smoothing=2, bottom_smoothing=6, precision=precision)

@@ -1,4 +1,4 @@
-"""The learned preconditioner on held-out scenes (SURVEY config 4): the
+"""The learned preconditioner on held-out scenes: the
 committed network (trained on seeds 1000-2999 at 128^3) in flexible PCG with
 MGPCG fallback, on seeds >= 3000 at 128^3 and 256^3.
 

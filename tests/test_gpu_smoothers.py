@@ -68,8 +68,7 @@ def _oracle_project(oracle, s, smoother, real, rel_tol, max_iters=-1, mean_f64=F
 
 @pytest.mark.parametrize("n,max_iters", [(32, -1), (64, -1), (64, 4)])
 def test_fraction_solve_dam_break(n, max_iters, oracle):
-    """smoother="fraction" on the dam-break scene (the config-1 scene at 32^3 and
-    64^3) to 1e-6, and for the 4 iterations of the harness's
+    """smoother="fraction" on the dam-break scene at 32^3 and 64^3 to 1e-6, and for the 4 iterations of the harness's
     solve(4, smoother="fraction") (example_fluid.py:43-49)."""
     s = scenes.dam_break(n, 0)
     it, pr = _gpu_project(s, "fraction", max_iters=max_iters)
